@@ -1,0 +1,175 @@
+#!/usr/bin/env python3
+"""Benchmark: particle-steps/s of the per-frame N-body step on one MI355X.
+
+Workload (N=1): the plain GPU configuration -- one collision of 2 x 131,072
+particles (70/10/20 disc/bulge/halo), FP32, theta = 0.5, softening 0.02,
+dt = 0.005, leaf 16, quadrupole group traversal.  Inputs are generated on the
+host and resident in HBM before the timed region; each timed step is a full
+step (bounds, Morton, sort, permute, tree build, aggregation, traversal,
+kick-drift), replayed from a hipGraph.
+
+Prints one JSON line.  `roofline` is the traversal kernel: 20 flop per
+interaction (interactions counted exactly by the kernel) over its average
+duration from HIP events on the engine stream, against the MI355X FP32 peak.
+`cpu_baseline` times the reference's own CPU octree step (compiled from its
+sources into oracle/_ref) on a bounded sample of the same input on the host
+cores; the same leg checks the GPU accelerations against an FP64 direct sum.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(REPO, "galaxy-collision-simulator_amd"))
+
+PEAK_FP32_TFLOPS = 157.3
+FLOP_PER_INTERACTION = 20
+
+CONFIGS = {
+    # name: (particles per galaxy, eps, dt, theta)
+    "plain-2x131072": (131072, 0.02, 0.005, 0.5),
+    "plain-2x8192": (8192, 0.05, 0.01, 0.5),
+    "precision-2x1048576": (1048576, 0.02, 0.005, 0.5),
+}
+
+
+def cpu_baseline(stars, p, acc_gpu, sample_seconds):
+    """Times the reference CPU octree on the host and checks the GPU
+    accelerations against an FP64 direct sum on a subsample."""
+    import numpy as np
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import oracle
+
+    n = len(stars)
+    # accuracy of the GPU step (FP64 direct sum on 4096 targets)
+    tgt = np.linspace(0, n - 1, 4096).astype(np.int64)
+    ref = oracle.direct_sum_f64(stars["x"], stars["y"], stars["z"], stars["mass"], p.G,
+                                float(p.eps2), tgt)
+    d = acc_gpu[tgt, :3].astype(np.float64) - ref[:, :3]
+    err = float(np.sqrt((d ** 2).sum() / (ref[:, :3] ** 2).sum()))
+    rel = np.linalg.norm(d, axis=1) / np.linalg.norm(ref[:, :3], axis=1)
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    kind = "reference" if oracle.ref_available() else "port"
+    step = (lambda s: oracle.ref_octree_step(s, p.G, float(p.eps2), p.dt)) if kind == "reference" \
+        else (lambda s: oracle.octree_step(s, p.G, float(p.eps2), p.dt, 0.8, 32))
+    s = stars
+    s, _, _ = step(s)  # warm-up
+    t0 = time.perf_counter()
+    k = 0
+    while True:
+        s, _, ph = step(s)
+        k += 1
+        if time.perf_counter() - t0 > sample_seconds or k >= 50:
+            break
+    el = time.perf_counter() - t0
+    return {
+        "value": n * k / el,
+        "unit": "particle-steps/s",
+        "cores": threads,
+        "kind": kind,
+        "sample": "%d steps of the reference CPU octree frame (theta=0.8, leaf 32, as compiled "
+                  "in gravity_aggregator.cpp/engine.cpp) on the same %d-particle input, "
+                  "OMP_NUM_THREADS=%d" % (k, n, threads),
+    }, {"rel_rms_vs_fp64": err, "p99_rel_vs_fp64": float(np.quantile(rel, 0.99)),
+        "targets": len(tgt)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--config", default="plain-2x131072", choices=sorted(CONFIGS))
+    ap.add_argument("--leaf", type=int, default=16)
+    ap.add_argument("--block", type=int, default=256)
+    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print("bench.py runs on one GPU: this build has no multi-GPU path "
+              "(independent collisions shard across GPUs without exchange; see DESIGN.md)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import gcs_amd
+
+    per, eps, dt, theta = CONFIGS[a.config]
+    stars = gcs_amd.collision(per, 1002)
+    n = len(stars)
+    eps2 = float(np.float32(eps) ** 2)
+    p = gcs_amd.params(G=1.0, eps2=eps2, dt=dt, theta=theta, leaf_size=a.leaf,
+                       block_size=a.block, use_graph=1)
+    e = gcs_amd.Engine(n)
+    e.load(stars)
+
+    # Accelerations of the initial state (for the accuracy check).
+    e.compute_accels(p)
+    acc0 = e.accels()
+    sorted0 = e.stars()
+
+    e.step(p, a.warmup)
+    e.sync()
+    t0 = time.perf_counter()
+    e.step(p, a.steps, sync=False)
+    e.sync()
+    el = time.perf_counter() - t0
+    ms_per_step = 1e3 * el / a.steps
+    value = n * a.steps / el
+
+    # Roofline of the traversal kernel: event-timed eager steps.
+    pe = gcs_amd.params(G=1.0, eps2=eps2, dt=dt, theta=theta, leaf_size=a.leaf,
+                        block_size=a.block, use_graph=0)
+    trav = []
+    stages = []
+    inter = None
+    for _ in range(5):
+        e.step(pe, 1)
+        st = e.stage_ms()
+        stages.append(st)
+        trav.append(float(st[5]))
+        inter = e.interactions()
+    trav_ms = float(np.mean(trav))
+    n_inter = inter[0] + inter[1]
+    achieved = FLOP_PER_INTERACTION * n_inter / (trav_ms * 1e-3) / 1e12
+    stage_ms = [float(x) for x in np.mean(np.array(stages), 0)]
+    e.close()
+
+    line = {
+        "metric": "particle-steps/s @theta=0.5, err<=1e-3 (plain GPU step)",
+        "value": value,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded two-galaxy generator)",
+        "config": {"workload": a.config, "particles": n, "theta": theta, "eps": eps, "dt": dt,
+                   "leaf": a.leaf, "block": a.block, "multipole": "quadrupole",
+                   "parallelism": "single GPU"},
+        "interactions_per_step": n_inter,
+        "interactions_per_s": n_inter / (trav_ms * 1e-3),
+        "stage_ms": dict(zip(["bounds_morton", "sort", "permute", "build", "aggregate",
+                              "traverse", "integrate"], stage_ms)),
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
+                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
+                     "kernel": "k_traverse_group", "kernel_ms": trav_ms,
+                     "flop_per_interaction": FLOP_PER_INTERACTION},
+        "cpu_baseline": None,
+    }
+    if not a.no_cpu:
+        cb, acc_check = cpu_baseline(sorted0, p, acc0, a.cpu_seconds)
+        line["cpu_baseline"] = cb
+        line["accuracy"] = acc_check
+        line["speedup_vs_cpu"] = value / cb["value"]
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

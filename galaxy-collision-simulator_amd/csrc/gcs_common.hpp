@@ -1,0 +1,73 @@
+// gcs_common.hpp -- layouts shared by the host engine and the gfx950 kernels.
+//
+// Device state (structure of arrays, one entry per particle, all instances
+// back to back; each instance's particles stay contiguous and in Morton order):
+//   posm   float4  x, y, z (FP32 "hi" position), mass
+//   vel    float4  vx, vy, vz, bit pattern of (type | r<<8 | g<<16 | b<<24)
+//   plo    float4  x_lo, y_lo, z_lo, 0   (compensated precision only)
+//   acc    float4  ax, ay, az, potential
+// Tree of instance s with n_s particles starting at offset o_s occupies node
+// indices [2*o_s - s, 2*o_s - s + 2*n_s - 1): internal nodes first, then
+// leaves, i.e. the reference's numbering (karras_bvh.cuh:114-124) shifted.
+//   nodeA  float4  com x, y, z, mass
+//   nodeB  int4    opening value (float bits), first, last, split
+//                  (particle indices, global; split = -1 for a leaf)
+//   nodeQ  2 x float4  Qxx, Qyy, Qzz, Qxy | Qxz, Qyz, width, 0
+//   boxLo/boxHi  float4  AABB of internal nodes
+//   parent int     parent node (-1 for a root)
+//   flag   int     arrival counter of the bottom-up aggregation
+#pragma once
+#include <cstdint>
+
+namespace gcs {
+
+constexpr int kWave = 64;
+constexpr int kTile = 1024;          // particles per sort tile
+constexpr int kTileThreads = 256;    // threads per sort block
+constexpr int kTileItems = kTile / kTileThreads;
+constexpr int kRadixBits = 8;
+constexpr int kRadix = 1 << kRadixBits;
+constexpr int kSortPasses = 8;       // 64 bits (bit 63 of a Morton key is 0)
+constexpr int kGroup = 64;           // particles per traversal group (one wave)
+constexpr int kScanChunk = 4096;     // entries per block in the offset scan
+constexpr int kStackCap = 8192;      // per-wave traversal stack entries
+constexpr int kNumXcd = 8;           // MI355X: 8 XCDs, blocks dealt round-robin
+
+// Sort tile: a run of at most kTile particles of one instance.
+struct TileInfo {
+    int start;      // first particle (global index)
+    int count;      // particles in the tile
+    int seg;        // instance slot
+    int hist_base;  // kRadix * (global index of the instance's first tile)
+    int seg_tiles;  // tiles of this instance
+    int tile_in_seg;
+    int pad0, pad1;
+};
+
+// Instance slot.
+struct SegInfo {
+    int start;      // first particle
+    int n;          // particles
+    int int_base;   // internal node of particle p: p + int_base (= start - slot)
+    int leaf_base;  // leaf node of particle p: p + leaf_base (= start + n - slot - 1)
+    int root;       // 2*start - slot
+    int tile0;      // global index of the first tile
+    int ntiles;
+    int pad;
+};
+
+// Traversal group: up to kGroup consecutive particles of one instance.
+struct GroupInfo {
+    int first;
+    int count;
+    int seg;
+    int pad;
+};
+
+// Error bits of the device error word.
+enum : unsigned {
+    kErrStack = 1u,
+    kErrNonFinite = 2u,
+};
+
+}  // namespace gcs

@@ -1,0 +1,1041 @@
+// gcs_engine.hip -- host engine and C ABI of the MI355X per-frame N-body step.
+//
+// Replaces the reference's cuda_engine.cu:219-451 (init, upload, the
+// per-frame step and read-back) with an engine object that owns its device
+// buffers and HIP stream, holds any number of independent instances back to
+// back, and queues each step without host synchronisation (optionally as a
+// replayed hipGraph).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "gcs.h"
+#include "gcs_common.hpp"
+#include "gcs_kernels.hpp"
+#include "gcs_sort.hpp"
+#include "gcs_traverse.hpp"
+
+using namespace gcs;
+
+static_assert(sizeof(gcs_star) == 32, "gcs_star must match the reference Star (types.h:5-12)");
+static_assert(sizeof(gcs_playback_star) == 16, "gcs_playback_star must match PlaybackStar");
+static_assert(kRadix == kTileThreads, "one digit per scatter thread");
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg)
+{
+    g_err = msg;
+    return code;
+}
+
+#define HIPCHK(expr)                                                                     \
+    do {                                                                                 \
+        hipError_t e_ = (expr);                                                          \
+        if (e_ != hipSuccess)                                                            \
+            return fail(e_ == hipErrorOutOfMemory ? GCS_ERR_OOM : GCS_ERR_HIP,           \
+                        std::string(#expr) + ": " + hipGetErrorString(e_));              \
+    } while (0)
+
+#define LAUNCHCHK()                                                                      \
+    do {                                                                                 \
+        hipError_t e_ = hipGetLastError();                                               \
+        if (e_ != hipSuccess)                                                            \
+            return fail(GCS_ERR_HIP, std::string("kernel launch: ") + hipGetErrorString(e_)); \
+    } while (0)
+
+template <typename T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t n = 0;
+};
+
+struct Instance {
+    int32_t id;
+    int32_t n;
+};
+
+size_t max_tiles(int64_t cap_n, int32_t cap_k) { return (size_t)(cap_n / kTile) + (size_t)cap_k + 1; }
+size_t max_groups(int64_t cap_n, int32_t cap_k) { return (size_t)(cap_n / kGroup) + (size_t)cap_k + 1; }
+size_t hist_entries(int64_t cap_n, int32_t cap_k) { return (size_t)kRadix * max_tiles(cap_n, cap_k); }
+size_t scan_blocks(size_t entries) { return (entries + kScanChunk - 1) / kScanChunk; }
+
+}  // namespace
+
+struct gcs_engine {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    int64_t cap_n = 0;
+    int32_t cap_k = 0;
+    int num_cu = 256;
+    int trav_blocks_cap = 0;  // persistent traversal blocks the stack memory covers
+
+    // state (double-buffered by the permute)
+    float4* posm[2] = {nullptr, nullptr};
+    float4* vel[2] = {nullptr, nullptr};
+    float4* plo[2] = {nullptr, nullptr};
+    int cur = 0;
+    float4* acc = nullptr;
+    uint64_t* keys[2] = {nullptr, nullptr};
+    uint32_t* vals[2] = {nullptr, nullptr};
+    uint64_t* keys_raw = nullptr;
+    // tree
+    float4* nodeA = nullptr;
+    int4* nodeB = nullptr;
+    float4* nodeQ = nullptr;
+    float4* boxLo = nullptr;
+    float4* boxHi = nullptr;
+    int* parent = nullptr;
+    int* flag = nullptr;
+    // tables
+    TileInfo* tiles = nullptr;
+    SegInfo* segs = nullptr;
+    GroupInfo* groups = nullptr;
+    uint32_t* hist = nullptr;
+    uint32_t* partial = nullptr;
+    uint32_t* seg_bounds = nullptr;
+    unsigned* qctr = nullptr;
+    unsigned long long* counters = nullptr;
+    unsigned* err = nullptr;
+    double* diag_part = nullptr;
+    double* diag_out = nullptr;
+    int* stack_mem = nullptr;
+    uint8_t* staging = nullptr;
+
+    // host mirror of the composition
+    std::vector<Instance> inst;
+    std::vector<SegInfo> h_segs;
+    std::vector<TileInfo> h_tiles;
+    std::vector<GroupInfo> h_groups;
+    int64_t n_total = 0;
+    int32_t next_id = 0;
+    bool have_tree = false;
+    bool capture_keys = false;
+    int last_mac = GCS_MAC_GROUP;
+
+    // graphs, one per state parity
+    hipGraphExec_t gexec[2] = {nullptr, nullptr};
+    gcs_params gparams;
+    bool graph_valid = false;
+
+    hipEvent_t ev[8] = {};
+    bool timed = false;
+};
+
+namespace {
+
+template <typename T>
+int dalloc(T** p, size_t count)
+{
+    *p = nullptr;
+    if (count == 0) count = 1;
+    hipError_t e = hipMalloc((void**)p, sizeof(T) * count);
+    if (e != hipSuccess) {
+        *p = nullptr;
+        return fail(GCS_ERR_OOM, std::string("hipMalloc of ") + std::to_string(sizeof(T) * count) +
+                                     " bytes: " + hipGetErrorString(e));
+    }
+    return GCS_OK;
+}
+
+template <typename T>
+void dfree(T*& p)
+{
+    if (p) (void)hipFree((void*)p);
+    p = nullptr;
+}
+
+void free_all(gcs_engine* e)
+{
+    for (int k = 0; k < 2; ++k) {
+        dfree(e->posm[k]); dfree(e->vel[k]); dfree(e->plo[k]);
+        dfree(e->keys[k]); dfree(e->vals[k]);
+        if (e->gexec[k]) (void)hipGraphExecDestroy(e->gexec[k]);
+        e->gexec[k] = nullptr;
+    }
+    dfree(e->acc); dfree(e->keys_raw);
+    dfree(e->nodeA); dfree(e->nodeB); dfree(e->nodeQ); dfree(e->boxLo); dfree(e->boxHi);
+    dfree(e->parent); dfree(e->flag);
+    dfree(e->tiles); dfree(e->segs); dfree(e->groups); dfree(e->hist); dfree(e->partial);
+    dfree(e->seg_bounds); dfree(e->qctr); dfree(e->counters); dfree(e->err);
+    dfree(e->diag_part); dfree(e->diag_out); dfree(e->stack_mem); dfree(e->staging);
+    for (auto& v : e->ev)
+        if (v) (void)hipEventDestroy(v), v = nullptr;
+}
+
+int grid_traverse_blocks(gcs_engine* e, int wpb)
+{
+    (void)wpb;
+    return e->trav_blocks_cap;
+}
+
+// Rebuilds the instance, tile and group tables after the composition changed.
+int rebuild_tables(gcs_engine* e)
+{
+    e->h_segs.clear();
+    e->h_tiles.clear();
+    e->h_groups.clear();
+    int start = 0;
+    for (size_t s = 0; s < e->inst.size(); ++s) {
+        const int n = e->inst[s].n;
+        SegInfo sg;
+        sg.start = start;
+        sg.n = n;
+        sg.int_base = start - (int)s;
+        sg.leaf_base = start + n - (int)s - 1;
+        sg.root = 2 * start - (int)s;
+        sg.tile0 = (int)e->h_tiles.size();
+        sg.ntiles = (n + kTile - 1) / kTile;
+        sg.pad = 0;
+        for (int j = 0; j < sg.ntiles; ++j) {
+            TileInfo t;
+            t.start = start + j * kTile;
+            t.count = std::min(kTile, n - j * kTile);
+            t.seg = (int)s;
+            t.hist_base = kRadix * sg.tile0;
+            t.seg_tiles = sg.ntiles;
+            t.tile_in_seg = j;
+            t.pad0 = t.pad1 = 0;
+            e->h_tiles.push_back(t);
+        }
+        for (int j = 0; j < n; j += kGroup) {
+            GroupInfo g;
+            g.first = start + j;
+            g.count = std::min(kGroup, n - j);
+            g.seg = (int)s;
+            g.pad = 0;
+            e->h_groups.push_back(g);
+        }
+        e->h_segs.push_back(sg);
+        start += n;
+    }
+    e->n_total = start;
+    if (!e->h_segs.empty()) {
+        HIPCHK(hipMemcpy(e->segs, e->h_segs.data(), sizeof(SegInfo) * e->h_segs.size(),
+                         hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(e->tiles, e->h_tiles.data(), sizeof(TileInfo) * e->h_tiles.size(),
+                         hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(e->groups, e->h_groups.data(), sizeof(GroupInfo) * e->h_groups.size(),
+                         hipMemcpyHostToDevice));
+    }
+    e->graph_valid = false;
+    e->have_tree = false;
+    return GCS_OK;
+}
+
+int check_params(const gcs_params* p)
+{
+    if (!p) return fail(GCS_ERR_INVALID, "params is null");
+    if (!std::isfinite(p->G) || !std::isfinite(p->eps2) || !std::isfinite(p->dt) ||
+        !std::isfinite(p->theta))
+        return fail(GCS_ERR_INVALID, "non-finite parameter");
+    if (p->eps2 < 0.0f) return fail(GCS_ERR_INVALID, "eps2 must be >= 0");
+    if (p->mac != GCS_MAC_REFERENCE && p->mac != GCS_MAC_GROUP)
+        return fail(GCS_ERR_INVALID, "unknown opening criterion");
+    if (p->mac == GCS_MAC_GROUP) {
+        if (!(p->eps2 > 0.0f)) return fail(GCS_ERR_INVALID, "group traversal needs eps2 > 0");
+        if (p->theta < 0.0f || p->theta > 1.0f)
+            return fail(GCS_ERR_INVALID, "theta must be in [0, 1]");
+        if (p->leaf_size < 1 || p->leaf_size > 4096)
+            return fail(GCS_ERR_INVALID, "leaf_size must be in [1, 4096]");
+        if (p->block_size != 64 && p->block_size != 128 && p->block_size != 256)
+            return fail(GCS_ERR_INVALID, "block_size must be 64, 128 or 256");
+    } else {
+        if (p->stack_depth < 2 || p->stack_depth > 1024)
+            return fail(GCS_ERR_INVALID, "stack_depth must be in [2, 1024]");
+    }
+    if (p->precision != GCS_PREC_PLAIN && p->precision != GCS_PREC_COMPENSATED)
+        return fail(GCS_ERR_INVALID, "unknown precision mode");
+    if (p->use_bounds && !(std::isfinite(p->global_min) && std::isfinite(p->global_max)))
+        return fail(GCS_ERR_INVALID, "non-finite bounds");
+    return GCS_OK;
+}
+
+template <bool POT, bool QUAD, int WPB>
+void launch_group(gcs_engine* e, int blocks, const gcs_params& p, float4* posm)
+{
+    k_traverse_group<POT, QUAD, WPB><<<blocks, WPB * kWave, 0, e->stream>>>(
+        e->groups, (int)e->h_groups.size(), e->segs, posm, e->nodeA, e->nodeB, e->nodeQ, e->acc,
+        e->stack_mem, e->qctr, e->counters, e->err, p.G, p.eps2, p.leaf_size);
+}
+
+template <int STACK>
+void launch_reference(gcs_engine* e, const gcs_params& p, float4* posm, int nblk)
+{
+    k_traverse_reference<STACK><<<nblk, kTileThreads, 0, e->stream>>>(
+        e->tiles, e->segs, posm, e->nodeA, e->nodeB, e->acc, e->counters, e->err, p.G, p.eps2,
+        std::min(p.stack_depth, STACK));
+}
+
+void mark(gcs_engine* e, int k)
+{
+    if (e->timed) (void)hipEventRecord(e->ev[k], e->stream);
+}
+
+// Queues one step on the stream.  Reads state[cur], leaves the result in
+// state[cur ^ 1] and flips cur.
+int enqueue_step(gcs_engine* e, const gcs_params& p, bool integrate)
+{
+    const int ntiles = (int)e->h_tiles.size();
+    const int nseg = (int)e->h_segs.size();
+    if (ntiles == 0) return GCS_OK;
+    const int qblocks = 4 * ntiles;
+    const int c = e->cur;
+    const bool comp = p.precision == GCS_PREC_COMPENSATED;
+    const bool group = p.mac == GCS_MAC_GROUP;
+
+    mark(e, 0);
+    k_reset<<<(nseg + 255) / 256, 256, 0, e->stream>>>(nseg, e->seg_bounds, e->qctr, e->counters);
+    if (!p.use_bounds)
+        k_bounds<<<qblocks, kTileThreads, 0, e->stream>>>(e->tiles, e->segs, e->posm[c],
+                                                           e->seg_bounds);
+    k_morton<<<qblocks, kTileThreads, 0, e->stream>>>(e->tiles, e->segs, e->posm[c],
+                                                       e->seg_bounds, p.use_bounds, p.global_min,
+                                                       p.global_max, e->keys[0], e->vals[0]);
+    LAUNCHCHK();
+    if (e->capture_keys)
+        HIPCHK(hipMemcpyAsync(e->keys_raw, e->keys[0], sizeof(uint64_t) * e->n_total,
+                              hipMemcpyDeviceToDevice, e->stream));
+    mark(e, 1);
+    const int64_t hn = (int64_t)kRadix * ntiles;
+    const int sblocks = (int)scan_blocks((size_t)hn);
+    for (int pass = 0; pass < kSortPasses; ++pass) {
+        const int shift = pass * kRadixBits;
+        const int src = pass & 1, dst = src ^ 1;
+        k_sort_count<<<ntiles, kTileThreads, 0, e->stream>>>(e->tiles, e->keys[src], e->hist,
+                                                              shift);
+        k_scan_reduce<<<sblocks, 256, 0, e->stream>>>(e->hist, hn, e->partial);
+        k_scan_partials<<<1, 256, 0, e->stream>>>(e->partial, sblocks);
+        k_scan_apply<<<sblocks, 256, 0, e->stream>>>(e->hist, hn, e->partial);
+        k_sort_scatter<<<ntiles, kTileThreads, 0, e->stream>>>(
+            e->tiles, e->keys[src], e->vals[src], e->keys[dst], e->vals[dst], e->hist, shift);
+    }
+    LAUNCHCHK();
+    static_assert(kSortPasses % 2 == 0, "sorted keys end in buffer 0");
+    mark(e, 2);
+    k_permute<<<qblocks, kTileThreads, 0, e->stream>>>(
+        e->tiles, e->segs, e->vals[0], e->posm[c], e->vel[c], e->plo[c], e->posm[c ^ 1],
+        e->vel[c ^ 1], e->plo[c ^ 1], comp ? 1 : 0, group ? 0 : 1, e->nodeA, e->nodeB, e->nodeQ);
+    e->cur = c ^ 1;
+    float4* posm = e->posm[e->cur];
+    mark(e, 3);
+    k_build<<<qblocks, kTileThreads, 0, e->stream>>>(e->tiles, e->segs, e->keys[0], e->nodeB,
+                                                     e->parent, e->flag);
+    mark(e, 4);
+    const float inv_theta = p.theta > 0.0f ? 1.0f / p.theta : __builtin_inff();
+    k_aggregate<<<qblocks, kTileThreads, 0, e->stream>>>(
+        e->tiles, e->segs, posm, e->nodeA, e->nodeB, e->nodeQ, e->boxLo, e->boxHi, e->parent,
+        e->flag, inv_theta, group ? 1 : 0);
+    LAUNCHCHK();
+    mark(e, 5);
+    if (group) {
+        const int wpb = p.block_size / kWave;
+        int blocks = std::min(grid_traverse_blocks(e, wpb) * (4 / wpb),
+                              (int)((e->h_groups.size() + wpb - 1) / wpb));
+        blocks = std::max(blocks, 1);
+        const bool pot = p.diagnostics != 0;
+        // Stack slots: blocks * wpb waves <= trav_blocks_cap * 4.
+        if (wpb == 4) {
+            if (pot) launch_group<true, true, 4>(e, blocks, p, posm);
+            else launch_group<false, true, 4>(e, blocks, p, posm);
+        } else if (wpb == 2) {
+            if (pot) launch_group<true, true, 2>(e, blocks, p, posm);
+            else launch_group<false, true, 2>(e, blocks, p, posm);
+        } else {
+            if (pot) launch_group<true, true, 1>(e, blocks, p, posm);
+            else launch_group<false, true, 1>(e, blocks, p, posm);
+        }
+    } else {
+        if (p.stack_depth <= 64) launch_reference<64>(e, p, posm, qblocks);
+        else if (p.stack_depth <= 128) launch_reference<128>(e, p, posm, qblocks);
+        else if (p.stack_depth <= 256) launch_reference<256>(e, p, posm, qblocks);
+        else launch_reference<1024>(e, p, posm, qblocks);
+    }
+    LAUNCHCHK();
+    mark(e, 6);
+    if (integrate) {
+        k_integrate<<<qblocks, kTileThreads, 0, e->stream>>>(
+            e->tiles, e->segs, e->acc, posm, e->vel[e->cur], e->plo[e->cur], p.dt, comp ? 1 : 0,
+            p.colours, p.diagnostics, e->diag_part, e->err);
+        if (p.diagnostics)
+            k_diag_final<<<nseg, 64, 0, e->stream>>>(e->segs, nseg, e->diag_part, e->diag_out);
+        LAUNCHCHK();
+    }
+    mark(e, 7);
+    e->have_tree = true;
+    e->last_mac = p.mac;
+    return GCS_OK;
+}
+
+int capture_graphs(gcs_engine* e, const gcs_params& p)
+{
+    for (int k = 0; k < 2; ++k) {
+        if (e->gexec[k]) (void)hipGraphExecDestroy(e->gexec[k]);
+        e->gexec[k] = nullptr;
+    }
+    const int saved = e->cur;
+    for (int k = 0; k < 2; ++k) {
+        e->cur = k;
+        hipGraph_t g = nullptr;
+        HIPCHK(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+        int rc = enqueue_step(e, p, true);
+        hipError_t ec = hipStreamEndCapture(e->stream, &g);
+        if (rc != GCS_OK) {
+            if (g) (void)hipGraphDestroy(g);
+            e->cur = saved;
+            return rc;
+        }
+        if (ec != hipSuccess) {
+            e->cur = saved;
+            return fail(GCS_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(ec));
+        }
+        hipError_t ei = hipGraphInstantiate(&e->gexec[k], g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (ei != hipSuccess) {
+            e->cur = saved;
+            return fail(GCS_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ei));
+        }
+    }
+    e->cur = saved;
+    e->gparams = p;
+    e->graph_valid = true;
+    return GCS_OK;
+}
+
+int find_slot(gcs_engine* e, int32_t id)
+{
+    for (size_t s = 0; s < e->inst.size(); ++s)
+        if (e->inst[s].id == id) return (int)s;
+    return -1;
+}
+
+int upload_at(gcs_engine* e, const gcs_star* stars, int64_t n, int64_t offset)
+{
+    if (n <= 0) return GCS_OK;
+    HIPCHK(hipMemcpyAsync(e->staging, stars, sizeof(gcs_star) * (size_t)n, hipMemcpyHostToDevice,
+                          e->stream));
+    const int c = e->cur;
+    k_unpack_stars<<<(unsigned)((n + 255) / 256), 256, 0, e->stream>>>(e->staging, n, offset,
+                                                                       e->posm[c], e->vel[c],
+                                                                       e->plo[c]);
+    LAUNCHCHK();
+    HIPCHK(hipStreamSynchronize(e->stream));
+    return GCS_OK;
+}
+
+}  // namespace
+
+// ============================================================================
+// C ABI
+// ============================================================================
+
+extern "C" {
+
+const char* gcs_last_error(void) { return g_err.c_str(); }
+
+void gcs_params_default(gcs_params* p)
+{
+    if (!p) return;
+    std::memset(p, 0, sizeof(*p));
+    p->G = 1.0f;
+    p->eps2 = 0.0025f;
+    p->dt = 0.01f;
+    p->theta = 0.5f;
+    p->mac = GCS_MAC_GROUP;
+    p->precision = GCS_PREC_PLAIN;
+    p->leaf_size = 16;
+    p->block_size = 256;
+    p->stack_depth = 256;
+    p->use_bounds = 0;
+    p->colours = 1;
+    p->diagnostics = 0;
+    p->use_graph = 1;
+}
+
+int64_t gcs_device_bytes(int64_t cap_n, int32_t cap_k)
+{
+    if (cap_n < 1 || cap_k < 1) return 0;
+    const int64_t n = cap_n;
+    int64_t b = 0;
+    b += 6 * 16 * n;           // posm, vel, plo x2
+    b += 16 * n;               // acc
+    b += 3 * 8 * n + 2 * 4 * n; // keys x2 + raw, vals x2
+    b += 2 * n * (16 + 16 + 32 + 16 + 16 + 4 + 4);  // nodes
+    b += (int64_t)(sizeof(TileInfo) * max_tiles(cap_n, cap_k) + sizeof(SegInfo) * cap_k +
+                   sizeof(GroupInfo) * max_groups(cap_n, cap_k));
+    b += (int64_t)(4 * hist_entries(cap_n, cap_k) + 4 * scan_blocks(hist_entries(cap_n, cap_k)));
+    b += 8 * cap_k + 64 + 5 * 8 * 4 * (int64_t)max_tiles(cap_n, cap_k) + 5 * 8 * cap_k;
+    b += 32 * n;               // staging
+    return b;
+}
+
+int gcs_create(int64_t cap_n, int32_t cap_k, int32_t device, gcs_engine** out)
+{
+    if (!out) return fail(GCS_ERR_INVALID, "out is null");
+    *out = nullptr;
+    if (cap_n < 1 || cap_k < 1 || cap_n > (int64_t)1 << 30)
+        return fail(GCS_ERR_INVALID, "capacity out of range");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+        return fail(GCS_ERR_NO_DEVICE, "no HIP device");
+    if (device < 0 || device >= ndev) return fail(GCS_ERR_NO_DEVICE, "device ordinal out of range");
+    HIPCHK(hipSetDevice(device));
+    gcs_engine* e = new (std::nothrow) gcs_engine();
+    if (!e) return fail(GCS_ERR_OOM, "host allocation failed");
+    e->device = device;
+    e->cap_n = cap_n;
+    e->cap_k = cap_k;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
+        e->num_cu = prop.multiProcessorCount;
+    int occ = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_traverse_group<true, true, 4>, 256, 0) !=
+            hipSuccess ||
+        occ < 1)
+        occ = 2;
+    e->trav_blocks_cap = e->num_cu * occ;
+    const size_t nt = max_tiles(cap_n, cap_k), ng = max_groups(cap_n, cap_k);
+    const size_t he = hist_entries(cap_n, cap_k);
+    int rc = GCS_OK;
+    const size_t n = (size_t)cap_n;
+    for (int k = 0; k < 2 && rc == GCS_OK; ++k) {
+        if ((rc = dalloc(&e->posm[k], n))) break;
+        if ((rc = dalloc(&e->vel[k], n))) break;
+        if ((rc = dalloc(&e->plo[k], n))) break;
+        if ((rc = dalloc(&e->keys[k], n))) break;
+        if ((rc = dalloc(&e->vals[k], n))) break;
+    }
+    if (!rc) rc = dalloc(&e->acc, n);
+    if (!rc) rc = dalloc(&e->keys_raw, n);
+    if (!rc) rc = dalloc(&e->nodeA, 2 * n);
+    if (!rc) rc = dalloc(&e->nodeB, 2 * n);
+    if (!rc) rc = dalloc(&e->nodeQ, 4 * n);
+    if (!rc) rc = dalloc(&e->boxLo, 2 * n);
+    if (!rc) rc = dalloc(&e->boxHi, 2 * n);
+    if (!rc) rc = dalloc(&e->parent, 2 * n);
+    if (!rc) rc = dalloc(&e->flag, 2 * n);
+    if (!rc) rc = dalloc(&e->tiles, nt);
+    if (!rc) rc = dalloc(&e->segs, (size_t)cap_k);
+    if (!rc) rc = dalloc(&e->groups, ng);
+    if (!rc) rc = dalloc(&e->hist, he);
+    if (!rc) rc = dalloc(&e->partial, scan_blocks(he) + 1);
+    if (!rc) rc = dalloc(&e->seg_bounds, 2 * (size_t)cap_k);
+    if (!rc) rc = dalloc(&e->qctr, kNumXcd);
+    if (!rc) rc = dalloc(&e->counters, 2);
+    if (!rc) rc = dalloc(&e->err, 1);
+    if (!rc) rc = dalloc(&e->diag_part, 5 * 4 * nt);
+    if (!rc) rc = dalloc(&e->diag_out, 5 * (size_t)cap_k);
+    if (!rc) rc = dalloc(&e->stack_mem, (size_t)e->trav_blocks_cap * 4 * kStackCap);
+    if (!rc) rc = dalloc(&e->staging, 32 * n);
+    if (rc) {
+        free_all(e);
+        delete e;
+        return rc;
+    }
+    if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess) {
+        free_all(e);
+        delete e;
+        return fail(GCS_ERR_HIP, "hipStreamCreate failed");
+    }
+    e->own_stream = true;
+    for (auto& v : e->ev)
+        if (hipEventCreate(&v) != hipSuccess) {
+            (void)hipStreamDestroy(e->stream);
+            free_all(e);
+            delete e;
+            return fail(GCS_ERR_HIP, "hipEventCreate failed");
+        }
+    if (hipMemset(e->err, 0, sizeof(unsigned)) != hipSuccess ||
+        hipMemset(e->counters, 0, 2 * sizeof(unsigned long long)) != hipSuccess) {
+        (void)hipStreamDestroy(e->stream);
+        free_all(e);
+        delete e;
+        return fail(GCS_ERR_HIP, "hipMemset failed");
+    }
+    gcs_params_default(&e->gparams);
+    *out = e;
+    return GCS_OK;
+}
+
+int gcs_destroy(gcs_engine* e)
+{
+    if (!e) return GCS_OK;
+    (void)hipSetDevice(e->device);
+    if (e->stream) (void)hipStreamSynchronize(e->stream);
+    free_all(e);
+    if (e->own_stream && e->stream) (void)hipStreamDestroy(e->stream);
+    delete e;
+    return GCS_OK;
+}
+
+int gcs_set_stream(gcs_engine* e, void* s)
+{
+    if (!e) return fail(GCS_ERR_INVALID, "engine is null");
+    HIPCHK(hipStreamSynchronize(e->stream));
+    if (s) {
+        if (e->own_stream) (void)hipStreamDestroy(e->stream);
+        e->stream = (hipStream_t)s;
+        e->own_stream = false;
+    } else if (!e->own_stream) {
+        HIPCHK(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
+        e->own_stream = true;
+    }
+    e->graph_valid = false;
+    return GCS_OK;
+}
+
+int gcs_load(gcs_engine* e, const gcs_star* stars, const int32_t* counts, int32_t k)
+{
+    if (!e || (!stars && k > 0) || (!counts && k > 0)) return fail(GCS_ERR_INVALID, "null argument");
+    if (k < 0 || k > e->cap_k) return fail(GCS_ERR_CAPACITY, "instance count exceeds capacity");
+    int64_t total = 0;
+    for (int32_t s = 0; s < k; ++s) {
+        if (counts[s] < 1) return fail(GCS_ERR_INVALID, "every instance needs at least 1 particle");
+        total += counts[s];
+    }
+    if (total > e->cap_n) return fail(GCS_ERR_CAPACITY, "particles exceed capacity");
+    HIPCHK(hipSetDevice(e->device));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    e->inst.clear();
+    for (int32_t s = 0; s < k; ++s) e->inst.push_back({s, counts[s]});
+    e->next_id = k;
+    e->cur = 0;
+    int rc = upload_at(e, stars, total, 0);
+    if (rc) return rc;
+    return rebuild_tables(e);
+}
+
+int gcs_instance_add(gcs_engine* e, const gcs_star* stars, int32_t n, int32_t* id)
+{
+    if (!e || !stars || n < 1) return fail(GCS_ERR_INVALID, "bad argument");
+    if ((int32_t)e->inst.size() + 1 > e->cap_k) return fail(GCS_ERR_CAPACITY, "instance slots full");
+    if (e->n_total + n > e->cap_n) return fail(GCS_ERR_CAPACITY, "particle capacity full");
+    HIPCHK(hipSetDevice(e->device));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    int rc = upload_at(e, stars, n, e->n_total);
+    if (rc) return rc;
+    const int32_t nid = e->next_id++;
+    e->inst.push_back({nid, n});
+    if (id) *id = nid;
+    return rebuild_tables(e);
+}
+
+int gcs_instance_remove(gcs_engine* e, int32_t id)
+{
+    if (!e) return fail(GCS_ERR_INVALID, "engine is null");
+    const int slot = find_slot(e, id);
+    if (slot < 0) return fail(GCS_ERR_NOT_FOUND, "unknown instance id");
+    HIPCHK(hipSetDevice(e->device));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    if (slot + 1 < (int)e->inst.size()) {
+        // Compact into the other state buffer, then switch to it.
+        const int c = e->cur, d = c ^ 1;
+        int64_t src = 0, dst = 0;
+        for (size_t s = 0; s < e->inst.size(); ++s) {
+            const int64_t n = e->inst[s].n;
+            if ((int)s != slot) {
+                HIPCHK(hipMemcpyAsync(e->posm[d] + dst, e->posm[c] + src, 16 * n,
+                                      hipMemcpyDeviceToDevice, e->stream));
+                HIPCHK(hipMemcpyAsync(e->vel[d] + dst, e->vel[c] + src, 16 * n,
+                                      hipMemcpyDeviceToDevice, e->stream));
+                HIPCHK(hipMemcpyAsync(e->plo[d] + dst, e->plo[c] + src, 16 * n,
+                                      hipMemcpyDeviceToDevice, e->stream));
+                dst += n;
+            }
+            src += n;
+        }
+        HIPCHK(hipStreamSynchronize(e->stream));
+        e->cur = d;
+    }
+    e->inst.erase(e->inst.begin() + slot);
+    return rebuild_tables(e);
+}
+
+int gcs_instance_count(gcs_engine* e, int32_t* k, int64_t* n_total)
+{
+    if (!e) return fail(GCS_ERR_INVALID, "engine is null");
+    if (k) *k = (int32_t)e->inst.size();
+    if (n_total) *n_total = e->n_total;
+    return GCS_OK;
+}
+
+int gcs_instance_info(gcs_engine* e, int32_t id, int32_t* n, int64_t* offset)
+{
+    if (!e) return fail(GCS_ERR_INVALID, "engine is null");
+    const int slot = find_slot(e, id);
+    if (slot < 0) return fail(GCS_ERR_NOT_FOUND, "unknown instance id");
+    if (n) *n = e->h_segs[slot].n;
+    if (offset) *offset = e->h_segs[slot].start;
+    return GCS_OK;
+}
+
+static int read_stars_range(gcs_engine* e, int64_t off, int64_t n, gcs_star* out)
+{
+    if (n <= 0) return GCS_OK;
+    HIPCHK(hipSetDevice(e->device));
+    const int c = e->cur;
+    k_pack_stars<<<(unsigned)((n + 255) / 256), 256, 0, e->stream>>>(n, off, e->posm[c], e->vel[c],
+                                                                      e->staging);
+    LAUNCHCHK();
+    HIPCHK(hipMemcpyAsync(out, e->staging, 32 * (size_t)n, hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    return GCS_OK;
+}
+
+int gcs_read_stars(gcs_engine* e, gcs_star* out)
+{
+    if (!e || !out) return fail(GCS_ERR_INVALID, "null argument");
+    return read_stars_range(e, 0, e->n_total, out);
+}
+
+int gcs_read_instance(gcs_engine* e, int32_t id, gcs_star* out)
+{
+    if (!e || !out) return fail(GCS_ERR_INVALID, "null argument");
+    const int slot = find_slot(e, id);
+    if (slot < 0) return fail(GCS_ERR_NOT_FOUND, "unknown instance id");
+    return read_stars_range(e, e->h_segs[slot].start, e->h_segs[slot].n, out);
+}
+
+int gcs_read_position_lo(gcs_engine* e, float* out)
+{
+    if (!e || !out) return fail(GCS_ERR_INVALID, "null argument");
+    HIPCHK(hipSetDevice(e->device));
+    std::vector<float4> tmp((size_t)e->n_total);
+    if (e->n_total)
+        HIPCHK(hipMemcpyAsync(tmp.data(), e->plo[e->cur], 16 * (size_t)e->n_total,
+                              hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    for (int64_t i = 0; i < e->n_total; ++i) {
+        out[3 * i] = tmp[(size_t)i].x;
+        out[3 * i + 1] = tmp[(size_t)i].y;
+        out[3 * i + 2] = tmp[(size_t)i].z;
+    }
+    return GCS_OK;
+}
+
+int gcs_read_playback(gcs_engine* e, gcs_playback_star* out)
+{
+    if (!e || !out) return fail(GCS_ERR_INVALID, "null argument");
+    const int64_t n = e->n_total;
+    if (n == 0) return GCS_OK;
+    HIPCHK(hipSetDevice(e->device));
+    const int c = e->cur;
+    k_pack_playback<<<(unsigned)((n + 255) / 256), 256, 0, e->stream>>>(n, 0, e->posm[c],
+                                                                         e->vel[c], e->staging);
+    LAUNCHCHK();
+    HIPCHK(hipMemcpyAsync(out, e->staging, 16 * (size_t)n, hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    return GCS_OK;
+}
+
+int gcs_step(gcs_engine* e, const gcs_params* p, int32_t nsteps)
+{
+    if (!e) return fail(GCS_ERR_INVALID, "engine is null");
+    int rc = check_params(p);
+    if (rc) return rc;
+    if (nsteps < 0) return fail(GCS_ERR_INVALID, "nsteps < 0");
+    if (e->n_total == 0 || nsteps == 0) return GCS_OK;
+    HIPCHK(hipSetDevice(e->device));
+    e->capture_keys = false;
+    if (p->use_graph) {
+        if (!e->graph_valid || std::memcmp(&e->gparams, p, sizeof(gcs_params)) != 0) {
+            e->timed = false;
+            rc = capture_graphs(e, *p);
+            if (rc) return rc;
+        }
+        for (int32_t s = 0; s < nsteps; ++s) {
+            HIPCHK(hipGraphLaunch(e->gexec[e->cur], e->stream));
+            e->cur ^= 1;
+        }
+        e->have_tree = true;
+        e->last_mac = p->mac;
+        return GCS_OK;
+    }
+    for (int32_t s = 0; s < nsteps; ++s) {
+        e->timed = (s == nsteps - 1);
+        rc = enqueue_step(e, *p, true);
+        e->timed = false;
+        if (rc) return rc;
+    }
+    return GCS_OK;
+}
+
+int gcs_sync(gcs_engine* e)
+{
+    if (!e) return fail(GCS_ERR_INVALID, "engine is null");
+    HIPCHK(hipSetDevice(e->device));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    unsigned h = 0;
+    HIPCHK(hipMemcpy(&h, e->err, sizeof(unsigned), hipMemcpyDeviceToHost));
+    if (h) {
+        HIPCHK(hipMemset(e->err, 0, sizeof(unsigned)));
+        if (h & kErrStack) return fail(GCS_ERR_STACK, "traversal stack overflow");
+        return fail(GCS_ERR_NONFINITE, "non-finite acceleration");
+    }
+    return GCS_OK;
+}
+
+int gcs_compute_accels(gcs_engine* e, const gcs_params* p)
+{
+    if (!e) return fail(GCS_ERR_INVALID, "engine is null");
+    int rc = check_params(p);
+    if (rc) return rc;
+    if (e->n_total == 0) return GCS_OK;
+    HIPCHK(hipSetDevice(e->device));
+    e->capture_keys = true;
+    e->timed = true;
+    rc = enqueue_step(e, *p, false);
+    e->capture_keys = false;
+    e->timed = false;
+    if (rc) return rc;
+    return gcs_sync(e);
+}
+
+int gcs_read_accels(gcs_engine* e, float* out)
+{
+    if (!e || !out) return fail(GCS_ERR_INVALID, "null argument");
+    HIPCHK(hipSetDevice(e->device));
+    if (e->n_total)
+        HIPCHK(hipMemcpyAsync(out, e->acc, 16 * (size_t)e->n_total, hipMemcpyDeviceToHost,
+                              e->stream));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    return GCS_OK;
+}
+
+int gcs_read_keys(gcs_engine* e, uint64_t* keys_unsorted, uint64_t* keys_sorted, int32_t* perm)
+{
+    if (!e) return fail(GCS_ERR_INVALID, "engine is null");
+    HIPCHK(hipSetDevice(e->device));
+    const size_t n = (size_t)e->n_total;
+    if (n == 0) return GCS_OK;
+    if (keys_unsorted)
+        HIPCHK(hipMemcpyAsync(keys_unsorted, e->keys_raw, 8 * n, hipMemcpyDeviceToHost, e->stream));
+    if (keys_sorted)
+        HIPCHK(hipMemcpyAsync(keys_sorted, e->keys[0], 8 * n, hipMemcpyDeviceToHost, e->stream));
+    if (perm)
+        HIPCHK(hipMemcpyAsync(perm, e->vals[0], 4 * n, hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    if (perm)
+        for (const SegInfo& sg : e->h_segs)
+            for (int i = sg.start; i < sg.start + sg.n; ++i) perm[i] -= sg.start;
+    return GCS_OK;
+}
+
+int gcs_read_tree(gcs_engine* e, int32_t id, int32_t* left, int32_t* right, int32_t* parent,
+                  float* com4, float* width, float* bmin, float* bmax)
+{
+    if (!e) return fail(GCS_ERR_INVALID, "engine is null");
+    const int slot = find_slot(e, id);
+    if (slot < 0) return fail(GCS_ERR_NOT_FOUND, "unknown instance id");
+    if (!e->have_tree) return fail(GCS_ERR_INVALID, "no tree has been built");
+    HIPCHK(hipSetDevice(e->device));
+    const SegInfo sg = e->h_segs[slot];
+    const int n = sg.n, nodes = 2 * n - 1, base = sg.root;
+    std::vector<float4> A((size_t)nodes), Q((size_t)2 * nodes), lo((size_t)nodes), hi((size_t)nodes),
+        P((size_t)n);
+    std::vector<int4> B((size_t)nodes);
+    std::vector<int> par((size_t)nodes);
+    HIPCHK(hipMemcpyAsync(A.data(), e->nodeA + base, 16 * (size_t)nodes, hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(hipMemcpyAsync(B.data(), e->nodeB + base, 16 * (size_t)nodes, hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(hipMemcpyAsync(Q.data(), e->nodeQ + 2 * (size_t)base, 32 * (size_t)nodes, hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(hipMemcpyAsync(lo.data(), e->boxLo + base, 16 * (size_t)nodes, hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(hipMemcpyAsync(hi.data(), e->boxHi + base, 16 * (size_t)nodes, hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(hipMemcpyAsync(par.data(), e->parent + base, 4 * (size_t)nodes, hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(hipMemcpyAsync(P.data(), e->posm[e->cur] + sg.start, 16 * (size_t)n, hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    for (int v = 0; v < nodes; ++v) {
+        const bool leaf = v >= n - 1;
+        if (parent) parent[v] = par[(size_t)v] < 0 ? -1 : par[(size_t)v] - base;
+        if (leaf) {
+            const float4 p = P[(size_t)(v - (n - 1))];
+            if (left) left[v] = -1;
+            if (right) right[v] = -1;
+            if (com4) { com4[4 * v] = p.x; com4[4 * v + 1] = p.y; com4[4 * v + 2] = p.z; com4[4 * v + 3] = p.w; }
+            if (width) width[v] = 0.0f;
+            if (bmin) { bmin[3 * v] = p.x; bmin[3 * v + 1] = p.y; bmin[3 * v + 2] = p.z; }
+            if (bmax) { bmax[3 * v] = p.x; bmax[3 * v + 1] = p.y; bmax[3 * v + 2] = p.z; }
+            continue;
+        }
+        const int4 b = B[(size_t)v];
+        const int f = b.y - sg.start, l = b.z - sg.start, s = b.w - sg.start;
+        if (left) left[v] = (f == s) ? (n - 1) + s : s;
+        if (right) right[v] = (l == s + 1) ? (n - 1) + s + 1 : s + 1;
+        if (com4) { const float4 a = A[(size_t)v]; com4[4 * v] = a.x; com4[4 * v + 1] = a.y; com4[4 * v + 2] = a.z; com4[4 * v + 3] = a.w; }
+        if (width) width[v] = Q[(size_t)2 * v + 1].z;
+        if (bmin) { bmin[3 * v] = lo[(size_t)v].x; bmin[3 * v + 1] = lo[(size_t)v].y; bmin[3 * v + 2] = lo[(size_t)v].z; }
+        if (bmax) { bmax[3 * v] = hi[(size_t)v].x; bmax[3 * v + 1] = hi[(size_t)v].y; bmax[3 * v + 2] = hi[(size_t)v].z; }
+    }
+    return GCS_OK;
+}
+
+int gcs_read_bounds(gcs_engine* e, int32_t id, float* mn, float* mx)
+{
+    if (!e) return fail(GCS_ERR_INVALID, "engine is null");
+    const int slot = find_slot(e, id);
+    if (slot < 0) return fail(GCS_ERR_NOT_FOUND, "unknown instance id");
+    uint32_t b[2];
+    HIPCHK(hipMemcpy(b, e->seg_bounds + 2 * slot, 8, hipMemcpyDeviceToHost));
+    if (mn) *mn = fdec(b[0]);
+    if (mx) *mx = fdec(b[1]);
+    return GCS_OK;
+}
+
+int gcs_read_interactions(gcs_engine* e, int64_t* out2)
+{
+    if (!e || !out2) return fail(GCS_ERR_INVALID, "null argument");
+    unsigned long long c[2];
+    HIPCHK(hipStreamSynchronize(e->stream));
+    HIPCHK(hipMemcpy(c, e->counters, sizeof(c), hipMemcpyDeviceToHost));
+    out2[0] = (int64_t)c[0];
+    out2[1] = (int64_t)c[1];
+    return GCS_OK;
+}
+
+int gcs_read_diagnostics(gcs_engine* e, int32_t id, double* out5)
+{
+    if (!e || !out5) return fail(GCS_ERR_INVALID, "null argument");
+    const int slot = find_slot(e, id);
+    if (slot < 0) return fail(GCS_ERR_NOT_FOUND, "unknown instance id");
+    HIPCHK(hipStreamSynchronize(e->stream));
+    HIPCHK(hipMemcpy(out5, e->diag_out + 5 * slot, 5 * sizeof(double), hipMemcpyDeviceToHost));
+    return GCS_OK;
+}
+
+int gcs_read_stage_ms(gcs_engine* e, float* out7)
+{
+    if (!e || !out7) return fail(GCS_ERR_INVALID, "null argument");
+    HIPCHK(hipStreamSynchronize(e->stream));
+    for (int k = 0; k < 7; ++k) {
+        float ms = 0.0f;
+        if (hipEventElapsedTime(&ms, e->ev[k], e->ev[k + 1]) != hipSuccess) ms = -1.0f;
+        out7[k] = ms;
+    }
+    return GCS_OK;
+}
+
+// ---- Stand-alone stages ------------------------------------------------------
+
+int gcs_morton_keys(const float* x, const float* y, const float* z, int64_t n, float mn, float mx,
+                    uint64_t* keys)
+{
+    if (n < 0 || (n > 0 && (!x || !y || !z || !keys))) return fail(GCS_ERR_INVALID, "bad argument");
+    if (n == 0) return GCS_OK;
+    if (n > (int64_t)1 << 30) return fail(GCS_ERR_CAPACITY, "too many keys");
+    gcs_engine* e = nullptr;
+    int rc = gcs_create(n, 1, 0, &e);
+    if (rc) return rc;
+    std::vector<gcs_star> st((size_t)n);
+    for (int64_t i = 0; i < n; ++i) {
+        std::memset(&st[(size_t)i], 0, sizeof(gcs_star));
+        st[(size_t)i].x = x[i]; st[(size_t)i].y = y[i]; st[(size_t)i].z = z[i];
+        st[(size_t)i].mass = 1.0f;
+    }
+    int32_t cnt = (int32_t)n;
+    rc = gcs_load(e, st.data(), &cnt, 1);
+    if (!rc) {
+        const int qblocks = 4 * (int)e->h_tiles.size();
+        k_morton<<<qblocks, kTileThreads, 0, e->stream>>>(e->tiles, e->segs, e->posm[e->cur],
+                                                           e->seg_bounds, 1, mn, mx, e->keys[0],
+                                                           e->vals[0]);
+        hipError_t le = hipGetLastError();
+        if (le != hipSuccess) rc = fail(GCS_ERR_HIP, hipGetErrorString(le));
+        else if (hipMemcpyAsync(keys, e->keys[0], 8 * (size_t)n, hipMemcpyDeviceToHost, e->stream) != hipSuccess ||
+                 hipStreamSynchronize(e->stream) != hipSuccess)
+            rc = fail(GCS_ERR_HIP, "copy of keys failed");
+    }
+    gcs_destroy(e);
+    return rc;
+}
+
+int gcs_sort_pairs(const uint64_t* keys, int64_t n, uint64_t* keys_out, int32_t* perm_out)
+{
+    if (n < 0 || (n > 0 && (!keys || !keys_out || !perm_out))) return fail(GCS_ERR_INVALID, "bad argument");
+    if (n == 0) return GCS_OK;
+    if (n > (int64_t)1 << 30) return fail(GCS_ERR_CAPACITY, "too many keys");
+    gcs_engine* e = nullptr;
+    int rc = gcs_create(n, 1, 0, &e);
+    if (rc) return rc;
+    e->inst.assign(1, Instance{0, (int32_t)n});
+    rc = rebuild_tables(e);
+    if (!rc) {
+        std::vector<uint32_t> iota((size_t)n);
+        for (int64_t i = 0; i < n; ++i) iota[(size_t)i] = (uint32_t)i;
+        if (hipMemcpy(e->keys[0], keys, 8 * (size_t)n, hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(e->vals[0], iota.data(), 4 * (size_t)n, hipMemcpyHostToDevice) != hipSuccess)
+            rc = fail(GCS_ERR_HIP, "upload failed");
+    }
+    if (!rc) {
+        const int ntiles = (int)e->h_tiles.size();
+        const int64_t hn = (int64_t)kRadix * ntiles;
+        const int sblocks = (int)scan_blocks((size_t)hn);
+        for (int pass = 0; pass < kSortPasses; ++pass) {
+            const int shift = pass * kRadixBits, src = pass & 1, dst = src ^ 1;
+            k_sort_count<<<ntiles, kTileThreads, 0, e->stream>>>(e->tiles, e->keys[src], e->hist, shift);
+            k_scan_reduce<<<sblocks, 256, 0, e->stream>>>(e->hist, hn, e->partial);
+            k_scan_partials<<<1, 256, 0, e->stream>>>(e->partial, sblocks);
+            k_scan_apply<<<sblocks, 256, 0, e->stream>>>(e->hist, hn, e->partial);
+            k_sort_scatter<<<ntiles, kTileThreads, 0, e->stream>>>(
+                e->tiles, e->keys[src], e->vals[src], e->keys[dst], e->vals[dst], e->hist, shift);
+        }
+        hipError_t le = hipGetLastError();
+        if (le != hipSuccess) rc = fail(GCS_ERR_HIP, hipGetErrorString(le));
+        else if (hipMemcpyAsync(keys_out, e->keys[0], 8 * (size_t)n, hipMemcpyDeviceToHost, e->stream) != hipSuccess ||
+                 hipMemcpyAsync(perm_out, e->vals[0], 4 * (size_t)n, hipMemcpyDeviceToHost, e->stream) != hipSuccess ||
+                 hipStreamSynchronize(e->stream) != hipSuccess)
+            rc = fail(GCS_ERR_HIP, "copy back failed");
+    }
+    gcs_destroy(e);
+    return rc;
+}
+
+int gcs_karras_tree(const uint64_t* sorted_keys, int32_t n, int32_t* left, int32_t* right,
+                    int32_t* parent)
+{
+    if (n < 1 || !sorted_keys || !left || !right || !parent) return fail(GCS_ERR_INVALID, "bad argument");
+    gcs_engine* e = nullptr;
+    int rc = gcs_create(n, 1, 0, &e);
+    if (rc) return rc;
+    e->inst.assign(1, Instance{0, n});
+    rc = rebuild_tables(e);
+    if (!rc && hipMemcpy(e->keys[0], sorted_keys, 8 * (size_t)n, hipMemcpyHostToDevice) != hipSuccess)
+        rc = fail(GCS_ERR_HIP, "upload failed");
+    if (!rc) {
+        const int qblocks = 4 * (int)e->h_tiles.size();
+        k_build<<<qblocks, kTileThreads, 0, e->stream>>>(e->tiles, e->segs, e->keys[0], e->nodeB,
+                                                         e->parent, e->flag);
+        hipError_t le = hipGetLastError();
+        if (le != hipSuccess) rc = fail(GCS_ERR_HIP, hipGetErrorString(le));
+    }
+    if (!rc) {
+        const int nodes = 2 * n - 1;
+        std::vector<int4> B((size_t)nodes);
+        if (hipMemcpyAsync(B.data(), e->nodeB, 16 * (size_t)nodes, hipMemcpyDeviceToHost, e->stream) != hipSuccess ||
+            hipMemcpyAsync(parent, e->parent, 4 * (size_t)nodes, hipMemcpyDeviceToHost, e->stream) != hipSuccess ||
+            hipStreamSynchronize(e->stream) != hipSuccess) {
+            rc = fail(GCS_ERR_HIP, "copy back failed");
+        } else {
+            for (int v = 0; v < nodes; ++v) {
+                if (v >= n - 1) {
+                    left[v] = right[v] = -1;
+                    continue;
+                }
+                const int f = B[(size_t)v].y, l = B[(size_t)v].z, s = B[(size_t)v].w;
+                left[v] = (f == s) ? (n - 1) + s : s;
+                right[v] = (l == s + 1) ? (n - 1) + s + 1 : s + 1;
+            }
+        }
+    }
+    gcs_destroy(e);
+    return rc;
+}
+
+}  // extern "C"

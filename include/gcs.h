@@ -1,0 +1,221 @@
+/*
+ * gcs.h -- C ABI of the MI355X (gfx950) per-frame N-body step.
+ *
+ * One engine holds the device-resident state of one or more independent
+ * collisions ("instances").  Each step runs, on the engine's HIP stream:
+ *
+ *   cube bounds -> 63-bit Morton keys -> stable LSD radix sort -> permute
+ *   -> Karras radix tree per instance -> bottom-up mass/COM/AABB
+ *   -> Barnes-Hut traversal -> kick-drift (+ colours, + diagnostics)
+ *
+ * The entry points replace the reference's cuda_engine.h (cuda_engine.h:5-32)
+ * and the kernels behind it (cuda_engine.cu, karras_bvh.cuh, morton.h); the
+ * ten reference-named C++ functions are provided over this ABI by
+ * gcs_cuda_engine.h.  All functions return GCS_OK (0) or a negative error
+ * code; gcs_last_error() gives the message of the calling thread's last
+ * failure.  No function throws.  An engine is not thread-safe: use one engine
+ * per host thread (or serialise calls).  Pointers marked "host" are ordinary
+ * host memory; the library copies through its own staging buffers.
+ */
+#ifndef GCS_H
+#define GCS_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- Records -------------------------------------------------------------
+ * Byte-identical to the reference's Star (types.h:5-12, 32 bytes) and
+ * PlaybackStar (types.h:14-18, 16 bytes, the simulation.bin record). */
+typedef struct gcs_star {
+    float x, y, z;
+    float vx, vy, vz;
+    float mass;
+    uint8_t type, r, g, b;
+} gcs_star;
+
+typedef struct gcs_playback_star {
+    float x, y, z;
+    uint8_t r, g, b;
+    uint8_t padding; /* carries `type` (main.cpp:266) */
+} gcs_playback_star;
+
+/* ---- Status codes ------------------------------------------------------ */
+enum {
+    GCS_OK = 0,
+    GCS_ERR_INVALID = -1,       /* bad argument */
+    GCS_ERR_NO_DEVICE = -2,     /* no HIP device */
+    GCS_ERR_OOM = -3,           /* device or host allocation failed */
+    GCS_ERR_HIP = -4,           /* a HIP runtime call failed */
+    GCS_ERR_CAPACITY = -5,      /* request exceeds the engine's capacity */
+    GCS_ERR_STACK = -6,         /* a traversal stack overflowed */
+    GCS_ERR_NONFINITE = -7,     /* a non-finite acceleration was produced */
+    GCS_ERR_NOT_FOUND = -8      /* unknown instance id */
+};
+
+/* ---- Opening criteria ----------------------------------------------------
+ * GCS_MAC_REFERENCE  the reference's GPU traversal (cuda_engine.cu:91-161):
+ *                    one particle per thread, DFS from the root, internal
+ *                    node accepted when width^2 < |com-p|^2 + eps^2 (theta is
+ *                    ignored: the reference's rule is theta = 1 on the
+ *                    softened distance), leaves skipped when within 0.1*eps.
+ * GCS_MAC_GROUP      64 consecutive sorted particles share one traversal; a
+ *                    node is a monopole for the group when the distance from
+ *                    its COM to the group's bounding box exceeds
+ *                    width/theta + |COM - box centre|; children holding at
+ *                    most leaf_size particles are summed directly; index
+ *                    self-exclusion.  Needs eps2 > 0. */
+enum { GCS_MAC_REFERENCE = 0, GCS_MAC_GROUP = 1 };
+
+/* Position precision: plain FP32, or a compensated FP32 hi/lo pair whose
+ * drift carries are formed in FP64. */
+enum { GCS_PREC_PLAIN = 0, GCS_PREC_COMPENSATED = 1 };
+
+typedef struct gcs_params {
+    float G;            /* gravitational constant */
+    float eps2;         /* Plummer softening squared (epsilon_sq) */
+    float dt;           /* time step */
+    float theta;        /* opening angle (GCS_MAC_GROUP) */
+    int32_t mac;        /* GCS_MAC_* */
+    int32_t precision;  /* GCS_PREC_* */
+    int32_t leaf_size;  /* direct-sum bucket size, 1..64 (GCS_MAC_GROUP) */
+    int32_t block_size; /* traversal threads per block: 64, 128, 256 */
+    int32_t stack_depth;/* per-thread stack (GCS_MAC_REFERENCE), 16..1024 */
+    int32_t use_bounds; /* 1: use global_min/global_max below for every
+                           instance (cuda_physics_step semantics);
+                           0: per-instance cube bounds from the device */
+    float global_min, global_max;
+    int32_t colours;    /* 1: update r,g,b from speed (cuda_engine.cu:185-215) */
+    int32_t diagnostics;/* 1: FP64 energy/momentum per instance each step */
+    int32_t use_graph;  /* 1: replay steps from a captured hipGraph */
+    int32_t reserved[5];
+} gcs_params;
+
+/* Defaults: G=1, eps2=0.0025, dt=0.01, theta=0.5, GCS_MAC_GROUP,
+ * GCS_PREC_PLAIN, leaf 16, block 256, stack 256, device bounds, colours on,
+ * diagnostics off, graph on. */
+void gcs_params_default(gcs_params* p);
+
+typedef struct gcs_engine gcs_engine;
+
+/* ---- Lifetime (cuda_engine.cu:219-285 cuda_init, :303-340 cuda_cleanup) --
+ * capacity_particles: total particles over all instances; capacity_instances:
+ * the most instances held at once.  device: HIP ordinal.  The engine creates
+ * its own stream unless gcs_set_stream is called. */
+int gcs_create(int64_t capacity_particles, int32_t capacity_instances,
+               int32_t device, gcs_engine** out);
+int gcs_destroy(gcs_engine* e);
+const char* gcs_last_error(void);
+/* hip_stream: a hipStream_t, or NULL for the engine's own stream. */
+int gcs_set_stream(gcs_engine* e, void* hip_stream);
+/* Bytes of device memory an engine of this capacity allocates. */
+int64_t gcs_device_bytes(int64_t capacity_particles, int32_t capacity_instances);
+
+/* ---- State in / out (cuda_engine.cu:342-352 upload, :441-449 read back) --
+ * gcs_load replaces the whole state with k instances whose particles are
+ * consecutive in `stars` (host), counts[i] each.  Instances keep their
+ * order; ids 0..k-1 are assigned. */
+int gcs_load(gcs_engine* e, const gcs_star* stars, const int32_t* counts,
+             int32_t k);
+/* Adds one instance at the end (continuous batching); *id receives its id. */
+int gcs_instance_add(gcs_engine* e, const gcs_star* stars, int32_t n,
+                     int32_t* id);
+/* Removes an instance; the others keep their ids and order. */
+int gcs_instance_remove(gcs_engine* e, int32_t id);
+int gcs_instance_count(gcs_engine* e, int32_t* k, int64_t* n_total);
+/* Particle count and the current offset of an instance in the state. */
+int gcs_instance_info(gcs_engine* e, int32_t id, int32_t* n, int64_t* offset);
+/* Reads every particle (host, n_total records), instance by instance, each
+ * in the device's current (Morton-sorted) order. */
+int gcs_read_stars(gcs_engine* e, gcs_star* out);
+int gcs_read_instance(gcs_engine* e, int32_t id, gcs_star* out);
+/* Compensated low words (x_lo, y_lo, z_lo per particle, host n_total*3). */
+int gcs_read_position_lo(gcs_engine* e, float* out);
+/* PlaybackStar frame of all particles (simulation.bin record, main.cpp:266). */
+int gcs_read_playback(gcs_engine* e, gcs_playback_star* out);
+
+/* ---- Stepping (cuda_engine.cu:379-451) -----------------------------------
+ * Advances every instance by nsteps full steps.  Returns when the work is
+ * queued on the stream; gcs_sync waits and reports device-side errors. */
+int gcs_step(gcs_engine* e, const gcs_params* p, int32_t nsteps);
+int gcs_sync(gcs_engine* e);
+/* Runs bounds, Morton, sort, permute, tree build, aggregation and traversal
+ * without integrating; accelerations are then readable. */
+int gcs_compute_accels(gcs_engine* e, const gcs_params* p);
+/* Host n_total*4 floats: ax, ay, az, potential (potential only for
+ * GCS_MAC_GROUP), in current particle order. */
+int gcs_read_accels(gcs_engine* e, float* out);
+
+/* ---- Inspection for parity tests -----------------------------------------
+ * Keys as encoded this step (before the sort, in pre-sort particle order),
+ * sorted keys, and the sort permutation (sorted position -> pre-sort
+ * position), each n_total entries, instance-local values for the
+ * permutation. */
+int gcs_read_keys(gcs_engine* e, uint64_t* keys_unsorted, uint64_t* keys_sorted,
+                  int32_t* perm);
+/* Tree of one instance in the reference's numbering (karras_bvh.cuh:114-124:
+ * internal nodes 0..n-2, leaves n-1..2n-2), 2n-1 entries each:
+ * left/right children (-1 for leaves), parent (-1 for the root), com4
+ * (com x,y,z, mass), width (max AABB extent), bmin/bmax (3 floats). */
+int gcs_read_tree(gcs_engine* e, int32_t id, int32_t* left, int32_t* right,
+                  int32_t* parent, float* com4, float* width, float* bmin,
+                  float* bmax);
+/* Cube bounds used for instance `id` in the last step. */
+int gcs_read_bounds(gcs_engine* e, int32_t id, float* mn, float* mx);
+/* Interaction counts of the last traversal: [0] monopole (node)
+ * interactions, [1] particle-particle interactions, summed over targets. */
+int gcs_read_interactions(gcs_engine* e, int64_t* out2);
+/* FP64 diagnostics of instance `id` from the last step run with
+ * diagnostics=1: kinetic energy, potential energy, momentum x, y, z, all at
+ * the force time (velocities re-synchronised by half a kick). */
+int gcs_read_diagnostics(gcs_engine* e, int32_t id, double* out5);
+/* Milliseconds of the last step's stages on the device (needs
+ * use_graph = 0): [0] bounds+morton, [1] sort, [2] permute, [3] build,
+ * [4] aggregate, [5] traverse, [6] integrate. */
+int gcs_read_stage_ms(gcs_engine* e, float* out7);
+
+/* ---- Stand-alone stage entry points (host buffers in and out) ----------- */
+int gcs_morton_keys(const float* x, const float* y, const float* z, int64_t n,
+                    float mn, float mx, uint64_t* keys);
+/* Stable ascending sort of 64-bit keys with 32-bit payload (the payload
+ * starts as 0..n-1): the semantics of cub::DeviceRadixSort::SortPairs
+ * (cuda_engine.cu:397) on all 64 bits. */
+int gcs_sort_pairs(const uint64_t* keys, int64_t n, uint64_t* keys_out,
+                   int32_t* perm_out);
+/* Karras 2012 radix tree over sorted keys (karras_bvh.cuh:73-131). */
+int gcs_karras_tree(const uint64_t* sorted_keys, int32_t n, int32_t* left,
+                    int32_t* right, int32_t* parent);
+
+/* ---- Initial conditions --------------------------------------------------
+ * Two-galaxy collision: per galaxy a Miyamoto-Nagai disc (a=3, b=0.3,
+ * M=1), a Hernquist bulge (a=0.5, M=0.25) and a Hernquist dark halo (a=12,
+ * M=5) truncated at r<60 (the bulge too), split 70/10/20 of each galaxy's
+ * count (disc = round(0.7 n), bulge = round(0.1 n), halo = the rest), equal
+ * mass per particle within a component, circular velocities from the
+ * enclosed mass plus a 5% Gaussian dispersion.  Galaxy A at rest at the
+ * origin; galaxy B tilted about x, offset and moving as given. */
+typedef struct gcs_collision_spec {
+    int32_t n_a, n_b;     /* particles per galaxy */
+    float offset[3];      /* position of galaxy B */
+    float velocity[3];    /* velocity of galaxy B */
+    float tilt_deg;       /* rotation of galaxy B about the x axis */
+    uint64_t seed;
+} gcs_collision_spec;
+
+/* The single-collision configuration: n per galaxy, B at (40,10,0) moving
+ * (-0.5,0,0), tilted 30 degrees. */
+void gcs_spec_default(int32_t n_per_galaxy, uint64_t seed, gcs_collision_spec* s);
+/* Randomised collision k of a stream: total N log-uniform in [n_min, n_max]
+ * split 50/50, |offset| uniform in [20,60] in a random direction, approach
+ * speed uniform in [0.2,0.8] toward galaxy A, tilt uniform in [0,90]. */
+void gcs_spec_random(uint64_t seed, int64_t k, int32_t n_min, int32_t n_max,
+                     gcs_collision_spec* s);
+/* Writes n_a + n_b records (galaxy A then B; each halo, disc, bulge). */
+int gcs_generate(const gcs_collision_spec* s, gcs_star* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,123 @@
+/*
+ * gcs_oracle.h -- CPU oracle for the per-frame N-body step.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker that the HIP path is
+ * compared against; it is never linked into, loaded by, or called from the
+ * product library.  Only tests/, __graft_entry__.smoke() and the cpu_baseline
+ * leg of bench.py may use it.
+ *
+ * Every function restates one piece of the reference
+ * (lakshya076/Galaxy-Collision-Simulator) in plain C; each definition cites the
+ * reference file:line it follows.  All arrays are caller-owned.
+ */
+#ifndef GCS_ORACLE_H
+#define GCS_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- Morton (morton.h:11-28) ------------------------------------------ */
+uint64_t orc_expand_bits(uint32_t v);
+uint32_t orc_f2u_sat(float f);
+uint64_t orc_morton(float x, float y, float z, float mn, float mx);
+void orc_morton_n(const float* x, const float* y, const float* z, int64_t n,
+                  float mn, float mx, uint64_t* keys);
+/* main.cpp:152-166: one scalar cube [min,max] over all three axes. */
+void orc_bounds_cube(const float* x, const float* y, const float* z, int64_t n,
+                     float* mn, float* mx);
+
+/* ---- Sort (cub::DeviceRadixSort::SortPairs semantics, cuda_engine.cu:397) */
+void orc_stable_sort_pairs(const uint64_t* keys, int64_t n, uint64_t* keys_out,
+                           int32_t* perm_out);
+
+/* ---- Karras 2012 radix tree (karras_bvh.cuh:23-131) -------------------- */
+void orc_karras_build(const uint64_t* skeys, int32_t n, int32_t* left,
+                      int32_t* right, int32_t* parent);
+/* Same tree from the definition (highest differing bit of the 96-bit key
+ * (key, index)), by recursive splitting.  Returns 0 on success. */
+int orc_radix_tree_definition(const uint64_t* skeys, int32_t n, int32_t* left,
+                              int32_t* right, int32_t* parent);
+/* Leaf range [first,last] and split of every node (from left/right). */
+void orc_node_ranges(int32_t n, const int32_t* left, const int32_t* right,
+                     int32_t* first, int32_t* last);
+
+/* ---- Bottom-up aggregation (karras_bvh.cuh:134-225) ---------------------
+ * com4: (2n-1)*4 floats (com x,y,z, mass); width: 2n-1; bmin/bmax: (2n-1)*3;
+ * crit: (2n-1) opening radius squared (w*inv_theta + |com - box centre|)^2,
+ * 0 for leaves.  quad (optional, (2n-1)*6): traceless quadrupole moment
+ * about the COM, components xx, yy, zz, xy, xz, yz (0 for leaves). */
+void orc_aggregate(int32_t n, const int32_t* left, const int32_t* right,
+                   const float* x, const float* y, const float* z,
+                   const float* m, float inv_theta, float* com4, float* width,
+                   float* bmin, float* bmax, float* crit, float* quad);
+
+/* ---- Reference GPU traversal semantics (cuda_engine.cu:91-161) ---------
+ * Per-particle DFS from root 0; softened distance; internal accepted when
+ * width^2 < d^2+eps^2; leaves skipped when d^2+eps^2 <= 1.01*eps^2.
+ * Decisions in float exactly as written; sums in double.  acc: n*4 doubles
+ * (ax, ay, az, 0); msum and mskip (optional, n doubles each) receive the
+ * mass each target interacted with and the mass of the leaves it skipped by
+ * the distance rule.  Returns the number of interactions. */
+int64_t orc_traverse_reference(int32_t n, const float* x, const float* y,
+                               const float* z, const int32_t* left,
+                               const int32_t* right, const float* com4,
+                               const float* width, float G, float eps2,
+                               double* acc, double* msum, double* mskip);
+
+/* ---- Group traversal with an opening radius (the MI355X kernel's rule) --
+ * Groups are runs of `group` consecutive sorted particles.  A node is taken
+ * as a monopole when the squared distance from its COM to the group's
+ * bounding box exceeds crit[node]; otherwise it is summed directly when it
+ * holds <= leaf particles, else opened.  Index self-exclusion.  acc: n*4
+ * doubles (ax, ay, az, phi).  With quad non-null accepted nodes add their
+ * quadrupole term.  counts[0] += node (multipole) interactions,
+ * counts[1] += particle-particle interactions (per target particle);
+ * msum (optional, n doubles) receives the mass each target interacted with
+ * (the total mass less its own when every particle is counted once). */
+void orc_traverse_group(int32_t n, const float* x, const float* y,
+                        const float* z, const float* m, const int32_t* left,
+                        const int32_t* right, const int32_t* first,
+                        const int32_t* last, const float* com4,
+                        const float* crit, const float* quad, int32_t group,
+                        int32_t leaf, float G, float eps2, double* acc,
+                        int64_t* counts, double* msum);
+
+/* ---- FP64 direct sum (index self-exclusion) ------------------------------
+ * Targets tgt[0..nt) (indices into the n sources).  acc: nt*4 doubles
+ * (ax, ay, az, phi). */
+void orc_direct_sum_f64(int64_t n, const double* x, const double* y,
+                        const double* z, const double* m, int64_t nt,
+                        const int64_t* tgt, double G, double eps2,
+                        double* acc);
+
+/* ---- Kick-drift and colour (cuda_engine.cu:164-216) -------------------- */
+void orc_integrate_reference(int64_t n, float* x, float* y, float* z,
+                             float* vx, float* vy, float* vz, uint8_t* r,
+                             uint8_t* g, uint8_t* b, const uint8_t* type,
+                             const float* ax, const float* ay, const float* az,
+                             float dt);
+
+/* ---- CPU octree step (engine.cpp:5-127, gravity_aggregator.cpp:8-83,
+ *      main.cpp:152-253) -----------------------------------------------
+ * Stars are AoS records of 32 bytes laid out as types.h:5-12.  One full
+ * frame: bounds, Morton sort (stable), octree insert with `leaf_cap`,
+ * mass/COM, Barnes-Hut query at `theta`, kick-drift.  When `acc_out` is
+ * non-null the accelerations (in post-sort order) are written there as n*3
+ * floats.  phase_s[6] receives seconds for bounds, sort, insert, mass, query,
+ * integrate.  Returns 0, or -1 when the octree cannot be built (more than
+ * leaf_cap coincident points) or a query stack overflows. */
+typedef struct orc_star {
+    float x, y, z, vx, vy, vz, mass;
+    uint8_t type, r, g, b;
+} orc_star;
+
+int orc_octree_step(orc_star* stars, int64_t n, float G, float eps2, float dt,
+                    float theta, int32_t leaf_cap, int32_t do_integrate,
+                    float* acc_out, double* phase_s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

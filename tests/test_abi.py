@@ -1,0 +1,84 @@
+"""The C-ABI library loads, exports every symbol include/*.h declares, and its
+host-only paths behave; no GPU compute is called."""
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def header_functions(name):
+    text = open(os.path.join(REPO, "include", name)).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    text = re.sub(r"//[^\n]*", "", text)
+    found = re.findall(r"\b([a-z_][a-z0-9_]*)\s*\([^;{]*\)\s*;", text)
+    return sorted(set(found) - {"static_assert", "sizeof"})
+
+
+def exported(lib_path, demangle):
+    out = subprocess.run(["nm", "-D", "--defined-only"] + (["-C"] if demangle else []) +
+                         [lib_path], stdout=subprocess.PIPE, text=True, check=True).stdout
+    return out
+
+
+def test_c_abi_symbols_exported(gcs):
+    names = header_functions("gcs.h")
+    assert len(names) >= 30
+    syms = exported(gcs.LIB_PATH, False)
+    for n in names:
+        assert re.search(r"\bT %s$" % n, syms, flags=re.M), "missing export " + n
+
+
+def test_cuda_engine_symbols_exported(gcs):
+    names = header_functions("gcs_cuda_engine.h")
+    assert set(names) == {"cuda_init", "cuda_cleanup", "cuda_allocate_pinned_stars",
+                          "cuda_free_pinned_stars", "cuda_allocate_pinned_playback",
+                          "cuda_free_pinned_playback", "cuda_upload_initial_stars",
+                          "cuda_register_vbo", "cuda_unregister_vbo", "cuda_get_center_of_mass",
+                          "cuda_physics_step"}
+    syms = exported(gcs.LIB_PATH, True)
+    # C++ linkage with the reference's parameter types (cuda_engine.h:5-32).
+    for sig in ["cuda_init(int)", "cuda_cleanup()", "cuda_allocate_pinned_stars(Star**, int)",
+                "cuda_free_pinned_stars(Star*)",
+                "cuda_allocate_pinned_playback(PlaybackStar**, int)",
+                "cuda_free_pinned_playback(PlaybackStar*)",
+                "cuda_upload_initial_stars(Star const*, int)", "cuda_register_vbo(unsigned int)",
+                "cuda_unregister_vbo()", "cuda_get_center_of_mass(float*, float*, float*)",
+                "cuda_physics_step(Star*, int, float, float, float, float, float, bool)"]:
+        assert sig in syms, "missing " + sig
+
+
+def test_record_layouts(gcs):
+    assert gcs.STAR.itemsize == 32 and gcs.PLAYBACK_STAR.itemsize == 16
+    assert gcs.STAR.fields["mass"][1] == 24 and gcs.STAR.fields["type"][1] == 28
+
+
+def test_params_defaults(gcs):
+    p = gcs.params()
+    assert p.G == 1.0 and p.theta == 0.5 and p.mac == gcs.MAC_GROUP and p.leaf_size == 16
+
+
+def test_device_bytes_scale(gcs):
+    a = gcs.lib().gcs_device_bytes(1 << 20, 1)
+    b = gcs.lib().gcs_device_bytes(1 << 21, 1)
+    assert 200 * (1 << 20) < a < 1000 * (1 << 20)
+    assert 1.9 < b / a < 2.1
+
+
+def test_no_device_is_an_error_not_a_crash(gcs):
+    import ctypes
+    import torch
+    if torch.cuda.device_count() > 0:
+        pytest.skip("a GPU is present")
+    h = ctypes.c_void_p()
+    rc = gcs.lib().gcs_create(1024, 1, 0, ctypes.byref(h))
+    assert rc == -2 and not h.value
+    assert b"device" in gcs.lib().gcs_last_error()
+
+
+def test_bake_driver_built(gcs):
+    path = os.path.join(os.path.dirname(gcs.LIB_PATH), "gcs_bake")
+    assert os.access(path, os.X_OK)

@@ -1,0 +1,359 @@
+"""HIP path vs the CPU oracle, through the C ABI.
+
+Bit-exact: Morton keys, the sort (keys and permutation), the Karras topology,
+node aggregates (COM, mass, AABB, width), the kick-drift and colours.
+Tolerance (written per test): accelerations -- 2e-5 relative RMS against the
+oracle's FP64 evaluation of the same interaction lists, <= 1e-3 relative RMS
+against the FP64 direct sum at theta = 0.5.
+"""
+import os
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+G = 1.0
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def rel_rms(a, b):
+    return float(np.sqrt(((a - b) ** 2).sum() / (b ** 2).sum()))
+
+
+def cloud(gcs, n, seed, spread=1.0):
+    r = np.random.default_rng(seed)
+    st = np.zeros(n, gcs.STAR)
+    for k in ("x", "y", "z"):
+        st[k] = (r.normal(size=n) * spread).astype(np.float32)
+    for k in ("vx", "vy", "vz"):
+        st[k] = r.normal(size=n).astype(np.float32) * 0.1
+    st["mass"] = r.uniform(0.5, 1.5, n).astype(np.float32) / n
+    st["type"] = r.integers(0, 3, n)
+    return st
+
+
+# ---- stand-alone stages --------------------------------------------------------
+
+def test_morton_keys_match_oracle(gcs, orc):
+    r = np.random.default_rng(1)
+    mn, mx = -7.25, 3.5
+    x = np.concatenate([r.uniform(-9, 6, 100000), [mn, mx, np.nan, np.inf, -np.inf]])
+    y = r.uniform(-9, 6, len(x))
+    z = r.uniform(-9, 6, len(x))
+    x, y, z = (a.astype(np.float32) for a in (x, y, z))
+    say("morton n=%d" % len(x))
+    assert np.array_equal(gcs.morton_keys(x, y, z, mn, mx), orc.morton_n(x, y, z, mn, mx))
+    # degenerate span
+    assert np.all(gcs.morton_keys(x[:100], y[:100], z[:100], 2.0, 2.0) == 0)
+
+
+@pytest.mark.parametrize("n", [1, 2, 63, 1023, 1024, 1025, 4097, 100003])
+def test_sort_pairs_match_oracle(gcs, orc, n):
+    r = np.random.default_rng(n)
+    keys = r.integers(0, 1 << 63, n, dtype=np.uint64)
+    keys[: n // 3] = keys[: n // 3] % np.uint64(13)
+    keys[n // 3: n // 2] = np.uint64(0x7FFFFFFFFFFFFFFF)
+    say("sort n=%d" % n)
+    ko, po = gcs.sort_pairs(keys)
+    eo, ep = orc.stable_sort_pairs(keys)
+    assert np.array_equal(ko, eo) and np.array_equal(po, ep)
+
+
+def test_sort_pairs_full_64_bits_and_presorted(gcs, orc):
+    r = np.random.default_rng(5)
+    keys = r.integers(0, 1 << 64, 2_000_000, dtype=np.uint64)  # top bit used too
+    for k in (keys, np.sort(keys), np.sort(keys)[::-1].copy()):
+        say("sort 2M")
+        ko, po = gcs.sort_pairs(k)
+        eo, ep = orc.stable_sort_pairs(k)
+        assert np.array_equal(ko, eo) and np.array_equal(po, ep)
+
+
+def karras_cases():
+    r = np.random.default_rng(3)
+    out = []
+    for n in (1, 2, 3, 5, 64, 1000, 5000):
+        out.append(np.sort(r.integers(0, 1 << 63, n, dtype=np.uint64)))
+        out.append(np.sort(r.integers(0, 4, n, dtype=np.uint64)))
+    out.append(np.zeros(3000, np.uint64))
+    out.append(np.sort(r.integers(0, 1 << 20, 262144, dtype=np.uint64)))
+    return out
+
+
+@pytest.mark.parametrize("case", range(16))
+def test_karras_tree_matches_oracle(gcs, orc, case):
+    keys = karras_cases()[case]
+    say("karras case %d n=%d" % (case, len(keys)))
+    l, r, p = gcs.karras_tree(keys)
+    el, er, ep = orc.karras_build(keys)
+    assert np.array_equal(l, el) and np.array_equal(r, er) and np.array_equal(p, ep)
+
+
+# ---- engine: one instance ----------------------------------------------------------
+
+def check_pipeline(gcs, orc, e, stars, p, iid=0, theta=0.5, bounds=None):
+    raw, srt, perm = e.keys()
+    cpu = orc.build_tree_pipeline(stars, theta, *(bounds or (None, None)))
+    if bounds is None:
+        mn, mx = e.bounds(iid)
+        assert (mn, mx) == (np.float32(cpu["mn"]), np.float32(cpu["mx"]))
+    assert np.array_equal(raw, cpu["keys"])
+    assert np.array_equal(srt, cpu["skeys"])
+    assert np.array_equal(perm, cpu["perm"])
+    t = e.tree(iid)
+    for k in ("left", "right", "parent"):
+        assert np.array_equal(t[k], cpu[k]), k
+    agg = cpu["agg"]
+    for k in ("com4", "width", "bmin", "bmax"):
+        assert np.array_equal(t[k].view(np.uint32), agg[k].view(np.uint32)), k
+    return cpu
+
+
+def test_engine_c1_group_parity(gcs, orc):
+    stars = gcs.collision(8192, 1001)
+    n = len(stars)
+    eps2 = np.float32(0.05) ** 2
+    p = gcs.params(G=G, eps2=float(eps2), dt=0.01, theta=0.5, leaf_size=16, diagnostics=1)
+    with gcs.Engine(n) as e:
+        e.load(stars)
+        say("engine C1 compute")
+        e.compute_accels(p)
+        cpu = check_pipeline(gcs, orc, e, stars, p)
+        acc = e.accels()
+        s = cpu["sorted"]
+        want, msum, cnt = orc.traverse_group(cpu["left"], cpu["right"], cpu["first"],
+                                             cpu["last"], s["x"], s["y"], s["z"], s["mass"],
+                                             cpu["agg"], G, float(eps2), 64, 16)
+        assert rel_rms(acc[:, :3], want[:, :3]) < 2e-5
+        assert rel_rms(acc[:, 3], want[:, 3]) < 2e-5
+        ref = orc.direct_sum_f64(s["x"], s["y"], s["z"], s["mass"], G, float(eps2))
+        err = rel_rms(acc[:, :3], ref[:, :3])
+        say("C1 rel rms vs FP64 direct sum: %.3g" % err)
+        assert err < 1e-3
+        assert rel_rms(acc[:, 3], ref[:, 3]) < 1e-4
+        nn, npart = e.interactions()
+        assert (nn, npart) == (int(cnt[0]), int(cnt[1]))
+
+
+def test_engine_reference_mode_parity(gcs, orc):
+    stars = cloud(gcs, 20000, 7)
+    eps2 = 0.0625
+    mn, mx = -6.0, 6.5
+    p = gcs.reference_params(G, eps2, 0.01, mn, mx)
+    with gcs.Engine(len(stars)) as e:
+        e.load(stars)
+        say("reference mode compute")
+        e.compute_accels(p)
+        cpu = check_pipeline(gcs, orc, e, stars, p, theta=1.0, bounds=(mn, mx))
+        s = cpu["sorted"]
+        acc = e.accels()
+        want, _, cnt = orc.traverse_reference(cpu["left"], cpu["right"], s["x"], s["y"], s["z"],
+                                              cpu["agg"]["com4"], cpu["agg"]["width"], G, eps2)
+        assert rel_rms(acc[:, :3], want[:, :3]) < 2e-5
+        assert e.interactions()[1] == cnt
+
+
+def test_integrate_and_colours_bit_exact(gcs, orc):
+    stars = cloud(gcs, 5000, 8)
+    stars["vx"] *= 2000.0  # speeds across the colour ramp
+    p = gcs.reference_params(G, 0.0625, 0.01, -8.0, 8.0, use_graph=0)
+    with gcs.Engine(len(stars)) as e:
+        e.load(stars)
+        e.compute_accels(p)
+        before = e.stars()
+        acc = e.accels()
+        say("integrate step")
+        e.step(p, 1)  # same order (stable sort of unchanged keys), same forces
+        after = e.stars()
+    want = orc.integrate_reference(before, acc[:, :3], 0.01)
+    assert np.array_equal(after.view(np.uint8), want.view(np.uint8))
+
+
+def test_c2_full_size(gcs, orc):
+    stars = gcs.collision(131072, 1002)
+    n = len(stars)
+    eps2 = float(np.float32(0.02) ** 2)
+    p = gcs.params(G=G, eps2=eps2, dt=0.005, theta=0.5, leaf_size=16)
+    with gcs.Engine(n) as e:
+        e.load(stars)
+        say("C2 compute")
+        e.compute_accels(p)
+        cpu = check_pipeline(gcs, orc, e, stars, p)
+        acc = e.accels()
+        s = cpu["sorted"]
+        tgt = np.linspace(0, n - 1, 4096).astype(np.int64)
+        ref = orc.direct_sum_f64(s["x"], s["y"], s["z"], s["mass"], G, eps2, tgt)
+        err = rel_rms(acc[tgt, :3], ref[:, :3])
+        say("C2 rel rms vs FP64 direct sum: %.3g" % err)
+        assert err < 1e-3
+        e.step(p, 5)
+        say("C2 5 steps ok")
+
+
+# ---- forests, graphs, composition ----------------------------------------------------
+
+def test_forest_equals_standalone(gcs, orc):
+    specs = [gcs.random_spec(11, k, 600, 9000) for k in range(6)]
+    parts = [gcs.generate(s) for s in specs]
+    parts.append(cloud(gcs, 1, 3))      # single particle instance
+    parts.append(cloud(gcs, 1025, 4))   # crosses a tile boundary
+    p = gcs.params(G=G, eps2=0.0025, dt=0.01, theta=0.5, leaf_size=8)
+    counts = [len(a) for a in parts]
+    with gcs.Engine(sum(counts), len(parts)) as e:
+        e.load(np.concatenate(parts), counts)
+        say("forest compute k=%d" % len(parts))
+        e.compute_accels(p)
+        acc_all = e.accels()
+        raw, srt, perm = e.keys()
+        trees = [e.tree(i) for i in range(len(parts))]
+    off = 0
+    for i, a in enumerate(parts):
+        with gcs.Engine(len(a)) as e1:
+            e1.load(a)
+            e1.compute_accels(p)
+            acc1 = e1.accels()
+            r1, s1, p1 = e1.keys()
+            t1 = e1.tree(0)
+        sl = slice(off, off + len(a))
+        assert np.array_equal(acc_all[sl].view(np.uint32), acc1.view(np.uint32)), i
+        assert np.array_equal(raw[sl], r1) and np.array_equal(srt[sl], s1)
+        assert np.array_equal(perm[sl], p1)
+        for k in ("left", "right", "parent", "com4"):
+            assert np.array_equal(trees[i][k], t1[k]), (i, k)
+        off += len(a)
+
+
+def test_graph_replays_match_eager(gcs):
+    stars = gcs.collision(4000, 21)
+    pg = gcs.params(G=G, eps2=0.0025, dt=0.01, use_graph=1, diagnostics=1)
+    pe = gcs.params(G=G, eps2=0.0025, dt=0.01, use_graph=0, diagnostics=1)
+    with gcs.Engine(len(stars)) as a, gcs.Engine(len(stars)) as b:
+        a.load(stars)
+        b.load(stars)
+        for rep in range(4):
+            say("graph replay %d" % rep)
+            a.step(pg, 3)
+            b.step(pe, 3)
+            _ = b.stars()  # eager work between replays
+            assert np.array_equal(a.stars().view(np.uint8), b.stars().view(np.uint8)), rep
+            assert np.array_equal(a.diagnostics(), b.diagnostics())
+
+
+def test_instance_remove_and_add(gcs):
+    parts = [gcs.collision(n, 30 + n) for n in (700, 1500, 900)]
+    p = gcs.params(G=G, eps2=0.0025, dt=0.01)
+    with gcs.Engine(20000, 8) as e:
+        e.load(np.concatenate(parts), [len(a) for a in parts])
+        e.step(p, 2)
+        keep0, keep2 = e.instance(0), e.instance(2)
+        e.remove(1)
+        assert e.count() == (2, len(parts[0]) + len(parts[2]))
+        assert np.array_equal(e.instance(0), keep0) and np.array_equal(e.instance(2), keep2)
+        new = gcs.collision(1200, 99)
+        iid = e.add(new)
+        assert iid == 3
+        say("composition step")
+        e.step(p, 3)
+        got = [e.instance(i) for i in (0, 2, 3)]
+    for start, g in zip((keep0, keep2, new), got):
+        with gcs.Engine(len(start)) as e1:
+            e1.load(start)
+            e1.step(p, 3)
+            assert np.array_equal(e1.stars().view(np.uint8), g.view(np.uint8))
+
+
+# ---- precision and diagnostics --------------------------------------------------------
+
+def test_compensated_positions(gcs):
+    stars = gcs.collision(3000, 41)
+    p = gcs.params(G=G, eps2=0.0025, dt=0.01, precision=gcs.PREC_COMPENSATED)
+    with gcs.Engine(len(stars)) as e:
+        e.load(stars)
+        say("compensated steps")
+        e.step(p, 20)
+        s = e.stars()
+        lo = e.position_lo()
+    hi = np.stack([s["x"], s["y"], s["z"]], 1).astype(np.float64)
+    ulp = np.spacing(np.abs(hi).astype(np.float32)).astype(np.float64)
+    assert np.all(np.abs(lo) <= 0.5 * ulp + 1e-30)
+    assert np.any(lo != 0)
+
+
+def test_diagnostics_match_host_sums(gcs, orc):
+    stars = gcs.collision(2000, 42)
+    eps2 = 0.0025
+    p = gcs.params(G=G, eps2=eps2, dt=0.01, diagnostics=1, use_graph=0)
+    with gcs.Engine(len(stars)) as e:
+        e.load(stars)
+        e.compute_accels(p)
+        s = e.stars()
+        acc = e.accels()
+        e.step(p, 1)
+        d = e.diagnostics()
+    m = s["mass"].astype(np.float64)
+    v = np.stack([s["vx"], s["vy"], s["vz"]], 1).astype(np.float64) + \
+        0.5 * 0.01 * acc[:, :3].astype(np.float64)
+    ke = 0.5 * (m * (v ** 2).sum(1)).sum()
+    ref = orc.direct_sum_f64(s["x"], s["y"], s["z"], s["mass"], G, eps2)
+    pe = 0.5 * (m * ref[:, 3]).sum()
+    assert np.isclose(d[0], ke, rtol=1e-9)
+    assert np.isclose(d[1], pe, rtol=1e-3)
+    assert np.allclose(d[2:], (m[:, None] * v).sum(0), rtol=1e-9, atol=1e-12)
+
+
+# ---- edge cases and errors ---------------------------------------------------------
+
+def test_coincident_particles(gcs):
+    st = np.zeros(300, gcs.STAR)
+    st["x"], st["y"], st["z"], st["mass"] = 1.5, -2.0, 0.25, 1.0
+    p = gcs.params(G=G, eps2=0.01, dt=0.01)
+    with gcs.Engine(300) as e:
+        e.load(st)
+        e.compute_accels(p)
+        raw, srt, perm = e.keys()
+        assert np.all(raw == 0) and np.array_equal(perm, np.arange(300))
+        assert np.all(e.accels()[:, :3] == 0)
+
+
+def test_stack_overflow_is_reported(gcs):
+    stars = cloud(gcs, 2000, 9)
+    p = gcs.reference_params(G, 1e-6, 0.01, -6.0, 6.0, stack_depth=2)
+    with gcs.Engine(len(stars)) as e:
+        e.load(stars)
+        with pytest.raises(gcs.GcsError) as ex:
+            e.step(p, 1)
+        assert ex.value.code == -6
+        e.step(gcs.params(G=G, eps2=0.01, dt=0.01), 1)  # the engine still works
+
+
+def test_bake_driver_writes_reference_frames(gcs, tmp_path):
+    exe = os.path.join(os.path.dirname(gcs.LIB_PATH), "gcs_bake")
+    out = tmp_path / "simulation.bin"
+    n_per = 3000
+    say("bake driver")
+    r = subprocess.run([exe, "3", "--n", str(n_per), "--seed", "5", "--eps2", "0.0025",
+                        "--out", str(out)], stdout=subprocess.PIPE, stderr=subprocess.STDOUT,
+                       text=True, timeout=120)
+    assert r.returncode == 0, r.stdout
+    data = np.fromfile(out, gcs.PLAYBACK_STAR)
+    assert len(data) == 3 * 2 * n_per
+    # Same frames from the engine with the same semantics.
+    stars = gcs.collision(n_per, 5)
+    with gcs.Engine(len(stars)) as e:
+        e.load(stars)
+        host = stars
+        for f in range(3):
+            mn = min(host["x"].min(), host["y"].min(), host["z"].min())
+            mx = max(host["x"].max(), host["y"].max(), host["z"].max())
+            e.step(gcs.reference_params(G, 0.0025, 0.01, float(mn), float(mx)), 1)
+            host = e.stars()
+            pb = e.playback()
+            assert np.array_equal(pb.view(np.uint8), data[f * 2 * n_per:(f + 1) * 2 * n_per]
+                                  .view(np.uint8)), f

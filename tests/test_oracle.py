@@ -1,0 +1,388 @@
+"""Pins the CPU oracle against closed forms, definitions, brute force and the
+compiled reference -- never against a retyped copy of its own formulas."""
+import numpy as np
+import pytest
+
+G = 1.0
+EPS2 = 0.0078125  # exactly representable in float32
+
+
+def rng(seed=0):
+    return np.random.default_rng(seed)
+
+
+# ---- Morton (morton.h:11-28) -------------------------------------------------
+
+def naive_expand(v):
+    v = np.asarray(v, np.uint64)
+    out = np.zeros_like(v)
+    for b in range(21):
+        out |= ((v >> np.uint64(b)) & np.uint64(1)) << np.uint64(3 * b)
+    return out
+
+
+def test_expand_bits_all_inputs(orc):
+    # With bounds [0, 2^21-1] the scale is exactly 1, so an integer coordinate
+    # v maps to ix = v and the key of (v, 0, 0) is expand(v) << 2.
+    v = np.arange(1 << 21, dtype=np.float32)
+    zero = np.zeros_like(v)
+    e = naive_expand(np.arange(1 << 21, dtype=np.uint64))
+    assert np.array_equal(orc.morton_n(v, zero, zero, 0.0, 2097151.0), e << np.uint64(2))
+    assert np.array_equal(orc.morton_n(zero, v, zero, 0.0, 2097151.0), e << np.uint64(1))
+    assert np.array_equal(orc.morton_n(zero, zero, v, 0.0, 2097151.0), e)
+
+
+def test_morton_known_answers(orc):
+    assert orc.expand_bits(1) == 1
+    assert orc.expand_bits(2) == 8
+    assert orc.expand_bits(0x1FFFFF) == 0x1249249249249249
+    assert orc.expand_bits(0xFFFFFFFF) == 0x1249249249249249  # masked to 21 bits
+    assert orc.morton(-3.0, -3.0, -3.0, -3.0, 5.0) == 0
+    assert orc.morton(5.0, 5.0, 5.0, -3.0, 5.0) == 0x7FFFFFFFFFFFFFFF
+    # x is the most significant axis.
+    assert orc.morton(5.0, -3.0, -3.0, -3.0, 5.0) == 0x4924924924924924
+    assert orc.morton(-3.0, -3.0, 5.0, -3.0, 5.0) == 0x1249249249249249
+
+
+def test_float_to_uint_saturates(orc):
+    f = orc.lib().orc_f2u_sat
+    assert f(float("nan")) == 0
+    assert f(-1.0) == 0
+    assert f(0.99) == 0
+    assert f(1.5) == 1
+    assert f(3.0e9) == 3000000000
+    assert f(4294967296.0) == 0xFFFFFFFF
+    assert f(float("inf")) == 0xFFFFFFFF
+
+
+def test_morton_edges_match_numpy_float32(orc):
+    # Brute-force the formula with numpy float32 arithmetic and a saturating
+    # conversion, including points below min, above max, x = min, x = max.
+    r = rng(1)
+    mn, mx = np.float32(-7.25), np.float32(3.5)
+    x = np.concatenate([r.uniform(-9, 6, 5000), [mn, mx, -7.25, 3.5]]).astype(np.float32)
+    y = r.uniform(-9, 6, len(x)).astype(np.float32)
+    z = r.uniform(-9, 6, len(x)).astype(np.float32)
+    scale = np.float32(2097151.0) / (mx - mn)
+
+    def conv(v):
+        q = ((v - mn) * scale).astype(np.float32).astype(np.float64)
+        q = np.where(np.isnan(q) | (q < 1), 0, np.where(q >= 2 ** 32, 2 ** 32 - 1, np.floor(q)))
+        return q.astype(np.uint64) & np.uint64(0x1FFFFF)
+
+    want = (naive_expand(conv(x)) << np.uint64(2)) | (naive_expand(conv(y)) << np.uint64(1)) \
+        | naive_expand(conv(z))
+    assert np.array_equal(orc.morton_n(x, y, z, float(mn), float(mx)), want)
+
+
+def test_morton_degenerate_span_is_zero(orc):
+    x = np.full(10, 2.0, np.float32)
+    assert np.all(orc.morton_n(x, x, x, 2.0, 2.0) == 0)
+
+
+def test_bounds_cube(orc):
+    r = rng(2)
+    x, y, z = (r.normal(size=1000).astype(np.float32) for _ in range(3))
+    mn, mx = orc.bounds_cube(x, y, z)
+    assert mn == min(x.min(), y.min(), z.min())
+    assert mx == max(x.max(), y.max(), z.max())
+
+
+# ---- Sort (CUB SortPairs semantics) ------------------------------------------
+
+@pytest.mark.parametrize("n", [1, 2, 17, 1000, 4096, 70001])
+def test_stable_sort_matches_numpy(orc, n):
+    r = rng(n)
+    keys = r.integers(0, 1 << 63, n, dtype=np.uint64)
+    keys[: n // 2] = keys[: n // 2] % np.uint64(97)  # many duplicates
+    ko, po = orc.stable_sort_pairs(keys)
+    want = np.argsort(keys, kind="stable")
+    assert np.array_equal(po, want)
+    assert np.array_equal(ko, keys[want])
+
+
+# ---- Karras radix tree (karras_bvh.cuh:23-131) --------------------------------
+
+def key_cases():
+    r = rng(3)
+    out = []
+    for n in (2, 3, 5, 8, 31, 64, 200, 1000):
+        out.append(np.sort(r.integers(0, 1 << 63, n, dtype=np.uint64)))
+        out.append(np.sort(r.integers(0, 4, n, dtype=np.uint64)))  # duplicate heavy
+    out.append(np.zeros(50, np.uint64))  # all equal
+    out.append(np.arange(100, dtype=np.uint64))
+    out.append(np.array([0, 0, 1, 1, 1, 2, 0x7FFFFFFFFFFFFFFF, 0x7FFFFFFFFFFFFFFF], np.uint64))
+    return out
+
+
+def check_tree_invariants(n, left, right, parent):
+    nodes = 2 * n - 1
+    assert parent[0] == -1
+    # every node but the root has exactly one parent, which lists it as a child
+    seen = np.zeros(nodes, int)
+    for v in range(n - 1):
+        for c in (left[v], right[v]):
+            assert 0 <= c < nodes
+            seen[c] += 1
+            assert parent[c] == v
+    assert seen[0] == 0 and np.all(seen[1:] == 1)
+    # leaf ranges of each internal node are contiguous and node i is an end
+    lo = np.zeros(nodes, int)
+    hi = np.zeros(nodes, int)
+    lo[n - 1:] = np.arange(n)
+    hi[n - 1:] = np.arange(n)
+    order = []
+    stack = [0]
+    while stack:
+        v = stack.pop()
+        if v < n - 1:
+            order.append(v)
+            stack += [left[v], right[v]]
+    for v in reversed(order):
+        assert hi[left[v]] + 1 == lo[right[v]]
+        lo[v], hi[v] = lo[left[v]], hi[right[v]]
+        assert v in (lo[v], hi[v])
+    assert lo[0] == 0 and hi[0] == n - 1
+
+
+@pytest.mark.parametrize("case", range(len(key_cases())))
+def test_karras_equals_definition(orc, case):
+    keys = key_cases()[case]
+    n = len(keys)
+    l1, r1, p1 = orc.karras_build(keys)
+    l2, r2, p2 = orc.radix_tree_definition(keys)
+    assert np.array_equal(l1, l2) and np.array_equal(r1, r2) and np.array_equal(p1, p2)
+    check_tree_invariants(n, l1, r1, p1)
+
+
+def test_karras_single_leaf(orc):
+    l, r, p = orc.karras_build(np.array([5], np.uint64))
+    assert list(l) == [-1] and list(r) == [-1] and list(p) == [-1]
+
+
+# ---- Aggregation (karras_bvh.cuh:134-225) --------------------------------------
+
+def small_cloud(n, seed):
+    r = rng(seed)
+    x, y, z = (r.normal(size=n).astype(np.float32) for _ in range(3))
+    m = r.uniform(0.5, 2.0, n).astype(np.float32)
+    return x, y, z, m
+
+
+def test_aggregate_against_fp64_subtree_sums(orc):
+    n = 3000
+    x, y, z, m = small_cloud(n, 4)
+    keys = orc.morton_n(x, y, z, *orc.bounds_cube(x, y, z))
+    sk, perm = orc.stable_sort_pairs(keys)
+    x, y, z, m = x[perm], y[perm], z[perm], m[perm]
+    left, right, parent = orc.karras_build(sk)
+    first, last = orc.node_ranges(n, left, right)
+    agg = orc.aggregate(left, right, x, y, z, m, 0.5)
+    pos = np.stack([x, y, z], 1).astype(np.float64)
+    for v in list(range(0, n - 1, 7)) + [0]:
+        sl = slice(first[v], last[v] + 1)
+        mm = m[sl].astype(np.float64)
+        M = mm.sum()
+        com = (pos[sl] * mm[:, None]).sum(0) / M
+        assert abs(agg["com4"][v, 3] - M) <= 1e-5 * M
+        ext = pos[sl].max(0) - pos[sl].min(0)
+        assert np.all(np.abs(agg["com4"][v, :3] - com) <= 1e-5 * max(ext.max(), 1e-6) + 1e-6)
+        assert np.array_equal(agg["bmin"][v], np.stack([x[sl], y[sl], z[sl]], 1).min(0))
+        assert np.array_equal(agg["bmax"][v], np.stack([x[sl], y[sl], z[sl]], 1).max(0))
+        w = (agg["bmax"][v] - agg["bmin"][v]).max()
+        assert agg["width"][v] == w
+        s = pos[sl] - com
+        s2 = (s * s).sum(1)
+        q = np.array([(mm * (3 * s[:, 0] ** 2 - s2)).sum(), (mm * (3 * s[:, 1] ** 2 - s2)).sum(),
+                      (mm * (3 * s[:, 2] ** 2 - s2)).sum(), (mm * 3 * s[:, 0] * s[:, 1]).sum(),
+                      (mm * 3 * s[:, 0] * s[:, 2]).sum(), (mm * 3 * s[:, 1] * s[:, 2]).sum()])
+        scale = (mm * s2).sum() * 3 + 1e-12
+        assert np.all(np.abs(agg["quad"][v] - q) <= 1e-4 * scale)
+    assert abs(agg["com4"][0, 3] - m.astype(np.float64).sum()) <= 1e-5 * m.sum()
+
+
+def test_aggregate_zero_mass_uses_box_centre(orc):
+    x = np.array([0.0, 1.0, 4.0], np.float32)
+    y = np.zeros(3, np.float32)
+    z = np.zeros(3, np.float32)
+    m = np.zeros(3, np.float32)
+    left, right, parent = orc.karras_build(np.array([1, 2, 3], np.uint64))
+    agg = orc.aggregate(left, right, x, y, z, m, 0.5)
+    assert agg["com4"][0, 0] == 2.0 and agg["com4"][0, 3] == 0.0
+
+
+# ---- Traversals ---------------------------------------------------------------
+
+def pipeline(orc, n, seed, theta=0.5):
+    x, y, z, m = small_cloud(n, seed)
+    st = np.zeros(n, orc.STAR)
+    st["x"], st["y"], st["z"], st["mass"] = x, y, z, m
+    return orc.build_tree_pipeline(st, theta)
+
+
+def rel_rms(a, b):
+    return np.sqrt(((a - b) ** 2).sum() / (b ** 2).sum())
+
+
+def test_group_traversal_theta0_is_direct_sum(orc):
+    t = pipeline(orc, 700, 5, theta=0.0)
+    s = t["sorted"]
+    acc, msum, cnt = orc.traverse_group(t["left"], t["right"], t["first"], t["last"], s["x"],
+                                        s["y"], s["z"], s["mass"], t["agg"], G, EPS2, 64, 8)
+    ref = orc.direct_sum_f64(s["x"], s["y"], s["z"], s["mass"], G, EPS2)
+    assert cnt[0] == 0
+    assert np.allclose(acc, ref, rtol=1e-10, atol=1e-10)
+
+
+def test_group_traversal_large_leaf_is_direct_sum(orc):
+    t = pipeline(orc, 300, 6)
+    s = t["sorted"]
+    acc, msum, cnt = orc.traverse_group(t["left"], t["right"], t["first"], t["last"], s["x"],
+                                        s["y"], s["z"], s["mass"], t["agg"], G, EPS2, 64, 300)
+    ref = orc.direct_sum_f64(s["x"], s["y"], s["z"], s["mass"], G, EPS2)
+    assert np.allclose(acc, ref, rtol=1e-10, atol=1e-10)
+
+
+def test_group_traversal_counts_every_mass_once(orc):
+    t = pipeline(orc, 5000, 7)
+    s = t["sorted"]
+    acc, msum, cnt = orc.traverse_group(t["left"], t["right"], t["first"], t["last"], s["x"],
+                                        s["y"], s["z"], s["mass"], t["agg"], G, EPS2, 64, 16)
+    M = s["mass"].astype(np.float64).sum()
+    assert np.allclose(msum + s["mass"], M, rtol=1e-5)
+    assert cnt[0] > 0 and cnt[1] > 0
+
+
+def test_group_traversal_accuracy_quadrupole(orc):
+    t = pipeline(orc, 6000, 8)
+    s = t["sorted"]
+    ref = orc.direct_sum_f64(s["x"], s["y"], s["z"], s["mass"], G, EPS2)
+    acc_q, _, _ = orc.traverse_group(t["left"], t["right"], t["first"], t["last"], s["x"], s["y"],
+                                     s["z"], s["mass"], t["agg"], G, EPS2, 64, 16, quad=True)
+    acc_m, _, _ = orc.traverse_group(t["left"], t["right"], t["first"], t["last"], s["x"], s["y"],
+                                     s["z"], s["mass"], t["agg"], G, EPS2, 64, 16, quad=False)
+    eq = rel_rms(acc_q[:, :3], ref[:, :3])
+    em = rel_rms(acc_m[:, :3], ref[:, :3])
+    assert eq < 1e-3
+    assert eq < em  # the quadrupole term reduces the error
+    assert rel_rms(acc_q[:, 3], ref[:, 3]) < 1e-4
+
+
+def test_plummer_two_body_closed_form(orc):
+    eps2 = 0.09
+    x = np.array([0.0, 2.0], np.float32)
+    y = np.zeros(2, np.float32)
+    z = np.zeros(2, np.float32)
+    m = np.array([3.0, 5.0], np.float32)
+    ref = orc.direct_sum_f64(x, y, z, m, G, eps2)
+    f = 2.0 / (4.0 + eps2) ** 1.5
+    assert np.isclose(ref[0, 0], 5.0 * f) and np.isclose(ref[1, 0], -3.0 * f)
+    assert np.isclose(ref[0, 3], -5.0 / np.sqrt(4.0 + eps2))
+    st = np.zeros(2, orc.STAR)
+    st["x"], st["mass"] = x, m
+    t = orc.build_tree_pipeline(st, 0.5)
+    s = t["sorted"]
+    acc, _, _ = orc.traverse_group(t["left"], t["right"], t["first"], t["last"], s["x"], s["y"],
+                                   s["z"], s["mass"], t["agg"], G, eps2, 64, 1)
+    want = orc.direct_sum_f64(s["x"], s["y"], s["z"], s["mass"], G, eps2)
+    assert np.allclose(acc, want, rtol=1e-12)
+    accr, _, _ = orc.traverse_reference(t["left"], t["right"], s["x"], s["y"], s["z"],
+                                        t["agg"]["com4"], t["agg"]["width"], G, eps2)
+    assert np.allclose(accr[:, :3], want[:, :3], rtol=1e-6)
+
+
+def test_direct_sum_momentum_conservation(orc):
+    x, y, z, m = small_cloud(500, 9)
+    a = orc.direct_sum_f64(x, y, z, m, G, EPS2)
+    F = (a[:, :3] * m[:, None].astype(np.float64)).sum(0)
+    assert np.all(np.abs(F) < 1e-10 * np.abs(a[:, :3] * m[:, None]).sum(0))
+
+
+def test_reference_traversal_partition(orc):
+    # The taken nodes and the leaves skipped by distance (cuda_engine.cu:134)
+    # partition the particles: their masses add up to the total exactly once.
+    t = pipeline(orc, 4000, 10)
+    s = t["sorted"]
+    eps2 = EPS2
+    acc, (msum, mskip), cnt = orc.traverse_reference(t["left"], t["right"], s["x"], s["y"],
+                                                     s["z"], t["agg"]["com4"],
+                                                     t["agg"]["width"], G, eps2)
+    M = s["mass"].astype(np.float64).sum()
+    assert np.allclose(msum + mskip, M, rtol=1e-5)
+    ref = orc.direct_sum_f64(s["x"], s["y"], s["z"], s["mass"], G, eps2)
+    assert rel_rms(acc[:, :3], ref[:, :3]) < 0.1  # theta = 1 monopole
+
+
+# ---- Integrate (cuda_engine.cu:164-216) -----------------------------------------
+
+def test_integrate_constant_acceleration_closed_form(orc):
+    n = 64
+    r = rng(11)
+    st = np.zeros(n, orc.STAR)
+    for k in ("x", "y", "z", "vx", "vy", "vz"):
+        st[k] = r.normal(size=n).astype(np.float32)
+    st["type"] = 1
+    a = r.normal(size=(n, 3)).astype(np.float32)
+    dt, K = 0.01, 50
+    s = st
+    for _ in range(K):
+        s = orc.integrate_reference(s, a, dt)
+    v0 = np.stack([st["vx"], st["vy"], st["vz"]], 1).astype(np.float64)
+    x0 = np.stack([st["x"], st["y"], st["z"]], 1).astype(np.float64)
+    v = v0 + K * a * dt
+    x = x0 + K * v0 * dt + a * dt * dt * K * (K + 1) / 2
+    assert np.allclose(np.stack([s["vx"], s["vy"], s["vz"]], 1), v, atol=1e-5)
+    assert np.allclose(np.stack([s["x"], s["y"], s["z"]], 1), x, atol=1e-5)
+
+
+@pytest.mark.parametrize("speed,t0,t2", [(0.0, (255, 50, 0), (0, 100, 255)),
+                                         (75.0, (255, 200, 0), (0, 200, 255)),
+                                         (300.0, (255, 255, 150), (200, 255, 255))])
+def test_colour_table_spot_values(orc, speed, t0, t2):
+    # cuda_engine.cu:185-215 comments: fire 50->200 then 200->255 / 0->150;
+    # ice 100->200 then 0->200 / 200->255.
+    st = np.zeros(3, orc.STAR)
+    st["type"] = [0, 2, 1]
+    st["vx"] = speed
+    st["r"], st["g"], st["b"] = 7, 8, 9
+    s = orc.integrate_reference(st, np.zeros((3, 3), np.float32), 0.0)
+    assert tuple(int(s[k][0]) for k in "rgb") == t0
+    assert tuple(int(s[k][1]) for k in "rgb") == t2
+    assert tuple(int(s[k][2]) for k in "rgb") == (7, 8, 9)
+
+
+# ---- CPU octree (engine.cpp, gravity_aggregator.cpp, main.cpp:152-253) --------
+
+def cloud_stars(orc, n, seed):
+    x, y, z, m = small_cloud(n, seed)
+    st = np.zeros(n, orc.STAR)
+    st["x"], st["y"], st["z"], st["mass"] = x, y, z, m / n
+    return st
+
+
+def test_octree_single_leaf_is_direct_sum(orc):
+    st = cloud_stars(orc, 300, 12)
+    s, acc, _ = orc.octree_step(st, G, EPS2, 0.0, theta=0.5, leaf_cap=512, integrate=False,
+                                want_acc=True)
+    ref = orc.direct_sum_f64(s["x"], s["y"], s["z"], s["mass"], G, EPS2)
+    assert rel_rms(acc, ref[:, :3]) < 1e-5
+
+
+def test_octree_error_bound(orc):
+    st = cloud_stars(orc, 6000, 13)
+    s, acc, _ = orc.octree_step(st, G, EPS2, 0.0, theta=0.5, leaf_cap=32, integrate=False,
+                                want_acc=True)
+    ref = orc.direct_sum_f64(s["x"], s["y"], s["z"], s["mass"], G, EPS2)
+    assert rel_rms(acc, ref[:, :3]) < 1e-2
+
+
+def test_octree_matches_compiled_reference(orc):
+    if not orc.ref_available():
+        pytest.skip("oracle/_ref not built (reference sources absent)")
+    st = cloud_stars(orc, 20000, 14)
+    st["vx"] = 0.1
+    a, acc_a, _ = orc.octree_step(st, G, 0.0025, 0.01, theta=0.8, leaf_cap=32, integrate=True,
+                                  want_acc=True)
+    b, acc_b, _ = orc.ref_octree_step(st, G, 0.0025, 0.01, integrate=True, want_acc=True)
+    assert np.array_equal(acc_a, acc_b)
+    assert np.array_equal(a.view(np.uint8), b.view(np.uint8))
