@@ -135,6 +135,7 @@ def main():
     n_inter = inter[0] + inter[1]
     achieved = FLOP_PER_INTERACTION * n_inter / (trav_ms * 1e-3) / 1e12
     stage_ms = [float(x) for x in np.mean(np.array(stages), 0)]
+    info = e.engine_info()
     e.close()
 
     line = {
@@ -162,6 +163,7 @@ def main():
                      "kernel": "k_traverse_group", "kernel_ms": trav_ms,
                      "flop_per_interaction": FLOP_PER_INTERACTION},
         "cpu_baseline": None,
+        "engine": info,
     }
     if not a.no_cpu:
         cb, acc_check = cpu_baseline(sorted0, p, acc0, a.cpu_seconds)

@@ -64,7 +64,6 @@ struct Instance {
 };
 
 size_t max_tiles(int64_t cap_n, int32_t cap_k) { return (size_t)(cap_n / kTile) + (size_t)cap_k + 1; }
-size_t max_groups(int64_t cap_n, int32_t cap_k) { return (size_t)(cap_n / kGroup) + (size_t)cap_k + 1; }
 size_t hist_entries(int64_t cap_n, int32_t cap_k) { return (size_t)kRadix * max_tiles(cap_n, cap_k); }
 size_t scan_blocks(size_t entries) { return (entries + kScanChunk - 1) / kScanChunk; }
 
@@ -88,9 +87,8 @@ struct gcs_engine {
     uint64_t* keys[2] = {nullptr, nullptr};
     uint32_t* vals[2] = {nullptr, nullptr};
     uint64_t* keys_raw = nullptr;
-    // tree
-    float4* nodeA = nullptr;
-    int4* nodeB = nullptr;
+    // tree (node: 2 float4 per node -- COM/mass, opening value/first/last/split)
+    float4* node = nullptr;
     float4* nodeQ = nullptr;
     float4* boxLo = nullptr;
     float4* boxHi = nullptr;
@@ -99,7 +97,11 @@ struct gcs_engine {
     // tables
     TileInfo* tiles = nullptr;
     SegInfo* segs = nullptr;
-    GroupInfo* groups = nullptr;
+    GroupInfo* groups = nullptr;  // traversal groups, rebuilt on the device each step
+    uint32_t* gmark = nullptr;
+    uint32_t* gidx = nullptr;
+    int* gcnt = nullptr;
+    int* ngroups = nullptr;
     uint32_t* hist = nullptr;
     uint32_t* partial = nullptr;
     uint32_t* seg_bounds = nullptr;
@@ -108,14 +110,13 @@ struct gcs_engine {
     unsigned* err = nullptr;
     double* diag_part = nullptr;
     double* diag_out = nullptr;
-    int* stack_mem = nullptr;
+    int* slot_mem = nullptr;  // per traversal wave: stack spill, node and particle lists
     uint8_t* staging = nullptr;
 
     // host mirror of the composition
     std::vector<Instance> inst;
     std::vector<SegInfo> h_segs;
     std::vector<TileInfo> h_tiles;
-    std::vector<GroupInfo> h_groups;
     int64_t n_total = 0;
     int32_t next_id = 0;
     bool have_tree = false;
@@ -163,11 +164,12 @@ void free_all(gcs_engine* e)
         e->gexec[k] = nullptr;
     }
     dfree(e->acc); dfree(e->keys_raw);
-    dfree(e->nodeA); dfree(e->nodeB); dfree(e->nodeQ); dfree(e->boxLo); dfree(e->boxHi);
+    dfree(e->node); dfree(e->nodeQ); dfree(e->boxLo); dfree(e->boxHi);
     dfree(e->parent); dfree(e->flag);
     dfree(e->tiles); dfree(e->segs); dfree(e->groups); dfree(e->hist); dfree(e->partial);
+    dfree(e->gmark); dfree(e->gidx); dfree(e->gcnt); dfree(e->ngroups);
     dfree(e->seg_bounds); dfree(e->qctr); dfree(e->counters); dfree(e->err);
-    dfree(e->diag_part); dfree(e->diag_out); dfree(e->stack_mem); dfree(e->staging);
+    dfree(e->diag_part); dfree(e->diag_out); dfree(e->slot_mem); dfree(e->staging);
     for (auto& v : e->ev)
         if (v) (void)hipEventDestroy(v), v = nullptr;
 }
@@ -183,7 +185,6 @@ int rebuild_tables(gcs_engine* e)
 {
     e->h_segs.clear();
     e->h_tiles.clear();
-    e->h_groups.clear();
     int start = 0;
     for (size_t s = 0; s < e->inst.size(); ++s) {
         const int n = e->inst[s].n;
@@ -207,14 +208,6 @@ int rebuild_tables(gcs_engine* e)
             t.pad0 = t.pad1 = 0;
             e->h_tiles.push_back(t);
         }
-        for (int j = 0; j < n; j += kGroup) {
-            GroupInfo g;
-            g.first = start + j;
-            g.count = std::min(kGroup, n - j);
-            g.seg = (int)s;
-            g.pad = 0;
-            e->h_groups.push_back(g);
-        }
         e->h_segs.push_back(sg);
         start += n;
     }
@@ -223,8 +216,6 @@ int rebuild_tables(gcs_engine* e)
         HIPCHK(hipMemcpy(e->segs, e->h_segs.data(), sizeof(SegInfo) * e->h_segs.size(),
                          hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(e->tiles, e->h_tiles.data(), sizeof(TileInfo) * e->h_tiles.size(),
-                         hipMemcpyHostToDevice));
-        HIPCHK(hipMemcpy(e->groups, e->h_groups.data(), sizeof(GroupInfo) * e->h_groups.size(),
                          hipMemcpyHostToDevice));
     }
     e->graph_valid = false;
@@ -245,8 +236,8 @@ int check_params(const gcs_params* p)
         if (!(p->eps2 > 0.0f)) return fail(GCS_ERR_INVALID, "group traversal needs eps2 > 0");
         if (p->theta < 0.0f || p->theta > 1.0f)
             return fail(GCS_ERR_INVALID, "theta must be in [0, 1]");
-        if (p->leaf_size < 1 || p->leaf_size > 4096)
-            return fail(GCS_ERR_INVALID, "leaf_size must be in [1, 4096]");
+        if (p->leaf_size < 1 || p->leaf_size > kMaxLeaf)
+            return fail(GCS_ERR_INVALID, "leaf_size must be in [1, 64]");
         if (p->block_size != 64 && p->block_size != 128 && p->block_size != 256)
             return fail(GCS_ERR_INVALID, "block_size must be 64, 128 or 256");
     } else {
@@ -264,15 +255,15 @@ template <bool POT, bool QUAD, int WPB>
 void launch_group(gcs_engine* e, int blocks, const gcs_params& p, float4* posm)
 {
     k_traverse_group<POT, QUAD, WPB><<<blocks, WPB * kWave, 0, e->stream>>>(
-        e->groups, (int)e->h_groups.size(), e->segs, posm, e->nodeA, e->nodeB, e->nodeQ, e->acc,
-        e->stack_mem, e->qctr, e->counters, e->err, p.G, p.eps2, p.leaf_size);
+        e->groups, e->ngroups, e->segs, posm, e->node, e->nodeQ, e->acc,
+        e->slot_mem, e->qctr, e->counters, e->err, p.G, p.eps2, p.leaf_size);
 }
 
 template <int STACK>
 void launch_reference(gcs_engine* e, const gcs_params& p, float4* posm, int nblk)
 {
     k_traverse_reference<STACK><<<nblk, kTileThreads, 0, e->stream>>>(
-        e->tiles, e->segs, posm, e->nodeA, e->nodeB, e->acc, e->counters, e->err, p.G, p.eps2,
+        e->tiles, e->segs, posm, e->node, e->acc, e->counters, e->err, p.G, p.eps2,
         std::min(p.stack_depth, STACK));
 }
 
@@ -324,24 +315,37 @@ int enqueue_step(gcs_engine* e, const gcs_params& p, bool integrate)
     mark(e, 2);
     k_permute<<<qblocks, kTileThreads, 0, e->stream>>>(
         e->tiles, e->segs, e->vals[0], e->posm[c], e->vel[c], e->plo[c], e->posm[c ^ 1],
-        e->vel[c ^ 1], e->plo[c ^ 1], comp ? 1 : 0, group ? 0 : 1, e->nodeA, e->nodeB, e->nodeQ);
+        e->vel[c ^ 1], e->plo[c ^ 1], comp ? 1 : 0, group ? 0 : 1, e->node, e->nodeQ, e->gmark,
+        e->gidx);
     e->cur = c ^ 1;
     float4* posm = e->posm[e->cur];
     mark(e, 3);
-    k_build<<<qblocks, kTileThreads, 0, e->stream>>>(e->tiles, e->segs, e->keys[0], e->nodeB,
-                                                     e->parent, e->flag);
+    k_build<<<qblocks, kTileThreads, 0, e->stream>>>(e->tiles, e->segs, e->keys[0], e->node,
+                                                     e->parent, e->flag, e->gmark, e->gidx,
+                                                     e->gcnt);
+    if (group) {
+        // Number the groups in particle order and write the group table.
+        const int gb = (int)scan_blocks((size_t)e->n_total);
+        k_scan_reduce<<<gb, 256, 0, e->stream>>>(e->gidx, e->n_total, e->partial);
+        k_scan_partials<<<1, 256, 0, e->stream>>>(e->partial, gb);
+        k_scan_apply<<<gb, 256, 0, e->stream>>>(e->gidx, e->n_total, e->partial);
+        k_group_emit<<<qblocks, kTileThreads, 0, e->stream>>>(
+            e->tiles, e->segs, e->gmark, e->gidx, e->gcnt, e->groups, e->ngroups,
+            (int)e->n_total);
+    }
     mark(e, 4);
     const float inv_theta = p.theta > 0.0f ? 1.0f / p.theta : __builtin_inff();
-    k_aggregate<<<qblocks, kTileThreads, 0, e->stream>>>(
-        e->tiles, e->segs, posm, e->nodeA, e->nodeB, e->nodeQ, e->boxLo, e->boxHi, e->parent,
-        e->flag, inv_theta, group ? 1 : 0);
+    k_aggregate<<<ntiles, kTile, 0, e->stream>>>(e->tiles, e->segs, posm, e->node, e->nodeQ,
+                                                 e->boxLo, e->boxHi, e->parent, e->flag,
+                                                 inv_theta, group ? 1 : 0);
     LAUNCHCHK();
     mark(e, 5);
     if (group) {
         const int wpb = p.block_size / kWave;
-        int blocks = std::min(grid_traverse_blocks(e, wpb) * (4 / wpb),
-                              (int)((e->h_groups.size() + wpb - 1) / wpb));
-        blocks = std::max(blocks, 1);
+        // Persistent: as many blocks as the per-wave memory covers; waves that
+        // find no group left exit at once.
+        const int blocks = std::max(1, std::min(grid_traverse_blocks(e, wpb) * (4 / wpb),
+                                                (int)((e->n_total + wpb - 1) / wpb)));
         const bool pot = p.diagnostics != 0;
         // Stack slots: blocks * wpb waves <= trav_blocks_cap * 4.
         if (wpb == 4) {
@@ -469,9 +473,9 @@ int64_t gcs_device_bytes(int64_t cap_n, int32_t cap_k)
     b += 6 * 16 * n;           // posm, vel, plo x2
     b += 16 * n;               // acc
     b += 3 * 8 * n + 2 * 4 * n; // keys x2 + raw, vals x2
-    b += 2 * n * (16 + 16 + 32 + 16 + 16 + 4 + 4);  // nodes
+    b += 2 * n * (32 + 32 + 16 + 16 + 4 + 4);  // nodes
     b += (int64_t)(sizeof(TileInfo) * max_tiles(cap_n, cap_k) + sizeof(SegInfo) * cap_k +
-                   sizeof(GroupInfo) * max_groups(cap_n, cap_k));
+                   sizeof(GroupInfo) * (cap_n + 1) + 12 * cap_n);
     b += (int64_t)(4 * hist_entries(cap_n, cap_k) + 4 * scan_blocks(hist_entries(cap_n, cap_k)));
     b += 8 * cap_k + 64 + 5 * 8 * 4 * (int64_t)max_tiles(cap_n, cap_k) + 5 * 8 * cap_k;
     b += 32 * n;               // staging
@@ -503,7 +507,7 @@ int gcs_create(int64_t cap_n, int32_t cap_k, int32_t device, gcs_engine** out)
         occ < 1)
         occ = 2;
     e->trav_blocks_cap = e->num_cu * occ;
-    const size_t nt = max_tiles(cap_n, cap_k), ng = max_groups(cap_n, cap_k);
+    const size_t nt = max_tiles(cap_n, cap_k);
     const size_t he = hist_entries(cap_n, cap_k);
     int rc = GCS_OK;
     const size_t n = (size_t)cap_n;
@@ -516,8 +520,7 @@ int gcs_create(int64_t cap_n, int32_t cap_k, int32_t device, gcs_engine** out)
     }
     if (!rc) rc = dalloc(&e->acc, n);
     if (!rc) rc = dalloc(&e->keys_raw, n);
-    if (!rc) rc = dalloc(&e->nodeA, 2 * n);
-    if (!rc) rc = dalloc(&e->nodeB, 2 * n);
+    if (!rc) rc = dalloc(&e->node, 4 * n);
     if (!rc) rc = dalloc(&e->nodeQ, 4 * n);
     if (!rc) rc = dalloc(&e->boxLo, 2 * n);
     if (!rc) rc = dalloc(&e->boxHi, 2 * n);
@@ -525,16 +528,20 @@ int gcs_create(int64_t cap_n, int32_t cap_k, int32_t device, gcs_engine** out)
     if (!rc) rc = dalloc(&e->flag, 2 * n);
     if (!rc) rc = dalloc(&e->tiles, nt);
     if (!rc) rc = dalloc(&e->segs, (size_t)cap_k);
-    if (!rc) rc = dalloc(&e->groups, ng);
+    if (!rc) rc = dalloc(&e->groups, n + 1);
+    if (!rc) rc = dalloc(&e->gmark, n);
+    if (!rc) rc = dalloc(&e->gidx, n);
+    if (!rc) rc = dalloc(&e->gcnt, n);
+    if (!rc) rc = dalloc(&e->ngroups, 1);
     if (!rc) rc = dalloc(&e->hist, he);
-    if (!rc) rc = dalloc(&e->partial, scan_blocks(he) + 1);
+    if (!rc) rc = dalloc(&e->partial, std::max(scan_blocks(he), scan_blocks(n)) + 1);
     if (!rc) rc = dalloc(&e->seg_bounds, 2 * (size_t)cap_k);
     if (!rc) rc = dalloc(&e->qctr, kNumXcd);
     if (!rc) rc = dalloc(&e->counters, 2);
     if (!rc) rc = dalloc(&e->err, 1);
     if (!rc) rc = dalloc(&e->diag_part, 5 * 4 * nt);
     if (!rc) rc = dalloc(&e->diag_out, 5 * (size_t)cap_k);
-    if (!rc) rc = dalloc(&e->stack_mem, (size_t)e->trav_blocks_cap * 4 * kStackCap);
+    if (!rc) rc = dalloc(&e->slot_mem, (size_t)e->trav_blocks_cap * 4 * kSlotInts);
     if (!rc) rc = dalloc(&e->staging, 32 * n);
     if (rc) {
         free_all(e);
@@ -840,12 +847,10 @@ int gcs_read_tree(gcs_engine* e, int32_t id, int32_t* left, int32_t* right, int3
     HIPCHK(hipSetDevice(e->device));
     const SegInfo sg = e->h_segs[slot];
     const int n = sg.n, nodes = 2 * n - 1, base = sg.root;
-    std::vector<float4> A((size_t)nodes), Q((size_t)2 * nodes), lo((size_t)nodes), hi((size_t)nodes),
-        P((size_t)n);
-    std::vector<int4> B((size_t)nodes);
+    std::vector<float4> AB((size_t)2 * nodes), Q((size_t)2 * nodes), lo((size_t)nodes),
+        hi((size_t)nodes), P((size_t)n);
     std::vector<int> par((size_t)nodes);
-    HIPCHK(hipMemcpyAsync(A.data(), e->nodeA + base, 16 * (size_t)nodes, hipMemcpyDeviceToHost, e->stream));
-    HIPCHK(hipMemcpyAsync(B.data(), e->nodeB + base, 16 * (size_t)nodes, hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(hipMemcpyAsync(AB.data(), e->node + 2 * (size_t)base, 32 * (size_t)nodes, hipMemcpyDeviceToHost, e->stream));
     HIPCHK(hipMemcpyAsync(Q.data(), e->nodeQ + 2 * (size_t)base, 32 * (size_t)nodes, hipMemcpyDeviceToHost, e->stream));
     HIPCHK(hipMemcpyAsync(lo.data(), e->boxLo + base, 16 * (size_t)nodes, hipMemcpyDeviceToHost, e->stream));
     HIPCHK(hipMemcpyAsync(hi.data(), e->boxHi + base, 16 * (size_t)nodes, hipMemcpyDeviceToHost, e->stream));
@@ -865,11 +870,13 @@ int gcs_read_tree(gcs_engine* e, int32_t id, int32_t* left, int32_t* right, int3
             if (bmax) { bmax[3 * v] = p.x; bmax[3 * v + 1] = p.y; bmax[3 * v + 2] = p.z; }
             continue;
         }
-        const int4 b = B[(size_t)v];
+        const float4 bf = AB[(size_t)2 * v + 1];
+        int4 b;
+        std::memcpy(&b, &bf, sizeof(b));
         const int f = b.y - sg.start, l = b.z - sg.start, s = b.w - sg.start;
         if (left) left[v] = (f == s) ? (n - 1) + s : s;
         if (right) right[v] = (l == s + 1) ? (n - 1) + s + 1 : s + 1;
-        if (com4) { const float4 a = A[(size_t)v]; com4[4 * v] = a.x; com4[4 * v + 1] = a.y; com4[4 * v + 2] = a.z; com4[4 * v + 3] = a.w; }
+        if (com4) { const float4 a = AB[(size_t)2 * v]; com4[4 * v] = a.x; com4[4 * v + 1] = a.y; com4[4 * v + 2] = a.z; com4[4 * v + 3] = a.w; }
         if (width) width[v] = Q[(size_t)2 * v + 1].z;
         if (bmin) { bmin[3 * v] = lo[(size_t)v].x; bmin[3 * v + 1] = lo[(size_t)v].y; bmin[3 * v + 2] = lo[(size_t)v].z; }
         if (bmax) { bmax[3 * v] = hi[(size_t)v].x; bmax[3 * v + 1] = hi[(size_t)v].y; bmax[3 * v + 2] = hi[(size_t)v].z; }
@@ -907,6 +914,25 @@ int gcs_read_diagnostics(gcs_engine* e, int32_t id, double* out5)
     if (slot < 0) return fail(GCS_ERR_NOT_FOUND, "unknown instance id");
     HIPCHK(hipStreamSynchronize(e->stream));
     HIPCHK(hipMemcpy(out5, e->diag_out + 5 * slot, 5 * sizeof(double), hipMemcpyDeviceToHost));
+    return GCS_OK;
+}
+
+int gcs_engine_info(gcs_engine* e, int64_t* out8)
+{
+    if (!e || !out8) return fail(GCS_ERR_INVALID, "null argument");
+    out8[0] = e->num_cu;
+    out8[1] = e->trav_blocks_cap;
+    out8[2] = (int64_t)e->trav_blocks_cap * 4;
+    out8[3] = (int64_t)e->trav_blocks_cap * 4 * kSlotInts * 4;
+    out8[4] = e->cap_n;
+    out8[5] = e->cap_k;
+    int ng = 0;
+    if (e->have_tree) {
+        HIPCHK(hipStreamSynchronize(e->stream));
+        HIPCHK(hipMemcpy(&ng, e->ngroups, sizeof(int), hipMemcpyDeviceToHost));
+    }
+    out8[6] = ng;
+    out8[7] = (int64_t)e->h_tiles.size();
     return GCS_OK;
 }
 
@@ -1010,15 +1036,16 @@ int gcs_karras_tree(const uint64_t* sorted_keys, int32_t n, int32_t* left, int32
         rc = fail(GCS_ERR_HIP, "upload failed");
     if (!rc) {
         const int qblocks = 4 * (int)e->h_tiles.size();
-        k_build<<<qblocks, kTileThreads, 0, e->stream>>>(e->tiles, e->segs, e->keys[0], e->nodeB,
-                                                         e->parent, e->flag);
+        k_build<<<qblocks, kTileThreads, 0, e->stream>>>(e->tiles, e->segs, e->keys[0], e->node,
+                                                         e->parent, e->flag, e->gmark, e->gidx,
+                                                         e->gcnt);
         hipError_t le = hipGetLastError();
         if (le != hipSuccess) rc = fail(GCS_ERR_HIP, hipGetErrorString(le));
     }
     if (!rc) {
         const int nodes = 2 * n - 1;
-        std::vector<int4> B((size_t)nodes);
-        if (hipMemcpyAsync(B.data(), e->nodeB, 16 * (size_t)nodes, hipMemcpyDeviceToHost, e->stream) != hipSuccess ||
+        std::vector<int4> B((size_t)2 * nodes);
+        if (hipMemcpyAsync(B.data(), e->node, 32 * (size_t)nodes, hipMemcpyDeviceToHost, e->stream) != hipSuccess ||
             hipMemcpyAsync(parent, e->parent, 4 * (size_t)nodes, hipMemcpyDeviceToHost, e->stream) != hipSuccess ||
             hipStreamSynchronize(e->stream) != hipSuccess) {
             rc = fail(GCS_ERR_HIP, "copy back failed");
@@ -1028,7 +1055,8 @@ int gcs_karras_tree(const uint64_t* sorted_keys, int32_t n, int32_t* left, int32
                     left[v] = right[v] = -1;
                     continue;
                 }
-                const int f = B[(size_t)v].y, l = B[(size_t)v].z, s = B[(size_t)v].w;
+                const int f = B[(size_t)2 * v + 1].y, l = B[(size_t)2 * v + 1].z,
+                          s = B[(size_t)2 * v + 1].w;
                 left[v] = (f == s) ? (n - 1) + s : s;
                 right[v] = (l == s + 1) ? (n - 1) + s + 1 : s + 1;
             }

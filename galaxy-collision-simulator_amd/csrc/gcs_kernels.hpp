@@ -105,22 +105,25 @@ __global__ __launch_bounds__(256) void k_permute(
     const uint32_t* __restrict__ perm, const float4* __restrict__ posm_in,
     const float4* __restrict__ vel_in, const float4* __restrict__ plo_in,
     float4* __restrict__ posm_out, float4* __restrict__ vel_out, float4* __restrict__ plo_out,
-    int compensated, int write_leaves, float4* __restrict__ nodeA, int4* __restrict__ nodeB,
-    float4* __restrict__ nodeQ)
+    int compensated, int write_leaves, float4* __restrict__ node, float4* __restrict__ nodeQ,
+    uint32_t* __restrict__ gmark, uint32_t* __restrict__ gidx)
 {
     const QuarterTile q = quarter_tile(tiles, segs);
     if (!q.valid) return;
+    gmark[q.i] = 0u;
+    gidx[q.i] = 0u;
     const int src = (int)perm[q.i];
     const float4 p = posm_in[src];
     posm_out[q.i] = p;
     vel_out[q.i] = vel_in[src];
     if (compensated) plo_out[q.i] = plo_in[src];
     if (write_leaves) {
-        const int node = q.i + segs[q.seg].leaf_base;
-        nodeA[node] = p;
-        nodeB[node] = make_int4(0, q.i, q.i, -1);
-        nodeQ[2 * node] = make_float4(0.f, 0.f, 0.f, 0.f);
-        nodeQ[2 * node + 1] = make_float4(0.f, 0.f, 0.f, 0.f);
+        const int v = q.i + segs[q.seg].leaf_base;
+        node[2 * v] = p;
+        node[2 * v + 1] = make_float4(0.0f, __int_as_float(q.i), __int_as_float(q.i),
+                                      __int_as_float(-1));
+        nodeQ[2 * v] = make_float4(0.f, 0.f, 0.f, 0.f);
+        nodeQ[2 * v + 1] = make_float4(0.f, 0.f, 0.f, 0.f);
     }
 }
 
@@ -136,18 +139,29 @@ __device__ __forceinline__ int delta(int i, int j, int n, const uint64_t* __rest
 // karras_bvh.cuh:73-131 -- one internal node per particle but the last of
 // each instance, built on instance-local indices so every instance's tree is
 // the stand-alone tree.  Writes range/split, parents and clears the arrival
-// flag of the node it builds.
+// flag of the node it builds.  Also marks traversal groups: the maximal
+// subtrees of at most kGroup particles (a child of at most kGroup under a
+// parent of more, or the root of a small instance) are flagged at their first
+// particle (gmark, gidx = 1) with their size (gcnt).
 __global__ __launch_bounds__(256) void k_build(const TileInfo* __restrict__ tiles,
                                                const SegInfo* __restrict__ segs,
                                                const uint64_t* __restrict__ skeys,
-                                               int4* __restrict__ nodeB, int* __restrict__ parent,
-                                               int* __restrict__ flag)
+                                               float4* __restrict__ node, int* __restrict__ parent,
+                                               int* __restrict__ flag, uint32_t* __restrict__ gmark,
+                                               uint32_t* __restrict__ gidx, int* __restrict__ gcnt)
 {
     const QuarterTile q = quarter_tile(tiles, segs);
     if (!q.valid) return;
     const SegInfo sg = segs[q.seg];
     const int n = sg.n, i = q.local;
-    if (i == 0) parent[sg.root] = -1;
+    if (i == 0) {
+        parent[sg.root] = -1;
+        if (n <= kGroup) {
+            gmark[sg.start] = 1u;
+            gidx[sg.start] = 1u;
+            gcnt[sg.start] = n;
+        }
+    }
     if (i >= n - 1) return;
     const uint64_t* k = skeys + sg.start;
     const int dir = (delta(i, i + 1, n, k) - delta(i, i - 1, n, k)) > 0 ? 1 : -1;
@@ -170,37 +184,58 @@ __global__ __launch_bounds__(256) void k_build(const TileInfo* __restrict__ tile
     const int gsplit = sg.start + split;
     const int cl = (lo == split) ? gsplit + sg.leaf_base : gsplit + sg.int_base;
     const int cr = (hi == split + 1) ? gsplit + 1 + sg.leaf_base : gsplit + 1 + sg.int_base;
-    nodeB[me] = make_int4(0, sg.start + lo, sg.start + hi, gsplit);
+    node[2 * me + 1] = make_float4(0.0f, __int_as_float(sg.start + lo),
+                                   __int_as_float(sg.start + hi), __int_as_float(gsplit));
     parent[cl] = me;
     parent[cr] = me;
     flag[me] = 0;
+    if (hi - lo + 1 > kGroup) {
+        const int nl = split - lo + 1, nr = hi - split;
+        if (nl <= kGroup) {
+            gmark[sg.start + lo] = 1u;
+            gidx[sg.start + lo] = 1u;
+            gcnt[sg.start + lo] = nl;
+        }
+        if (nr <= kGroup) {
+            gmark[gsplit + 1] = 1u;
+            gidx[gsplit + 1] = 1u;
+            gcnt[gsplit + 1] = nr;
+        }
+    }
 }
 
-// Adds a child's traceless quadrupole about the parent COM (same operation
-// order as the oracle): Q += Qc + mc*(3 s s^T - |s|^2 I), s = child COM -
-// parent COM; components xx, yy, zz, xy, xz, yz.
-__device__ __forceinline__ void quad_shift(float q[6], const float qc[6], float cx, float cy,
-                                           float cz, float mc, float px, float py, float pz)
+// Writes the group table in particle (Morton) order from the exclusive scan
+// of the group marks, and the number of groups.
+__global__ __launch_bounds__(256) void k_group_emit(const TileInfo* __restrict__ tiles,
+                                                    const SegInfo* __restrict__ segs,
+                                                    const uint32_t* __restrict__ gmark,
+                                                    const uint32_t* __restrict__ gidx,
+                                                    const int* __restrict__ gcnt,
+                                                    GroupInfo* __restrict__ groups,
+                                                    int* __restrict__ ngroups, int n_total)
 {
-#pragma clang fp contract(off)
-    const float sx = cx - px, sy = cy - py, sz = cz - pz;
-    const float s2 = sx * sx + sy * sy + sz * sz;
-    q[0] = q[0] + (qc[0] + mc * (3.0f * sx * sx - s2));
-    q[1] = q[1] + (qc[1] + mc * (3.0f * sy * sy - s2));
-    q[2] = q[2] + (qc[2] + mc * (3.0f * sz * sz - s2));
-    q[3] = q[3] + (qc[3] + mc * (3.0f * sx * sy));
-    q[4] = q[4] + (qc[4] + mc * (3.0f * sx * sz));
-    q[5] = q[5] + (qc[5] + mc * (3.0f * sy * sz));
+    const QuarterTile q = quarter_tile(tiles, segs);
+    if (!q.valid) return;
+    const uint32_t m = gmark[q.i];
+    if (m) {
+        GroupInfo g;
+        g.first = q.i;
+        g.count = gcnt[q.i];
+        g.seg = q.seg;
+        g.pad = 0;
+        groups[gidx[q.i]] = g;
+    }
+    if (q.i == n_total - 1) *ngroups = (int)(gidx[q.i] + m);
 }
 
 struct ChildData {
     float cx, cy, cz, m;
     float lx, ly, lz, hx, hy, hz;
-    float q[6];
+    float q0, q1, q2, q3, q4, q5;
 };
 
 __device__ __forceinline__ void load_child(bool leaf, int idx, const float4* __restrict__ posm,
-                                           const float4* nodeA, const float4* nodeQ,
+                                           const float4* node, const float4* nodeQ,
                                            const float4* boxLo, const float4* boxHi,
                                            ChildData& c)
 {
@@ -210,86 +245,128 @@ __device__ __forceinline__ void load_child(bool leaf, int idx, const float4* __r
         c.cy = c.ly = c.hy = p.y;
         c.cz = c.lz = c.hz = p.z;
         c.m = p.w;
-#pragma unroll
-        for (int k = 0; k < 6; ++k) c.q[k] = 0.0f;
+        c.q0 = c.q1 = c.q2 = c.q3 = c.q4 = c.q5 = 0.0f;
     } else {
-        const float4 a = nodeA[idx];
+        const float4 a = node[2 * idx];
         const float4 lo = boxLo[idx], hi = boxHi[idx];
-        const float4 q0 = nodeQ[2 * idx], q1 = nodeQ[2 * idx + 1];
+        const float4 qa = nodeQ[2 * idx], qb = nodeQ[2 * idx + 1];
         c.cx = a.x; c.cy = a.y; c.cz = a.z; c.m = a.w;
         c.lx = lo.x; c.ly = lo.y; c.lz = lo.z;
         c.hx = hi.x; c.hy = hi.y; c.hz = hi.z;
-        c.q[0] = q0.x; c.q[1] = q0.y; c.q[2] = q0.z; c.q[3] = q0.w;
-        c.q[4] = q1.x; c.q[5] = q1.y;
+        c.q0 = qa.x; c.q1 = qa.y; c.q2 = qa.z; c.q3 = qa.w;
+        c.q4 = qb.x; c.q5 = qb.y;
     }
 }
 
-// karras_bvh.cuh:175-225 -- bottom-up merge.  One thread per leaf climbs;
-// the first arrival at a node stops, the second merges its two children.
-// The arrival counter is an agent-scope acquire-release RMW, so the children
-// written by the other thread (possibly on another XCD, whose L2 is not
-// coherent with this one) are visible before they are read -- the ordering
-// the reference's plain atomicAdd leaves out.
-//   AABB = union; width = max extent; m = ml + mr;
-//   com = (cl*ml + cr*mr)/m, or the AABB centre when m == 0.
-// Extension: opening value = (width*inv_theta + |com - AABB centre|)^2 for
-// the group traversal (mode 1), or width for the reference traversal
-// (mode 0); quadrupole moments by the parallel-axis shift.
-__global__ __launch_bounds__(256) void k_aggregate(
-    const TileInfo* __restrict__ tiles, const SegInfo* __restrict__ segs,
-    const float4* __restrict__ posm, float4* nodeA, int4* nodeB, float4* nodeQ, float4* boxLo,
-    float4* boxHi, const int* __restrict__ parent, int* flag, float inv_theta, int mode)
+// Adds a child's traceless quadrupole about the parent COM (same operation
+// order as the oracle): Q += Qc + mc*(3 s s^T - |s|^2 I), s = child COM -
+// parent COM; components xx, yy, zz, xy, xz, yz.
+__device__ __forceinline__ void quad_shift(float q[6], const ChildData& c, float px, float py,
+                                           float pz)
 {
 #pragma clang fp contract(off)
-    const QuarterTile q = quarter_tile(tiles, segs);
-    if (!q.valid) return;
-    const SegInfo sg = segs[q.seg];
-    int cur = parent[q.i + sg.leaf_base];
+    const float sx = c.cx - px, sy = c.cy - py, sz = c.cz - pz;
+    const float s2 = sx * sx + sy * sy + sz * sz;
+    q[0] = q[0] + (c.q0 + c.m * (3.0f * sx * sx - s2));
+    q[1] = q[1] + (c.q1 + c.m * (3.0f * sy * sy - s2));
+    q[2] = q[2] + (c.q2 + c.m * (3.0f * sz * sz - s2));
+    q[3] = q[3] + (c.q3 + c.m * (3.0f * sx * sy));
+    q[4] = q[4] + (c.q4 + c.m * (3.0f * sx * sz));
+    q[5] = q[5] + (c.q5 + c.m * (3.0f * sy * sz));
+}
+
+// Merge of one internal node from its two children (karras_bvh.cuh:190-221):
+//   AABB = union; width = max extent; m = ml + mr;
+//   com = (cl*ml + cr*mr)/m, or the AABB centre when m == 0.
+// Extension: opening value = (width*inv_theta + |com - AABB centre|)^2 for the
+// group traversal (mode 1), or width for the reference traversal (mode 0);
+// quadrupole moments by the parallel-axis shift.
+__device__ __forceinline__ void merge_node(int cur, int int_base, const float4* __restrict__ posm,
+                                           float4* node, float4* nodeQ, float4* boxLo,
+                                           float4* boxHi, float inv_theta, int mode)
+{
+#pragma clang fp contract(off)
+    const float4 bb = node[2 * cur + 1];
+    const int first = __float_as_int(bb.y), last = __float_as_int(bb.z),
+              split = __float_as_int(bb.w);
+    ChildData l, r;
+    load_child(first == split, first == split ? split : split + int_base, posm, node, nodeQ,
+               boxLo, boxHi, l);
+    load_child(last == split + 1, last == split + 1 ? split + 1 : split + 1 + int_base, posm,
+               node, nodeQ, boxLo, boxHi, r);
+    const float lx = fminf(l.lx, r.lx), ly = fminf(l.ly, r.ly), lz = fminf(l.lz, r.lz);
+    const float hx = fmaxf(l.hx, r.hx), hy = fmaxf(l.hy, r.hy), hz = fmaxf(l.hz, r.hz);
+    const float w = fmaxf(hx - lx, fmaxf(hy - ly, hz - lz));
+    const float mt = l.m + r.m;
+    float cx, cy, cz;
+    if (mt > 0.0f) {
+        cx = (l.cx * l.m + r.cx * r.m) / mt;
+        cy = (l.cy * l.m + r.cy * r.m) / mt;
+        cz = (l.cz * l.m + r.cz * r.m) / mt;
+    } else {
+        cx = (lx + hx) * 0.5f;
+        cy = (ly + hy) * 0.5f;
+        cz = (lz + hz) * 0.5f;
+    }
+    float open;
+    if (mode == 1) {
+        const float dx = cx - (lx + hx) * 0.5f;
+        const float dy = cy - (ly + hy) * 0.5f;
+        const float dz = cz - (lz + hz) * 0.5f;
+        const float off = sqrtf(dx * dx + dy * dy + dz * dz);
+        const float ro = w * inv_theta + off;
+        open = ro * ro;
+    } else {
+        open = w;
+    }
+    float qq[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    quad_shift(qq, l, cx, cy, cz);
+    quad_shift(qq, r, cx, cy, cz);
+    node[2 * cur] = make_float4(cx, cy, cz, mt);
+    node[2 * cur + 1].x = open;
+    nodeQ[2 * cur] = make_float4(qq[0], qq[1], qq[2], qq[3]);
+    nodeQ[2 * cur + 1] = make_float4(qq[4], qq[5], w, 0.0f);
+    boxLo[cur] = make_float4(lx, ly, lz, 0.0f);
+    boxHi[cur] = make_float4(hx, hy, hz, 0.0f);
+}
+
+// karras_bvh.cuh:175-225 -- bottom-up aggregation, one block per tile and one
+// thread per leaf.  A thread climbs from its leaf; at each node the first of
+// the two arrivals stops and the second merges the node.  Nodes whose
+// particle range lies inside the block's tile are only ever reached by this
+// block, so their arrival counter is an LDS word with workgroup-scope
+// acquire-release; only nodes whose range crosses the tile use the global
+// counter with agent-scope acquire-release, which makes the children written
+// on another XCD (whose L2 is not coherent with this one) visible -- the
+// ordering the reference's plain atomicAdd leaves out.
+__global__ __launch_bounds__(kTile) void k_aggregate(
+    const TileInfo* __restrict__ tiles, const SegInfo* __restrict__ segs,
+    const float4* __restrict__ posm, float4* node, float4* nodeQ, float4* boxLo, float4* boxHi,
+    const int* __restrict__ parent, int* flag, float inv_theta, int mode)
+{
+    __shared__ int lflag[kTile];
+    const TileInfo t = tiles[blockIdx.x];
+    lflag[threadIdx.x] = 0;
+    __syncthreads();
+    if ((int)threadIdx.x >= t.count) return;
+    const SegInfo sg = segs[t.seg];
+    const int t0 = t.start, t1 = t.start + t.count - 1;
+    int cur = parent[t.start + (int)threadIdx.x + sg.leaf_base];
     while (cur != -1) {
-        const int old = __hip_atomic_fetch_add(&flag[cur], 1, __ATOMIC_ACQ_REL,
-                                               __HIP_MEMORY_SCOPE_AGENT);
+        const float4 bb = node[2 * cur + 1];
+        const int first = __float_as_int(bb.y), last = __float_as_int(bb.z);
+        int old;
+        if (first >= t0 && last <= t1) {
+            // An internal node is numbered by one end of its range.
+            const int slot = cur - sg.int_base - t0;
+            old = __hip_atomic_fetch_add(&lflag[slot], 1, __ATOMIC_ACQ_REL,
+                                         __HIP_MEMORY_SCOPE_WORKGROUP);
+        } else {
+            old = __hip_atomic_fetch_add(&flag[cur], 1, __ATOMIC_ACQ_REL,
+                                         __HIP_MEMORY_SCOPE_AGENT);
+        }
         if (old == 0) return;
-        const int4 b = nodeB[cur];
-        const int first = b.y, last = b.z, split = b.w;
-        ChildData l, r;
-        load_child(first == split, first == split ? split : split + sg.int_base, posm, nodeA,
-                   nodeQ, boxLo, boxHi, l);
-        load_child(last == split + 1, last == split + 1 ? split + 1 : split + 1 + sg.int_base,
-                   posm, nodeA, nodeQ, boxLo, boxHi, r);
-        const float lx = fminf(l.lx, r.lx), ly = fminf(l.ly, r.ly), lz = fminf(l.lz, r.lz);
-        const float hx = fmaxf(l.hx, r.hx), hy = fmaxf(l.hy, r.hy), hz = fmaxf(l.hz, r.hz);
-        const float w = fmaxf(hx - lx, fmaxf(hy - ly, hz - lz));
-        const float mt = l.m + r.m;
-        float cx, cy, cz;
-        if (mt > 0.0f) {
-            cx = (l.cx * l.m + r.cx * r.m) / mt;
-            cy = (l.cy * l.m + r.cy * r.m) / mt;
-            cz = (l.cz * l.m + r.cz * r.m) / mt;
-        } else {
-            cx = (lx + hx) * 0.5f;
-            cy = (ly + hy) * 0.5f;
-            cz = (lz + hz) * 0.5f;
-        }
-        float open;
-        if (mode == 1) {
-            const float dx = cx - (lx + hx) * 0.5f;
-            const float dy = cy - (ly + hy) * 0.5f;
-            const float dz = cz - (lz + hz) * 0.5f;
-            const float off = sqrtf(dx * dx + dy * dy + dz * dz);
-            const float ro = w * inv_theta + off;
-            open = ro * ro;
-        } else {
-            open = w;
-        }
-        float qq[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-        quad_shift(qq, l.q, l.cx, l.cy, l.cz, l.m, cx, cy, cz);
-        quad_shift(qq, r.q, r.cx, r.cy, r.cz, r.m, cx, cy, cz);
-        nodeA[cur] = make_float4(cx, cy, cz, mt);
-        nodeB[cur].x = __float_as_int(open);
-        nodeQ[2 * cur] = make_float4(qq[0], qq[1], qq[2], qq[3]);
-        nodeQ[2 * cur + 1] = make_float4(qq[4], qq[5], w, 0.0f);
-        boxLo[cur] = make_float4(lx, ly, lz, 0.0f);
-        boxHi[cur] = make_float4(hx, hy, hz, 0.0f);
+        merge_node(cur, sg.int_base, posm, node, nodeQ, boxLo, boxHi, inv_theta, mode);
         cur = parent[cur];
     }
 }

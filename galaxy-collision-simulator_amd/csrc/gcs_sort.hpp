@@ -75,26 +75,31 @@ __global__ __launch_bounds__(256) void k_scan_reduce(const uint32_t* __restrict_
     if (threadIdx.x == 0) partial[blockIdx.x] = ws[0] + ws[1] + ws[2] + ws[3];
 }
 
-// Exclusive scan of the chunk sums in one block (up to 256*16 chunks).
+// Exclusive scan of the chunk sums in one block, 4096 at a time with a
+// running carry (any number of chunks).
 __global__ __launch_bounds__(256) void k_scan_partials(uint32_t* partial, int nb)
 {
     __shared__ uint32_t ws[4];
     constexpr int kPer = 16;
-    uint32_t v[kPer];
-    uint32_t s = 0;
+    uint32_t carry = 0;
+    for (int base = 0; base < nb; base += 256 * kPer) {
+        uint32_t v[kPer];
+        uint32_t s = 0;
 #pragma unroll
-    for (int k = 0; k < kPer; ++k) {
-        const int i = threadIdx.x * kPer + k;
-        v[k] = i < nb ? partial[i] : 0u;
-        s += v[k];
-    }
-    uint32_t tot;
-    uint32_t run = block_exclusive_scan(s, &tot, ws);
+        for (int k = 0; k < kPer; ++k) {
+            const int i = base + (int)threadIdx.x * kPer + k;
+            v[k] = i < nb ? partial[i] : 0u;
+            s += v[k];
+        }
+        uint32_t tot;
+        uint32_t run = block_exclusive_scan(s, &tot, ws) + carry;
 #pragma unroll
-    for (int k = 0; k < kPer; ++k) {
-        const int i = threadIdx.x * kPer + k;
-        if (i < nb) partial[i] = run;
-        run += v[k];
+        for (int k = 0; k < kPer; ++k) {
+            const int i = base + (int)threadIdx.x * kPer + k;
+            if (i < nb) partial[i] = run;
+            run += v[k];
+        }
+        carry += tot;
     }
 }
 

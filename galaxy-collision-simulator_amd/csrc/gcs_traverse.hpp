@@ -1,22 +1,34 @@
 // gcs_traverse.hpp -- Barnes-Hut traversals.
 //
-// k_traverse_group: one wave per group of up to 64 consecutive (Morton-sorted)
-// particles of one instance.  The wave walks the instance's tree once for the
-// whole group:
-//   pop up to 64 nodes (one per lane) from the wave's stack;
-//   node accepted  when the squared distance from its COM to the group's
-//                  bounding box exceeds its opening value (w/theta + offset)^2:
-//                  appended to the LDS multipole list;
-//   node opened    otherwise: each child with <= leaf particles is appended to
-//                  the LDS range list (summed directly), larger children are
-//                  pushed;
-//   lists are evaluated when 64 entries are pending: every lane accumulates
-//   its own particle against each entry, read from LDS as a broadcast.
-// Persistent grid: waves take groups from eight queues, one per XCD, each
-// holding a contiguous run of groups (so neighbouring groups, which touch the
-// same nodes, share an L2), and steal from the other queues when theirs is
-// empty.  Self-interaction contributes zero force (zero separation) and its
-// potential term is removed at the end, which is index self-exclusion.
+// k_traverse_group: one wave per group -- a maximal subtree of at most 64
+// particles (consecutive in Morton order, so spatially compact).  For each
+// group the wave
+//   phase 1  walks the instance's tree once for the whole group, 64 nodes per
+//            round from an LDS stack (spilling its oldest half to global
+//            memory when full).  A popped node is
+//              accepted  when the squared distance from its COM to the group's
+//                        bounding box exceeds its opening value
+//                        (width/theta + |COM - box centre|)^2: its index goes
+//                        to the node list;
+//              summed    otherwise, when it holds <= leaf particles: its
+//                        particles go to the particle list;
+//              opened    otherwise: single-particle children go to the
+//                        particle list, larger children are pushed.
+//            Both lists live in a per-wave slot of global memory (L2-resident;
+//            a full list is evaluated early, so no list overflows);
+//   phase 2  streams each list in chunks of 64 entries: every lane gathers one
+//            source (node COM, mass and quadrupole, or particle position and
+//            mass) into LDS while the next chunk's gathers are in flight, and
+//            every lane then accumulates its own particle against all 64
+//            entries read back as broadcasts.
+// The interaction set is a function of the tree, the group and the
+// parameters only, and every sum runs in list order, so results do not depend
+// on scheduling.  Persistent grid: waves take groups from eight queues, one
+// per XCD, each holding a contiguous run of groups (neighbouring groups touch
+// the same nodes, which then stay in that XCD's L2), and steal from the other
+// queues when theirs is empty.  The particle itself is in its own particle
+// list: zero separation gives zero force, and its potential term is removed at
+// the end, which is index self-exclusion.
 //
 // k_traverse_reference: the reference's per-particle traversal
 // (cuda_engine.cu:91-161) with the same decisions, for the cuda_engine.h entry
@@ -26,16 +38,17 @@
 
 namespace gcs {
 
-constexpr int kNodeBuf = 128;
-constexpr int kRangeBuf = 192;
+constexpr int kLdsStack = 1024;    // LDS stack entries per wave
+constexpr int kNodeList = 4096;    // node-list entries per wave slot
+constexpr int kPartList = 16384;   // particle-list entries per wave slot
+constexpr int kMaxLeaf = 64;       // largest direct-sum bucket
+constexpr int kSlotInts = kStackCap + kNodeList + kPartList;
 
 struct WaveLds {
-    float4 na[kNodeBuf];
-    float4 nq0[kNodeBuf];
-    float2 nq1[kNodeBuf];
-    int2 rng[kRangeBuf];
-    int rpre[kRangeBuf];
-    float4 src[kWave];
+    int stk[kLdsStack];
+    float4 ca[kWave];
+    float4 cq0[kWave];
+    float2 cq1[kWave];
 };
 
 struct Acc {
@@ -91,73 +104,89 @@ __device__ __forceinline__ void interact_n(Acc& a, float px, float py, float pz,
     if (POT) a.ph += __builtin_fmaf(0.5f, t, minv);
 }
 
+// Phase 2 over the node list nl[0..k): gathers of chunk c+1 are issued before
+// chunk c is evaluated, and the indices one chunk further ahead.
 template <bool POT, bool QUAD>
-__device__ __forceinline__ void eval_nodes(WaveLds& L, int k, Acc& a, float px, float py, float pz,
-                                           float eps2)
+__device__ __forceinline__ void eval_node_list(WaveLds& L, const int* nl, int k,
+                                               const float4* __restrict__ node,
+                                               const float4* __restrict__ nodeQ, Acc& a,
+                                               float px, float py, float pz, float eps2)
 {
-    int j = 0;
-    for (; j + 1 < k; j += 2) {
-        const float4 c0 = L.na[j], c1 = L.na[j + 1];
-        float4 q00 = make_float4(0, 0, 0, 0), q01 = q00;
-        float2 q10 = make_float2(0, 0), q11 = q10;
-        if (QUAD) {
-            q00 = L.nq0[j]; q01 = L.nq0[j + 1];
-            q10 = L.nq1[j]; q11 = L.nq1[j + 1];
-        }
-        interact_n<POT, QUAD>(a, px, py, pz, c0, q00, q10, eps2);
-        interact_n<POT, QUAD>(a, px, py, pz, c1, q01, q11, eps2);
+    const int l = lane_id();
+    float4 A = make_float4(0.f, 0.f, 0.f, 0.f), Q0 = A, Q1 = A;
+    const int idx = l < k ? nl[l] : -1;
+    int idx_next = l + kWave < k ? nl[l + kWave] : -1;
+    if (idx >= 0) {
+        A = node[2 * idx];
+        if (QUAD) { Q0 = nodeQ[2 * idx]; Q1 = nodeQ[2 * idx + 1]; }
     }
-    if (j < k) {
-        float4 q0 = make_float4(0, 0, 0, 0);
-        float2 q1 = make_float2(0, 0);
-        if (QUAD) { q0 = L.nq0[j]; q1 = L.nq1[j]; }
-        interact_n<POT, QUAD>(a, px, py, pz, L.na[j], q0, q1, eps2);
+    for (int base = 0; base < k; base += kWave) {
+        L.ca[l] = A;
+        if (QUAD) {
+            L.cq0[l] = Q0;
+            L.cq1[l] = make_float2(Q1.x, Q1.y);
+        }
+        __builtin_amdgcn_wave_barrier();
+        A = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (idx_next >= 0) {
+            A = node[2 * idx_next];
+            if (QUAD) { Q0 = nodeQ[2 * idx_next]; Q1 = nodeQ[2 * idx_next + 1]; }
+        }
+        const int j2 = base + 2 * kWave + l;
+        idx_next = j2 < k ? nl[j2] : -1;
+        const int m = uniform(min(kWave, k - base));
+        int t = 0;
+        for (; t + 1 < m; t += 2) {
+            const float4 c0 = L.ca[t], c1 = L.ca[t + 1];
+            float4 q00 = make_float4(0, 0, 0, 0), q01 = q00;
+            float2 q10 = make_float2(0, 0), q11 = q10;
+            if (QUAD) {
+                q00 = L.cq0[t]; q01 = L.cq0[t + 1];
+                q10 = L.cq1[t]; q11 = L.cq1[t + 1];
+            }
+            interact_n<POT, QUAD>(a, px, py, pz, c0, q00, q10, eps2);
+            interact_n<POT, QUAD>(a, px, py, pz, c1, q01, q11, eps2);
+        }
+        if (t < m) {
+            float4 q0 = make_float4(0, 0, 0, 0);
+            float2 q1 = make_float2(0, 0);
+            if (QUAD) { q0 = L.cq0[t]; q1 = L.cq1[t]; }
+            interact_n<POT, QUAD>(a, px, py, pz, L.ca[t], q0, q1, eps2);
+        }
+        __builtin_amdgcn_wave_barrier();
     }
 }
 
-// Sums the pending ranges; returns the number of source particles.
+// Phase 2 over the particle list pl[0..k), same pipeline.
 template <bool POT>
-__device__ __forceinline__ int eval_ranges(WaveLds& L, int k, Acc& a, float px, float py,
-                                           float pz, float eps2,
-                                           const float4* __restrict__ posm)
+__device__ __forceinline__ void eval_part_list(WaveLds& L, const int* pl, int k,
+                                               const float4* __restrict__ posm, Acc& a,
+                                               float px, float py, float pz, float eps2)
 {
     const int l = lane_id();
-    int c0 = l < k ? L.rng[l].y : 0;
-    int c1 = l + 64 < k ? L.rng[l + 64].y : 0;
-    int c2 = l + 128 < k ? L.rng[l + 128].y : 0;
-    const int i0 = wave_scan_incl(c0), i1 = wave_scan_incl(c1), i2 = wave_scan_incl(c2);
-    const int t0 = __shfl(i0, 63), t1 = __shfl(i1, 63), t2 = __shfl(i2, 63);
-    if (l < k) L.rpre[l] = i0 - c0;
-    if (l + 64 < k) L.rpre[l + 64] = t0 + i1 - c1;
-    if (l + 128 < k) L.rpre[l + 128] = t0 + t1 + i2 - c2;
-    const int total = t0 + t1 + t2;
-    __builtin_amdgcn_wave_barrier();
-    for (int base = 0; base < total; base += kWave) {
-        const int f = base + l;
-        float4 s = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (f < total) {
-            int r = 0;
-#pragma unroll
-            for (int step = 128; step > 0; step >>= 1) {
-                const int c = r + step;
-                if (c < k && L.rpre[c] <= f) r = c;
-            }
-            s = posm[L.rng[r].x + (f - L.rpre[r])];
-        }
+    float4 S = make_float4(0.f, 0.f, 0.f, 0.f);
+    const int idx = l < k ? pl[l] : -1;
+    int idx_next = l + kWave < k ? pl[l + kWave] : -1;
+    if (idx >= 0) S = posm[idx];
+    for (int base = 0; base < k; base += kWave) {
+        L.ca[l] = S;
         __builtin_amdgcn_wave_barrier();
-        L.src[l] = s;
-        __builtin_amdgcn_wave_barrier();
-        const int m = min(kWave, total - base);
-        int j = 0;
-        for (; j + 1 < m; j += 2) {
-            const float4 s0 = L.src[j], s1 = L.src[j + 1];
+        S = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (idx_next >= 0) S = posm[idx_next];
+        const int j2 = base + 2 * kWave + l;
+        idx_next = j2 < k ? pl[j2] : -1;
+        const int m = uniform(min(kWave, k - base));
+        int t = 0;
+        for (; t + 3 < m; t += 4) {
+            const float4 s0 = L.ca[t], s1 = L.ca[t + 1], s2 = L.ca[t + 2], s3 = L.ca[t + 3];
             interact_p<POT>(a, px, py, pz, s0, eps2);
             interact_p<POT>(a, px, py, pz, s1, eps2);
+            interact_p<POT>(a, px, py, pz, s2, eps2);
+            interact_p<POT>(a, px, py, pz, s3, eps2);
         }
-        if (j < m) interact_p<POT>(a, px, py, pz, L.src[j], eps2);
+        for (; t < m; ++t) interact_p<POT>(a, px, py, pz, L.ca[t], eps2);
         __builtin_amdgcn_wave_barrier();
     }
-    return total;
 }
 
 __device__ __forceinline__ float box_dist2(float lx, float ly, float lz, float hx, float hy,
@@ -172,22 +201,27 @@ __device__ __forceinline__ float box_dist2(float lx, float ly, float lz, float h
 
 template <bool POT, bool QUAD, int WPB>
 __global__ __launch_bounds__(WPB * 64) void k_traverse_group(
-    const GroupInfo* __restrict__ groups, int ngroups, const SegInfo* __restrict__ segs,
-    const float4* __restrict__ posm, const float4* __restrict__ nodeA,
-    const int4* __restrict__ nodeB, const float4* __restrict__ nodeQ, float4* __restrict__ acc,
-    int* __restrict__ stack_mem, unsigned* qctr, unsigned long long* counters, unsigned* err,
-    float G, float eps2, int leaf)
+    const GroupInfo* __restrict__ groups, const int* __restrict__ ngroups_dev,
+    const SegInfo* __restrict__ segs, const float4* __restrict__ posm,
+    const float4* __restrict__ node,
+    const float4* __restrict__ nodeQ, float4* __restrict__ acc, int* __restrict__ slot_mem,
+    unsigned* qctr, unsigned long long* counters, unsigned* err, float G, float eps2, int leaf)
 {
     __shared__ WaveLds lds[WPB];
     const int l = lane_id();
     const int w = threadIdx.x >> 6;
     WaveLds& L = lds[w];
-    int* stk = stack_mem + (size_t)(blockIdx.x * WPB + w) * kStackCap;
+    int* const spill = slot_mem + (size_t)(blockIdx.x * WPB + w) * kSlotInts;
+    int* const nl = spill + kStackCap;
+    int* const pl = nl + kNodeList;
     const int xcd = blockIdx.x % kNumXcd;
+    const int ngroups = uniform(*ngroups_dev);
     const int per = (ngroups + kNumXcd - 1) / kNumXcd;
     unsigned long long n_node = 0, n_part = 0;
     int q = 0;
     const float self_inv = __builtin_amdgcn_rsqf(eps2);
+    const uint64_t lt = (1ull << l) - 1ull;
+    bool overflow = false;
 
     for (;;) {
         int g = -1;
@@ -212,84 +246,110 @@ __global__ __launch_bounds__(WPB * 64) void k_traverse_group(
         const float lx = wave_min(pp.x), ly = wave_min(pp.y), lz = wave_min(pp.z);
         const float hx = wave_max(pp.x), hy = wave_max(pp.y), hz = wave_max(pp.z);
         Acc a = {0.f, 0.f, 0.f, 0.f};
-        int nb = 0, nr = 0, top = 0, nnodes = 0, nparts = 0;
-        if (sg.n <= leaf) {
-            if (l == 0) L.rng[0] = make_int2(sg.start, sg.n);
-            nr = 1;
+        int nn = 0, np = 0, top = 0, spilled = 0;
+        long long tot_nodes = 0, tot_parts = 0;
+        if (sg.n == 1) {
+            if (l == 0) pl[0] = sg.start;
+            np = 1;
         } else {
-            if (l == 0) stk[0] = sg.root;
+            if (l == 0) L.stk[0] = sg.root;
             top = 1;
         }
         __builtin_amdgcn_wave_barrier();
-        bool overflow = false;
-        while (top > 0) {
+        while (top > 0 || spilled > 0) {
+            if (top == 0) {
+                // Reload the most recently spilled half.
+                const int c = min(spilled, kLdsStack / 2);
+                for (int k = l; k < c; k += kWave) L.stk[k] = spill[spilled - c + k];
+                spilled = uniform(spilled - c);
+                top = uniform(c);
+                __builtin_amdgcn_wave_barrier();
+            }
             const int cnt = min(top, kWave);
             top -= cnt;
             const bool have = l < cnt;
             int v = 0;
-            if (have) v = stk[top + l];
-            float4 A = make_float4(0, 0, 0, 0);
+            if (have) v = L.stk[top + l];
+            float4 A = make_float4(0.f, 0.f, 0.f, 0.f);
             int4 B = make_int4(0, 0, 0, 0);
-            bool accept = false;
+            bool accept = false, self_direct = false, open = false;
             if (have) {
-                A = nodeA[v];
-                B = nodeB[v];
-                accept = box_dist2(lx, ly, lz, hx, hy, hz, A.x, A.y, A.z) > __int_as_float(B.x);
+                A = node[2 * v];
+                const float4 bb = node[2 * v + 1];
+                B = make_int4(__float_as_int(bb.x), __float_as_int(bb.y), __float_as_int(bb.z),
+                              __float_as_int(bb.w));
+                accept = box_dist2(lx, ly, lz, hx, hy, hz, A.x, A.y, A.z) > bb.x;
+                self_direct = !accept && (B.z - B.y + 1) <= leaf;
+                open = !accept && !self_direct;
             }
-            const bool open = have && !accept;
+            // accepted nodes -> node list
             const uint64_t mA = ballot(accept);
-            if (accept) {
-                const int s = nb + mbcnt(mA);
-                L.na[s] = A;
-                if (QUAD) {
-                    const float4 q0 = nodeQ[2 * v], q1 = nodeQ[2 * v + 1];
-                    L.nq0[s] = q0;
-                    L.nq1[s] = make_float2(q1.x, q1.y);
-                }
+            const int na = __popcll(mA);
+            if (nn + na > kNodeList) {
+                eval_node_list<POT, QUAD>(L, nl, nn, node, nodeQ, a, pp.x, pp.y, pp.z, eps2);
+                tot_nodes += nn;
+                nn = 0;
             }
-            nb += __popcll(mA);
-            const int lcount = B.w - B.y + 1, rcount = B.z - B.w;
-            const bool lr = open && lcount <= leaf, rr = open && rcount <= leaf;
-            const bool lp = open && !lr, rp = open && !rr;
-            const uint64_t m1 = ballot(lr), m2 = ballot(rr);
-            if (lr) L.rng[nr + mbcnt(m1)] = make_int2(B.y, lcount);
-            nr += __popcll(m1);
-            if (rr) L.rng[nr + mbcnt(m2)] = make_int2(B.w + 1, rcount);
-            nr += __popcll(m2);
+            if (accept) nl[nn + __popcll(mA & lt)] = v;
+            nn = uniform(nn + na);
+            // particles: the node's own (summed) or its single-particle children
+            const bool lleaf = open && B.y == B.w, rleaf = open && B.z == B.w + 1;
+            const int c_self = self_direct ? B.z - B.y + 1 : 0;
+            const int cnt_p = c_self + (lleaf ? 1 : 0) + (rleaf ? 1 : 0);
+            const int incl = wave_scan_incl(cnt_p);
+            const int tp = uniform(__shfl(incl, 63));
+            if (np + tp > kPartList) {
+                eval_part_list<POT>(L, pl, np, posm, a, pp.x, pp.y, pp.z, eps2);
+                tot_parts += np;
+                np = 0;
+            }
+            int o = np + incl - cnt_p;
+            for (int k = 0; k < c_self; ++k) pl[o + k] = B.y + k;
+            o += c_self;
+            if (lleaf) pl[o++] = B.w;
+            if (rleaf) pl[o] = B.w + 1;
+            np = uniform(np + tp);
+            // larger children -> stack
+            const bool lp = open && !lleaf, rp = open && !rleaf;
             const uint64_t m3 = ballot(lp), m4 = ballot(rp);
-            const int np = __popcll(m3) + __popcll(m4);
-            if (top + np > kStackCap) {
-                overflow = true;
-                break;
+            const int npush = __popcll(m3) + __popcll(m4);
+            if (top + npush > kLdsStack) {
+                // Spill the oldest half of the LDS stack to global memory.
+                const int h = kLdsStack / 2;
+                if (spilled + h > kStackCap) {
+                    overflow = true;
+                    break;
+                }
+                for (int k = l; k < h; k += kWave) spill[spilled + k] = L.stk[k];
+                __builtin_amdgcn_wave_barrier();
+                for (int k = l; k < top - h; k += kWave) {
+                    const int t = L.stk[k + h];
+                    __builtin_amdgcn_wave_barrier();
+                    L.stk[k] = t;
+                }
+                spilled = uniform(spilled + h);
+                top -= h;
             }
-            if (lp) stk[top + mbcnt(m3)] = B.w + sg.int_base;
+            if (lp) L.stk[top + __popcll(m3 & lt)] = B.w + sg.int_base;
             top += __popcll(m3);
-            if (rp) stk[top + mbcnt(m4)] = B.w + 1 + sg.int_base;
-            top += __popcll(m4);
-            __builtin_amdgcn_wave_barrier();
-            if (nb >= kWave) {
-                eval_nodes<POT, QUAD>(L, nb, a, pp.x, pp.y, pp.z, eps2);
-                nnodes += nb;
-                nb = 0;
-            }
-            if (nr >= kWave) {
-                nparts += eval_ranges<POT>(L, nr, a, pp.x, pp.y, pp.z, eps2, posm);
-                nr = 0;
-            }
+            if (rp) L.stk[top + __popcll(m4 & lt)] = B.w + 1 + sg.int_base;
+            top = uniform(top + __popcll(m4));
             __builtin_amdgcn_wave_barrier();
         }
-        if (overflow && l == 0) atomicOr(err, kErrStack);
-        if (nb > 0) {
-            eval_nodes<POT, QUAD>(L, nb, a, pp.x, pp.y, pp.z, eps2);
-            nnodes += nb;
+        if (overflow) {
+            if (l == 0) atomicOr(err, kErrStack);
+            break;
         }
-        if (nr > 0) nparts += eval_ranges<POT>(L, nr, a, pp.x, pp.y, pp.z, eps2, posm);
+        if (nn > 0) eval_node_list<POT, QUAD>(L, nl, nn, node, nodeQ, a, pp.x, pp.y, pp.z, eps2);
+        if (np > 0) eval_part_list<POT>(L, pl, np, posm, a, pp.x, pp.y, pp.z, eps2);
+        tot_nodes += nn;
+        tot_parts += np;
         if (l < gi.count) {
             const float ph = POT ? -G * (a.ph - pp.w * self_inv) : 0.0f;
             acc[p] = make_float4(G * a.ax, G * a.ay, G * a.az, ph);
         }
-        n_node += (unsigned long long)nnodes * (unsigned long long)gi.count;
-        n_part += (unsigned long long)(nparts - 1) * (unsigned long long)gi.count;
+        n_node += (unsigned long long)tot_nodes * (unsigned long long)gi.count;
+        n_part += (unsigned long long)(tot_parts - 1) * (unsigned long long)gi.count;
         __builtin_amdgcn_wave_barrier();
     }
     if (l == 0) {
@@ -302,15 +362,14 @@ __global__ __launch_bounds__(WPB * 64) void k_traverse_group(
 // pushed before right; leaf taken when d^2+eps^2 > 1.01*eps^2; internal node
 // taken when width^2 < d^2+eps^2; force G*m*rsqrt(d2)^3 * d.  `limit` (at most
 // STACK) is the stack depth; a traversal that would exceed it stops and sets
-// the stack error bit.  Decisions are
-// computed exactly as written (no contraction); the accumulation uses the
-// multiply-adds nvcc forms.
+// the stack error bit.  Decisions are computed exactly as written (no
+// contraction); the accumulation uses the multiply-adds nvcc forms.
 template <int STACK>
 __global__ __launch_bounds__(256) void k_traverse_reference(
     const TileInfo* __restrict__ tiles, const SegInfo* __restrict__ segs,
-    const float4* __restrict__ posm, const float4* __restrict__ nodeA,
-    const int4* __restrict__ nodeB, float4* __restrict__ acc, unsigned long long* counters,
-    unsigned* err, float G, float eps2, int limit)
+    const float4* __restrict__ posm, const float4* __restrict__ node,
+    float4* __restrict__ acc, unsigned long long* counters, unsigned* err, float G, float eps2,
+    int limit)
 {
 #pragma clang fp contract(off)
     const TileInfo t = tiles[blockIdx.x >> 2];
@@ -327,25 +386,26 @@ __global__ __launch_bounds__(256) void k_traverse_reference(
         bool overflow = false;
         while (top > 0) {
             const int v = stack[--top];
-            const float4 A = nodeA[v];
-            const int4 B = nodeB[v];
+            const float4 A = node[2 * v];
+            const float4 bb = node[2 * v + 1];
+            const int first = __float_as_int(bb.y), last = __float_as_int(bb.z),
+                      split = __float_as_int(bb.w);
             const float dx = A.x - p.x, dy = A.y - p.y, dz = A.z - p.z;
             const float d2 = dx * dx + dy * dy + dz * dz + eps2;
             bool take;
-            if (B.w < 0) {
+            if (split < 0) {
                 take = d2 > eps2 * 1.01f;
             } else {
-                const float width = __int_as_float(B.x);
+                const float width = bb.x;
                 take = width * width < d2 * 1.0f;
                 if (!take) {
                     if (top + 2 > limit) {
                         overflow = true;
                         break;
                     }
-                    const int split = B.w;
-                    stack[top++] = (B.y == split) ? split + sg.leaf_base : split + sg.int_base;
-                    stack[top++] = (B.z == split + 1) ? split + 1 + sg.leaf_base
-                                                      : split + 1 + sg.int_base;
+                    stack[top++] = (first == split) ? split + sg.leaf_base : split + sg.int_base;
+                    stack[top++] = (last == split + 1) ? split + 1 + sg.leaf_base
+                                                       : split + 1 + sg.int_base;
                 }
             }
             if (take) {

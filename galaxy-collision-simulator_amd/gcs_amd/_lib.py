@@ -91,6 +91,7 @@ SIGNATURES = {
     "gcs_read_interactions": ([_P, _P], _I32),
     "gcs_read_diagnostics": ([_P, _I32, _P], _I32),
     "gcs_read_stage_ms": ([_P, _P], _I32),
+    "gcs_engine_info": ([_P, _P], _I32),
     "gcs_morton_keys": ([_P, _P, _P, _I64, _F, _F, _P], _I32),
     "gcs_sort_pairs": ([_P, _I64, _P, _P], _I32),
     "gcs_karras_tree": ([_P, _I32, _P, _P, _P], _I32),

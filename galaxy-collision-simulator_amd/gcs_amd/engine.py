@@ -178,6 +178,13 @@ class Engine:
         L.check(L.lib().gcs_read_diagnostics(self._h, int(iid), L.ptr(out)))
         return out
 
+    def engine_info(self):
+        out = np.zeros(8, np.int64)
+        L.check(L.lib().gcs_engine_info(self._h, L.ptr(out)))
+        keys = ["num_cu", "trav_blocks", "trav_waves", "trav_slot_bytes", "capacity",
+                "instance_capacity", "groups", "tiles"]
+        return dict(zip(keys, (int(x) for x in out)))
+
     def stage_ms(self):
         out = np.zeros(7, np.float32)
         L.check(L.lib().gcs_read_stage_ms(self._h, L.ptr(out)))

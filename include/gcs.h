@@ -62,11 +62,12 @@ enum {
  *                    ignored: the reference's rule is theta = 1 on the
  *                    softened distance), leaves skipped when within 0.1*eps.
  * GCS_MAC_GROUP      64 consecutive sorted particles share one traversal; a
- *                    node is a monopole for the group when the distance from
- *                    its COM to the group's bounding box exceeds
- *                    width/theta + |COM - box centre|; children holding at
- *                    most leaf_size particles are summed directly; index
- *                    self-exclusion.  Needs eps2 > 0. */
+ *                    node is taken as a multipole (mass and quadrupole) for
+ *                    the group when the distance from its COM to the group's
+ *                    bounding box exceeds width/theta + |COM - box centre|;
+ *                    otherwise a node of at most leaf_size particles is
+ *                    summed directly and a larger one is opened; index
+ *                    self-exclusion.  Needs eps2 > 0 and theta <= 1. */
 enum { GCS_MAC_REFERENCE = 0, GCS_MAC_GROUP = 1 };
 
 /* Position precision: plain FP32, or a compensated FP32 hi/lo pair whose
@@ -171,6 +172,10 @@ int gcs_read_interactions(gcs_engine* e, int64_t* out2);
  * diagnostics=1: kinetic energy, potential energy, momentum x, y, z, all at
  * the force time (velocities re-synchronised by half a kick). */
 int gcs_read_diagnostics(gcs_engine* e, int32_t id, double* out5);
+/* Engine facts: [0] compute units, [1] persistent traversal blocks, [2]
+ * traversal wave slots, [3] bytes of per-wave traversal memory, [4] particle
+ * capacity, [5] instance capacity, [6] traversal groups, [7] sort tiles. */
+int gcs_engine_info(gcs_engine* e, int64_t* out8);
 /* Milliseconds of the last step's stages on the device (needs
  * use_graph = 0): [0] bounds+morton, [1] sort, [2] permute, [3] build,
  * [4] aggregate, [5] traverse, [6] integrate. */

@@ -447,16 +447,17 @@ void orc_traverse_group(int32_t n, const float* x, const float* y,
                         const int32_t* last, const float* com4,
                         const float* crit, const float* quad, int32_t group,
                         int32_t leaf, float G, float eps2, double* acc,
-                        int64_t* counts, double* msum)
+                        int64_t* counts, double* msum, double* aabs,
+                        int64_t* gcost, int32_t ngl, const int32_t* glist)
 {
     if (n <= 0) return;
-    int32_t ngroups = (n + group - 1) / group;
+    int32_t ngroups = glist ? ngl : (n + group - 1) / group;
     int64_t cnode = 0, cpart = 0;
     const double Gd = G, e2 = eps2;
 #pragma omp parallel for schedule(dynamic, 1) reduction(+ : cnode, cpart)
     for (int32_t g = 0; g < ngroups; ++g) {
-        int32_t g0 = g * group;
-        int32_t g1 = g0 + group < n ? g0 + group : n;
+        int32_t g0 = glist ? glist[2 * g] : g * group;
+        int32_t g1 = glist ? g0 + glist[2 * g + 1] : (g0 + group < n ? g0 + group : n);
         float lo[3] = {x[g0], y[g0], z[g0]}, hi[3] = {x[g0], y[g0], z[g0]};
         for (int32_t i = g0; i < g1; ++i) {
             lo[0] = fminf(lo[0], x[i]); hi[0] = fmaxf(hi[0], x[i]);
@@ -464,12 +465,14 @@ void orc_traverse_group(int32_t n, const float* x, const float* y,
             lo[2] = fminf(lo[2], z[i]); hi[2] = fmaxf(hi[2], z[i]);
         }
         int32_t cap = 4096;
+        int64_t gc_nodes = 0, gc_parts = 0;
         int32_t* stack = (int32_t*)malloc(sizeof(int32_t) * (size_t)cap);
         int top = 0;
         stack[top++] = 0;
         for (int32_t i = g0; i < g1; ++i) {
             acc[4 * i + 0] = acc[4 * i + 1] = acc[4 * i + 2] = acc[4 * i + 3] = 0.0;
             if (msum) msum[i] = 0.0;
+            if (aabs) aabs[i] = 0.0;
         }
         while (top > 0) {
             int32_t v = stack[--top];
@@ -497,9 +500,15 @@ void orc_traverse_group(int32_t n, const float* x, const float* y,
                     acc[4 * i + 2] += Gd * (dz * f - qz * inv5);
                     acc[4 * i + 3] -= Gd * (mm * inv + 0.5 * dqd * inv5);
                     if (msum) msum[i] += mm;
+                    if (aabs)
+                        aabs[i] += fabs(Gd) * sqrt((dx * f - qx * inv5) * (dx * f - qx * inv5) +
+                                                   (dy * f - qy * inv5) * (dy * f - qy * inv5) +
+                                                   (dz * f - qz * inv5) * (dz * f - qz * inv5));
                 }
                 cnode += g1 - g0;
+                gc_nodes += 1;
             } else if (!internal || cnt <= leaf) {
+                gc_parts += cnt;
                 for (int32_t j = first[v]; j <= last[v]; ++j) {
                     for (int32_t i = g0; i < g1; ++i) {
                         if (i == j) continue;
@@ -513,9 +522,13 @@ void orc_traverse_group(int32_t n, const float* x, const float* y,
                         acc[4 * i + 2] += dz * f;
                         acc[4 * i + 3] -= Gd * m[j] * inv;
                         if (msum) msum[i] += m[j];
+                        if (aabs) aabs[i] += fabs(f) * sqrt(dx * dx + dy * dy + dz * dz);
                     }
                 }
-                cpart += (int64_t)cnt * (g1 - g0);
+                /* Self-interactions are excluded from the count. */
+                int32_t ov0 = first[v] > g0 ? first[v] : g0;
+                int32_t ov1 = last[v] + 1 < g1 ? last[v] + 1 : g1;
+                cpart += (int64_t)cnt * (g1 - g0) - (ov1 > ov0 ? ov1 - ov0 : 0);
             } else {
                 if (top + 2 > cap) {
                     cap *= 2;
@@ -526,9 +539,42 @@ void orc_traverse_group(int32_t n, const float* x, const float* y,
             }
         }
         free(stack);
+        if (gcost) {
+            gcost[2 * g] = gc_nodes;
+            gcost[2 * g + 1] = gc_parts;
+        }
     }
     counts[0] += cnode;
     counts[1] += cpart;
+}
+
+/* Groups taken from the tree: the maximal subtrees holding at most gmax
+ * particles, i.e. every node with count <= gmax whose parent holds more (or
+ * the root when n <= gmax), in particle order.  out: (first, count) pairs.
+ * Returns the number of groups. */
+int32_t orc_tree_groups(int32_t n, const int32_t* left, const int32_t* right,
+                        const int32_t* first, const int32_t* last, int32_t gmax,
+                        int32_t* out)
+{
+    if (n <= 0) return 0;
+    int32_t ng = 0;
+    int32_t* stack = (int32_t*)malloc(sizeof(int32_t) * (size_t)(2 * n));
+    int top = 0;
+    stack[top++] = 0;
+    while (top > 0) {
+        int32_t v = stack[--top];
+        int32_t cnt = last[v] - first[v] + 1;
+        if (cnt <= gmax) {
+            out[2 * ng] = first[v];
+            out[2 * ng + 1] = cnt;
+            ++ng;
+        } else {
+            stack[top++] = right[v];
+            stack[top++] = left[v];
+        }
+    }
+    free(stack);
+    return ng;
 }
 
 void orc_direct_sum_f64(int64_t n, const double* x, const double* y,

@@ -73,16 +73,29 @@ int64_t orc_traverse_reference(int32_t n, const float* x, const float* y,
  * holds <= leaf particles, else opened.  Index self-exclusion.  acc: n*4
  * doubles (ax, ay, az, phi).  With quad non-null accepted nodes add their
  * quadrupole term.  counts[0] += node (multipole) interactions,
- * counts[1] += particle-particle interactions (per target particle);
+ * counts[1] += particle-particle interactions (per target particle, self
+ * excluded);
  * msum (optional, n doubles) receives the mass each target interacted with
- * (the total mass less its own when every particle is counted once). */
+ * (the total mass less its own when every particle is counted once); aabs
+ * (optional, n doubles) the sum of the magnitudes of the acceleration terms,
+ * the scale of the rounding error of any summation order; gcost
+ * (optional, 2 per group) the node and particle list lengths of each group. */
 void orc_traverse_group(int32_t n, const float* x, const float* y,
                         const float* z, const float* m, const int32_t* left,
                         const int32_t* right, const int32_t* first,
                         const int32_t* last, const float* com4,
                         const float* crit, const float* quad, int32_t group,
                         int32_t leaf, float G, float eps2, double* acc,
-                        int64_t* counts, double* msum);
+                        int64_t* counts, double* msum, double* aabs,
+                        int64_t* gcost, int32_t ngl, const int32_t* glist);
+/* With glist non-null the groups are the ngl (first, count) pairs of glist
+ * instead of runs of `group` particles.
+ * Tree groups: the maximal subtrees of at most gmax particles (every node of
+ * count <= gmax whose parent holds more), in particle order, as (first,
+ * count) pairs.  Returns their number. */
+int32_t orc_tree_groups(int32_t n, const int32_t* left, const int32_t* right,
+                        const int32_t* first, const int32_t* last, int32_t gmax,
+                        int32_t* out);
 
 /* ---- FP64 direct sum (index self-exclusion) ------------------------------
  * Targets tgt[0..nt) (indices into the n sources).  acc: nt*4 doubles

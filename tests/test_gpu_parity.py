@@ -50,8 +50,11 @@ def test_morton_keys_match_oracle(gcs, orc):
     x, y, z = (a.astype(np.float32) for a in (x, y, z))
     say("morton n=%d" % len(x))
     assert np.array_equal(gcs.morton_keys(x, y, z, mn, mx), orc.morton_n(x, y, z, mn, mx))
-    # degenerate span
-    assert np.all(gcs.morton_keys(x[:100], y[:100], z[:100], 2.0, 2.0) == 0)
+    # degenerate span: scale = +inf; points on the span give 0, others saturate
+    assert np.array_equal(gcs.morton_keys(x[:100], y[:100], z[:100], 2.0, 2.0),
+                          orc.morton_n(x[:100], y[:100], z[:100], 2.0, 2.0))
+    two = np.full(50, 2.0, np.float32)
+    assert np.all(gcs.morton_keys(two, two, two, 2.0, 2.0) == 0)
 
 
 @pytest.mark.parametrize("n", [1, 2, 63, 1023, 1024, 1025, 4097, 100003])
@@ -128,18 +131,28 @@ def test_engine_c1_group_parity(gcs, orc):
         cpu = check_pipeline(gcs, orc, e, stars, p)
         acc = e.accels()
         s = cpu["sorted"]
-        want, msum, cnt = orc.traverse_group(cpu["left"], cpu["right"], cpu["first"],
-                                             cpu["last"], s["x"], s["y"], s["z"], s["mass"],
-                                             cpu["agg"], G, float(eps2), 64, 16)
-        assert rel_rms(acc[:, :3], want[:, :3]) < 2e-5
+        grp = orc.tree_groups(cpu["left"], cpu["right"], cpu["first"], cpu["last"], 64)
+        want, msum, cnt, aabs = orc.traverse_group(cpu["left"], cpu["right"], cpu["first"],
+                                                   cpu["last"], s["x"], s["y"], s["z"],
+                                                   s["mass"], cpu["agg"], G, float(eps2), 64, 16,
+                                                   want_abs=True, groups=grp)
+        assert e.engine_info()["groups"] == len(grp)
+        # Same interaction lists: identical counts.  Same sums: the FP32
+        # accumulation error is bounded by a multiple of unit roundoff times
+        # the sum of term magnitudes (3e-5 ~ 500 u here; a dropped or doubled
+        # term of the lists is larger than this for any particle).
+        nn, npart = e.interactions()
+        assert (nn, npart) == (int(cnt[0]), int(cnt[1]))
+        dev = np.abs(acc[:, :3] - want[:, :3]).max(1) / aabs
+        say("C1 vs oracle lists: rel rms %.3g, max dev/sum|terms| %.3g"
+            % (rel_rms(acc[:, :3], want[:, :3]), dev.max()))
+        assert dev.max() < 3e-5
         assert rel_rms(acc[:, 3], want[:, 3]) < 2e-5
         ref = orc.direct_sum_f64(s["x"], s["y"], s["z"], s["mass"], G, float(eps2))
         err = rel_rms(acc[:, :3], ref[:, :3])
         say("C1 rel rms vs FP64 direct sum: %.3g" % err)
         assert err < 1e-3
         assert rel_rms(acc[:, 3], ref[:, 3]) < 1e-4
-        nn, npart = e.interactions()
-        assert (nn, npart) == (int(cnt[0]), int(cnt[1]))
 
 
 def test_engine_reference_mode_parity(gcs, orc):
@@ -310,7 +323,9 @@ def test_diagnostics_match_host_sums(gcs, orc):
 
 # ---- edge cases and errors ---------------------------------------------------------
 
-def test_coincident_particles(gcs):
+def test_coincident_particles(gcs, orc):
+    # All at one point: every key equal, the tree splits by index only, and
+    # every pair is at zero separation (zero force).
     st = np.zeros(300, gcs.STAR)
     st["x"], st["y"], st["z"], st["mass"] = 1.5, -2.0, 0.25, 1.0
     p = gcs.params(G=G, eps2=0.01, dt=0.01)
@@ -318,7 +333,11 @@ def test_coincident_particles(gcs):
         e.load(st)
         e.compute_accels(p)
         raw, srt, perm = e.keys()
-        assert np.all(raw == 0) and np.array_equal(perm, np.arange(300))
+        assert np.all(raw == raw[0]) and np.array_equal(perm, np.arange(300))
+        assert raw[0] == orc.morton(1.5, -2.0, 0.25, -2.0, 1.5)
+        t = e.tree(0)
+        el, er, ep = orc.karras_build(srt)
+        assert np.array_equal(t["left"], el) and np.array_equal(t["parent"], ep)
         assert np.all(e.accels()[:, :3] == 0)
 
 

@@ -268,6 +268,35 @@ def test_group_traversal_accuracy_quadrupole(orc):
     assert rel_rms(acc_q[:, 3], ref[:, 3]) < 1e-4
 
 
+@pytest.mark.parametrize("gmax", [1, 16, 64])
+def test_tree_groups_definition(orc, gmax):
+    # Groups partition the particles; each is the range of a tree node holding
+    # at most gmax particles whose parent holds more (or the whole tree).
+    t = pipeline(orc, 3000, 15)
+    n = 3000
+    grp = orc.tree_groups(t["left"], t["right"], t["first"], t["last"], gmax)
+    assert grp[0, 0] == 0 and grp[-1, 0] + grp[-1, 1] == n
+    assert np.all(grp[1:, 0] == grp[:-1, 0] + grp[:-1, 1])
+    assert np.all((grp[:, 1] >= 1) & (grp[:, 1] <= gmax))
+    ranges = {(int(t["first"][v]), int(t["last"][v])): v for v in range(2 * n - 1)}
+    for f, c in grp:
+        v = ranges[(int(f), int(f + c - 1))]
+        par = t["parent"][v]
+        assert par >= 0 and t["last"][par] - t["first"][par] + 1 > gmax
+
+
+def test_group_traversal_tree_groups_accuracy(orc):
+    t = pipeline(orc, 6000, 16)
+    s = t["sorted"]
+    grp = orc.tree_groups(t["left"], t["right"], t["first"], t["last"], 64)
+    acc, msum, cnt = orc.traverse_group(t["left"], t["right"], t["first"], t["last"], s["x"],
+                                        s["y"], s["z"], s["mass"], t["agg"], G, EPS2, 64, 16,
+                                        groups=grp)
+    ref = orc.direct_sum_f64(s["x"], s["y"], s["z"], s["mass"], G, EPS2)
+    assert rel_rms(acc[:, :3], ref[:, :3]) < 1e-3
+    assert np.allclose(msum + s["mass"], s["mass"].astype(np.float64).sum(), rtol=1e-5)
+
+
 def test_plummer_two_body_closed_form(orc):
     eps2 = 0.09
     x = np.array([0.0, 2.0], np.float32)
