@@ -76,6 +76,49 @@ def cpu_baseline(stars, p, acc_gpu, sample_seconds):
         "targets": len(tgt)}
 
 
+def batched_run(n_inst, steps, warmup, leaf, seed=1004, plain_steps=10, plain_sample=32):
+    """Batched form: n_inst collisions (N log-uniform in [4096, 65536]) advanced
+    together as one forest, against the same collisions run one at a time
+    through the plain path (a sample of plain_sample of them, plain_steps steps
+    each, extrapolated per particle-step).  Returns a dict."""
+    import numpy as np
+    import gcs_amd
+
+    specs = [gcs_amd.random_spec(seed, k, 4096, 65536) for k in range(n_inst)]
+    parts = [gcs_amd.generate(sp) for sp in specs]
+    counts = [len(a) for a in parts]
+    total = int(sum(counts))
+    p = gcs_amd.params(G=1.0, eps2=float(np.float32(0.02) ** 2), dt=0.005, theta=0.5,
+                       leaf_size=leaf, precision=gcs_amd.PREC_COMPENSATED, use_graph=1)
+    e = gcs_amd.Engine(total, n_inst)
+    e.load(np.concatenate(parts), counts)
+    e.step(p, warmup)
+    t0 = time.perf_counter()
+    e.step(p, steps, sync=False)
+    e.sync()
+    el = time.perf_counter() - t0
+    e.close()
+    batched = total * steps / el
+    # One at a time: each collision alone (fresh engine, graph-replayed steps).
+    ps = 0
+    pt = 0.0
+    for a in parts[:plain_sample]:
+        e1 = gcs_amd.Engine(len(a))
+        e1.load(a)
+        e1.step(p, 2)
+        t1 = time.perf_counter()
+        e1.step(p, plain_steps, sync=False)
+        e1.sync()
+        pt += time.perf_counter() - t1
+        ps += len(a) * plain_steps
+        e1.close()
+    plain = ps / pt
+    return {"instances": n_inst, "particles": total, "steps": steps,
+            "ms_per_step": 1e3 * el / steps, "value": batched,
+            "plain_one_at_a_time": plain, "speedup_vs_plain": batched / plain,
+            "plain_sample": "%d collisions x %d steps" % (min(plain_sample, n_inst), plain_steps)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -86,6 +129,10 @@ def main():
     ap.add_argument("--block", type=int, default=256)
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--batched", type=int, default=256,
+                    help="instances of the batched form measured beside the plain step "
+                         "(0 = skip)")
+    ap.add_argument("--batched-steps", type=int, default=30)
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py runs on one GPU: this build has no multi-GPU path "
@@ -165,6 +212,8 @@ def main():
         "cpu_baseline": None,
         "engine": info,
     }
+    if a.batched > 0:
+        line["batched"] = batched_run(a.batched, a.batched_steps, 3, a.leaf)
     if not a.no_cpu:
         cb, acc_check = cpu_baseline(sorted0, p, acc0, a.cpu_seconds)
         line["cpu_baseline"] = cb

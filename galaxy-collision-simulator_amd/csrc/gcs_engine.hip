@@ -102,6 +102,8 @@ struct gcs_engine {
     uint32_t* gidx = nullptr;
     int* gcnt = nullptr;
     int* ngroups = nullptr;
+    int* front = nullptr;   // aggregation frontier: instance s uses [2*start_s, 2*(start_s+n_s))
+    int* nfront = nullptr;
     uint32_t* hist = nullptr;
     uint32_t* partial = nullptr;
     uint32_t* seg_bounds = nullptr;
@@ -168,6 +170,7 @@ void free_all(gcs_engine* e)
     dfree(e->parent); dfree(e->flag);
     dfree(e->tiles); dfree(e->segs); dfree(e->groups); dfree(e->hist); dfree(e->partial);
     dfree(e->gmark); dfree(e->gidx); dfree(e->gcnt); dfree(e->ngroups);
+    dfree(e->front); dfree(e->nfront);
     dfree(e->seg_bounds); dfree(e->qctr); dfree(e->counters); dfree(e->err);
     dfree(e->diag_part); dfree(e->diag_out); dfree(e->slot_mem); dfree(e->staging);
     for (auto& v : e->ev)
@@ -285,7 +288,8 @@ int enqueue_step(gcs_engine* e, const gcs_params& p, bool integrate)
     const bool group = p.mac == GCS_MAC_GROUP;
 
     mark(e, 0);
-    k_reset<<<(nseg + 255) / 256, 256, 0, e->stream>>>(nseg, e->seg_bounds, e->qctr, e->counters);
+    k_reset<<<(nseg + 255) / 256, 256, 0, e->stream>>>(nseg, e->seg_bounds, e->qctr, e->counters,
+                                                        e->nfront);
     if (!p.use_bounds)
         k_bounds<<<qblocks, kTileThreads, 0, e->stream>>>(e->tiles, e->segs, e->posm[c],
                                                            e->seg_bounds);
@@ -336,8 +340,11 @@ int enqueue_step(gcs_engine* e, const gcs_params& p, bool integrate)
     mark(e, 4);
     const float inv_theta = p.theta > 0.0f ? 1.0f / p.theta : __builtin_inff();
     k_aggregate<<<ntiles, kTile, 0, e->stream>>>(e->tiles, e->segs, posm, e->node, e->nodeQ,
-                                                 e->boxLo, e->boxHi, e->parent, e->flag,
-                                                 inv_theta, group ? 1 : 0);
+                                                 e->boxLo, e->boxHi, e->parent, e->front,
+                                                 e->nfront, inv_theta, group ? 1 : 0);
+    k_aggregate_top<<<nseg, 1024, 0, e->stream>>>(e->segs, posm, e->node, e->nodeQ, e->boxLo,
+                                                  e->boxHi, e->parent, e->flag, e->front,
+                                                  e->nfront, inv_theta, group ? 1 : 0);
     LAUNCHCHK();
     mark(e, 5);
     if (group) {
@@ -533,6 +540,8 @@ int gcs_create(int64_t cap_n, int32_t cap_k, int32_t device, gcs_engine** out)
     if (!rc) rc = dalloc(&e->gidx, n);
     if (!rc) rc = dalloc(&e->gcnt, n);
     if (!rc) rc = dalloc(&e->ngroups, 1);
+    if (!rc) rc = dalloc(&e->front, 2 * n + 2);
+    if (!rc) rc = dalloc(&e->nfront, (size_t)cap_k);
     if (!rc) rc = dalloc(&e->hist, he);
     if (!rc) rc = dalloc(&e->partial, std::max(scan_blocks(he), scan_blocks(n)) + 1);
     if (!rc) rc = dalloc(&e->seg_bounds, 2 * (size_t)cap_k);

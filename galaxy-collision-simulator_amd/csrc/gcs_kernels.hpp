@@ -29,14 +29,15 @@ __device__ __forceinline__ QuarterTile quarter_tile(const TileInfo* __restrict__
     return q;
 }
 
-// Start-of-step reset: cube bounds to empty, work-queue and interaction
-// counters to zero.  A kernel rather than hipMemsetAsync so the step can be
+// Start-of-step reset: cube bounds to empty, work-queue, interaction and
+// aggregation-frontier counters to zero.  A kernel rather than hipMemsetAsync so the step can be
 // captured into a hipGraph.
 __global__ void k_reset(int nseg, uint32_t* seg_bounds, unsigned* qctr,
-                        unsigned long long* counters)
+                        unsigned long long* counters, int* nfront)
 {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s < nseg) {
+        nfront[s] = 0;
         seg_bounds[2 * s] = 0xFFFFFFFFu;
         seg_bounds[2 * s + 1] = 0u;
     }
@@ -228,146 +229,260 @@ __global__ __launch_bounds__(256) void k_group_emit(const TileInfo* __restrict__
     if (q.i == n_total - 1) *ngroups = (int)(gidx[q.i] + m);
 }
 
-struct ChildData {
+// Aggregate of one node: COM, mass, AABB, quadrupole (xx, yy, zz, xy, xz, yz).
+struct NodeData {
     float cx, cy, cz, m;
     float lx, ly, lz, hx, hy, hz;
     float q0, q1, q2, q3, q4, q5;
 };
 
-__device__ __forceinline__ void load_child(bool leaf, int idx, const float4* __restrict__ posm,
-                                           const float4* node, const float4* nodeQ,
-                                           const float4* boxLo, const float4* boxHi,
-                                           ChildData& c)
+__device__ __forceinline__ NodeData leaf_data(float4 p)
 {
-    if (leaf) {
-        const float4 p = posm[idx];
-        c.cx = c.lx = c.hx = p.x;
-        c.cy = c.ly = c.hy = p.y;
-        c.cz = c.lz = c.hz = p.z;
-        c.m = p.w;
-        c.q0 = c.q1 = c.q2 = c.q3 = c.q4 = c.q5 = 0.0f;
-    } else {
-        const float4 a = node[2 * idx];
-        const float4 lo = boxLo[idx], hi = boxHi[idx];
-        const float4 qa = nodeQ[2 * idx], qb = nodeQ[2 * idx + 1];
-        c.cx = a.x; c.cy = a.y; c.cz = a.z; c.m = a.w;
-        c.lx = lo.x; c.ly = lo.y; c.lz = lo.z;
-        c.hx = hi.x; c.hy = hi.y; c.hz = hi.z;
-        c.q0 = qa.x; c.q1 = qa.y; c.q2 = qa.z; c.q3 = qa.w;
-        c.q4 = qb.x; c.q5 = qb.y;
-    }
+    NodeData d;
+    d.cx = d.lx = d.hx = p.x;
+    d.cy = d.ly = d.hy = p.y;
+    d.cz = d.lz = d.hz = p.z;
+    d.m = p.w;
+    d.q0 = d.q1 = d.q2 = d.q3 = d.q4 = d.q5 = 0.0f;
+    return d;
+}
+
+__device__ __forceinline__ NodeData load_node_data(int v, const float4* node, const float4* nodeQ,
+                                                   const float4* boxLo, const float4* boxHi)
+{
+    const float4 a = node[2 * v];
+    const float4 lo = boxLo[v], hi = boxHi[v];
+    const float4 qa = nodeQ[2 * v], qb = nodeQ[2 * v + 1];
+    NodeData d;
+    d.cx = a.x; d.cy = a.y; d.cz = a.z; d.m = a.w;
+    d.lx = lo.x; d.ly = lo.y; d.lz = lo.z;
+    d.hx = hi.x; d.hy = hi.y; d.hz = hi.z;
+    d.q0 = qa.x; d.q1 = qa.y; d.q2 = qa.z; d.q3 = qa.w;
+    d.q4 = qb.x; d.q5 = qb.y;
+    return d;
 }
 
 // Adds a child's traceless quadrupole about the parent COM (same operation
 // order as the oracle): Q += Qc + mc*(3 s s^T - |s|^2 I), s = child COM -
-// parent COM; components xx, yy, zz, xy, xz, yz.
-__device__ __forceinline__ void quad_shift(float q[6], const ChildData& c, float px, float py,
-                                           float pz)
+// parent COM.
+__device__ __forceinline__ void quad_shift(NodeData& o, const NodeData& c)
 {
 #pragma clang fp contract(off)
-    const float sx = c.cx - px, sy = c.cy - py, sz = c.cz - pz;
+    const float sx = c.cx - o.cx, sy = c.cy - o.cy, sz = c.cz - o.cz;
     const float s2 = sx * sx + sy * sy + sz * sz;
-    q[0] = q[0] + (c.q0 + c.m * (3.0f * sx * sx - s2));
-    q[1] = q[1] + (c.q1 + c.m * (3.0f * sy * sy - s2));
-    q[2] = q[2] + (c.q2 + c.m * (3.0f * sz * sz - s2));
-    q[3] = q[3] + (c.q3 + c.m * (3.0f * sx * sy));
-    q[4] = q[4] + (c.q4 + c.m * (3.0f * sx * sz));
-    q[5] = q[5] + (c.q5 + c.m * (3.0f * sy * sz));
+    o.q0 = o.q0 + (c.q0 + c.m * (3.0f * sx * sx - s2));
+    o.q1 = o.q1 + (c.q1 + c.m * (3.0f * sy * sy - s2));
+    o.q2 = o.q2 + (c.q2 + c.m * (3.0f * sz * sz - s2));
+    o.q3 = o.q3 + (c.q3 + c.m * (3.0f * sx * sy));
+    o.q4 = o.q4 + (c.q4 + c.m * (3.0f * sx * sz));
+    o.q5 = o.q5 + (c.q5 + c.m * (3.0f * sy * sz));
 }
 
-// Merge of one internal node from its two children (karras_bvh.cuh:190-221):
+// Merge of one internal node from its children (karras_bvh.cuh:190-221):
 //   AABB = union; width = max extent; m = ml + mr;
 //   com = (cl*ml + cr*mr)/m, or the AABB centre when m == 0.
 // Extension: opening value = (width*inv_theta + |com - AABB centre|)^2 for the
 // group traversal (mode 1), or width for the reference traversal (mode 0);
 // quadrupole moments by the parallel-axis shift.
-__device__ __forceinline__ void merge_node(int cur, int int_base, const float4* __restrict__ posm,
-                                           float4* node, float4* nodeQ, float4* boxLo,
-                                           float4* boxHi, float inv_theta, int mode)
+__device__ __forceinline__ NodeData merge(const NodeData& l, const NodeData& r, float inv_theta,
+                                          int mode, float& w, float& open)
 {
 #pragma clang fp contract(off)
-    const float4 bb = node[2 * cur + 1];
-    const int first = __float_as_int(bb.y), last = __float_as_int(bb.z),
-              split = __float_as_int(bb.w);
-    ChildData l, r;
-    load_child(first == split, first == split ? split : split + int_base, posm, node, nodeQ,
-               boxLo, boxHi, l);
-    load_child(last == split + 1, last == split + 1 ? split + 1 : split + 1 + int_base, posm,
-               node, nodeQ, boxLo, boxHi, r);
-    const float lx = fminf(l.lx, r.lx), ly = fminf(l.ly, r.ly), lz = fminf(l.lz, r.lz);
-    const float hx = fmaxf(l.hx, r.hx), hy = fmaxf(l.hy, r.hy), hz = fmaxf(l.hz, r.hz);
-    const float w = fmaxf(hx - lx, fmaxf(hy - ly, hz - lz));
-    const float mt = l.m + r.m;
-    float cx, cy, cz;
-    if (mt > 0.0f) {
-        cx = (l.cx * l.m + r.cx * r.m) / mt;
-        cy = (l.cy * l.m + r.cy * r.m) / mt;
-        cz = (l.cz * l.m + r.cz * r.m) / mt;
+    NodeData o;
+    o.lx = fminf(l.lx, r.lx); o.ly = fminf(l.ly, r.ly); o.lz = fminf(l.lz, r.lz);
+    o.hx = fmaxf(l.hx, r.hx); o.hy = fmaxf(l.hy, r.hy); o.hz = fmaxf(l.hz, r.hz);
+    w = fmaxf(o.hx - o.lx, fmaxf(o.hy - o.ly, o.hz - o.lz));
+    o.m = l.m + r.m;
+    if (o.m > 0.0f) {
+        o.cx = (l.cx * l.m + r.cx * r.m) / o.m;
+        o.cy = (l.cy * l.m + r.cy * r.m) / o.m;
+        o.cz = (l.cz * l.m + r.cz * r.m) / o.m;
     } else {
-        cx = (lx + hx) * 0.5f;
-        cy = (ly + hy) * 0.5f;
-        cz = (lz + hz) * 0.5f;
+        o.cx = (o.lx + o.hx) * 0.5f;
+        o.cy = (o.ly + o.hy) * 0.5f;
+        o.cz = (o.lz + o.hz) * 0.5f;
     }
-    float open;
     if (mode == 1) {
-        const float dx = cx - (lx + hx) * 0.5f;
-        const float dy = cy - (ly + hy) * 0.5f;
-        const float dz = cz - (lz + hz) * 0.5f;
+        const float dx = o.cx - (o.lx + o.hx) * 0.5f;
+        const float dy = o.cy - (o.ly + o.hy) * 0.5f;
+        const float dz = o.cz - (o.lz + o.hz) * 0.5f;
         const float off = sqrtf(dx * dx + dy * dy + dz * dz);
         const float ro = w * inv_theta + off;
         open = ro * ro;
     } else {
         open = w;
     }
-    float qq[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-    quad_shift(qq, l, cx, cy, cz);
-    quad_shift(qq, r, cx, cy, cz);
-    node[2 * cur] = make_float4(cx, cy, cz, mt);
-    node[2 * cur + 1].x = open;
-    nodeQ[2 * cur] = make_float4(qq[0], qq[1], qq[2], qq[3]);
-    nodeQ[2 * cur + 1] = make_float4(qq[4], qq[5], w, 0.0f);
-    boxLo[cur] = make_float4(lx, ly, lz, 0.0f);
-    boxHi[cur] = make_float4(hx, hy, hz, 0.0f);
+    o.q0 = o.q1 = o.q2 = o.q3 = o.q4 = o.q5 = 0.0f;
+    quad_shift(o, l);
+    quad_shift(o, r);
+    return o;
 }
 
-// karras_bvh.cuh:175-225 -- bottom-up aggregation, one block per tile and one
-// thread per leaf.  A thread climbs from its leaf; at each node the first of
-// the two arrivals stops and the second merges the node.  Nodes whose
-// particle range lies inside the block's tile are only ever reached by this
-// block, so their arrival counter is an LDS word with workgroup-scope
-// acquire-release; only nodes whose range crosses the tile use the global
-// counter with agent-scope acquire-release, which makes the children written
-// on another XCD (whose L2 is not coherent with this one) visible -- the
-// ordering the reference's plain atomicAdd leaves out.
+__device__ __forceinline__ void store_node(int v, const NodeData& d, float w, float open,
+                                           float4* node, float4* nodeQ, float4* boxLo,
+                                           float4* boxHi)
+{
+    node[2 * v] = make_float4(d.cx, d.cy, d.cz, d.m);
+    node[2 * v + 1].x = open;
+    nodeQ[2 * v] = make_float4(d.q0, d.q1, d.q2, d.q3);
+    nodeQ[2 * v + 1] = make_float4(d.q4, d.q5, w, 0.0f);
+    boxLo[v] = make_float4(d.lx, d.ly, d.lz, 0.0f);
+    boxHi[v] = make_float4(d.hx, d.hy, d.hz, 0.0f);
+}
+
+// karras_bvh.cuh:175-225 -- bottom-up aggregation, in two kernels.  A thread
+// climbs from a leaf carrying its subtree's aggregate in registers; at each
+// node the first of the two arrivals stops, the second loads only the sibling
+// and merges.
+//
+// k_aggregate: one block per tile, one thread per leaf.  Nodes whose particle
+// range lies inside the tile are only reached by this block: their arrival
+// flags are bits of an LDS word (workgroup-scope acquire-release), their
+// aggregates are kept in LDS (slot = the end of the range that numbers the
+// node, minus the tile start) besides global memory, and the tile's particles
+// are staged in LDS, so sibling reads stay on the CU.  A thread that reaches a
+// node whose range crosses the tile appends the child it came from to its
+// instance's frontier list and stops.
+//
+// k_aggregate_top: one block per instance finishes the tile-crossing nodes,
+// climbing from the frontier with arrival counters in global memory at
+// workgroup scope -- every arrival at a node of an instance is in that
+// instance's block, so no write-back or invalidate across XCDs is needed (the
+// ordering the reference's plain atomicAdd, with no fence, leaves out); the
+// children written by k_aggregate are visible because it completed before
+// this launch.
 __global__ __launch_bounds__(kTile) void k_aggregate(
     const TileInfo* __restrict__ tiles, const SegInfo* __restrict__ segs,
     const float4* __restrict__ posm, float4* node, float4* nodeQ, float4* boxLo, float4* boxHi,
-    const int* __restrict__ parent, int* flag, float inv_theta, int mode)
+    const int* __restrict__ parent, int* __restrict__ front, int* __restrict__ nfront,
+    float inv_theta, int mode)
 {
-    __shared__ int lflag[kTile];
+    __shared__ uint32_t lflag[kTile / 32];
+    __shared__ float4 ldat[kTile][4];
+    __shared__ float4 lpos[kTile];
     const TileInfo t = tiles[blockIdx.x];
-    lflag[threadIdx.x] = 0;
+    const int tid = threadIdx.x;
+    if (tid < kTile / 32) lflag[tid] = 0u;
+    if (tid < t.count) lpos[tid] = posm[t.start + tid];
     __syncthreads();
-    if ((int)threadIdx.x >= t.count) return;
+    if (tid >= t.count) return;
     const SegInfo sg = segs[t.seg];
     const int t0 = t.start, t1 = t.start + t.count - 1;
-    int cur = parent[t.start + (int)threadIdx.x + sg.leaf_base];
+    int child = t.start + tid + sg.leaf_base;
+    NodeData mine = leaf_data(lpos[tid]);
+    int cur = parent[child];
+    // The node record and parent of the next level are loaded one level ahead.
+    float4 bb = make_float4(0.f, 0.f, 0.f, 0.f);
+    int par = -1;
+    if (cur != -1) {
+        bb = node[2 * cur + 1];
+        par = parent[cur];
+    }
     while (cur != -1) {
-        const float4 bb = node[2 * cur + 1];
-        const int first = __float_as_int(bb.y), last = __float_as_int(bb.z);
-        int old;
-        if (first >= t0 && last <= t1) {
-            // An internal node is numbered by one end of its range.
-            const int slot = cur - sg.int_base - t0;
-            old = __hip_atomic_fetch_add(&lflag[slot], 1, __ATOMIC_ACQ_REL,
-                                         __HIP_MEMORY_SCOPE_WORKGROUP);
-        } else {
-            old = __hip_atomic_fetch_add(&flag[cur], 1, __ATOMIC_ACQ_REL,
-                                         __HIP_MEMORY_SCOPE_AGENT);
+        const int first = __float_as_int(bb.y), last = __float_as_int(bb.z),
+                  split = __float_as_int(bb.w);
+        if (first < t0 || last > t1) {
+            const int k = atomicAdd(&nfront[t.seg], 1);
+            front[2 * sg.start + k] = child;
+            return;
         }
-        if (old == 0) return;
-        merge_node(cur, sg.int_base, posm, node, nodeQ, boxLo, boxHi, inv_theta, mode);
-        cur = parent[cur];
+        float4 bb_next = make_float4(0.f, 0.f, 0.f, 0.f);
+        int par_next = -1;
+        if (par != -1) {
+            bb_next = node[2 * par + 1];
+            par_next = parent[par];
+        }
+        const int slot = cur - sg.int_base - t0;
+        const uint32_t bit = 1u << (slot & 31);
+        const uint32_t old = __hip_atomic_fetch_or(&lflag[slot >> 5], bit, __ATOMIC_ACQ_REL,
+                                                   __HIP_MEMORY_SCOPE_WORKGROUP);
+        if (!(old & bit)) return;
+        const bool lleaf = first == split, rleaf = last == split + 1;
+        const int lidx = lleaf ? split + sg.leaf_base : split + sg.int_base;
+        const bool mine_left = child == lidx;
+        // The sibling: a leaf of this tile, or an intra-tile node in LDS.
+        const bool sleaf = mine_left ? rleaf : lleaf;
+        const int sp = mine_left ? split + 1 : split;  // its particle / numbering end
+        NodeData sib;
+        if (sleaf) {
+            sib = leaf_data(lpos[sp - t0]);
+        } else {
+            const int ss = sp - t0;
+            const float4 a = ldat[ss][0], b = ldat[ss][1], c = ldat[ss][2], d = ldat[ss][3];
+            sib.cx = a.x; sib.cy = a.y; sib.cz = a.z; sib.m = a.w;
+            sib.lx = b.x; sib.ly = b.y; sib.lz = b.z; sib.hx = b.w;
+            sib.hy = c.x; sib.hz = c.y; sib.q0 = c.z; sib.q1 = c.w;
+            sib.q2 = d.x; sib.q3 = d.y; sib.q4 = d.z; sib.q5 = d.w;
+        }
+        float w, open;
+        mine = mine_left ? merge(mine, sib, inv_theta, mode, w, open)
+                         : merge(sib, mine, inv_theta, mode, w, open);
+        ldat[slot][0] = make_float4(mine.cx, mine.cy, mine.cz, mine.m);
+        ldat[slot][1] = make_float4(mine.lx, mine.ly, mine.lz, mine.hx);
+        ldat[slot][2] = make_float4(mine.hy, mine.hz, mine.q0, mine.q1);
+        ldat[slot][3] = make_float4(mine.q2, mine.q3, mine.q4, mine.q5);
+        store_node(cur, mine, w, open, node, nodeQ, boxLo, boxHi);
+        child = cur;
+        cur = par;
+        bb = bb_next;
+        par = par_next;
+    }
+}
+
+__global__ __launch_bounds__(1024) void k_aggregate_top(
+    const SegInfo* __restrict__ segs, const float4* __restrict__ posm, float4* node,
+    float4* nodeQ, float4* boxLo, float4* boxHi, const int* __restrict__ parent, int* flag,
+    const int* __restrict__ front, const int* __restrict__ nfront, float inv_theta, int mode)
+{
+    const int s = blockIdx.x;
+    const int nf = nfront[s];
+    if (nf == 0) return;
+    const SegInfo sg = segs[s];
+    for (int k = threadIdx.x; k < nf; k += blockDim.x) {
+        int child = front[2 * sg.start + k];
+        const bool cleaf = child >= sg.start + sg.leaf_base;
+        NodeData mine = cleaf ? leaf_data(posm[child - sg.leaf_base])
+                              : load_node_data(child, node, nodeQ, boxLo, boxHi);
+        int cur = parent[child];
+        float4 bb = node[2 * cur + 1];
+        int par = parent[cur];
+        while (cur != -1) {
+            const int first = __float_as_int(bb.y), last = __float_as_int(bb.z),
+                      split = __float_as_int(bb.w);
+            const bool lleaf = first == split, rleaf = last == split + 1;
+            const int lidx = lleaf ? split + sg.leaf_base : split + sg.int_base;
+            const bool mine_left = child == lidx;
+            const bool sleaf = mine_left ? rleaf : lleaf;
+            const int sp = mine_left ? split + 1 : split;
+            // A sibling whose range lies in one tile (or a leaf) was completed
+            // by k_aggregate: load it before the arrival counter.  A tile-crossing
+            // sibling is written in this launch and is read after the acquire.
+            const int sf = mine_left ? split + 1 : first, sl = mine_left ? last : split;
+            const bool sfinal =
+                sleaf || ((sf - sg.start) / kTile) == ((sl - sg.start) / kTile);
+            NodeData sib;
+            if (sfinal)
+                sib = sleaf ? leaf_data(posm[sp])
+                            : load_node_data(sp + sg.int_base, node, nodeQ, boxLo, boxHi);
+            float4 bb_next = make_float4(0.f, 0.f, 0.f, 0.f);
+            int par_next = -1;
+            if (par != -1) {
+                bb_next = node[2 * par + 1];
+                par_next = parent[par];
+            }
+            const int old = __hip_atomic_fetch_add(&flag[cur], 1, __ATOMIC_ACQ_REL,
+                                                   __HIP_MEMORY_SCOPE_WORKGROUP);
+            if (old == 0) break;
+            if (!sfinal) sib = load_node_data(sp + sg.int_base, node, nodeQ, boxLo, boxHi);
+            float w, open;
+            mine = mine_left ? merge(mine, sib, inv_theta, mode, w, open)
+                             : merge(sib, mine, inv_theta, mode, w, open);
+            store_node(cur, mine, w, open, node, nodeQ, boxLo, boxHi);
+            child = cur;
+            cur = par;
+            bb = bb_next;
+            par = par_next;
+        }
     }
 }
 

@@ -105,12 +105,16 @@ __device__ __forceinline__ void interact_n(Acc& a, float px, float py, float pz,
 }
 
 // Phase 2 over the node list nl[0..k): gathers of chunk c+1 are issued before
-// chunk c is evaluated, and the indices one chunk further ahead.
+// chunk c is evaluated, and the indices one chunk further ahead.  With ks > 1
+// the wave holds ks copies of the group's targets (sub-wave h = lane / (64/ks))
+// and sub-wave h takes chunk entries h, h + ks, h + 2*ks, ...; entries past the
+// list end were gathered as zero-mass sources and add nothing.
 template <bool POT, bool QUAD>
 __device__ __forceinline__ void eval_node_list(WaveLds& L, const int* nl, int k,
                                                const float4* __restrict__ node,
                                                const float4* __restrict__ nodeQ, Acc& a,
-                                               float px, float py, float pz, float eps2)
+                                               float px, float py, float pz, float eps2, int ks,
+                                               int h)
 {
     const int l = lane_id();
     float4 A = make_float4(0.f, 0.f, 0.f, 0.f), Q0 = A, Q1 = A;
@@ -128,40 +132,45 @@ __device__ __forceinline__ void eval_node_list(WaveLds& L, const int* nl, int k,
         }
         __builtin_amdgcn_wave_barrier();
         A = make_float4(0.f, 0.f, 0.f, 0.f);
+        Q0 = A;
+        Q1 = A;
         if (idx_next >= 0) {
             A = node[2 * idx_next];
             if (QUAD) { Q0 = nodeQ[2 * idx_next]; Q1 = nodeQ[2 * idx_next + 1]; }
         }
         const int j2 = base + 2 * kWave + l;
         idx_next = j2 < k ? nl[j2] : -1;
-        const int m = uniform(min(kWave, k - base));
+        const int trips = uniform((min(kWave, k - base) + ks - 1) / ks);
         int t = 0;
-        for (; t + 1 < m; t += 2) {
-            const float4 c0 = L.ca[t], c1 = L.ca[t + 1];
+        for (; t + 1 < trips; t += 2) {
+            const int e0 = t * ks + h, e1 = e0 + ks;
+            const float4 c0 = L.ca[e0], c1 = L.ca[e1];
             float4 q00 = make_float4(0, 0, 0, 0), q01 = q00;
             float2 q10 = make_float2(0, 0), q11 = q10;
             if (QUAD) {
-                q00 = L.cq0[t]; q01 = L.cq0[t + 1];
-                q10 = L.cq1[t]; q11 = L.cq1[t + 1];
+                q00 = L.cq0[e0]; q01 = L.cq0[e1];
+                q10 = L.cq1[e0]; q11 = L.cq1[e1];
             }
             interact_n<POT, QUAD>(a, px, py, pz, c0, q00, q10, eps2);
             interact_n<POT, QUAD>(a, px, py, pz, c1, q01, q11, eps2);
         }
-        if (t < m) {
+        if (t < trips) {
+            const int e0 = t * ks + h;
             float4 q0 = make_float4(0, 0, 0, 0);
             float2 q1 = make_float2(0, 0);
-            if (QUAD) { q0 = L.cq0[t]; q1 = L.cq1[t]; }
-            interact_n<POT, QUAD>(a, px, py, pz, L.ca[t], q0, q1, eps2);
+            if (QUAD) { q0 = L.cq0[e0]; q1 = L.cq1[e0]; }
+            interact_n<POT, QUAD>(a, px, py, pz, L.ca[e0], q0, q1, eps2);
         }
         __builtin_amdgcn_wave_barrier();
     }
 }
 
-// Phase 2 over the particle list pl[0..k), same pipeline.
+// Phase 2 over the particle list pl[0..k), same pipeline and split.
 template <bool POT>
 __device__ __forceinline__ void eval_part_list(WaveLds& L, const int* pl, int k,
                                                const float4* __restrict__ posm, Acc& a,
-                                               float px, float py, float pz, float eps2)
+                                               float px, float py, float pz, float eps2, int ks,
+                                               int h)
 {
     const int l = lane_id();
     float4 S = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -175,16 +184,18 @@ __device__ __forceinline__ void eval_part_list(WaveLds& L, const int* pl, int k,
         if (idx_next >= 0) S = posm[idx_next];
         const int j2 = base + 2 * kWave + l;
         idx_next = j2 < k ? pl[j2] : -1;
-        const int m = uniform(min(kWave, k - base));
+        const int trips = uniform((min(kWave, k - base) + ks - 1) / ks);
         int t = 0;
-        for (; t + 3 < m; t += 4) {
-            const float4 s0 = L.ca[t], s1 = L.ca[t + 1], s2 = L.ca[t + 2], s3 = L.ca[t + 3];
+        for (; t + 3 < trips; t += 4) {
+            const int e0 = t * ks + h;
+            const float4 s0 = L.ca[e0], s1 = L.ca[e0 + ks], s2 = L.ca[e0 + 2 * ks],
+                         s3 = L.ca[e0 + 3 * ks];
             interact_p<POT>(a, px, py, pz, s0, eps2);
             interact_p<POT>(a, px, py, pz, s1, eps2);
             interact_p<POT>(a, px, py, pz, s2, eps2);
             interact_p<POT>(a, px, py, pz, s3, eps2);
         }
-        for (; t < m; ++t) interact_p<POT>(a, px, py, pz, L.ca[t], eps2);
+        for (; t < trips; ++t) interact_p<POT>(a, px, py, pz, L.ca[t * ks + h], eps2);
         __builtin_amdgcn_wave_barrier();
     }
 }
@@ -241,7 +252,11 @@ __global__ __launch_bounds__(WPB * 64) void k_traverse_group(
 
         const GroupInfo gi = groups[g];
         const SegInfo sg = segs[gi.seg];
-        const int p = gi.first + min(l, gi.count - 1);
+        // Small groups: ks copies of the targets, each taking 1/ks of every list.
+        const int ks = gi.count <= 8 ? 8 : gi.count <= 16 ? 4 : gi.count <= 32 ? 2 : 1;
+        const int sub = kWave / ks;
+        const int h = l / sub, tl = l & (sub - 1);
+        const int p = gi.first + min(tl, gi.count - 1);
         const float4 pp = posm[p];
         const float lx = wave_min(pp.x), ly = wave_min(pp.y), lz = wave_min(pp.z);
         const float hx = wave_max(pp.x), hy = wave_max(pp.y), hz = wave_max(pp.z);
@@ -286,7 +301,7 @@ __global__ __launch_bounds__(WPB * 64) void k_traverse_group(
             const uint64_t mA = ballot(accept);
             const int na = __popcll(mA);
             if (nn + na > kNodeList) {
-                eval_node_list<POT, QUAD>(L, nl, nn, node, nodeQ, a, pp.x, pp.y, pp.z, eps2);
+                eval_node_list<POT, QUAD>(L, nl, nn, node, nodeQ, a, pp.x, pp.y, pp.z, eps2, ks, h);
                 tot_nodes += nn;
                 nn = 0;
             }
@@ -299,7 +314,7 @@ __global__ __launch_bounds__(WPB * 64) void k_traverse_group(
             const int incl = wave_scan_incl(cnt_p);
             const int tp = uniform(__shfl(incl, 63));
             if (np + tp > kPartList) {
-                eval_part_list<POT>(L, pl, np, posm, a, pp.x, pp.y, pp.z, eps2);
+                eval_part_list<POT>(L, pl, np, posm, a, pp.x, pp.y, pp.z, eps2, ks, h);
                 tot_parts += np;
                 np = 0;
             }
@@ -340,11 +355,18 @@ __global__ __launch_bounds__(WPB * 64) void k_traverse_group(
             if (l == 0) atomicOr(err, kErrStack);
             break;
         }
-        if (nn > 0) eval_node_list<POT, QUAD>(L, nl, nn, node, nodeQ, a, pp.x, pp.y, pp.z, eps2);
-        if (np > 0) eval_part_list<POT>(L, pl, np, posm, a, pp.x, pp.y, pp.z, eps2);
+        if (nn > 0) eval_node_list<POT, QUAD>(L, nl, nn, node, nodeQ, a, pp.x, pp.y, pp.z, eps2, ks, h);
+        if (np > 0) eval_part_list<POT>(L, pl, np, posm, a, pp.x, pp.y, pp.z, eps2, ks, h);
         tot_nodes += nn;
         tot_parts += np;
-        if (l < gi.count) {
+        // Combine the sub-wave partial sums (fixed order).
+        for (int o = sub; o < kWave; o <<= 1) {
+            a.ax += __shfl_xor(a.ax, o);
+            a.ay += __shfl_xor(a.ay, o);
+            a.az += __shfl_xor(a.az, o);
+            if (POT) a.ph += __shfl_xor(a.ph, o);
+        }
+        if (h == 0 && tl < gi.count) {
             const float ph = POT ? -G * (a.ph - pp.w * self_inv) : 0.0f;
             acc[p] = make_float4(G * a.ax, G * a.ay, G * a.az, ph);
         }
