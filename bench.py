@@ -3,7 +3,7 @@
 
 Workload (N=1): the plain GPU configuration -- one collision of 2 x 131,072
 particles (70/10/20 disc/bulge/halo), FP32, theta = 0.5, softening 0.02,
-dt = 0.005, leaf 16, quadrupole group traversal.  Inputs are generated on the
+dt = 0.005, leaf 8, quadrupole group traversal.  Inputs are generated on the
 host and resident in HBM before the timed region; each timed step is a full
 step (bounds, Morton, sort, permute, tree build, aggregation, traversal,
 kick-drift), replayed from a hipGraph.
@@ -125,8 +125,9 @@ def main():
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--config", default="plain-2x131072", choices=sorted(CONFIGS))
-    ap.add_argument("--leaf", type=int, default=16)
+    ap.add_argument("--leaf", type=int, default=8)
     ap.add_argument("--block", type=int, default=256)
+    ap.add_argument("--wps", type=int, default=3, help="traversal waves per SIMD")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--batched", type=int, default=256,
@@ -148,7 +149,7 @@ def main():
     n = len(stars)
     eps2 = float(np.float32(eps) ** 2)
     p = gcs_amd.params(G=1.0, eps2=eps2, dt=dt, theta=theta, leaf_size=a.leaf,
-                       block_size=a.block, use_graph=1)
+                       block_size=a.block, waves_per_simd=a.wps, use_graph=1)
     e = gcs_amd.Engine(n)
     e.load(stars)
 
@@ -168,7 +169,7 @@ def main():
 
     # Roofline of the traversal kernel: event-timed eager steps.
     pe = gcs_amd.params(G=1.0, eps2=eps2, dt=dt, theta=theta, leaf_size=a.leaf,
-                        block_size=a.block, use_graph=0)
+                        block_size=a.block, waves_per_simd=a.wps, use_graph=0)
     trav = []
     stages = []
     inter = None
@@ -199,7 +200,8 @@ def main():
         "dtype": "f32",
         "data": "synthetic (seeded two-galaxy generator)",
         "config": {"workload": a.config, "particles": n, "theta": theta, "eps": eps, "dt": dt,
-                   "leaf": a.leaf, "block": a.block, "multipole": "quadrupole",
+                   "leaf": a.leaf, "block": a.block, "waves_per_simd": a.wps,
+                   "multipole": "quadrupole",
                    "parallelism": "single GPU"},
         "interactions_per_step": n_inter,
         "interactions_per_s": n_inter / (trav_ms * 1e-3),

@@ -177,12 +177,6 @@ void free_all(gcs_engine* e)
         if (v) (void)hipEventDestroy(v), v = nullptr;
 }
 
-int grid_traverse_blocks(gcs_engine* e, int wpb)
-{
-    (void)wpb;
-    return e->trav_blocks_cap;
-}
-
 // Rebuilds the instance, tile and group tables after the composition changed.
 int rebuild_tables(gcs_engine* e)
 {
@@ -243,6 +237,8 @@ int check_params(const gcs_params* p)
             return fail(GCS_ERR_INVALID, "leaf_size must be in [1, 64]");
         if (p->block_size != 64 && p->block_size != 128 && p->block_size != 256)
             return fail(GCS_ERR_INVALID, "block_size must be 64, 128 or 256");
+        if (p->waves_per_simd < 1 || p->waves_per_simd > 4)
+            return fail(GCS_ERR_INVALID, "waves_per_simd must be in [1, 4]");
     } else {
         if (p->stack_depth < 2 || p->stack_depth > 1024)
             return fail(GCS_ERR_INVALID, "stack_depth must be in [2, 1024]");
@@ -351,8 +347,9 @@ int enqueue_step(gcs_engine* e, const gcs_params& p, bool integrate)
         const int wpb = p.block_size / kWave;
         // Persistent: as many blocks as the per-wave memory covers; waves that
         // find no group left exit at once.
-        const int blocks = std::max(1, std::min(grid_traverse_blocks(e, wpb) * (4 / wpb),
-                                                (int)((e->n_total + wpb - 1) / wpb)));
+        // waves_per_simd * 4 SIMDs * CUs waves, within the per-wave memory.
+        const int waves = std::min(e->trav_blocks_cap * 4, e->num_cu * 4 * p.waves_per_simd);
+        const int blocks = std::max(1, std::min(waves / wpb, (int)((e->n_total + wpb - 1) / wpb)));
         const bool pot = p.diagnostics != 0;
         // Stack slots: blocks * wpb waves <= trav_blocks_cap * 4.
         if (wpb == 4) {
@@ -463,13 +460,14 @@ void gcs_params_default(gcs_params* p)
     p->theta = 0.5f;
     p->mac = GCS_MAC_GROUP;
     p->precision = GCS_PREC_PLAIN;
-    p->leaf_size = 16;
+    p->leaf_size = 8;
     p->block_size = 256;
     p->stack_depth = 256;
     p->use_bounds = 0;
     p->colours = 1;
     p->diagnostics = 0;
     p->use_graph = 1;
+    p->waves_per_simd = 3;
 }
 
 int64_t gcs_device_bytes(int64_t cap_n, int32_t cap_k)

@@ -15,7 +15,7 @@ INCLUDE = os.path.join(REPO, "include")
 LIB = os.path.join(PKG_DIR, "libgcs_amd.so")
 BAKE = os.path.join(PKG_DIR, "gcs_bake")
 
-SOURCES = ["gcs_engine.hip", "gcs_generator.cpp", "gcs_cuda_engine.cpp"]
+SOURCES = ["gcs_engine.hip", "gcs_generator.cpp", "gcs_cuda_engine.cpp", "gcs_server.cpp"]
 
 # -packed-fp32-ops: no packed FP32 instructions are generated.
 # Correctly rounded float divide and sqrt: the Morton scale and the

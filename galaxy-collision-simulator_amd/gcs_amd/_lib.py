@@ -43,13 +43,31 @@ class Params(ctypes.Structure):
                 ("use_bounds", ctypes.c_int32), ("global_min", ctypes.c_float),
                 ("global_max", ctypes.c_float), ("colours", ctypes.c_int32),
                 ("diagnostics", ctypes.c_int32), ("use_graph", ctypes.c_int32),
-                ("reserved", ctypes.c_int32 * 5)]
+                ("waves_per_simd", ctypes.c_int32), ("reserved", ctypes.c_int32 * 4)]
 
 
 class CollisionSpec(ctypes.Structure):
     _fields_ = [("n_a", ctypes.c_int32), ("n_b", ctypes.c_int32),
                 ("offset", ctypes.c_float * 3), ("velocity", ctypes.c_float * 3),
                 ("tilt_deg", ctypes.c_float), ("seed", ctypes.c_uint64)]
+
+
+class Job(ctypes.Structure):
+    _fields_ = [("id", ctypes.c_int64), ("n", ctypes.c_int32), ("steps", ctypes.c_int32),
+                ("seed", ctypes.c_uint64), ("arrival_s", ctypes.c_double)]
+
+
+class JobResult(ctypes.Structure):
+    _fields_ = [("id", ctypes.c_int64), ("n", ctypes.c_int32), ("steps", ctypes.c_int32),
+                ("arrival_s", ctypes.c_double), ("admit_s", ctypes.c_double),
+                ("finish_s", ctypes.c_double)]
+
+
+class ServerConfig(ctypes.Structure):
+    _fields_ = [("hbm_budget_bytes", ctypes.c_int64), ("max_resident", ctypes.c_int64),
+                ("max_jobs", ctypes.c_int32), ("steps_per_round", ctypes.c_int32),
+                ("gen_threads", ctypes.c_int32), ("keep_final", ctypes.c_int32),
+                ("params", Params)]
 
 
 class GcsError(RuntimeError):
@@ -98,6 +116,13 @@ SIGNATURES = {
     "gcs_spec_default": ([_I32, ctypes.c_uint64, _P], None),
     "gcs_spec_random": ([ctypes.c_uint64, _I64, _I32, _I32, _P], None),
     "gcs_generate": ([_P, _P], _I32),
+    "gcs_server_create": ([_P, _P], _I32),
+    "gcs_server_destroy": ([_P], _I32),
+    "gcs_server_submit": ([_P, _P, _I32], _I32),
+    "gcs_server_run": ([_P, ctypes.c_double], _I32),
+    "gcs_server_results": ([_P, _P, _I32, _P], _I32),
+    "gcs_server_stats": ([_P, _P], _I32),
+    "gcs_server_final": ([_P, ctypes.c_int64, _P, _I32], _I32),
 }
 
 

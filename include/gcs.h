@@ -91,12 +91,13 @@ typedef struct gcs_params {
     int32_t colours;    /* 1: update r,g,b from speed (cuda_engine.cu:185-215) */
     int32_t diagnostics;/* 1: FP64 energy/momentum per instance each step */
     int32_t use_graph;  /* 1: replay steps from a captured hipGraph */
-    int32_t reserved[5];
+    int32_t waves_per_simd; /* resident traversal waves per SIMD, 1..4 (GCS_MAC_GROUP) */
+    int32_t reserved[4];
 } gcs_params;
 
 /* Defaults: G=1, eps2=0.0025, dt=0.01, theta=0.5, GCS_MAC_GROUP,
- * GCS_PREC_PLAIN, leaf 16, block 256, stack 256, device bounds, colours on,
- * diagnostics off, graph on. */
+ * GCS_PREC_PLAIN, leaf 8, block 256, stack 256, device bounds, colours on,
+ * diagnostics off, graph on, 3 traversal waves per SIMD. */
 void gcs_params_default(gcs_params* p);
 
 typedef struct gcs_engine gcs_engine;
@@ -219,6 +220,58 @@ void gcs_spec_random(uint64_t seed, int64_t k, int32_t n_min, int32_t n_max,
                      gcs_collision_spec* s);
 /* Writes n_a + n_b records (galaxy A then B; each halo, disc, bulge). */
 int gcs_generate(const gcs_collision_spec* s, gcs_star* out);
+
+/* ---- Served form: continuous batching -----------------------------------
+ * A server owns one engine.  Jobs (a randomised collision of n particles
+ * advanced `steps` steps) arrive at given times; worker threads generate
+ * their initial conditions on arrival; the stepping loop admits generated
+ * jobs in arrival order while the resident particles fit the HBM state
+ * budget and the instance cap, advances every resident job together (one
+ * launch sequence per step, hipGraph-replayed) in rounds that end when the
+ * next job finishes or `steps_per_round` steps have run, and retires finished
+ * jobs.  With max_jobs = 1 the same loop runs jobs one at a time through the
+ * plain path. */
+typedef struct gcs_job {
+    int64_t id;
+    int32_t n;          /* particles (split 50/50 between the two galaxies) */
+    int32_t steps;
+    uint64_t seed;      /* gcs_spec_random(seed, id, n, n) fixes the scenario */
+    double arrival_s;   /* seconds after the start of gcs_server_run */
+} gcs_job;
+
+typedef struct gcs_job_result {
+    int64_t id;
+    int32_t n, steps;
+    double arrival_s, admit_s, finish_s; /* seconds after the start */
+} gcs_job_result;
+
+typedef struct gcs_server_config {
+    int64_t hbm_budget_bytes;     /* device memory for the engine (state + trees) */
+    int64_t max_resident;         /* resident-particle cap, 0 = what the budget holds */
+    int32_t max_jobs;             /* resident-job cap (1 = one at a time) */
+    int32_t steps_per_round;      /* steps between admission checks */
+    int32_t gen_threads;          /* initial-condition worker threads */
+    int32_t keep_final;           /* 1: keep every finished job's final state */
+    gcs_params params;
+} gcs_server_config;
+
+typedef struct gcs_server gcs_server;
+
+int gcs_server_create(const gcs_server_config* c, gcs_server** out);
+int gcs_server_destroy(gcs_server* s);
+/* Adds jobs to the arrival schedule (before or between runs). */
+int gcs_server_submit(gcs_server* s, const gcs_job* jobs, int32_t n);
+/* Runs the loop for `duration_s` seconds of wall time, arrivals following the
+ * wall clock, or (duration_s <= 0) until every submitted job has finished. */
+int gcs_server_run(gcs_server* s, double duration_s);
+/* Finished jobs in completion order. */
+int gcs_server_results(gcs_server* s, gcs_job_result* out, int32_t cap, int32_t* n_out);
+/* [0] particle-steps advanced, [1] run seconds, [2] jobs finished,
+ * [3] jobs admitted, [4] jobs waiting at the end, [5] mean resident
+ * particles per step, [6] rounds, [7] resident particle cap. */
+int gcs_server_stats(gcs_server* s, double* out8);
+/* Final state of a finished job (keep_final = 1), n records. */
+int gcs_server_final(gcs_server* s, int64_t id, gcs_star* out, int32_t n);
 
 #ifdef __cplusplus
 }

@@ -58,7 +58,7 @@ def test_record_layouts(gcs):
 
 def test_params_defaults(gcs):
     p = gcs.params()
-    assert p.G == 1.0 and p.theta == 0.5 and p.mac == gcs.MAC_GROUP and p.leaf_size == 16
+    assert p.G == 1.0 and p.theta == 0.5 and p.mac == gcs.MAC_GROUP and p.leaf_size == 8
 
 
 def test_device_bytes_scale(gcs):
