@@ -119,6 +119,43 @@ def batched_run(n_inst, steps, warmup, leaf, seed=1004, plain_steps=10, plain_sa
             "plain_sample": "%d collisions x %d steps" % (min(plain_sample, n_inst), plain_steps)}
 
 
+def served_run(window, rate, max_resident, leaf, seed=1005, budget=200e9, steps_per_round=25):
+    """Served form: Poisson arrivals at `rate` jobs/s over `window` seconds
+    (N log-uniform in [8192, 524288], steps uniform in [200, 2000]), admitted
+    under the HBM budget and a resident-particle target, continuously batched.
+    The comparator runs the same job list one job at a time (max_jobs = 1)
+    over the same window.  Latency = finish - arrival, over finished jobs."""
+    import numpy as np
+    import gcs_amd
+
+    jobs = gcs_amd.poisson_jobs(seed, rate, window, 8192, 524288, 200, 2000)
+    eps2 = float(np.float32(0.02) ** 2)
+    out = {"window_s": window, "rate_jobs_per_s": rate, "jobs_offered": len(jobs),
+           "offered_particle_steps_per_s": sum(n * k for _, n, k, _ in jobs) / window,
+           "hbm_budget_bytes": budget}
+    for name, kw in (("served", dict(max_resident=max_resident)),
+                     ("one_at_a_time", dict(max_jobs=1, max_resident=524288))):
+        with gcs_amd.Server(hbm_budget_bytes=budget, steps_per_round=steps_per_round,
+                            gen_threads=8, G=1.0, eps2=eps2, dt=0.005, theta=0.5,
+                            leaf_size=leaf, precision=gcs_amd.PREC_COMPENSATED, use_graph=1,
+                            **kw) as srv:
+            srv.submit(jobs, seed)
+            srv.run(window)
+            st = srv.stats()
+            lat = np.array([r["finish"] - r["arrival"] for r in srv.results()])
+        out[name] = {
+            "value": st["particle_steps"] / st["run_s"], "unit": "particle-steps/s",
+            "jobs_finished": int(st["finished"]), "jobs_admitted": int(st["admitted"]),
+            "jobs_waiting": int(st["waiting"]), "mean_resident_particles": st["mean_resident"],
+            "resident_cap": int(st["resident_cap"]), "rounds": int(st["rounds"]),
+            "p50_latency_s": float(np.quantile(lat, 0.5)) if len(lat) else None,
+            "p99_latency_s": float(np.quantile(lat, 0.99)) if len(lat) else None,
+        }
+        print("served leg %s: %s" % (name, json.dumps(out[name])), file=sys.stderr, flush=True)
+    out["speedup_vs_one_at_a_time"] = out["served"]["value"] / out["one_at_a_time"]["value"]
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -134,6 +171,11 @@ def main():
                     help="instances of the batched form measured beside the plain step "
                          "(0 = skip)")
     ap.add_argument("--batched-steps", type=int, default=30)
+    ap.add_argument("--served", type=float, default=20.0,
+                    help="seconds of the served form (each of two legs; 0 = skip)")
+    ap.add_argument("--served-rate", type=float, default=40.0)
+    ap.add_argument("--served-resident", type=int, default=1 << 24,
+                    help="resident-particle admission target of the served form")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py runs on one GPU: this build has no multi-GPU path "
@@ -216,6 +258,8 @@ def main():
     }
     if a.batched > 0:
         line["batched"] = batched_run(a.batched, a.batched_steps, 3, a.leaf)
+    if a.served > 0:
+        line["served"] = served_run(a.served, a.served_rate, a.served_resident, a.leaf)
     if not a.no_cpu:
         cb, acc_check = cpu_baseline(sorted0, p, acc0, a.cpu_seconds)
         line["cpu_baseline"] = cb

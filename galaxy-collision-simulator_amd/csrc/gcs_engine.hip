@@ -16,6 +16,7 @@
 #include <vector>
 
 #include "gcs.h"
+#include "gcs_error.hpp"
 #include "gcs_common.hpp"
 #include "gcs_kernels.hpp"
 #include "gcs_sort.hpp"
@@ -36,6 +37,12 @@ int fail(int code, const std::string& msg)
     g_err = msg;
     return code;
 }
+
+}  // namespace
+
+int gcs_detail::set_error(int code, const char* msg) { return fail(code, msg ? msg : ""); }
+
+namespace {
 
 #define HIPCHK(expr)                                                                     \
     do {                                                                                 \

@@ -22,6 +22,7 @@
 #include <vector>
 
 #include "gcs.h"
+#include "gcs_error.hpp"
 
 namespace {
 
@@ -60,7 +61,10 @@ struct gcs_server {
 
     std::vector<size_t> resident;
     int64_t resident_particles = 0;
-    size_t next_release = 0, next_admit = 0;
+    // jobs[0, next_release) have arrived; jobs[0, next_gen) are queued for
+    // generation (a bounded look-ahead past the admission head keeps host
+    // memory bounded under overload); jobs[0, next_admit) are admitted.
+    size_t next_release = 0, next_gen = 0, next_admit = 0, lookahead = 32;
     std::vector<gcs_job_result> results;
     std::map<int64_t, std::vector<gcs_star>> finals;
     double particle_steps = 0.0, run_s = 0.0, resident_sum = 0.0;
@@ -137,10 +141,12 @@ extern "C" {
 
 int gcs_server_create(const gcs_server_config* c, gcs_server** out)
 {
-    if (!c || !out) return GCS_ERR_INVALID;
+    using gcs_detail::set_error;
+    if (!c || !out) return set_error(GCS_ERR_INVALID, "null argument");
     *out = nullptr;
     if (c->max_jobs < 1 || c->steps_per_round < 1 || c->hbm_budget_bytes <= 0)
-        return GCS_ERR_INVALID;
+        return set_error(GCS_ERR_INVALID, "server config: max_jobs, steps_per_round and "
+                                          "hbm_budget_bytes must be positive");
     gcs_server* s = new (std::nothrow) gcs_server();
     if (!s) return GCS_ERR_OOM;
     s->cfg = *c;
@@ -149,13 +155,13 @@ int gcs_server_create(const gcs_server_config* c, gcs_server** out)
     const int32_t kcap = std::max<int32_t>(1, std::min<int32_t>(c->max_jobs, 1 << 16));
     const int64_t b1 = gcs_device_bytes(1 << 20, kcap), b2 = gcs_device_bytes(1 << 21, kcap);
     const int64_t per = std::max<int64_t>(1, (b2 - b1) >> 20);
-    const int64_t fixed = std::max<int64_t>(0, b1 - per * (1 << 20)) + (int64_t)1 << 30;
+    const int64_t fixed = std::max<int64_t>(0, b1 - per * (1 << 20)) + ((int64_t)1 << 30);
     int64_t cap = (c->hbm_budget_bytes - fixed) / per;
     if (c->max_resident > 0) cap = std::min(cap, c->max_resident);
     cap = std::min<int64_t>(cap, (int64_t)1 << 30);
     if (cap < 1) {
         delete s;
-        return GCS_ERR_CAPACITY;
+        return set_error(GCS_ERR_CAPACITY, "HBM budget holds no particles");
     }
     s->cap_particles = cap;
     s->cap_jobs = kcap;
@@ -165,6 +171,7 @@ int gcs_server_create(const gcs_server_config* c, gcs_server** out)
         return rc;
     }
     const int nt = std::max(1, std::min(32, c->gen_threads));
+    s->lookahead = (size_t)std::max(32, 4 * nt);
     for (int t = 0; t < nt; ++t) s->workers.emplace_back([s] { s->worker(); });
     *out = s;
     return GCS_OK;
@@ -186,11 +193,14 @@ int gcs_server_destroy(gcs_server* s)
 
 int gcs_server_submit(gcs_server* s, const gcs_job* jobs, int32_t n)
 {
-    if (!s || (!jobs && n > 0) || n < 0) return GCS_ERR_INVALID;
+    using gcs_detail::set_error;
+    if (!s || (!jobs && n > 0) || n < 0) return set_error(GCS_ERR_INVALID, "bad job array");
     std::lock_guard<std::mutex> lk(s->mu);
-    for (int32_t i = 0; i < n; ++i) {
+    for (int32_t i = 0; i < n; ++i)
         if (jobs[i].n < 2 || jobs[i].steps < 1 || jobs[i].n > s->cap_particles)
-            return GCS_ERR_INVALID;
+            return set_error(jobs[i].n > s->cap_particles ? GCS_ERR_CAPACITY : GCS_ERR_INVALID,
+                             "job needs n >= 2, steps >= 1 and n within the resident cap");
+    for (int32_t i = 0; i < n; ++i) {
         Job j;
         j.spec = jobs[i];
         s->jobs.push_back(std::move(j));
@@ -208,14 +218,16 @@ int gcs_server_run(gcs_server* s, double duration_s)
     int rc = GCS_OK;
     for (;;) {
         const double t = now_s(t0);
-        // Release arrivals to the generators.
+        // Release arrivals; hand the look-ahead window to the generators.
         {
             std::lock_guard<std::mutex> lk(s->mu);
             while (s->next_release < s->jobs.size() &&
-                   s->jobs[s->next_release].spec.arrival_s <= t) {
-                s->gen_queue.push_back(s->next_release);
-                s->jobs[s->next_release].queued = true;
+                   s->jobs[s->next_release].spec.arrival_s <= t)
                 ++s->next_release;
+            while (s->next_gen < s->next_release && s->next_gen < s->next_admit + s->lookahead) {
+                s->gen_queue.push_back(s->next_gen);
+                s->jobs[s->next_gen].queued = true;
+                ++s->next_gen;
             }
         }
         s->cv_gen.notify_all();

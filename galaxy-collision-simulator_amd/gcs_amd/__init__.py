@@ -7,8 +7,9 @@ from ._lib import (STAR, PLAYBACK_STAR, MAC_GROUP, MAC_REFERENCE, PREC_COMPENSAT
                    GcsError, LIB_PATH, lib)
 from .engine import (Engine, collision, generate, karras_tree, morton_keys, params,
                      random_spec, reference_params, sort_pairs)
+from .server import Server, poisson_jobs
 
 __all__ = ["STAR", "PLAYBACK_STAR", "MAC_GROUP", "MAC_REFERENCE", "PREC_COMPENSATED",
            "PREC_PLAIN", "GcsError", "LIB_PATH", "lib", "Engine", "collision", "generate",
            "karras_tree", "morton_keys", "params", "random_spec", "reference_params",
-           "sort_pairs"]
+           "sort_pairs", "Server", "poisson_jobs"]

@@ -79,6 +79,36 @@ def test_no_device_is_an_error_not_a_crash(gcs):
     assert b"device" in gcs.lib().gcs_last_error()
 
 
+def test_server_config_errors(gcs):
+    """Server set-up failures come back as codes with a message, before any
+    device work: bad config -> -1, a budget that holds no particles -> -5;
+    on a machine with no GPU a valid config then fails with -2."""
+    import ctypes
+    import torch
+    from gcs_amd import _lib as L
+
+    def create(**kw):
+        c = L.ServerConfig()
+        c.hbm_budget_bytes, c.max_jobs, c.steps_per_round, c.gen_threads = 8 << 30, 16, 8, 1
+        c.params = gcs.params()
+        for k, v in kw.items():
+            setattr(c, k, v)
+        h = ctypes.c_void_p()
+        rc = gcs.lib().gcs_server_create(ctypes.byref(c), ctypes.byref(h))
+        assert not h.value or rc == 0
+        if h.value:
+            gcs.lib().gcs_server_destroy(h)
+        return rc, gcs.lib().gcs_last_error()
+
+    rc, msg = create(max_jobs=0)
+    assert rc == -1 and b"max_jobs" in msg
+    rc, msg = create(hbm_budget_bytes=1 << 20)
+    assert rc == -5 and b"budget" in msg
+    if torch.cuda.device_count() == 0:
+        rc, msg = create()
+        assert rc == -2 and b"device" in msg
+
+
 def test_bake_driver_built(gcs):
     path = os.path.join(os.path.dirname(gcs.LIB_PATH), "gcs_bake")
     assert os.access(path, os.X_OK)
