@@ -228,18 +228,32 @@ def ref_available(fast=False):
     return os.path.exists(REF_FAST_SO if fast else REF_SO)
 
 
-def ref_octree_step(stars, G, eps2, dt, integrate=True, want_acc=False, fast=False):
-    """One CPU frame of the reference's own octree code (theta = 0.8 and leaf
-    cap 32 as compiled into gravity_aggregator.cpp / engine.cpp)."""
+def _ref_lib(fast=False):
     path = REF_FAST_SO if fast else REF_SO
     if path not in _ref:
         L = ctypes.CDLL(path)
         L.ref_octree_step.argtypes = [_P, ctypes.c_int, _F, _F, _F, ctypes.c_int, _P, _P]
         L.ref_octree_step.restype = ctypes.c_int
         L.ref_star_size.restype = ctypes.c_int
+        L.ref_morton_codes.argtypes = [_P, _P, _P, ctypes.c_int, _F, _F, _P]
+        L.ref_morton_codes.restype = None
         assert L.ref_star_size() == 32
         _ref[path] = L
-    L = _ref[path]
+    return _ref[path]
+
+
+def ref_morton(x, y, z, mn, mx):
+    """The reference's compiled get_morton_code (morton.h:21-28) on n points."""
+    x, y, z = c32(x), c32(y), c32(z)
+    out = np.zeros(len(x), np.uint64)
+    _ref_lib().ref_morton_codes(P(x), P(y), P(z), len(x), mn, mx, P(out))
+    return out
+
+
+def ref_octree_step(stars, G, eps2, dt, integrate=True, want_acc=False, fast=False):
+    """One CPU frame of the reference's own octree code (theta = 0.8 and leaf
+    cap 32 as compiled into gravity_aggregator.cpp / engine.cpp)."""
+    L = _ref_lib(fast)
     s = np.array(stars, dtype=STAR, copy=True)
     acc = np.zeros((len(s), 3), np.float32) if want_acc else None
     ph = np.zeros(6, np.float64)

@@ -50,6 +50,13 @@ double secs(std::chrono::steady_clock::time_point a, std::chrono::steady_clock::
 
 extern "C" int ref_star_size() { return (int)sizeof(Star); }
 
+// The reference's Morton encoding (morton.h get_morton_code), n points.
+extern "C" void ref_morton_codes(const float* x, const float* y, const float* z, int n,
+                                 float min_val, float max_val, uint64_t* out)
+{
+    for (int i = 0; i < n; ++i) out[i] = get_morton_code(x[i], y[i], z[i], min_val, max_val);
+}
+
 // One CPU frame of the reference (main.cpp:152-253) on `stars` in place.
 // acc_out (n*3 floats, post-sort order) and phase_s[6] are optional.
 // Returns 0, or -1 when the arena runs out.

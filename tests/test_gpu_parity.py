@@ -57,6 +57,17 @@ def test_morton_keys_match_oracle(gcs, orc):
     assert np.all(gcs.morton_keys(two, two, two, 2.0, 2.0) == 0)
 
 
+def test_morton_keys_match_reference_golden(gcs):
+    """HIP Morton keys == the reference's compiled get_morton_code (fixture)."""
+    g = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden",
+                             "morton_ref.npz"))
+    for i in range(int(g["cases"])):
+        mn, mx = (float(v) for v in g["bounds%d" % i])
+        say("morton golden case %d" % i)
+        k = gcs.morton_keys(g["x%d" % i], g["y%d" % i], g["z%d" % i], mn, mx)
+        assert np.array_equal(k, g["keys%d" % i]), i
+
+
 @pytest.mark.parametrize("n", [1, 2, 63, 1023, 1024, 1025, 4097, 100003])
 def test_sort_pairs_match_oracle(gcs, orc, n):
     r = np.random.default_rng(n)

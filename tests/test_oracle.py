@@ -1,5 +1,7 @@
 """Pins the CPU oracle against closed forms, definitions, brute force and the
 compiled reference -- never against a retyped copy of its own formulas."""
+import os
+
 import numpy as np
 import pytest
 
@@ -415,3 +417,29 @@ def test_octree_matches_compiled_reference(orc):
     b, acc_b, _ = orc.ref_octree_step(st, G, 0.0025, 0.01, integrate=True, want_acc=True)
     assert np.array_equal(acc_a, acc_b)
     assert np.array_equal(a.view(np.uint8), b.view(np.uint8))
+
+
+# ---- golden fixtures written from the reference's compiled code ----------------
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def test_morton_matches_reference_golden(orc):
+    """Oracle Morton == the reference's get_morton_code (morton.h:21-28) as
+    compiled, on 4 bound spans including on-bound and next-to-bound points."""
+    g = np.load(os.path.join(GOLDEN, "morton_ref.npz"))
+    for i in range(int(g["cases"])):
+        mn, mx = (float(v) for v in g["bounds%d" % i])
+        k = orc.morton_n(g["x%d" % i], g["y%d" % i], g["z%d" % i], mn, mx)
+        assert np.array_equal(k, g["keys%d" % i]), i
+
+
+def test_octree_matches_reference_golden(orc):
+    """Oracle CPU octree frame == the reference's engine.cpp/gravity_aggregator.cpp
+    frame (main.cpp:152-253) as compiled: accelerations and state bit-exact."""
+    g = np.load(os.path.join(GOLDEN, "octree_ref.npz"))
+    G, eps2, dt = (float(v) for v in g["params"])
+    s, acc, _ = orc.octree_step(g["stars"], G, eps2, dt, theta=0.8, leaf_cap=32,
+                                integrate=True, want_acc=True)
+    assert np.array_equal(acc, g["acc"])
+    assert np.array_equal(s.view(np.uint8), g["after"].view(np.uint8))
