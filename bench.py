@@ -26,6 +26,14 @@ sys.path.insert(0, os.path.join(REPO, "galaxy-collision-simulator_amd"))
 
 PEAK_FP32_TFLOPS = 157.3
 FLOP_PER_INTERACTION = 20
+PEAK_HBM_GBS = 8000.0
+TREE_BYTES_PER_PARTICLE = 376  # Morton 24 + sort 24 + permute 68 + init 88 + build 24 + agg 148
+
+# HBM bytes per k_traverse_group launch, from rocprofv3 FETCH_SIZE (x2.000) and
+# WRITE_SIZE (x1.000) passes scaled by a 256 MiB copy (profiles/
+# k_traverse_group_counters_plain-2x131072.md); valid for the configuration
+# it was measured on.
+TRAVERSE_TRAFFIC = {("plain-2x131072", 8, 256, 3): 237532485}
 
 CONFIGS = {
     # name: (particles per galaxy, eps, dt, theta)
@@ -225,6 +233,8 @@ def main():
     n_inter = inter[0] + inter[1]
     achieved = FLOP_PER_INTERACTION * n_inter / (trav_ms * 1e-3) / 1e12
     stage_ms = [float(x) for x in np.mean(np.array(stages), 0)]
+    tree_ms = sum(stage_ms[:5])
+    tree_gbs = TREE_BYTES_PER_PARTICLE * n / (tree_ms * 1e-3) / 1e9
     info = e.engine_info()
     e.close()
 
@@ -250,9 +260,15 @@ def main():
         "stage_ms": dict(zip(["bounds_morton", "sort", "permute", "build", "aggregate",
                               "traverse", "integrate"], stage_ms)),
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
-                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
+                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS,
+                     "traffic": TRAVERSE_TRAFFIC.get((a.config, a.leaf, a.block, a.wps)),
                      "kernel": "k_traverse_group", "kernel_ms": trav_ms,
                      "flop_per_interaction": FLOP_PER_INTERACTION},
+        "roofline_tree": {"bound": "hbm", "unit": "GB/s", "peak": PEAK_HBM_GBS,
+                          "achieved": tree_gbs, "frac": tree_gbs / PEAK_HBM_GBS,
+                          "bytes_per_particle": TREE_BYTES_PER_PARTICLE,
+                          "stages": "bounds/morton, sort, permute, build, aggregate",
+                          "ms": tree_ms},
         "cpu_baseline": None,
         "engine": info,
     }

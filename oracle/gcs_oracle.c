@@ -11,21 +11,27 @@
  *
  * Parity pins (tests/test_oracle.py):
  *   orc_expand_bits / orc_morton   naive per-bit interleave over all 2^21
- *                                  inputs, known-answer values, edge values
+ *                                  inputs, known-answer values, edge values,
+ *                                  the reference's compiled get_morton_code
+ *                                  (tests/golden/morton_ref.npz)
  *   orc_stable_sort_pairs          numpy stable argsort
  *   orc_karras_build               orc_radix_tree_definition (split at the
  *                                  highest differing bit), tree invariants
  *   orc_aggregate                  FP64 subtree sums, exact AABB min/max
  *   orc_traverse_reference         theta->0 limit equals the direct sum
  *   orc_traverse_group             theta=0 and leaf>=n equal the direct sum;
- *                                  Plummer two-body closed form
+ *                                  Plummer two-body closed form; every mass
+ *                                  counted once; error bound vs FP64
+ *   orc_tree_groups                maximal-subtree definition by brute force
  *   orc_direct_sum_f64             Plummer two-body closed form, Newton's
  *                                  third law (zero total force)
  *   orc_integrate_reference        constant-acceleration closed form, colour
  *                                  table spot values
  *   orc_octree_step                leaf_cap >= n reduces to the direct sum;
- *                                  error bound vs FP64 at theta=0.5
- *                                  the compiled reference (oracle/_ref)
+ *                                  error bound vs FP64 at theta=0.5;
+ *                                  bit-exact against the compiled reference
+ *                                  (oracle/_ref) and its golden frame
+ *                                  (tests/golden/octree_ref.npz)
  */
 #include "gcs_oracle.h"
 
