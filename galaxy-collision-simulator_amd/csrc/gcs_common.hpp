@@ -29,7 +29,7 @@ constexpr int kRadixBits = 8;
 constexpr int kRadix = 1 << kRadixBits;
 constexpr int kSortPasses = 8;       // 64 bits (bit 63 of a Morton key is 0)
 constexpr int kGroup = 64;           // particles per traversal group (one wave)
-constexpr int kScanChunk = 4096;     // entries per block in the offset scan
+constexpr int kGroupChunk = 128;    // quarter tiles per group-numbering chunk
 constexpr int kStackCap = 8192;      // per-wave traversal stack entries
 constexpr int kNumXcd = 8;           // MI355X: 8 XCDs, blocks dealt round-robin
 
@@ -38,10 +38,8 @@ struct TileInfo {
     int start;      // first particle (global index)
     int count;      // particles in the tile
     int seg;        // instance slot
-    int hist_base;  // kRadix * (global index of the instance's first tile)
-    int seg_tiles;  // tiles of this instance
     int tile_in_seg;
-    int pad0, pad1;
+    int pad0, pad1, pad2, pad3;
 };
 
 // Instance slot.
@@ -53,7 +51,7 @@ struct SegInfo {
     int root;       // 2*start - slot
     int tile0;      // global index of the first tile
     int ntiles;
-    int pad;
+    int chunk0;     // first sort-histogram chunk (32 tiles each)
 };
 
 // Traversal group: up to kGroup consecutive particles of one instance.

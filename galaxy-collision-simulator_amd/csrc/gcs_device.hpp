@@ -117,4 +117,25 @@ __device__ __forceinline__ uint32_t pack_rgbt(uint32_t type, uint32_t r, uint32_
     return type | (r << 8) | (g << 16) | (b << 24);
 }
 
+// Block-wide exclusive scan of one value per thread (256 threads; wsum is a
+// 4-entry LDS scratch).  *total receives the block sum.
+__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* total,
+                                                         uint32_t* wsum)
+{
+    const int l = lane_id(), w = threadIdx.x >> 6;
+    uint32_t incl = (uint32_t)wave_scan_incl((int)v);
+    if (l == 63) wsum[w] = incl;
+    __syncthreads();
+    uint32_t base = 0, tot = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const uint32_t s = wsum[k];
+        if (k < w) base += s;
+        tot += s;
+    }
+    __syncthreads();
+    *total = tot;
+    return base + incl - v;
+}
+
 }  // namespace gcs

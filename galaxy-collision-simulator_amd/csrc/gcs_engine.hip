@@ -72,7 +72,15 @@ struct Instance {
 
 size_t max_tiles(int64_t cap_n, int32_t cap_k) { return (size_t)(cap_n / kTile) + (size_t)cap_k + 1; }
 size_t hist_entries(int64_t cap_n, int32_t cap_k) { return (size_t)kRadix * max_tiles(cap_n, cap_k); }
-size_t scan_blocks(size_t entries) { return (entries + kScanChunk - 1) / kScanChunk; }
+// Sort-histogram chunks: at most one partial chunk per instance.
+size_t max_chunks(int64_t cap_n, int32_t cap_k)
+{
+    return max_tiles(cap_n, cap_k) / kChunkTiles + (size_t)cap_k + 1;
+}
+size_t max_gchunks(int64_t cap_n, int32_t cap_k)
+{
+    return (4 * max_tiles(cap_n, cap_k) + kGroupChunk - 1) / kGroupChunk;
+}
 
 }  // namespace
 
@@ -106,14 +114,16 @@ struct gcs_engine {
     SegInfo* segs = nullptr;
     GroupInfo* groups = nullptr;  // traversal groups, rebuilt on the device each step
     uint32_t* gmark = nullptr;
-    uint32_t* gidx = nullptr;
+    uint32_t* gq = nullptr;      // groups per quarter tile
+    uint32_t* gchunk = nullptr;  // groups per kGroupChunk quarter tiles
     int* gcnt = nullptr;
     int* ngroups = nullptr;
     int* front = nullptr;   // aggregation frontier: instance s uses [2*start_s, 2*(start_s+n_s))
     int* nfront = nullptr;
-    uint32_t* hist = nullptr;
-    uint32_t* partial = nullptr;
+    uint32_t* hist = nullptr;   // per-tile digit counts of the current pass
+    uint32_t* chist = nullptr;  // per-chunk digit counts, one block per pass
     uint32_t* seg_bounds = nullptr;
+    uint32_t* bpart = nullptr;  // per quarter tile (min, max) encodings
     unsigned* qctr = nullptr;
     unsigned long long* counters = nullptr;
     unsigned* err = nullptr;
@@ -128,6 +138,7 @@ struct gcs_engine {
     std::vector<TileInfo> h_tiles;
     int64_t n_total = 0;
     int32_t next_id = 0;
+    int nchunks = 0;  // sort-histogram chunks in use
     bool have_tree = false;
     bool capture_keys = false;
     int last_mac = GCS_MAC_GROUP;
@@ -175,10 +186,10 @@ void free_all(gcs_engine* e)
     dfree(e->acc); dfree(e->keys_raw);
     dfree(e->node); dfree(e->nodeQ); dfree(e->boxLo); dfree(e->boxHi);
     dfree(e->parent); dfree(e->flag);
-    dfree(e->tiles); dfree(e->segs); dfree(e->groups); dfree(e->hist); dfree(e->partial);
-    dfree(e->gmark); dfree(e->gidx); dfree(e->gcnt); dfree(e->ngroups);
+    dfree(e->tiles); dfree(e->segs); dfree(e->groups); dfree(e->hist); dfree(e->chist);
+    dfree(e->gmark); dfree(e->gq); dfree(e->gchunk); dfree(e->gcnt); dfree(e->ngroups);
     dfree(e->front); dfree(e->nfront);
-    dfree(e->seg_bounds); dfree(e->qctr); dfree(e->counters); dfree(e->err);
+    dfree(e->seg_bounds); dfree(e->bpart); dfree(e->qctr); dfree(e->counters); dfree(e->err);
     dfree(e->diag_part); dfree(e->diag_out); dfree(e->slot_mem); dfree(e->staging);
     for (auto& v : e->ev)
         if (v) (void)hipEventDestroy(v), v = nullptr;
@@ -189,7 +200,7 @@ int rebuild_tables(gcs_engine* e)
 {
     e->h_segs.clear();
     e->h_tiles.clear();
-    int start = 0;
+    int start = 0, chunks = 0;
     for (size_t s = 0; s < e->inst.size(); ++s) {
         const int n = e->inst[s].n;
         SegInfo sg;
@@ -200,22 +211,22 @@ int rebuild_tables(gcs_engine* e)
         sg.root = 2 * start - (int)s;
         sg.tile0 = (int)e->h_tiles.size();
         sg.ntiles = (n + kTile - 1) / kTile;
-        sg.pad = 0;
+        sg.chunk0 = chunks;
+        chunks += (sg.ntiles + kChunkTiles - 1) / kChunkTiles;
         for (int j = 0; j < sg.ntiles; ++j) {
             TileInfo t;
             t.start = start + j * kTile;
             t.count = std::min(kTile, n - j * kTile);
             t.seg = (int)s;
-            t.hist_base = kRadix * sg.tile0;
-            t.seg_tiles = sg.ntiles;
             t.tile_in_seg = j;
-            t.pad0 = t.pad1 = 0;
+            t.pad0 = t.pad1 = t.pad2 = t.pad3 = 0;
             e->h_tiles.push_back(t);
         }
         e->h_segs.push_back(sg);
         start += n;
     }
     e->n_total = start;
+    e->nchunks = chunks;
     if (!e->h_segs.empty()) {
         HIPCHK(hipMemcpy(e->segs, e->h_segs.data(), sizeof(SegInfo) * e->h_segs.size(),
                          hipMemcpyHostToDevice));
@@ -280,6 +291,25 @@ void mark(gcs_engine* e, int k)
 
 // Queues one step on the stream.  Reads state[cur], leaves the result in
 // state[cur ^ 1] and flips cur.
+// The eight sort passes over keys[0]/vals[0] (sorted result back in
+// buffer 0).  The chunk histograms of all passes must be zero on entry.
+void enqueue_sort(gcs_engine* e)
+{
+    const int ntiles = (int)e->h_tiles.size();
+    const size_t cstride = (size_t)e->nchunks * kRadix;
+    for (int pass = 0; pass < kSortPasses; ++pass) {
+        const int shift = pass * kRadixBits;
+        const int src = pass & 1, dst = src ^ 1;
+        uint32_t* ch = e->chist + (size_t)pass * cstride;
+        k_sort_count<<<ntiles, kTileThreads, 0, e->stream>>>(e->tiles, e->segs, e->keys[src],
+                                                              e->hist, ch, shift);
+        k_sort_scatter<<<ntiles, kTileThreads, 0, e->stream>>>(
+            e->tiles, e->segs, e->keys[src], e->vals[src], e->keys[dst], e->vals[dst], e->hist,
+            ch, shift);
+    }
+    static_assert(kSortPasses % 2 == 0, "sorted keys end in buffer 0");
+}
+
 int enqueue_step(gcs_engine* e, const gcs_params& p, bool integrate)
 {
     const int ntiles = (int)e->h_tiles.size();
@@ -291,11 +321,17 @@ int enqueue_step(gcs_engine* e, const gcs_params& p, bool integrate)
     const bool group = p.mac == GCS_MAC_GROUP;
 
     mark(e, 0);
-    k_reset<<<(nseg + 255) / 256, 256, 0, e->stream>>>(nseg, e->seg_bounds, e->qctr, e->counters,
-                                                        e->nfront);
-    if (!p.use_bounds)
+    const int64_t cwords = (int64_t)kSortPasses * e->nchunks * kRadix;
+    const int ngchunk = (4 * ntiles + kGroupChunk - 1) / kGroupChunk;
+    const int rblocks = (int)std::max<int64_t>((nseg + 255) / 256,
+                                               std::min<int64_t>(1024, (cwords + 255) / 256));
+    k_reset<<<rblocks, 256, 0, e->stream>>>(nseg, e->seg_bounds, e->qctr, e->counters, e->nfront,
+                                            e->chist, cwords);
+    if (!p.use_bounds) {
         k_bounds<<<qblocks, kTileThreads, 0, e->stream>>>(e->tiles, e->segs, e->posm[c],
-                                                           e->seg_bounds);
+                                                           e->bpart);
+        k_bounds_reduce<<<nseg, 256, 0, e->stream>>>(e->segs, e->bpart, e->seg_bounds);
+    }
     k_morton<<<qblocks, kTileThreads, 0, e->stream>>>(e->tiles, e->segs, e->posm[c],
                                                        e->seg_bounds, p.use_bounds, p.global_min,
                                                        p.global_max, e->keys[0], e->vals[0]);
@@ -304,40 +340,23 @@ int enqueue_step(gcs_engine* e, const gcs_params& p, bool integrate)
         HIPCHK(hipMemcpyAsync(e->keys_raw, e->keys[0], sizeof(uint64_t) * e->n_total,
                               hipMemcpyDeviceToDevice, e->stream));
     mark(e, 1);
-    const int64_t hn = (int64_t)kRadix * ntiles;
-    const int sblocks = (int)scan_blocks((size_t)hn);
-    for (int pass = 0; pass < kSortPasses; ++pass) {
-        const int shift = pass * kRadixBits;
-        const int src = pass & 1, dst = src ^ 1;
-        k_sort_count<<<ntiles, kTileThreads, 0, e->stream>>>(e->tiles, e->keys[src], e->hist,
-                                                              shift);
-        k_scan_reduce<<<sblocks, 256, 0, e->stream>>>(e->hist, hn, e->partial);
-        k_scan_partials<<<1, 256, 0, e->stream>>>(e->partial, sblocks);
-        k_scan_apply<<<sblocks, 256, 0, e->stream>>>(e->hist, hn, e->partial);
-        k_sort_scatter<<<ntiles, kTileThreads, 0, e->stream>>>(
-            e->tiles, e->keys[src], e->vals[src], e->keys[dst], e->vals[dst], e->hist, shift);
-    }
+    enqueue_sort(e);
     LAUNCHCHK();
-    static_assert(kSortPasses % 2 == 0, "sorted keys end in buffer 0");
     mark(e, 2);
     k_permute<<<qblocks, kTileThreads, 0, e->stream>>>(
         e->tiles, e->segs, e->vals[0], e->posm[c], e->vel[c], e->plo[c], e->posm[c ^ 1],
         e->vel[c ^ 1], e->plo[c ^ 1], comp ? 1 : 0, group ? 0 : 1, e->node, e->nodeQ, e->gmark,
-        e->gidx);
+        e->gq);
     e->cur = c ^ 1;
     float4* posm = e->posm[e->cur];
     mark(e, 3);
     k_build<<<qblocks, kTileThreads, 0, e->stream>>>(e->tiles, e->segs, e->keys[0], e->node,
-                                                     e->parent, e->flag, e->gmark, e->gidx,
-                                                     e->gcnt);
+                                                     e->parent, e->flag, e->gmark, e->gq, e->gcnt);
     if (group) {
         // Number the groups in particle order and write the group table.
-        const int gb = (int)scan_blocks((size_t)e->n_total);
-        k_scan_reduce<<<gb, 256, 0, e->stream>>>(e->gidx, e->n_total, e->partial);
-        k_scan_partials<<<1, 256, 0, e->stream>>>(e->partial, gb);
-        k_scan_apply<<<gb, 256, 0, e->stream>>>(e->gidx, e->n_total, e->partial);
+        k_group_chunksum<<<ngchunk, kGroupChunk, 0, e->stream>>>(e->gq, qblocks, e->gchunk);
         k_group_emit<<<qblocks, kTileThreads, 0, e->stream>>>(
-            e->tiles, e->segs, e->gmark, e->gidx, e->gcnt, e->groups, e->ngroups,
+            e->tiles, e->segs, e->gmark, e->gq, e->gchunk, e->gcnt, e->groups, e->ngroups,
             (int)e->n_total);
     }
     mark(e, 4);
@@ -488,7 +507,9 @@ int64_t gcs_device_bytes(int64_t cap_n, int32_t cap_k)
     b += 2 * n * (32 + 32 + 16 + 16 + 4 + 4);  // nodes
     b += (int64_t)(sizeof(TileInfo) * max_tiles(cap_n, cap_k) + sizeof(SegInfo) * cap_k +
                    sizeof(GroupInfo) * (cap_n + 1) + 12 * cap_n);
-    b += (int64_t)(4 * hist_entries(cap_n, cap_k) + 4 * scan_blocks(hist_entries(cap_n, cap_k)));
+    b += (int64_t)(4 * hist_entries(cap_n, cap_k) +
+                   4 * (size_t)kSortPasses * kRadix * max_chunks(cap_n, cap_k));
+    b += (int64_t)(4 * 4 * max_tiles(cap_n, cap_k) + 4 * max_gchunks(cap_n, cap_k));
     b += 8 * cap_k + 64 + 5 * 8 * 4 * (int64_t)max_tiles(cap_n, cap_k) + 5 * 8 * cap_k;
     b += 32 * n;               // staging
     return b;
@@ -542,14 +563,16 @@ int gcs_create(int64_t cap_n, int32_t cap_k, int32_t device, gcs_engine** out)
     if (!rc) rc = dalloc(&e->segs, (size_t)cap_k);
     if (!rc) rc = dalloc(&e->groups, n + 1);
     if (!rc) rc = dalloc(&e->gmark, n);
-    if (!rc) rc = dalloc(&e->gidx, n);
+    if (!rc) rc = dalloc(&e->gq, 4 * nt);
+    if (!rc) rc = dalloc(&e->gchunk, max_gchunks(cap_n, cap_k));
     if (!rc) rc = dalloc(&e->gcnt, n);
     if (!rc) rc = dalloc(&e->ngroups, 1);
     if (!rc) rc = dalloc(&e->front, 2 * n + 2);
     if (!rc) rc = dalloc(&e->nfront, (size_t)cap_k);
     if (!rc) rc = dalloc(&e->hist, he);
-    if (!rc) rc = dalloc(&e->partial, std::max(scan_blocks(he), scan_blocks(n)) + 1);
+    if (!rc) rc = dalloc(&e->chist, (size_t)kSortPasses * kRadix * max_chunks(cap_n, cap_k));
     if (!rc) rc = dalloc(&e->seg_bounds, 2 * (size_t)cap_k);
+    if (!rc) rc = dalloc(&e->bpart, 8 * nt);
     if (!rc) rc = dalloc(&e->qctr, kNumXcd);
     if (!rc) rc = dalloc(&e->counters, 2);
     if (!rc) rc = dalloc(&e->err, 1);
@@ -1014,19 +1037,13 @@ int gcs_sort_pairs(const uint64_t* keys, int64_t n, uint64_t* keys_out, int32_t*
             rc = fail(GCS_ERR_HIP, "upload failed");
     }
     if (!rc) {
-        const int ntiles = (int)e->h_tiles.size();
-        const int64_t hn = (int64_t)kRadix * ntiles;
-        const int sblocks = (int)scan_blocks((size_t)hn);
-        for (int pass = 0; pass < kSortPasses; ++pass) {
-            const int shift = pass * kRadixBits, src = pass & 1, dst = src ^ 1;
-            k_sort_count<<<ntiles, kTileThreads, 0, e->stream>>>(e->tiles, e->keys[src], e->hist, shift);
-            k_scan_reduce<<<sblocks, 256, 0, e->stream>>>(e->hist, hn, e->partial);
-            k_scan_partials<<<1, 256, 0, e->stream>>>(e->partial, sblocks);
-            k_scan_apply<<<sblocks, 256, 0, e->stream>>>(e->hist, hn, e->partial);
-            k_sort_scatter<<<ntiles, kTileThreads, 0, e->stream>>>(
-                e->tiles, e->keys[src], e->vals[src], e->keys[dst], e->vals[dst], e->hist, shift);
+        hipError_t le = hipMemsetAsync(e->chist, 0,
+                                       sizeof(uint32_t) * kSortPasses * kRadix * e->nchunks,
+                                       e->stream);
+        if (le == hipSuccess) {
+            enqueue_sort(e);
+            le = hipGetLastError();
         }
-        hipError_t le = hipGetLastError();
         if (le != hipSuccess) rc = fail(GCS_ERR_HIP, hipGetErrorString(le));
         else if (hipMemcpyAsync(keys_out, e->keys[0], 8 * (size_t)n, hipMemcpyDeviceToHost, e->stream) != hipSuccess ||
                  hipMemcpyAsync(perm_out, e->vals[0], 4 * (size_t)n, hipMemcpyDeviceToHost, e->stream) != hipSuccess ||
@@ -1051,7 +1068,7 @@ int gcs_karras_tree(const uint64_t* sorted_keys, int32_t n, int32_t* left, int32
     if (!rc) {
         const int qblocks = 4 * (int)e->h_tiles.size();
         k_build<<<qblocks, kTileThreads, 0, e->stream>>>(e->tiles, e->segs, e->keys[0], e->node,
-                                                         e->parent, e->flag, e->gmark, e->gidx,
+                                                         e->parent, e->flag, e->gmark, e->gq,
                                                          e->gcnt);
         hipError_t le = hipGetLastError();
         if (le != hipSuccess) rc = fail(GCS_ERR_HIP, hipGetErrorString(le));

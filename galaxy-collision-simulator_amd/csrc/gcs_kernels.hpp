@@ -29,28 +29,35 @@ __device__ __forceinline__ QuarterTile quarter_tile(const TileInfo* __restrict__
     return q;
 }
 
-// Start-of-step reset: cube bounds to empty, work-queue, interaction and
-// aggregation-frontier counters to zero.  A kernel rather than hipMemsetAsync so the step can be
-// captured into a hipGraph.
+// Start-of-step reset: cube bounds to empty; work-queue, interaction and
+// aggregation-frontier counters and the sort's chunk histograms (all passes)
+// to zero.  A kernel rather than
+// hipMemsetAsync so the step can be captured into a hipGraph.
 __global__ void k_reset(int nseg, uint32_t* seg_bounds, unsigned* qctr,
-                        unsigned long long* counters, int* nfront)
+                        unsigned long long* counters, int* nfront, uint32_t* chist,
+                        int64_t chist_words)
 {
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s < nseg) {
-        nfront[s] = 0;
-        seg_bounds[2 * s] = 0xFFFFFFFFu;
-        seg_bounds[2 * s + 1] = 0u;
+    const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    if (tid < nseg) {
+        nfront[tid] = 0;
+        seg_bounds[2 * tid] = 0xFFFFFFFFu;
+        seg_bounds[2 * tid + 1] = 0u;
     }
-    if (blockIdx.x == 0 && threadIdx.x < kNumXcd) qctr[threadIdx.x] = 0u;
-    if (blockIdx.x == 0 && threadIdx.x < 2) counters[threadIdx.x] = 0ull;
+    if (tid < kNumXcd) qctr[tid] = 0u;
+    if (tid < 2) counters[tid] = 0ull;
+    for (int64_t i = tid; i < chist_words; i += stride) chist[i] = 0u;
 }
 
-// main.cpp:152-166 -- per-instance cube [min over x,y,z, max over x,y,z].
-// NaN coordinates are ignored, as the reference's `<` comparisons ignore them.
+// main.cpp:152-166 -- per-instance cube [min over x,y,z, max over x,y,z], in
+// two kernels: k_bounds writes each quarter tile's (min, max) as
+// order-preserving integer encodings, k_bounds_reduce folds an instance's
+// quarter tiles (no atomics, so no contention on one address).  NaN
+// coordinates are ignored, as the reference's `<` comparisons ignore them.
 __global__ __launch_bounds__(256) void k_bounds(const TileInfo* __restrict__ tiles,
                                                 const SegInfo* __restrict__ segs,
                                                 const float4* __restrict__ posm,
-                                                uint32_t* seg_bounds)
+                                                uint32_t* __restrict__ bpart)
 {
     const QuarterTile q = quarter_tile(tiles, segs);
     float lo = __builtin_inff(), hi = -__builtin_inff();
@@ -61,9 +68,33 @@ __global__ __launch_bounds__(256) void k_bounds(const TileInfo* __restrict__ til
         if (lo != lo) lo = __builtin_inff();
         if (hi != hi) hi = -__builtin_inff();
     }
+    uint32_t elo = wave_min(fenc(lo)), ehi = wave_max(fenc(hi));
+    __shared__ uint32_t slo[4], shi[4];
+    const int w = threadIdx.x >> 6;
+    if (lane_id() == 0) {
+        slo[w] = elo;
+        shi[w] = ehi;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        bpart[2 * blockIdx.x] = min(min(slo[0], slo[1]), min(slo[2], slo[3]));
+        bpart[2 * blockIdx.x + 1] = max(max(shi[0], shi[1]), max(shi[2], shi[3]));
+    }
+}
+
+__global__ __launch_bounds__(256) void k_bounds_reduce(const SegInfo* __restrict__ segs,
+                                                       const uint32_t* __restrict__ bpart,
+                                                       uint32_t* __restrict__ seg_bounds)
+{
+    const SegInfo sg = segs[blockIdx.x];
+    uint32_t lo = 0xFFFFFFFFu, hi = 0u;
+    for (int k = 4 * sg.tile0 + (int)threadIdx.x; k < 4 * (sg.tile0 + sg.ntiles); k += 256) {
+        lo = min(lo, bpart[2 * k]);
+        hi = max(hi, bpart[2 * k + 1]);
+    }
     lo = wave_min(lo);
     hi = wave_max(hi);
-    __shared__ float slo[4], shi[4];
+    __shared__ uint32_t slo[4], shi[4];
     const int w = threadIdx.x >> 6;
     if (lane_id() == 0) {
         slo[w] = lo;
@@ -71,10 +102,8 @@ __global__ __launch_bounds__(256) void k_bounds(const TileInfo* __restrict__ til
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        lo = fminf(fminf(slo[0], slo[1]), fminf(slo[2], slo[3]));
-        hi = fmaxf(fmaxf(shi[0], shi[1]), fmaxf(shi[2], shi[3]));
-        atomicMin(&seg_bounds[2 * q.seg], fenc(lo));
-        atomicMax(&seg_bounds[2 * q.seg + 1], fenc(hi));
+        seg_bounds[2 * blockIdx.x] = min(min(slo[0], slo[1]), min(slo[2], slo[3]));
+        seg_bounds[2 * blockIdx.x + 1] = max(max(shi[0], shi[1]), max(shi[2], shi[3]));
     }
 }
 
@@ -107,12 +136,12 @@ __global__ __launch_bounds__(256) void k_permute(
     const float4* __restrict__ vel_in, const float4* __restrict__ plo_in,
     float4* __restrict__ posm_out, float4* __restrict__ vel_out, float4* __restrict__ plo_out,
     int compensated, int write_leaves, float4* __restrict__ node, float4* __restrict__ nodeQ,
-    uint32_t* __restrict__ gmark, uint32_t* __restrict__ gidx)
+    uint32_t* __restrict__ gmark, uint32_t* __restrict__ gq)
 {
     const QuarterTile q = quarter_tile(tiles, segs);
+    if (threadIdx.x == 0) gq[blockIdx.x] = 0u;
     if (!q.valid) return;
     gmark[q.i] = 0u;
-    gidx[q.i] = 0u;
     const int src = (int)perm[q.i];
     const float4 p = posm_in[src];
     posm_out[q.i] = p;
@@ -137,19 +166,33 @@ __device__ __forceinline__ int delta(int i, int j, int n, const uint64_t* __rest
     return clz64s(a ^ b);
 }
 
+// Marks a traversal group starting at global particle x with `count`
+// particles and counts it in its quarter tile.
+__device__ __forceinline__ void mark_group(const SegInfo& sg, int x, int count,
+                                           uint32_t* __restrict__ gmark, uint32_t* __restrict__ gq,
+                                           int* __restrict__ gcnt)
+{
+    gmark[x] = 1u;
+    gcnt[x] = count;
+    const int off = x - sg.start;
+    const int qt = 4 * (sg.tile0 + off / kTile) + (off % kTile) / kTileThreads;
+    atomicAdd(&gq[qt], 1u);
+}
+
 // karras_bvh.cuh:73-131 -- one internal node per particle but the last of
 // each instance, built on instance-local indices so every instance's tree is
 // the stand-alone tree.  Writes range/split, parents and clears the arrival
 // flag of the node it builds.  Also marks traversal groups: the maximal
 // subtrees of at most kGroup particles (a child of at most kGroup under a
 // parent of more, or the root of a small instance) are flagged at their first
-// particle (gmark, gidx = 1) with their size (gcnt).
+// particle (gmark = 1) with their size (gcnt), and counted per quarter tile
+// (gq) for the numbering.
 __global__ __launch_bounds__(256) void k_build(const TileInfo* __restrict__ tiles,
                                                const SegInfo* __restrict__ segs,
                                                const uint64_t* __restrict__ skeys,
                                                float4* __restrict__ node, int* __restrict__ parent,
                                                int* __restrict__ flag, uint32_t* __restrict__ gmark,
-                                               uint32_t* __restrict__ gidx, int* __restrict__ gcnt)
+                                               uint32_t* __restrict__ gq, int* __restrict__ gcnt)
 {
     const QuarterTile q = quarter_tile(tiles, segs);
     if (!q.valid) return;
@@ -157,11 +200,7 @@ __global__ __launch_bounds__(256) void k_build(const TileInfo* __restrict__ tile
     const int n = sg.n, i = q.local;
     if (i == 0) {
         parent[sg.root] = -1;
-        if (n <= kGroup) {
-            gmark[sg.start] = 1u;
-            gidx[sg.start] = 1u;
-            gcnt[sg.start] = n;
-        }
+        if (n <= kGroup) mark_group(sg, sg.start, n, gmark, gq, gcnt);
     }
     if (i >= n - 1) return;
     const uint64_t* k = skeys + sg.start;
@@ -192,41 +231,58 @@ __global__ __launch_bounds__(256) void k_build(const TileInfo* __restrict__ tile
     flag[me] = 0;
     if (hi - lo + 1 > kGroup) {
         const int nl = split - lo + 1, nr = hi - split;
-        if (nl <= kGroup) {
-            gmark[sg.start + lo] = 1u;
-            gidx[sg.start + lo] = 1u;
-            gcnt[sg.start + lo] = nl;
-        }
-        if (nr <= kGroup) {
-            gmark[gsplit + 1] = 1u;
-            gidx[gsplit + 1] = 1u;
-            gcnt[gsplit + 1] = nr;
-        }
+        if (nl <= kGroup) mark_group(sg, sg.start + lo, nl, gmark, gq, gcnt);
+        if (nr <= kGroup) mark_group(sg, gsplit + 1, nr, gmark, gq, gcnt);
     }
 }
 
-// Writes the group table in particle (Morton) order from the exclusive scan
-// of the group marks, and the number of groups.
+// Groups per chunk of kGroupChunk quarter tiles.
+__global__ __launch_bounds__(kGroupChunk) void k_group_chunksum(const uint32_t* __restrict__ gq,
+                                                                 int nq,
+                                                                 uint32_t* __restrict__ gchunk)
+{
+    static_assert(kGroupChunk == 2 * kWave, "two waves per chunk");
+    __shared__ uint32_t w1;
+    const int q = blockIdx.x * kGroupChunk + (int)threadIdx.x;
+    const uint32_t v = (uint32_t)wave_sum(q < nq ? (int)gq[q] : 0);
+    if (threadIdx.x == kWave) w1 = v;
+    __syncthreads();
+    if (threadIdx.x == 0) gchunk[blockIdx.x] = v + w1;
+}
+
+// Writes the group table in particle (Morton) order, and the number of
+// groups.  A quarter tile's first group number is the groups of earlier
+// chunks, plus those of earlier quarter tiles of its chunk, plus the marks
+// before each particle within the quarter tile (block scan).
 __global__ __launch_bounds__(256) void k_group_emit(const TileInfo* __restrict__ tiles,
                                                     const SegInfo* __restrict__ segs,
                                                     const uint32_t* __restrict__ gmark,
-                                                    const uint32_t* __restrict__ gidx,
+                                                    const uint32_t* __restrict__ gq,
+                                                    const uint32_t* __restrict__ gchunk,
                                                     const int* __restrict__ gcnt,
                                                     GroupInfo* __restrict__ groups,
                                                     int* __restrict__ ngroups, int n_total)
 {
+    __shared__ uint32_t ws[4];
     const QuarterTile q = quarter_tile(tiles, segs);
-    if (!q.valid) return;
-    const uint32_t m = gmark[q.i];
+    const int b = blockIdx.x, c = b / kGroupChunk;
+    uint32_t part = 0u;
+    for (int k = threadIdx.x; k < c; k += 256) part += gchunk[k];
+    for (int k = c * kGroupChunk + (int)threadIdx.x; k < b; k += 256) part += gq[k];
+    uint32_t base;
+    block_exclusive_scan(part, &base, ws);
+    const uint32_t m = q.valid ? gmark[q.i] : 0u;
+    uint32_t nb;
+    const uint32_t ex = block_exclusive_scan(m, &nb, ws);
     if (m) {
         GroupInfo g;
         g.first = q.i;
         g.count = gcnt[q.i];
         g.seg = q.seg;
         g.pad = 0;
-        groups[gidx[q.i]] = g;
+        groups[base + ex] = g;
     }
-    if (q.i == n_total - 1) *ngroups = (int)(gidx[q.i] + m);
+    if (q.valid && q.i == n_total - 1) *ngroups = (int)(base + nb);
 }
 
 // Aggregate of one node: COM, mass, AABB, quadrupole (xx, yy, zz, xy, xz, yz).
@@ -345,12 +401,12 @@ __device__ __forceinline__ void store_node(int v, const NodeData& d, float w, fl
 // instance's frontier list and stops.
 //
 // k_aggregate_top: one block per instance finishes the tile-crossing nodes,
-// climbing from the frontier with arrival counters in global memory at
-// workgroup scope -- every arrival at a node of an instance is in that
-// instance's block, so no write-back or invalidate across XCDs is needed (the
-// ordering the reference's plain atomicAdd, with no fence, leaves out); the
-// children written by k_aggregate are visible because it completed before
-// this launch.
+// climbing from the frontier with arrival marks at workgroup scope (bits in
+// LDS, or counters in global memory for very large instances) -- every
+// arrival at a node of an instance is in that instance's block, so no
+// write-back or invalidate across XCDs is needed (the ordering the
+// reference's plain atomicAdd, with no fence, leaves out); the children
+// written by k_aggregate are visible because it completed before this launch.
 __global__ __launch_bounds__(kTile) void k_aggregate(
     const TileInfo* __restrict__ tiles, const SegInfo* __restrict__ segs,
     const float4* __restrict__ posm, float4* node, float4* nodeQ, float4* boxLo, float4* boxHi,
@@ -429,15 +485,24 @@ __global__ __launch_bounds__(kTile) void k_aggregate(
     }
 }
 
+// Instances of up to this many particles keep the top pass's arrival bits in
+// LDS (128 KiB); larger ones use the global arrival counters.
+constexpr int kTopLdsNodes = 1 << 20;
+
 __global__ __launch_bounds__(1024) void k_aggregate_top(
     const SegInfo* __restrict__ segs, const float4* __restrict__ posm, float4* node,
     float4* nodeQ, float4* boxLo, float4* boxHi, const int* __restrict__ parent, int* flag,
     const int* __restrict__ front, const int* __restrict__ nfront, float inv_theta, int mode)
 {
+    __shared__ uint32_t lbits[kTopLdsNodes / 32];
     const int s = blockIdx.x;
     const int nf = nfront[s];
     if (nf == 0) return;
     const SegInfo sg = segs[s];
+    const bool in_lds = sg.n <= kTopLdsNodes;
+    if (in_lds)
+        for (int k = threadIdx.x; k < (sg.n + 31) / 32; k += blockDim.x) lbits[k] = 0u;
+    __syncthreads();
     for (int k = threadIdx.x; k < nf; k += blockDim.x) {
         int child = front[2 * sg.start + k];
         const bool cleaf = child >= sg.start + sg.leaf_base;
@@ -455,7 +520,7 @@ __global__ __launch_bounds__(1024) void k_aggregate_top(
             const bool sleaf = mine_left ? rleaf : lleaf;
             const int sp = mine_left ? split + 1 : split;
             // A sibling whose range lies in one tile (or a leaf) was completed
-            // by k_aggregate: load it before the arrival counter.  A tile-crossing
+            // by k_aggregate: load it before the arrival check.  A tile-crossing
             // sibling is written in this launch and is read after the acquire.
             const int sf = mine_left ? split + 1 : first, sl = mine_left ? last : split;
             const bool sfinal =
@@ -470,9 +535,31 @@ __global__ __launch_bounds__(1024) void k_aggregate_top(
                 bb_next = node[2 * par + 1];
                 par_next = parent[par];
             }
-            const int old = __hip_atomic_fetch_add(&flag[cur], 1, __ATOMIC_ACQ_REL,
-                                                   __HIP_MEMORY_SCOPE_WORKGROUP);
-            if (old == 0) break;
+            // Arrival: a thread that sees the sibling's mark already set is the
+            // second and continues without publishing its own node (no one
+            // else reads it in this launch), so the climb that finishes last
+            // never waits for its stores.  Otherwise it marks the node with a
+            // release (its record is then visible to the sibling's thread).
+            bool second;
+            if (in_lds) {
+                const int slot = cur - sg.int_base - sg.start;
+                uint32_t* w = &lbits[slot >> 5];
+                const uint32_t bit = 1u << (slot & 31);
+                second = (__hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) &
+                          bit) != 0u;
+                if (!second)
+                    second = (__hip_atomic_fetch_or(w, bit, __ATOMIC_ACQ_REL,
+                                                    __HIP_MEMORY_SCOPE_WORKGROUP) &
+                              bit) != 0u;
+            } else {
+                second = __hip_atomic_load(&flag[cur], __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_WORKGROUP) != 0;
+                if (!second)
+                    second = __hip_atomic_fetch_add(&flag[cur], 1, __ATOMIC_ACQ_REL,
+                                                    __HIP_MEMORY_SCOPE_WORKGROUP) != 0;
+            }
+            if (!second) break;
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
             if (!sfinal) sib = load_node_data(sp + sg.int_base, node, nodeQ, boxLo, boxHi);
             float w, open;
             mine = mine_left ? merge(mine, sib, inv_theta, mode, w, open)
