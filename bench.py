@@ -43,22 +43,30 @@ CONFIGS = {
 }
 
 
-def cpu_baseline(stars, p, acc_gpu, sample_seconds):
-    """Times the reference CPU octree on the host and checks the GPU
-    accelerations against an FP64 direct sum on a subsample."""
+def accuracy_check(stars, p, acc_gpu, targets):
+    """CPU checker of the GPU accelerations: FP64 direct sum (the oracle) on
+    `targets` evenly spaced particles; relative RMS and p99 per-particle
+    relative error."""
     import numpy as np
     sys.path.insert(0, os.path.join(REPO, "oracle"))
     import oracle
 
     n = len(stars)
-    # accuracy of the GPU step (FP64 direct sum on 4096 targets)
-    tgt = np.linspace(0, n - 1, 4096).astype(np.int64)
+    tgt = np.linspace(0, n - 1, targets).astype(np.int64)
     ref = oracle.direct_sum_f64(stars["x"], stars["y"], stars["z"], stars["mass"], p.G,
                                 float(p.eps2), tgt)
     d = acc_gpu[tgt, :3].astype(np.float64) - ref[:, :3]
     err = float(np.sqrt((d ** 2).sum() / (ref[:, :3] ** 2).sum()))
     rel = np.linalg.norm(d, axis=1) / np.linalg.norm(ref[:, :3], axis=1)
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {"rel_rms_vs_fp64": err, "p99_rel_vs_fp64": float(np.quantile(rel, 0.99)),
+            "targets": len(tgt)}
+
+
+def cpu_octree_rate(stars, p, sample_seconds, max_steps=50):
+    """The reference's compiled CPU octree frame (oracle/_ref) timed on the
+    host cores; particle-steps/s."""
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import oracle
     kind = "reference" if oracle.ref_available() else "port"
     step = (lambda s: oracle.ref_octree_step(s, p.G, float(p.eps2), p.dt)) if kind == "reference" \
         else (lambda s: oracle.octree_step(s, p.G, float(p.eps2), p.dt, 0.8, 32))
@@ -67,21 +75,88 @@ def cpu_baseline(stars, p, acc_gpu, sample_seconds):
     t0 = time.perf_counter()
     k = 0
     while True:
-        s, _, ph = step(s)
+        s, _, _ = step(s)
         k += 1
-        if time.perf_counter() - t0 > sample_seconds or k >= 50:
+        if time.perf_counter() - t0 > sample_seconds or k >= max_steps:
             break
-    el = time.perf_counter() - t0
+    return len(stars) * k / (time.perf_counter() - t0), k, kind
+
+
+def cpu_baseline(stars, p, acc_gpu, sample_seconds):
+    """Times the reference CPU octree on the host and checks the GPU
+    accelerations against an FP64 direct sum on a subsample."""
+    n = len(stars)
+    check = accuracy_check(stars, p, acc_gpu, 4096)
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    rate, k, kind = cpu_octree_rate(stars, p, sample_seconds)
     return {
-        "value": n * k / el,
+        "value": rate,
         "unit": "particle-steps/s",
         "cores": threads,
         "kind": kind,
         "sample": "%d steps of the reference CPU octree frame (theta=0.8, leaf 32, as compiled "
                   "in gravity_aggregator.cpp/engine.cpp) on the same %d-particle input, "
                   "OMP_NUM_THREADS=%d" % (k, n, threads),
-    }, {"rel_rms_vs_fp64": err, "p99_rel_vs_fp64": float(np.quantile(rel, 0.99)),
-        "targets": len(tgt)}
+    }, check
+
+
+def precision_study(steps, targets, cpu_seconds, seed=1003):
+    """The precision form: one collision of 2 x 1,048,576 particles, eps 0.02,
+    dt 0.005.  For theta 0.5 and 0.7: the initial accelerations against an
+    FP64 direct sum on `targets` particles (CPU checker), and the relative
+    total-energy drift over `steps` steps with plain FP32 and with compensated
+    positions (FP64 diagnostics on the GPU, first and last step).  Also the
+    GPU step rate next to the reference CPU octree on the same input."""
+    import numpy as np
+    import gcs_amd
+
+    stars = gcs_amd.collision(1048576, seed)
+    n = len(stars)
+    eps2 = float(np.float32(0.02) ** 2)
+    out = {"particles": n, "steps": steps, "runs": []}
+    for theta in (0.5, 0.7):
+        base = gcs_amd.params(G=1.0, eps2=eps2, dt=0.005, theta=theta, use_graph=1)
+        with gcs_amd.Engine(n) as e:
+            e.load(stars)
+            base, _ = e.autotune(base, reps=2)
+            e.compute_accels(base)
+            acc = e.accels()
+            s0 = e.stars()
+        run = {"theta": theta, "leaf": base.leaf_size, "block": base.block_size,
+               "waves_per_simd": base.waves_per_simd}
+        run.update(accuracy_check(s0, base, acc, targets))
+        for prec, name in ((gcs_amd.PREC_PLAIN, "plain"),
+                           (gcs_amd.PREC_COMPENSATED, "compensated")):
+            kw = dict(G=1.0, eps2=eps2, dt=0.005, theta=theta, leaf_size=base.leaf_size,
+                      block_size=base.block_size, waves_per_simd=base.waves_per_simd,
+                      precision=prec, use_graph=1)
+            pd = gcs_amd.params(diagnostics=1, **kw)
+            pn = gcs_amd.params(diagnostics=0, **kw)
+            with gcs_amd.Engine(n) as e:
+                e.load(stars)
+                e.step(pd, 1)
+                d0 = e.diagnostics()
+                t0 = time.perf_counter()
+                e.step(pn, steps - 2, sync=False)
+                e.sync()
+                el = time.perf_counter() - t0
+                e.step(pd, 1)
+                d1 = e.diagnostics()
+            e0, e1 = d0[0] + d0[1], d1[0] + d1[1]
+            run[name] = {"energy_first": e0, "energy_last": e1,
+                         "energy_drift_rel": (e1 - e0) / abs(e0),
+                         "momentum_change": float(np.linalg.norm(d1[2:5] - d0[2:5])),
+                         "ms_per_step": 1e3 * el / (steps - 2)}
+        out["runs"].append(run)
+        print("precision theta %.1f: %s" % (theta, json.dumps(run)), file=sys.stderr, flush=True)
+    plain05 = out["runs"][0]["plain"]["ms_per_step"]
+    cpu_p = gcs_amd.params(G=1.0, eps2=eps2, dt=0.005)
+    rate, k, kind = cpu_octree_rate(stars, cpu_p, cpu_seconds, max_steps=5)
+    out["gpu_particle_steps_per_s"] = n / (plain05 * 1e-3)
+    out["cpu_octree"] = {"value": rate, "unit": "particle-steps/s", "kind": kind, "steps": k,
+                         "cores": int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))}
+    out["speedup_vs_cpu"] = out["gpu_particle_steps_per_s"] / rate
+    return out
 
 
 def batched_run(n_inst, steps, warmup, leaf, seed=1004, plain_steps=10, plain_sample=32):
@@ -100,6 +175,7 @@ def batched_run(n_inst, steps, warmup, leaf, seed=1004, plain_steps=10, plain_sa
                        leaf_size=leaf, precision=gcs_amd.PREC_COMPENSATED, use_graph=1)
     e = gcs_amd.Engine(total, n_inst)
     e.load(np.concatenate(parts), counts)
+    p, _ = e.autotune(p, reps=2)
     e.step(p, warmup)
     t0 = time.perf_counter()
     e.step(p, steps, sync=False)
@@ -123,6 +199,8 @@ def batched_run(n_inst, steps, warmup, leaf, seed=1004, plain_steps=10, plain_sa
     plain = ps / pt
     return {"instances": n_inst, "particles": total, "steps": steps,
             "ms_per_step": 1e3 * el / steps, "value": batched,
+            "autotuned": {"leaf": p.leaf_size, "block": p.block_size,
+                          "waves_per_simd": p.waves_per_simd},
             "plain_one_at_a_time": plain, "speedup_vs_plain": batched / plain,
             "plain_sample": "%d collisions x %d steps" % (min(plain_sample, n_inst), plain_steps)}
 
@@ -173,12 +251,17 @@ def main():
     ap.add_argument("--leaf", type=int, default=8)
     ap.add_argument("--block", type=int, default=256)
     ap.add_argument("--wps", type=int, default=3, help="traversal waves per SIMD")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="use --leaf/--block/--wps as given instead of the autotuner's choice")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--batched", type=int, default=256,
                     help="instances of the batched form measured beside the plain step "
                          "(0 = skip)")
     ap.add_argument("--batched-steps", type=int, default=30)
+    ap.add_argument("--precision", type=int, default=0,
+                    help="steps of the 2 x 1,048,576 precision study (0 = skip)")
+    ap.add_argument("--precision-targets", type=int, default=65536)
     ap.add_argument("--served", type=float, default=20.0,
                     help="seconds of the served form (each of two legs; 0 = skip)")
     ap.add_argument("--served-rate", type=float, default=40.0)
@@ -202,6 +285,9 @@ def main():
                        block_size=a.block, waves_per_simd=a.wps, use_graph=1)
     e = gcs_amd.Engine(n)
     e.load(stars)
+    if not a.no_autotune:
+        p, _ = e.autotune(p, reps=3)
+        a.leaf, a.block, a.wps = p.leaf_size, p.block_size, p.waves_per_simd
 
     # Accelerations of the initial state (for the accuracy check).
     e.compute_accels(p)
@@ -253,6 +339,7 @@ def main():
         "data": "synthetic (seeded two-galaxy generator)",
         "config": {"workload": a.config, "particles": n, "theta": theta, "eps": eps, "dt": dt,
                    "leaf": a.leaf, "block": a.block, "waves_per_simd": a.wps,
+                   "autotuned": not a.no_autotune,
                    "multipole": "quadrupole",
                    "parallelism": "single GPU"},
         "interactions_per_step": n_inter,
@@ -274,6 +361,8 @@ def main():
     }
     if a.batched > 0:
         line["batched"] = batched_run(a.batched, a.batched_steps, 3, a.leaf)
+    if a.precision > 0:
+        line["precision"] = precision_study(a.precision, a.precision_targets, a.cpu_seconds)
     if a.served > 0:
         line["served"] = served_run(a.served, a.served_rate, a.served_resident, a.leaf)
     if not a.no_cpu:

@@ -133,6 +133,15 @@ class Engine:
     def compute_accels(self, p):
         L.check(L.lib().gcs_compute_accels(self._h, ctypes.byref(p)))
 
+    def autotune(self, p, reps=3):
+        """Fastest traversal settings for the current state (gcs_autotune):
+        returns (params, ms per accelerations-only pass)."""
+        best = L.Params()
+        ms = ctypes.c_float()
+        L.check(L.lib().gcs_autotune(self._h, ctypes.byref(p), int(reps), ctypes.byref(best),
+                                     ctypes.byref(ms)))
+        return best, ms.value
+
     def accels(self):
         out = np.zeros((self.count()[1], 4), np.float32)
         L.check(L.lib().gcs_read_accels(self._h, L.ptr(out)))

@@ -146,6 +146,16 @@ int gcs_sync(gcs_engine* e);
 /* Runs bounds, Morton, sort, permute, tree build, aggregation and traversal
  * without integrating; accelerations are then readable. */
 int gcs_compute_accels(gcs_engine* e, const gcs_params* p);
+/* Autotuner (no counterpart in the reference).  Times accelerations-only
+ * passes of the current state (positions are not advanced; the state is
+ * re-sorted as every step does) for each candidate setting and writes the
+ * fastest into *best (base with those fields replaced) and its mean time per
+ * pass in ms into *best_ms.  Group traversal: leaf_size {8, 12, 16} x
+ * block_size {64, 128, 256} x waves_per_simd {2, 3, 4}; reference traversal:
+ * stack_depth {64, 128, 256, 1024}, skipping depths that overflow.  reps
+ * timed passes per candidate after one warm-up pass, 1 <= reps <= 100. */
+int gcs_autotune(gcs_engine* e, const gcs_params* base, int32_t reps,
+                 gcs_params* best, float* best_ms);
 /* Host n_total*4 floats: ax, ay, az, potential (potential only for
  * GCS_MAC_GROUP), in current particle order. */
 int gcs_read_accels(gcs_engine* e, float* out);

@@ -363,6 +363,29 @@ def test_stack_overflow_is_reported(gcs):
         e.step(gcs.params(G=G, eps2=0.01, dt=0.01), 1)  # the engine still works
 
 
+def test_autotune_picks_a_candidate_and_keeps_the_state(gcs):
+    stars = gcs.collision(8192, 44)
+    p = gcs.params(G=G, eps2=0.0025, dt=0.01, use_graph=1)
+    with gcs.Engine(len(stars)) as e:
+        e.load(stars)
+        say("autotune group")
+        best, ms = e.autotune(p, reps=2)
+        assert ms > 0
+        assert best.leaf_size in (8, 12, 16) and best.block_size in (64, 128, 256)
+        assert best.waves_per_simd in (2, 3, 4) and best.use_graph == 1
+        assert best.theta == p.theta and best.eps2 == p.eps2
+        # positions unchanged (re-sorted only)
+        after = np.sort(e.stars(), order=["x", "y", "z"])
+        before = np.sort(stars, order=["x", "y", "z"])
+        assert np.array_equal(after.view(np.uint8), before.view(np.uint8))
+        e.step(best, 2)
+        say("autotune reference")
+        r = gcs.reference_params(G, 0.0025, 0.01, -120.0, 120.0)
+        best_r, ms_r = e.autotune(r, reps=1)
+        assert best_r.stack_depth in (64, 128, 256, 1024) and ms_r > 0
+        e.step(best_r, 1)
+
+
 def test_bake_driver_writes_reference_frames(gcs, tmp_path):
     exe = os.path.join(os.path.dirname(gcs.LIB_PATH), "gcs_bake")
     out = tmp_path / "simulation.bin"
