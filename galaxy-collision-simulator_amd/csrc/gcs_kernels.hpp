@@ -290,6 +290,7 @@ struct NodeData {
     float cx, cy, cz, m;
     float lx, ly, lz, hx, hy, hz;
     float q0, q1, q2, q3, q4, q5;
+    float qt;  // trace of the raw second moment, sum m |x - COM|^2
 };
 
 __device__ __forceinline__ NodeData leaf_data(float4 p)
@@ -299,7 +300,7 @@ __device__ __forceinline__ NodeData leaf_data(float4 p)
     d.cy = d.ly = d.hy = p.y;
     d.cz = d.lz = d.hz = p.z;
     d.m = p.w;
-    d.q0 = d.q1 = d.q2 = d.q3 = d.q4 = d.q5 = 0.0f;
+    d.q0 = d.q1 = d.q2 = d.q3 = d.q4 = d.q5 = d.qt = 0.0f;
     return d;
 }
 
@@ -314,13 +315,13 @@ __device__ __forceinline__ NodeData load_node_data(int v, const float4* node, co
     d.lx = lo.x; d.ly = lo.y; d.lz = lo.z;
     d.hx = hi.x; d.hy = hi.y; d.hz = hi.z;
     d.q0 = qa.x; d.q1 = qa.y; d.q2 = qa.z; d.q3 = qa.w;
-    d.q4 = qb.x; d.q5 = qb.y;
+    d.q4 = qb.x; d.q5 = qb.y; d.qt = qb.w;
     return d;
 }
 
-// Adds a child's traceless quadrupole about the parent COM (same operation
-// order as the oracle): Q += Qc + mc*(3 s s^T - |s|^2 I), s = child COM -
-// parent COM.
+// Adds a child's second moments about the parent COM (same operation order
+// as the oracle): the traceless quadrupole Q += Qc + mc*(3 s s^T - |s|^2 I)
+// and the trace T += Tc + mc*|s|^2, s = child COM - parent COM.
 __device__ __forceinline__ void quad_shift(NodeData& o, const NodeData& c)
 {
 #pragma clang fp contract(off)
@@ -332,6 +333,7 @@ __device__ __forceinline__ void quad_shift(NodeData& o, const NodeData& c)
     o.q3 = o.q3 + (c.q3 + c.m * (3.0f * sx * sy));
     o.q4 = o.q4 + (c.q4 + c.m * (3.0f * sx * sz));
     o.q5 = o.q5 + (c.q5 + c.m * (3.0f * sy * sz));
+    o.qt = o.qt + (c.qt + c.m * s2);
 }
 
 // Merge of one internal node from its children (karras_bvh.cuh:190-221):
@@ -368,7 +370,7 @@ __device__ __forceinline__ NodeData merge(const NodeData& l, const NodeData& r, 
     } else {
         open = w;
     }
-    o.q0 = o.q1 = o.q2 = o.q3 = o.q4 = o.q5 = 0.0f;
+    o.q0 = o.q1 = o.q2 = o.q3 = o.q4 = o.q5 = o.qt = 0.0f;
     quad_shift(o, l);
     quad_shift(o, r);
     return o;
@@ -381,7 +383,7 @@ __device__ __forceinline__ void store_node(int v, const NodeData& d, float w, fl
     node[2 * v] = make_float4(d.cx, d.cy, d.cz, d.m);
     node[2 * v + 1].x = open;
     nodeQ[2 * v] = make_float4(d.q0, d.q1, d.q2, d.q3);
-    nodeQ[2 * v + 1] = make_float4(d.q4, d.q5, w, 0.0f);
+    nodeQ[2 * v + 1] = make_float4(d.q4, d.q5, w, d.qt);
     boxLo[v] = make_float4(d.lx, d.ly, d.lz, 0.0f);
     boxHi[v] = make_float4(d.hx, d.hy, d.hz, 0.0f);
 }
@@ -415,6 +417,7 @@ __global__ __launch_bounds__(kTile) void k_aggregate(
 {
     __shared__ uint32_t lflag[kTile / 32];
     __shared__ float4 ldat[kTile][4];
+    __shared__ float lqt[kTile];
     __shared__ float4 lpos[kTile];
     const TileInfo t = tiles[blockIdx.x];
     const int tid = threadIdx.x;
@@ -465,6 +468,7 @@ __global__ __launch_bounds__(kTile) void k_aggregate(
         } else {
             const int ss = sp - t0;
             const float4 a = ldat[ss][0], b = ldat[ss][1], c = ldat[ss][2], d = ldat[ss][3];
+            sib.qt = lqt[ss];
             sib.cx = a.x; sib.cy = a.y; sib.cz = a.z; sib.m = a.w;
             sib.lx = b.x; sib.ly = b.y; sib.lz = b.z; sib.hx = b.w;
             sib.hy = c.x; sib.hz = c.y; sib.q0 = c.z; sib.q1 = c.w;
@@ -477,6 +481,7 @@ __global__ __launch_bounds__(kTile) void k_aggregate(
         ldat[slot][1] = make_float4(mine.lx, mine.ly, mine.lz, mine.hx);
         ldat[slot][2] = make_float4(mine.hy, mine.hz, mine.q0, mine.q1);
         ldat[slot][3] = make_float4(mine.q2, mine.q3, mine.q4, mine.q5);
+        lqt[slot] = mine.qt;
         store_node(cur, mine, w, open, node, nodeQ, boxLo, boxHi);
         child = cur;
         cur = par;

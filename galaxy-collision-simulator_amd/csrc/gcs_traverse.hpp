@@ -48,7 +48,7 @@ struct WaveLds {
     int stk[kLdsStack];
     float4 ca[kWave];
     float4 cq0[kWave];
-    float2 cq1[kWave];
+    float4 cq1[kWave];  // Qxz, Qyz, (width), T
 };
 
 struct Acc {
@@ -71,11 +71,14 @@ __device__ __forceinline__ void interact_p(Acc& a, float px, float py, float pz,
     if (POT) a.ph += minv;
 }
 
-// Node source with its traceless quadrupole Q (softened expansion):
-//   a += (M/r^3 + 5/2 (d.Qd)/r^7) d - Qd/r^5,  phi += M/r + (d.Qd)/(2 r^5).
+// Node source with its traceless quadrupole Q and second-moment trace T,
+// expanded to second order in the softened kernel 1/R, R^2 = |d|^2 + eps^2
+// (the kernel's Laplacian -3 eps^2/R^5 brings in T):
+//   a += (M/R^3 + 5/2 (d.Qd - T eps^2)/R^7) d - Qd/R^5,
+//   phi += M/R + (d.Qd - T eps^2)/(2 R^5).
 template <bool POT, bool QUAD>
 __device__ __forceinline__ void interact_n(Acc& a, float px, float py, float pz, float4 c,
-                                           float4 q0, float2 q1, float eps2)
+                                           float4 q0, float4 q1, float eps2)
 {
     const float dx = c.x - px, dy = c.y - py, dz = c.z - pz;
     const float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, __builtin_fmaf(dz, dz, eps2)));
@@ -93,7 +96,8 @@ __device__ __forceinline__ void interact_n(Acc& a, float px, float py, float pz,
     const float qx = __builtin_fmaf(q0.x, dx, __builtin_fmaf(q0.w, dy, q1.x * dz));
     const float qy = __builtin_fmaf(q0.w, dx, __builtin_fmaf(q0.y, dy, q1.y * dz));
     const float qz = __builtin_fmaf(q1.x, dx, __builtin_fmaf(q1.y, dy, q0.z * dz));
-    const float dqd = __builtin_fmaf(dx, qx, __builtin_fmaf(dy, qy, dz * qz));
+    const float dqd =
+        __builtin_fmaf(-q1.w, eps2, __builtin_fmaf(dx, qx, __builtin_fmaf(dy, qy, dz * qz)));
     const float inv3 = inv * inv2;
     const float inv5 = inv3 * inv2;
     const float t = dqd * inv5;
@@ -128,7 +132,7 @@ __device__ __forceinline__ void eval_node_list(WaveLds& L, const int* nl, int k,
         L.ca[l] = A;
         if (QUAD) {
             L.cq0[l] = Q0;
-            L.cq1[l] = make_float2(Q1.x, Q1.y);
+            L.cq1[l] = Q1;
         }
         __builtin_amdgcn_wave_barrier();
         A = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -146,7 +150,7 @@ __device__ __forceinline__ void eval_node_list(WaveLds& L, const int* nl, int k,
             const int e0 = t * ks + h, e1 = e0 + ks;
             const float4 c0 = L.ca[e0], c1 = L.ca[e1];
             float4 q00 = make_float4(0, 0, 0, 0), q01 = q00;
-            float2 q10 = make_float2(0, 0), q11 = q10;
+            float4 q10 = q00, q11 = q00;
             if (QUAD) {
                 q00 = L.cq0[e0]; q01 = L.cq0[e1];
                 q10 = L.cq1[e0]; q11 = L.cq1[e1];
@@ -157,7 +161,7 @@ __device__ __forceinline__ void eval_node_list(WaveLds& L, const int* nl, int k,
         if (t < trips) {
             const int e0 = t * ks + h;
             float4 q0 = make_float4(0, 0, 0, 0);
-            float2 q1 = make_float2(0, 0);
+            float4 q1 = q0;
             if (QUAD) { q0 = L.cq0[e0]; q1 = L.cq1[e0]; }
             interact_n<POT, QUAD>(a, px, py, pz, L.ca[e0], q0, q1, eps2);
         }

@@ -299,10 +299,11 @@ void orc_node_ranges(int32_t n, const int32_t* left, const int32_t* right,
     free(order);
 }
 
-/* Adds a child's traceless quadrupole moment about the parent's COM:
- * Q += Qc + mc*(3 s s^T - |s|^2 I), s = child COM - parent COM, with the
- * components ordered xx, yy, zz, xy, xz, yz.  Same operation order as the
- * HIP aggregation kernel. */
+/* Adds a child's second moments about the parent's COM: the traceless
+ * quadrupole Q += Qc + mc*(3 s s^T - |s|^2 I) and the trace of the raw second
+ * moment T += Tc + mc*|s|^2, s = child COM - parent COM, with the components
+ * ordered xx, yy, zz, xy, xz, yz, T.  Same operation order as the HIP
+ * aggregation kernel. */
 static void quad_shift(float* q, const float* qc, const float* cc, const float* cp)
 {
     float sx = cc[0] - cp[0], sy = cc[1] - cp[1], sz = cc[2] - cp[2];
@@ -314,6 +315,7 @@ static void quad_shift(float* q, const float* qc, const float* cc, const float* 
     q[3] = q[3] + (qc[3] + mc * (3.0f * sx * sy));
     q[4] = q[4] + (qc[4] + mc * (3.0f * sx * sz));
     q[5] = q[5] + (qc[5] + mc * (3.0f * sy * sz));
+    q[6] = q[6] + (qc[6] + mc * s2);
 }
 
 /* karras_bvh.cuh:134-164 (leaves) and :175-225 (internal merge).  The merge
@@ -338,7 +340,7 @@ void orc_aggregate(int32_t n, const int32_t* left, const int32_t* right,
         width[v] = 0.0f;
         crit[v] = 0.0f;
         if (quad)
-            for (int c = 0; c < 6; ++c) quad[6 * v + c] = 0.0f;
+            for (int c = 0; c < 7; ++c) quad[7 * v + c] = 0.0f;
         bmin[3 * v + 0] = bmax[3 * v + 0] = x[i];
         bmin[3 * v + 1] = bmax[3 * v + 1] = y[i];
         bmin[3 * v + 2] = bmax[3 * v + 2] = z[i];
@@ -372,10 +374,10 @@ void orc_aggregate(int32_t n, const int32_t* left, const int32_t* right,
         float ro = w * inv_theta + off;
         crit[v] = ro * ro;
         if (quad) {
-            float q[6] = {0, 0, 0, 0, 0, 0};
-            quad_shift(q, &quad[6 * l], &com4[4 * l], &com4[4 * v]);
-            quad_shift(q, &quad[6 * r], &com4[4 * r], &com4[4 * v]);
-            for (int c = 0; c < 6; ++c) quad[6 * v + c] = q[c];
+            float q[7] = {0, 0, 0, 0, 0, 0, 0};
+            quad_shift(q, &quad[7 * l], &com4[4 * l], &com4[4 * v]);
+            quad_shift(q, &quad[7 * r], &com4[4 * r], &com4[4 * v]);
+            for (int c = 0; c < 7; ++c) quad[7 * v + c] = q[c];
         }
     }
     free(order);
@@ -488,9 +490,9 @@ void orc_traverse_group(int32_t n, const float* x, const float* y,
                 box_dist2(lo, hi, com4[4 * v], com4[4 * v + 1], com4[4 * v + 2]) > crit[v]) {
                 double cx = com4[4 * v], cy = com4[4 * v + 1], cz = com4[4 * v + 2];
                 double mm = com4[4 * v + 3];
-                double q[6] = {0, 0, 0, 0, 0, 0};
+                double q[7] = {0, 0, 0, 0, 0, 0, 0};
                 if (quad)
-                    for (int c = 0; c < 6; ++c) q[c] = quad[6 * (int64_t)v + c];
+                    for (int c = 0; c < 7; ++c) q[c] = quad[7 * (int64_t)v + c];
                 for (int32_t i = g0; i < g1; ++i) {
                     double dx = cx - (double)x[i], dy = cy - (double)y[i], dz = cz - (double)z[i];
                     double r2 = dx * dx + dy * dy + dz * dz + e2;
@@ -499,7 +501,10 @@ void orc_traverse_group(int32_t n, const float* x, const float* y,
                     double qx = q[0] * dx + q[3] * dy + q[4] * dz;
                     double qy = q[3] * dx + q[1] * dy + q[5] * dz;
                     double qz = q[4] * dx + q[5] * dy + q[2] * dz;
-                    double dqd = dx * qx + dy * qy + dz * qz;
+                    /* Softened kernel: the Laplacian of 1/sqrt(r^2+eps^2) is
+                     * -3 eps^2/R^5, so the trace of the second moment adds
+                     * -T eps^2 next to d.Q.d. */
+                    double dqd = dx * qx + dy * qy + dz * qz - q[6] * e2;
                     double f = mm * inv3 + 2.5 * dqd * inv5 * inv2;
                     acc[4 * i + 0] += Gd * (dx * f - qx * inv5);
                     acc[4 * i + 1] += Gd * (dy * f - qy * inv5);

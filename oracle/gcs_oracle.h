@@ -46,8 +46,9 @@ void orc_node_ranges(int32_t n, const int32_t* left, const int32_t* right,
 /* ---- Bottom-up aggregation (karras_bvh.cuh:134-225) ---------------------
  * com4: (2n-1)*4 floats (com x,y,z, mass); width: 2n-1; bmin/bmax: (2n-1)*3;
  * crit: (2n-1) opening radius squared (w*inv_theta + |com - box centre|)^2,
- * 0 for leaves.  quad (optional, (2n-1)*6): traceless quadrupole moment
- * about the COM, components xx, yy, zz, xy, xz, yz (0 for leaves). */
+ * 0 for leaves.  quad (optional, (2n-1)*7): traceless quadrupole moment
+ * about the COM, components xx, yy, zz, xy, xz, yz, then the trace T of the
+ * raw second moment sum m |x - COM|^2 (0 for leaves). */
 void orc_aggregate(int32_t n, const int32_t* left, const int32_t* right,
                    const float* x, const float* y, const float* z,
                    const float* m, float inv_theta, float* com4, float* width,
@@ -72,7 +73,8 @@ int64_t orc_traverse_reference(int32_t n, const float* x, const float* y,
  * bounding box exceeds crit[node]; otherwise it is summed directly when it
  * holds <= leaf particles, else opened.  Index self-exclusion.  acc: n*4
  * doubles (ax, ay, az, phi).  With quad non-null accepted nodes add their
- * quadrupole term.  counts[0] += node (multipole) interactions,
+ * quadrupole term of the softened potential, whose second-order part is
+ * (d.Q.d - T eps^2)/(2 R^5), R^2 = |d|^2 + eps^2.  counts[0] += node (multipole) interactions,
  * counts[1] += particle-particle interactions (per target particle, self
  * excluded);
  * msum (optional, n doubles) receives the mass each target interacted with

@@ -136,7 +136,7 @@ def aggregate(left, right, x, y, z, m, theta):
     bmin = np.zeros((k, 3), np.float32)
     bmax = np.zeros((k, 3), np.float32)
     crit = np.zeros(k, np.float32)
-    quad = np.zeros((k, 6), np.float32)
+    quad = np.zeros((k, 7), np.float32)
     inv_theta = np.float32(np.inf) if theta == 0 else np.float32(1.0) / np.float32(theta)
     lib().orc_aggregate(n, P(left), P(right), P(x), P(y), P(z), P(m), float(inv_theta), P(com4),
                         P(width), P(bmin), P(bmax), P(crit), P(quad))

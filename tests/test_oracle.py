@@ -197,7 +197,8 @@ def test_aggregate_against_fp64_subtree_sums(orc):
         s2 = (s * s).sum(1)
         q = np.array([(mm * (3 * s[:, 0] ** 2 - s2)).sum(), (mm * (3 * s[:, 1] ** 2 - s2)).sum(),
                       (mm * (3 * s[:, 2] ** 2 - s2)).sum(), (mm * 3 * s[:, 0] * s[:, 1]).sum(),
-                      (mm * 3 * s[:, 0] * s[:, 2]).sum(), (mm * 3 * s[:, 1] * s[:, 2]).sum()])
+                      (mm * 3 * s[:, 0] * s[:, 2]).sum(), (mm * 3 * s[:, 1] * s[:, 2]).sum(),
+                      (mm * s2).sum()])
         scale = (mm * s2).sum() * 3 + 1e-12
         assert np.all(np.abs(agg["quad"][v] - q) <= 1e-4 * scale)
     assert abs(agg["com4"][0, 3] - m.astype(np.float64).sum()) <= 1e-5 * m.sum()
@@ -268,6 +269,43 @@ def test_group_traversal_accuracy_quadrupole(orc):
     assert eq < 1e-3
     assert eq < em  # the quadrupole term reduces the error
     assert rel_rms(acc_q[:, 3], ref[:, 3]) < 1e-4
+
+
+def _multipole_error(orc, h, eps, D=0.1, k=12, seed=3):
+    """Acceleration at a massless target a distance D from a random cluster of
+    size h, from the cluster's root multipole (accepted at once: theta is
+    huge), against the exact softened sum over the cluster (FP64)."""
+    r = np.random.default_rng(seed)
+    pts = r.uniform(-1.0, 1.0, (k, 3)) * h
+    st = np.zeros(k + 1, orc.STAR)
+    st["x"][:k], st["y"][:k], st["z"][:k] = pts[:, 0], pts[:, 1], pts[:, 2]
+    st["mass"][:k] = r.uniform(0.5, 1.5, k)
+    st["x"][k], st["y"][k], st["z"][k] = D * 0.6, D * 0.8, 0.0
+    t = orc.build_tree_pipeline(st, 1e30)
+    s = t["sorted"]
+    i = int(np.nonzero(s["mass"] == 0)[0][0])
+    eps2 = float(np.float32(eps) ** 2)
+    acc, _, cnt = orc.traverse_group(t["left"], t["right"], t["first"], t["last"], s["x"], s["y"],
+                                     s["z"], s["mass"], t["agg"], G, eps2, 64, 1,
+                                     groups=np.array([[i, 1]], np.int32))
+    assert cnt[0] == 1 and cnt[1] == 0  # one multipole interaction
+    d = np.stack([s["x"], s["y"], s["z"]], 1).astype(np.float64)
+    dd = d - d[i]
+    R2 = (dd ** 2).sum(1) + eps2
+    exact = (G * s["mass"][:, None] * dd / R2[:, None] ** 1.5).sum(0)
+    return np.linalg.norm(acc[i, :3] - exact) / np.linalg.norm(exact)
+
+
+def test_softened_multipole_is_exact_to_second_order(orc):
+    """With the trace term the expansion of the softened potential is exact
+    through second order, so halving the cluster size cuts the error by about
+    8 (third-order remainder); without it the error would fall only by about
+    4, the missing T*eps^2 term being second order.  Softening comparable to
+    the distance makes the difference visible."""
+    for eps in (0.05, 0.1):
+        e1 = _multipole_error(orc, 0.004, eps)
+        e2 = _multipole_error(orc, 0.002, eps)
+        assert e1 / e2 > 6.5, (eps, e1, e2)
 
 
 @pytest.mark.parametrize("gmax", [1, 16, 64])
