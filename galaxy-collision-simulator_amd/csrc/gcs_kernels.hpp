@@ -490,16 +490,104 @@ __global__ __launch_bounds__(kTile) void k_aggregate(
     }
 }
 
+// One climber of the top pass, from the completed node `child`: at each
+// parent the first of the two arrivals stops and the second merges and
+// goes on, for at most `levels` levels.  Returns the last node it completed
+// when it stopped because of the level limit (with more levels to go), else
+// -1.
+__device__ __forceinline__ int top_climb(const SegInfo& sg, int child, int levels,
+                                         const float4* __restrict__ posm, float4* node,
+                                         float4* nodeQ, float4* boxLo, float4* boxHi,
+                                         const int* __restrict__ parent, int* flag,
+                                         uint32_t* lbits, bool in_lds, float inv_theta, int mode)
+{
+    const bool cleaf = child >= sg.start + sg.leaf_base;
+    NodeData mine = cleaf ? leaf_data(posm[child - sg.leaf_base])
+                          : load_node_data(child, node, nodeQ, boxLo, boxHi);
+    int cur = parent[child];
+    if (cur == -1) return -1;
+    float4 bb = node[2 * cur + 1];
+    int par = parent[cur];
+    for (int lv = 0;; ++lv) {
+        if (lv == levels) return child;
+        const int first = __float_as_int(bb.y), last = __float_as_int(bb.z),
+                  split = __float_as_int(bb.w);
+        const bool lleaf = first == split, rleaf = last == split + 1;
+        const int lidx = lleaf ? split + sg.leaf_base : split + sg.int_base;
+        const bool mine_left = child == lidx;
+        const bool sleaf = mine_left ? rleaf : lleaf;
+        const int sp = mine_left ? split + 1 : split;
+        // A sibling whose range lies in one tile (or a leaf) was completed by
+        // k_aggregate: load it before the arrival check.  A tile-crossing
+        // sibling is written in this launch and is read after the acquire.
+        const int sf = mine_left ? split + 1 : first, sl = mine_left ? last : split;
+        const bool sfinal = sleaf || ((sf - sg.start) / kTile) == ((sl - sg.start) / kTile);
+        NodeData sib;
+        if (sfinal)
+            sib = sleaf ? leaf_data(posm[sp])
+                        : load_node_data(sp + sg.int_base, node, nodeQ, boxLo, boxHi);
+        float4 bb_next = make_float4(0.f, 0.f, 0.f, 0.f);
+        int par_next = -1;
+        if (par != -1 && lv + 1 < levels) {
+            bb_next = node[2 * par + 1];
+            par_next = parent[par];
+        }
+        // Arrival: a thread that sees the sibling's mark already set is the
+        // second and continues without publishing a mark of its own; otherwise
+        // it marks the node with a release (its record is then visible to the
+        // sibling's thread).
+        bool second;
+        if (in_lds) {
+            const int slot = cur - sg.int_base - sg.start;
+            uint32_t* w = &lbits[slot >> 5];
+            const uint32_t bit = 1u << (slot & 31);
+            second =
+                (__hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) & bit) != 0u;
+            if (!second)
+                second = (__hip_atomic_fetch_or(w, bit, __ATOMIC_ACQ_REL,
+                                                __HIP_MEMORY_SCOPE_WORKGROUP) &
+                          bit) != 0u;
+        } else {
+            second = __hip_atomic_load(&flag[cur], __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_WORKGROUP) != 0;
+            if (!second)
+                second = __hip_atomic_fetch_add(&flag[cur], 1, __ATOMIC_ACQ_REL,
+                                                __HIP_MEMORY_SCOPE_WORKGROUP) != 0;
+        }
+        if (!second) return -1;
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+        if (!sfinal) sib = load_node_data(sp + sg.int_base, node, nodeQ, boxLo, boxHi);
+        float w, open;
+        mine = mine_left ? merge(mine, sib, inv_theta, mode, w, open)
+                         : merge(sib, mine, inv_theta, mode, w, open);
+        store_node(cur, mine, w, open, node, nodeQ, boxLo, boxHi);
+        if (par == -1) return -1;
+        child = cur;
+        cur = par;
+        if (lv + 1 < levels) {
+            bb = bb_next;
+            par = par_next;
+        }
+    }
+}
+
 // Instances of up to this many particles keep the top pass's arrival bits in
 // LDS (128 KiB); larger ones use the global arrival counters.
 constexpr int kTopLdsNodes = 1 << 20;
 
+// While the frontier holds more climbers than the block has threads, every
+// climber advances one level per round and the ones that go on (second
+// arrivals) form the next, much shorter list; then each remaining climber
+// runs to the end.  A thread thus never holds back climbers that have not
+// started behind one long climb.  The lists live in the instance's 2n-entry
+// frontier area: the current one in the first half, the next in the second.
 __global__ __launch_bounds__(1024) void k_aggregate_top(
     const SegInfo* __restrict__ segs, const float4* __restrict__ posm, float4* node,
     float4* nodeQ, float4* boxLo, float4* boxHi, const int* __restrict__ parent, int* flag,
-    const int* __restrict__ front, const int* __restrict__ nfront, float inv_theta, int mode)
+    int* front, const int* __restrict__ nfront, float inv_theta, int mode)
 {
     __shared__ uint32_t lbits[kTopLdsNodes / 32];
+    __shared__ int lcount;
     const int s = blockIdx.x;
     const int nf = nfront[s];
     if (nf == 0) return;
@@ -507,75 +595,28 @@ __global__ __launch_bounds__(1024) void k_aggregate_top(
     const bool in_lds = sg.n <= kTopLdsNodes;
     if (in_lds)
         for (int k = threadIdx.x; k < (sg.n + 31) / 32; k += blockDim.x) lbits[k] = 0u;
-    __syncthreads();
-    for (int k = threadIdx.x; k < nf; k += blockDim.x) {
-        int child = front[2 * sg.start + k];
-        const bool cleaf = child >= sg.start + sg.leaf_base;
-        NodeData mine = cleaf ? leaf_data(posm[child - sg.leaf_base])
-                              : load_node_data(child, node, nodeQ, boxLo, boxHi);
-        int cur = parent[child];
-        float4 bb = node[2 * cur + 1];
-        int par = parent[cur];
-        while (cur != -1) {
-            const int first = __float_as_int(bb.y), last = __float_as_int(bb.z),
-                      split = __float_as_int(bb.w);
-            const bool lleaf = first == split, rleaf = last == split + 1;
-            const int lidx = lleaf ? split + sg.leaf_base : split + sg.int_base;
-            const bool mine_left = child == lidx;
-            const bool sleaf = mine_left ? rleaf : lleaf;
-            const int sp = mine_left ? split + 1 : split;
-            // A sibling whose range lies in one tile (or a leaf) was completed
-            // by k_aggregate: load it before the arrival check.  A tile-crossing
-            // sibling is written in this launch and is read after the acquire.
-            const int sf = mine_left ? split + 1 : first, sl = mine_left ? last : split;
-            const bool sfinal =
-                sleaf || ((sf - sg.start) / kTile) == ((sl - sg.start) / kTile);
-            NodeData sib;
-            if (sfinal)
-                sib = sleaf ? leaf_data(posm[sp])
-                            : load_node_data(sp + sg.int_base, node, nodeQ, boxLo, boxHi);
-            float4 bb_next = make_float4(0.f, 0.f, 0.f, 0.f);
-            int par_next = -1;
-            if (par != -1) {
-                bb_next = node[2 * par + 1];
-                par_next = parent[par];
-            }
-            // Arrival: a thread that sees the sibling's mark already set is the
-            // second and continues without publishing its own node (no one
-            // else reads it in this launch), so the climb that finishes last
-            // never waits for its stores.  Otherwise it marks the node with a
-            // release (its record is then visible to the sibling's thread).
-            bool second;
-            if (in_lds) {
-                const int slot = cur - sg.int_base - sg.start;
-                uint32_t* w = &lbits[slot >> 5];
-                const uint32_t bit = 1u << (slot & 31);
-                second = (__hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) &
-                          bit) != 0u;
-                if (!second)
-                    second = (__hip_atomic_fetch_or(w, bit, __ATOMIC_ACQ_REL,
-                                                    __HIP_MEMORY_SCOPE_WORKGROUP) &
-                              bit) != 0u;
-            } else {
-                second = __hip_atomic_load(&flag[cur], __ATOMIC_RELAXED,
-                                           __HIP_MEMORY_SCOPE_WORKGROUP) != 0;
-                if (!second)
-                    second = __hip_atomic_fetch_add(&flag[cur], 1, __ATOMIC_ACQ_REL,
-                                                    __HIP_MEMORY_SCOPE_WORKGROUP) != 0;
-            }
-            if (!second) break;
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-            if (!sfinal) sib = load_node_data(sp + sg.int_base, node, nodeQ, boxLo, boxHi);
-            float w, open;
-            mine = mine_left ? merge(mine, sib, inv_theta, mode, w, open)
-                             : merge(sib, mine, inv_theta, mode, w, open);
-            store_node(cur, mine, w, open, node, nodeQ, boxLo, boxHi);
-            child = cur;
-            cur = par;
-            bb = bb_next;
-            par = par_next;
+    int* la = front + 2 * (size_t)sg.start;
+    int* lb = la + sg.n;
+    int count = nf;
+    while (count > (int)blockDim.x) {
+        if (threadIdx.x == 0) lcount = 0;
+        __syncthreads();
+        for (int k = threadIdx.x; k < count; k += blockDim.x) {
+            const int v = top_climb(sg, la[k], 1, posm, node, nodeQ, boxLo, boxHi, parent, flag,
+                                    lbits, in_lds, inv_theta, mode);
+            if (v >= 0) lb[atomicAdd(&lcount, 1)] = v;
         }
+        __syncthreads();
+        count = lcount;
+        int* t = la;
+        la = lb;
+        lb = t;
+        __syncthreads();  // every thread has read lcount before the next reset
     }
+    __syncthreads();
+    for (int k = threadIdx.x; k < count; k += blockDim.x)
+        top_climb(sg, la[k], 1 << 30, posm, node, nodeQ, boxLo, boxHi, parent, flag, lbits,
+                  in_lds, inv_theta, mode);
 }
 
 __device__ __forceinline__ uint8_t to_u8(float f) { return (uint8_t)f2u_sat(f); }
