@@ -490,6 +490,16 @@ __global__ __launch_bounds__(kTile) void k_aggregate(
     }
 }
 
+// Arrival marks of the top pass: bits in LDS indexed by node (instances of
+// up to kTopLdsNodes particles), an LDS hash of the tile-crossing nodes (at
+// most kTopHashMax of them: the table stays at most 3/4 full), or counters in
+// global memory.
+constexpr int kTopLdsNodes = 1 << 20;
+constexpr int kTopHashBits = 15;
+constexpr int kTopHashMax = 3 << (kTopHashBits - 2);
+enum : int { kMarksBits = 0, kMarksHash = 1, kMarksGlobal = 2 };
+static_assert((1 << kTopHashBits) == kTopLdsNodes / 32, "hash and bitmap share the LDS array");
+
 // One climber of the top pass, from the completed node `child`: at each
 // parent the first of the two arrivals stops and the second merges and
 // goes on, for at most `levels` levels.  Returns the last node it completed
@@ -499,7 +509,7 @@ __device__ __forceinline__ int top_climb(const SegInfo& sg, int child, int level
                                          const float4* __restrict__ posm, float4* node,
                                          float4* nodeQ, float4* boxLo, float4* boxHi,
                                          const int* __restrict__ parent, int* flag,
-                                         uint32_t* lbits, bool in_lds, float inv_theta, int mode)
+                                         uint32_t* lmem, int marks, float inv_theta, int mode)
 {
     const bool cleaf = child >= sg.start + sg.leaf_base;
     NodeData mine = cleaf ? leaf_data(posm[child - sg.leaf_base])
@@ -537,9 +547,9 @@ __device__ __forceinline__ int top_climb(const SegInfo& sg, int child, int level
         // it marks the node with a release (its record is then visible to the
         // sibling's thread).
         bool second;
-        if (in_lds) {
+        if (marks == kMarksBits) {
             const int slot = cur - sg.int_base - sg.start;
-            uint32_t* w = &lbits[slot >> 5];
+            uint32_t* w = &lmem[slot >> 5];
             const uint32_t bit = 1u << (slot & 31);
             second =
                 (__hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) & bit) != 0u;
@@ -547,6 +557,33 @@ __device__ __forceinline__ int top_climb(const SegInfo& sg, int child, int level
                 second = (__hip_atomic_fetch_or(w, bit, __ATOMIC_ACQ_REL,
                                                 __HIP_MEMORY_SCOPE_WORKGROUP) &
                           bit) != 0u;
+        } else if (marks == kMarksHash) {
+            // Open addressing on the node index: the first arrival claims a
+            // slot with its parent's index, the second finds it.
+            const uint32_t key = (uint32_t)cur;
+            uint32_t slot = (key * 0x9E3779B1u) >> (32 - kTopHashBits);
+            for (;;) {
+                const uint32_t k = __hip_atomic_load(&lmem[slot], __ATOMIC_RELAXED,
+                                                     __HIP_MEMORY_SCOPE_WORKGROUP);
+                if (k == key) {
+                    second = true;
+                    break;
+                }
+                if (k == 0xFFFFFFFFu) {
+                    uint32_t expect = 0xFFFFFFFFu;
+                    if (__hip_atomic_compare_exchange_strong(
+                            &lmem[slot], &expect, key, __ATOMIC_ACQ_REL, __ATOMIC_ACQUIRE,
+                            __HIP_MEMORY_SCOPE_WORKGROUP)) {
+                        second = false;
+                        break;
+                    }
+                    if (expect == key) {
+                        second = true;
+                        break;
+                    }
+                }
+                slot = (slot + 1u) & ((1u << kTopHashBits) - 1u);
+            }
         } else {
             second = __hip_atomic_load(&flag[cur], __ATOMIC_RELAXED,
                                        __HIP_MEMORY_SCOPE_WORKGROUP) != 0;
@@ -571,10 +608,6 @@ __device__ __forceinline__ int top_climb(const SegInfo& sg, int child, int level
     }
 }
 
-// Instances of up to this many particles keep the top pass's arrival bits in
-// LDS (128 KiB); larger ones use the global arrival counters.
-constexpr int kTopLdsNodes = 1 << 20;
-
 // While the frontier holds more climbers than the block has threads, every
 // climber advances one level per round and the ones that go on (second
 // arrivals) form the next, much shorter list; then each remaining climber
@@ -586,15 +619,20 @@ __global__ __launch_bounds__(1024) void k_aggregate_top(
     float4* nodeQ, float4* boxLo, float4* boxHi, const int* __restrict__ parent, int* flag,
     int* front, const int* __restrict__ nfront, float inv_theta, int mode)
 {
-    __shared__ uint32_t lbits[kTopLdsNodes / 32];
+    __shared__ uint32_t lmem[kTopLdsNodes / 32];
     __shared__ int lcount;
     const int s = blockIdx.x;
     const int nf = nfront[s];
     if (nf == 0) return;
     const SegInfo sg = segs[s];
-    const bool in_lds = sg.n <= kTopLdsNodes;
-    if (in_lds)
-        for (int k = threadIdx.x; k < (sg.n + 31) / 32; k += blockDim.x) lbits[k] = 0u;
+    // nf - 1 tile-crossing nodes get a mark.
+    const int marks = sg.n <= kTopLdsNodes ? kMarksBits
+                      : nf <= kTopHashMax  ? kMarksHash
+                                           : kMarksGlobal;
+    if (marks == kMarksBits)
+        for (int k = threadIdx.x; k < (sg.n + 31) / 32; k += blockDim.x) lmem[k] = 0u;
+    else if (marks == kMarksHash)
+        for (int k = threadIdx.x; k < (1 << kTopHashBits); k += blockDim.x) lmem[k] = 0xFFFFFFFFu;
     int* la = front + 2 * (size_t)sg.start;
     int* lb = la + sg.n;
     int count = nf;
@@ -603,7 +641,7 @@ __global__ __launch_bounds__(1024) void k_aggregate_top(
         __syncthreads();
         for (int k = threadIdx.x; k < count; k += blockDim.x) {
             const int v = top_climb(sg, la[k], 1, posm, node, nodeQ, boxLo, boxHi, parent, flag,
-                                    lbits, in_lds, inv_theta, mode);
+                                    lmem, marks, inv_theta, mode);
             if (v >= 0) lb[atomicAdd(&lcount, 1)] = v;
         }
         __syncthreads();
@@ -615,8 +653,8 @@ __global__ __launch_bounds__(1024) void k_aggregate_top(
     }
     __syncthreads();
     for (int k = threadIdx.x; k < count; k += blockDim.x)
-        top_climb(sg, la[k], 1 << 30, posm, node, nodeQ, boxLo, boxHi, parent, flag, lbits,
-                  in_lds, inv_theta, mode);
+        top_climb(sg, la[k], 1 << 30, posm, node, nodeQ, boxLo, boxHi, parent, flag, lmem, marks,
+                  inv_theta, mode);
 }
 
 __device__ __forceinline__ uint8_t to_u8(float f) { return (uint8_t)f2u_sat(f); }

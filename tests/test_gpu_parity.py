@@ -221,6 +221,22 @@ def test_c2_full_size(gcs, orc):
         say("C2 5 steps ok")
 
 
+def test_large_instance_aggregates(gcs, orc):
+    """An instance of more than 2^20 particles: the top aggregation pass keeps
+    its arrival marks in an LDS hash instead of a bitmap.  Topology and
+    aggregates bit-exact against the oracle."""
+    stars = gcs.collision(600000, 1006)
+    n = len(stars)
+    eps2 = float(np.float32(0.02) ** 2)
+    p = gcs.params(G=G, eps2=eps2, dt=0.005, theta=0.5, leaf_size=8)
+    with gcs.Engine(n) as e:
+        e.load(stars)
+        say("large instance compute n=%d" % n)
+        e.compute_accels(p)
+        check_pipeline(gcs, orc, e, stars, p)
+        e.step(p, 2)
+
+
 # ---- forests, graphs, composition ----------------------------------------------------
 
 def test_forest_equals_standalone(gcs, orc):
