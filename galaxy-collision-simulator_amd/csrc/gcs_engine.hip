@@ -129,6 +129,9 @@ struct gcs_engine {
     unsigned* err = nullptr;
     double* diag_part = nullptr;
     double* diag_out = nullptr;
+    float* amax_part = nullptr;  // per quarter tile: largest |a|
+    float* dt_inst = nullptr;    // per instance: adaptive step
+    double* tsim = nullptr;      // per instance: simulated time
     int* slot_mem = nullptr;  // per traversal wave: stack spill, node and particle lists
     uint8_t* staging = nullptr;
 
@@ -190,7 +193,7 @@ void free_all(gcs_engine* e)
     dfree(e->gmark); dfree(e->gq); dfree(e->gchunk); dfree(e->gcnt); dfree(e->ngroups);
     dfree(e->front); dfree(e->nfront);
     dfree(e->seg_bounds); dfree(e->bpart); dfree(e->qctr); dfree(e->counters); dfree(e->err);
-    dfree(e->diag_part); dfree(e->diag_out); dfree(e->slot_mem); dfree(e->staging);
+    dfree(e->diag_part); dfree(e->diag_out); dfree(e->amax_part); dfree(e->dt_inst); dfree(e->tsim); dfree(e->slot_mem); dfree(e->staging);
     for (auto& v : e->ev)
         if (v) (void)hipEventDestroy(v), v = nullptr;
 }
@@ -263,6 +266,10 @@ int check_params(const gcs_params* p)
     }
     if (p->precision != GCS_PREC_PLAIN && p->precision != GCS_PREC_COMPENSATED)
         return fail(GCS_ERR_INVALID, "unknown precision mode");
+    if (!std::isfinite(p->dt_eta) || p->dt_eta < 0.0f)
+        return fail(GCS_ERR_INVALID, "dt_eta must be >= 0");
+    if (p->dt_eta > 0.0f && !(p->eps2 > 0.0f))
+        return fail(GCS_ERR_INVALID, "the adaptive step needs eps2 > 0");
     if (p->use_bounds && !(std::isfinite(p->global_min) && std::isfinite(p->global_max)))
         return fail(GCS_ERR_INVALID, "non-finite bounds");
     return GCS_OK;
@@ -397,9 +404,17 @@ int enqueue_step(gcs_engine* e, const gcs_params& p, bool integrate)
     LAUNCHCHK();
     mark(e, 6);
     if (integrate) {
+        const bool adaptive = p.dt_eta > 0.0f;
+        if (adaptive) {
+            k_accmax<<<qblocks, kTileThreads, 0, e->stream>>>(e->tiles, e->segs, e->acc,
+                                                               e->amax_part);
+            k_adapt_dt<<<nseg, 256, 0, e->stream>>>(e->segs, e->amax_part, p.dt, p.dt_eta,
+                                                     std::sqrt(p.eps2), e->dt_inst);
+        }
         k_integrate<<<qblocks, kTileThreads, 0, e->stream>>>(
-            e->tiles, e->segs, e->acc, posm, e->vel[e->cur], e->plo[e->cur], p.dt, comp ? 1 : 0,
-            p.colours, p.diagnostics, e->diag_part, e->err);
+            e->tiles, e->segs, e->acc, posm, e->vel[e->cur], e->plo[e->cur], p.dt,
+            adaptive ? e->dt_inst : nullptr, comp ? 1 : 0, p.colours, p.diagnostics,
+            e->diag_part, e->amax_part, e->tsim, e->err);
         if (p.diagnostics)
             k_diag_final<<<nseg, 64, 0, e->stream>>>(e->segs, nseg, e->diag_part, e->diag_out);
         LAUNCHCHK();
@@ -511,6 +526,7 @@ int64_t gcs_device_bytes(int64_t cap_n, int32_t cap_k)
                    4 * (size_t)kSortPasses * kRadix * max_chunks(cap_n, cap_k));
     b += (int64_t)(4 * 4 * max_tiles(cap_n, cap_k) + 4 * max_gchunks(cap_n, cap_k));
     b += 8 * cap_k + 64 + 5 * 8 * 4 * (int64_t)max_tiles(cap_n, cap_k) + 5 * 8 * cap_k;
+    b += 4 * 4 * (int64_t)max_tiles(cap_n, cap_k) + 12 * (int64_t)cap_k;  // amax, dt, time
     b += 32 * n;               // staging
     return b;
 }
@@ -578,6 +594,9 @@ int gcs_create(int64_t cap_n, int32_t cap_k, int32_t device, gcs_engine** out)
     if (!rc) rc = dalloc(&e->err, 1);
     if (!rc) rc = dalloc(&e->diag_part, 5 * 4 * nt);
     if (!rc) rc = dalloc(&e->diag_out, 5 * (size_t)cap_k);
+    if (!rc) rc = dalloc(&e->amax_part, 4 * nt);
+    if (!rc) rc = dalloc(&e->dt_inst, (size_t)cap_k);
+    if (!rc) rc = dalloc(&e->tsim, (size_t)cap_k);
     if (!rc) rc = dalloc(&e->slot_mem, (size_t)e->trav_blocks_cap * 4 * kSlotInts);
     if (!rc) rc = dalloc(&e->staging, 32 * n);
     if (rc) {
@@ -655,6 +674,7 @@ int gcs_load(gcs_engine* e, const gcs_star* stars, const int32_t* counts, int32_
     e->cur = 0;
     int rc = upload_at(e, stars, total, 0);
     if (rc) return rc;
+    if (k > 0) HIPCHK(hipMemset(e->tsim, 0, sizeof(double) * (size_t)k));
     return rebuild_tables(e);
 }
 
@@ -667,6 +687,7 @@ int gcs_instance_add(gcs_engine* e, const gcs_star* stars, int32_t n, int32_t* i
     HIPCHK(hipStreamSynchronize(e->stream));
     int rc = upload_at(e, stars, n, e->n_total);
     if (rc) return rc;
+    HIPCHK(hipMemset(e->tsim + e->inst.size(), 0, sizeof(double)));
     const int32_t nid = e->next_id++;
     e->inst.push_back({nid, n});
     if (id) *id = nid;
@@ -699,6 +720,12 @@ int gcs_instance_remove(gcs_engine* e, int32_t id)
         }
         HIPCHK(hipStreamSynchronize(e->stream));
         e->cur = d;
+    }
+    if (slot + 1 < (int)e->inst.size()) {
+        std::vector<double> t(e->inst.size());
+        HIPCHK(hipMemcpy(t.data(), e->tsim, sizeof(double) * t.size(), hipMemcpyDeviceToHost));
+        t.erase(t.begin() + slot);
+        HIPCHK(hipMemcpy(e->tsim, t.data(), sizeof(double) * t.size(), hipMemcpyHostToDevice));
     }
     e->inst.erase(e->inst.begin() + slot);
     return rebuild_tables(e);
@@ -747,6 +774,80 @@ int gcs_read_instance(gcs_engine* e, int32_t id, gcs_star* out)
     const int slot = find_slot(e, id);
     if (slot < 0) return fail(GCS_ERR_NOT_FOUND, "unknown instance id");
     return read_stars_range(e, e->h_segs[slot].start, e->h_segs[slot].n, out);
+}
+
+// ---- Checkpoint / resume of one instance ----------------------------------
+// Layout: 32-byte header (magic, version, n, simulated time, 2 reserved
+// words), n gcs_star
+// records in the instance's current order, then n float4 low parts of the
+// compensated positions (x_lo, y_lo, z_lo, 0).  The current order and the low
+// parts are the whole state a step carries over (the tree, keys and bounds
+// are rebuilt every step), so a restored instance continues bit for bit.
+namespace {
+constexpr char kCkptMagic[8] = {'G', 'C', 'S', 'C', 'K', 'P', 'T', '1'};
+struct CkptHeader {
+    char magic[8];
+    int32_t version;
+    int32_t n;
+    double sim_time;
+    int32_t reserved[2];
+};
+static_assert(sizeof(CkptHeader) == 32, "checkpoint header is 32 bytes");
+}  // namespace
+
+int64_t gcs_checkpoint_bytes(int32_t n)
+{
+    if (n < 1) return 0;
+    return (int64_t)sizeof(CkptHeader) + (int64_t)n * (32 + 16);
+}
+
+int gcs_checkpoint(gcs_engine* e, int32_t id, void* buf, int64_t cap)
+{
+    if (!e || !buf) return fail(GCS_ERR_INVALID, "null argument");
+    const int slot = find_slot(e, id);
+    if (slot < 0) return fail(GCS_ERR_NOT_FOUND, "unknown instance id");
+    const int32_t n = e->h_segs[slot].n;
+    if (cap < gcs_checkpoint_bytes(n)) return fail(GCS_ERR_CAPACITY, "checkpoint buffer too small");
+    CkptHeader h;
+    std::memcpy(h.magic, kCkptMagic, 8);
+    h.version = 1;
+    h.n = n;
+    std::memset(h.reserved, 0, sizeof(h.reserved));
+    HIPCHK(hipSetDevice(e->device));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    HIPCHK(hipMemcpy(&h.sim_time, e->tsim + slot, sizeof(double), hipMemcpyDeviceToHost));
+    uint8_t* out = static_cast<uint8_t*>(buf);
+    std::memcpy(out, &h, sizeof(h));
+    int rc = read_stars_range(e, e->h_segs[slot].start, n, reinterpret_cast<gcs_star*>(out + 32));
+    if (rc) return rc;
+    HIPCHK(hipSetDevice(e->device));
+    HIPCHK(hipMemcpyAsync(out + 32 + 32 * (size_t)n, e->plo[e->cur] + e->h_segs[slot].start,
+                          16 * (size_t)n, hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    return GCS_OK;
+}
+
+int gcs_restore(gcs_engine* e, const void* buf, int64_t bytes, int32_t* id)
+{
+    if (!e || !buf) return fail(GCS_ERR_INVALID, "null argument");
+    if (bytes < (int64_t)sizeof(CkptHeader)) return fail(GCS_ERR_INVALID, "truncated checkpoint");
+    CkptHeader h;
+    std::memcpy(&h, buf, sizeof(h));
+    if (std::memcmp(h.magic, kCkptMagic, 8) != 0 || h.version != 1 || h.n < 1)
+        return fail(GCS_ERR_INVALID, "not a checkpoint");
+    if (bytes < gcs_checkpoint_bytes(h.n)) return fail(GCS_ERR_INVALID, "truncated checkpoint");
+    const uint8_t* in = static_cast<const uint8_t*>(buf);
+    int32_t nid = -1;
+    int rc = gcs_instance_add(e, reinterpret_cast<const gcs_star*>(in + 32), h.n, &nid);
+    if (rc) return rc;
+    const int slot = find_slot(e, nid);
+    HIPCHK(hipMemcpyAsync(e->plo[e->cur] + e->h_segs[slot].start, in + 32 + 32 * (size_t)h.n,
+                          16 * (size_t)h.n, hipMemcpyHostToDevice, e->stream));
+    HIPCHK(hipMemcpyAsync(e->tsim + slot, &h.sim_time, sizeof(double), hipMemcpyHostToDevice,
+                          e->stream));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    if (id) *id = nid;
+    return GCS_OK;
 }
 
 int gcs_read_position_lo(gcs_engine* e, float* out)
@@ -1024,6 +1125,34 @@ int gcs_read_diagnostics(gcs_engine* e, int32_t id, double* out5)
     if (slot < 0) return fail(GCS_ERR_NOT_FOUND, "unknown instance id");
     HIPCHK(hipStreamSynchronize(e->stream));
     HIPCHK(hipMemcpy(out5, e->diag_out + 5 * slot, 5 * sizeof(double), hipMemcpyDeviceToHost));
+    return GCS_OK;
+}
+
+int gcs_read_max_accel(gcs_engine* e, int32_t id, float* amax)
+{
+    if (!e || !amax) return fail(GCS_ERR_INVALID, "null argument");
+    const int slot = find_slot(e, id);
+    if (slot < 0) return fail(GCS_ERR_NOT_FOUND, "unknown instance id");
+    const SegInfo& sg = e->h_segs[slot];
+    std::vector<float> part((size_t)4 * sg.ntiles);
+    HIPCHK(hipSetDevice(e->device));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    HIPCHK(hipMemcpy(part.data(), e->amax_part + 4 * (size_t)sg.tile0, sizeof(float) * part.size(),
+                     hipMemcpyDeviceToHost));
+    float m = 0.0f;
+    for (float v : part) m = std::max(m, v);
+    *amax = m;
+    return GCS_OK;
+}
+
+int gcs_read_sim_time(gcs_engine* e, int32_t id, double* t)
+{
+    if (!e || !t) return fail(GCS_ERR_INVALID, "null argument");
+    const int slot = find_slot(e, id);
+    if (slot < 0) return fail(GCS_ERR_NOT_FOUND, "unknown instance id");
+    HIPCHK(hipSetDevice(e->device));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    HIPCHK(hipMemcpy(t, e->tsim + slot, sizeof(double), hipMemcpyDeviceToHost));
     return GCS_OK;
 }
 

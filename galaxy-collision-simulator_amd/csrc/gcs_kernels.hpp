@@ -694,14 +694,72 @@ __device__ __forceinline__ uint32_t speed_colour(uint32_t rgbt, float vx, float 
 // hi = fl32(X), lo = fl32(X - hi).  Diagnostics (FP64, per quarter tile):
 // kinetic and potential energy and momentum at the force time, with the
 // velocity re-synchronised to v_old + a*dt/2.
+// Largest |a| of the block's particles (all threads must call).
+__device__ __forceinline__ float block_max_accel(const float4& a, bool valid)
+{
+    __shared__ float sm[4];
+    float m2 = valid ? __builtin_fmaf(a.z, a.z, __builtin_fmaf(a.y, a.y, a.x * a.x)) : 0.0f;
+    if (!(m2 == m2)) m2 = __builtin_inff();
+    m2 = wave_max(m2);
+    if (lane_id() == 0) sm[threadIdx.x >> 6] = m2;
+    __syncthreads();
+    const float r = fmaxf(fmaxf(sm[0], sm[1]), fmaxf(sm[2], sm[3]));
+    __syncthreads();
+    return sqrtf(r);
+}
+
+// Per quarter tile: the largest |a| (the quantity the reference reduces with
+// atomicMax at cuda_engine.cu:157-160 and never reads).
+__global__ __launch_bounds__(256) void k_accmax(const TileInfo* __restrict__ tiles,
+                                                const SegInfo* __restrict__ segs,
+                                                const float4* __restrict__ acc,
+                                                float* __restrict__ amax_part)
+{
+    const QuarterTile q = quarter_tile(tiles, segs);
+    const float4 a = q.valid ? acc[q.i] : make_float4(0.f, 0.f, 0.f, 0.f);
+    const float m = block_max_accel(a, q.valid);
+    if (threadIdx.x == 0) amax_part[blockIdx.x] = m;
+}
+
+// Adaptive step of each instance: dt_s = min(dt_max, sqrt(2 eta eps / max|a|)),
+// eps the softening length.
+__global__ __launch_bounds__(256) void k_adapt_dt(const SegInfo* __restrict__ segs,
+                                                  const float* __restrict__ amax_part,
+                                                  float dt_max, float eta, float eps,
+                                                  float* __restrict__ dt_inst)
+{
+    const SegInfo sg = segs[blockIdx.x];
+    float m = 0.0f;
+    for (int b = 4 * sg.tile0 + (int)threadIdx.x; b < 4 * (sg.tile0 + sg.ntiles); b += 256)
+        m = fmaxf(m, amax_part[b]);
+    m = wave_max(m);
+    __shared__ float sm[4];
+    if (lane_id() == 0) sm[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        m = fmaxf(fmaxf(sm[0], sm[1]), fmaxf(sm[2], sm[3]));
+        dt_inst[blockIdx.x] = m > 0.0f ? fminf(dt_max, sqrtf(2.0f * eta * eps / m)) : dt_max;
+    }
+}
+
 __global__ __launch_bounds__(256) void k_integrate(
     const TileInfo* __restrict__ tiles, const SegInfo* __restrict__ segs,
     const float4* __restrict__ acc, float4* __restrict__ posm, float4* __restrict__ vel,
-    float4* __restrict__ plo, float dt, int compensated, int colours, int diagnostics,
-    double* __restrict__ diag_part, unsigned* err)
+    float4* __restrict__ plo, float dt_fixed, const float* __restrict__ dt_inst, int compensated,
+    int colours, int diagnostics, double* __restrict__ diag_part, float* __restrict__ amax_part,
+    double* __restrict__ tsim, unsigned* err)
 {
     const QuarterTile q = quarter_tile(tiles, segs);
+    const float dt = dt_inst ? dt_inst[q.seg] : dt_fixed;
     double d[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    {
+        const float4 a0 = q.valid ? acc[q.i] : make_float4(0.f, 0.f, 0.f, 0.f);
+        const float m = block_max_accel(a0, q.valid);
+        if (threadIdx.x == 0) {
+            amax_part[blockIdx.x] = m;
+            if ((int)blockIdx.x == 4 * segs[q.seg].tile0) tsim[q.seg] += (double)dt;
+        }
+    }
     if (q.valid) {
         const float4 a = acc[q.i];
         float4 v = vel[q.i];

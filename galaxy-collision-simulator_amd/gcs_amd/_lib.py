@@ -43,7 +43,8 @@ class Params(ctypes.Structure):
                 ("use_bounds", ctypes.c_int32), ("global_min", ctypes.c_float),
                 ("global_max", ctypes.c_float), ("colours", ctypes.c_int32),
                 ("diagnostics", ctypes.c_int32), ("use_graph", ctypes.c_int32),
-                ("waves_per_simd", ctypes.c_int32), ("reserved", ctypes.c_int32 * 4)]
+                ("waves_per_simd", ctypes.c_int32), ("dt_eta", ctypes.c_float),
+                ("reserved", ctypes.c_int32 * 3)]
 
 
 class CollisionSpec(ctypes.Structure):
@@ -117,6 +118,11 @@ SIGNATURES = {
     "gcs_spec_random": ([ctypes.c_uint64, _I64, _I32, _I32, _P], None),
     "gcs_generate": ([_P, _P], _I32),
     "gcs_autotune": ([_P, _P, _I32, _P, _P], _I32),
+    "gcs_read_max_accel": ([_P, _I32, _P], _I32),
+    "gcs_read_sim_time": ([_P, _I32, _P], _I32),
+    "gcs_checkpoint_bytes": ([_I32], ctypes.c_int64),
+    "gcs_checkpoint": ([_P, _I32, _P, ctypes.c_int64], _I32),
+    "gcs_restore": ([_P, _P, ctypes.c_int64, _P], _I32),
     "gcs_server_create": ([_P, _P], _I32),
     "gcs_server_destroy": ([_P], _I32),
     "gcs_server_submit": ([_P, _P, _I32], _I32),

@@ -111,6 +111,21 @@ class Engine:
         L.check(L.lib().gcs_read_instance(self._h, int(iid), L.ptr(out)))
         return out
 
+    def checkpoint(self, iid):
+        """Checkpoint of one instance (gcs_checkpoint) as bytes."""
+        n, _ = self.info(iid)
+        size = L.lib().gcs_checkpoint_bytes(n)
+        buf = ctypes.create_string_buffer(size)
+        L.check(L.lib().gcs_checkpoint(self._h, int(iid), buf, size))
+        return buf.raw
+
+    def restore(self, data):
+        """Appends the instance held in a checkpoint; returns its id."""
+        iid = ctypes.c_int32()
+        buf = ctypes.create_string_buffer(data, len(data))
+        L.check(L.lib().gcs_restore(self._h, buf, len(data), ctypes.byref(iid)))
+        return iid.value
+
     def position_lo(self):
         out = np.zeros((self.count()[1], 3), np.float32)
         L.check(L.lib().gcs_read_position_lo(self._h, L.ptr(out)))
@@ -186,6 +201,16 @@ class Engine:
         out = np.zeros(5, np.float64)
         L.check(L.lib().gcs_read_diagnostics(self._h, int(iid), L.ptr(out)))
         return out
+
+    def max_accel(self, iid=0):
+        out = ctypes.c_float()
+        L.check(L.lib().gcs_read_max_accel(self._h, int(iid), ctypes.byref(out)))
+        return out.value
+
+    def sim_time(self, iid=0):
+        out = ctypes.c_double()
+        L.check(L.lib().gcs_read_sim_time(self._h, int(iid), ctypes.byref(out)))
+        return out.value
 
     def engine_info(self):
         out = np.zeros(8, np.int64)

@@ -92,12 +92,14 @@ typedef struct gcs_params {
     int32_t diagnostics;/* 1: FP64 energy/momentum per instance each step */
     int32_t use_graph;  /* 1: replay steps from a captured hipGraph */
     int32_t waves_per_simd; /* resident traversal waves per SIMD, 1..4 (GCS_MAC_GROUP) */
-    int32_t reserved[4];
+    float dt_eta;       /* > 0: adaptive step per instance, min(dt, sqrt(2 dt_eta eps /
+                           max|a|)) with eps = sqrt(eps2); 0: fixed dt */
+    int32_t reserved[3];
 } gcs_params;
 
 /* Defaults: G=1, eps2=0.0025, dt=0.01, theta=0.5, GCS_MAC_GROUP,
  * GCS_PREC_PLAIN, leaf 8, block 256, stack 256, device bounds, colours on,
- * diagnostics off, graph on, 3 traversal waves per SIMD. */
+ * diagnostics off, graph on, 3 traversal waves per SIMD, fixed dt. */
 void gcs_params_default(gcs_params* p);
 
 typedef struct gcs_engine gcs_engine;
@@ -135,6 +137,14 @@ int gcs_read_stars(gcs_engine* e, gcs_star* out);
 int gcs_read_instance(gcs_engine* e, int32_t id, gcs_star* out);
 /* Compensated low words (x_lo, y_lo, z_lo per particle, host n_total*3). */
 int gcs_read_position_lo(gcs_engine* e, float* out);
+/* Checkpoint / resume of one instance (no counterpart in the reference:
+ * job preemption and restart).  A checkpoint holds the instance's records in
+ * its current order and the low parts of its compensated positions; an
+ * instance restored from it (appended to any engine) continues bit for bit
+ * as the original would.  gcs_checkpoint_bytes gives the buffer size. */
+int64_t gcs_checkpoint_bytes(int32_t n);
+int gcs_checkpoint(gcs_engine* e, int32_t id, void* buf, int64_t cap);
+int gcs_restore(gcs_engine* e, const void* buf, int64_t bytes, int32_t* id);
 /* PlaybackStar frame of all particles (simulation.bin record, main.cpp:266). */
 int gcs_read_playback(gcs_engine* e, gcs_playback_star* out);
 
@@ -183,6 +193,11 @@ int gcs_read_interactions(gcs_engine* e, int64_t* out2);
  * diagnostics=1: kinetic energy, potential energy, momentum x, y, z, all at
  * the force time (velocities re-synchronised by half a kick). */
 int gcs_read_diagnostics(gcs_engine* e, int32_t id, double* out5);
+/* Largest |a| over instance `id` in the last step (the value the reference
+ * reduces at cuda_engine.cu:157-160), and the instance's simulated time (the
+ * sum of its steps' dt since it was loaded or added). */
+int gcs_read_max_accel(gcs_engine* e, int32_t id, float* amax);
+int gcs_read_sim_time(gcs_engine* e, int32_t id, double* t);
 /* Engine facts: [0] compute units, [1] persistent traversal blocks, [2]
  * traversal wave slots, [3] bytes of per-wave traversal memory, [4] particle
  * capacity, [5] instance capacity, [6] traversal groups, [7] sort tiles. */

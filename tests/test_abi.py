@@ -109,6 +109,11 @@ def test_server_config_errors(gcs):
         assert rc == -2 and b"device" in msg
 
 
+def test_checkpoint_size(gcs):
+    assert gcs.lib().gcs_checkpoint_bytes(0) == 0
+    assert gcs.lib().gcs_checkpoint_bytes(1000) == 32 + 1000 * 48
+
+
 def test_bake_driver_built(gcs):
     path = os.path.join(os.path.dirname(gcs.LIB_PATH), "gcs_bake")
     assert os.access(path, os.X_OK)

@@ -402,6 +402,63 @@ def test_autotune_picks_a_candidate_and_keeps_the_state(gcs):
         e.step(best_r, 1)
 
 
+def test_checkpoint_resume_is_bit_exact(gcs):
+    a = gcs.generate(gcs.random_spec(21, 0, 5000, 5000))
+    b = gcs.generate(gcs.random_spec(21, 1, 3000, 3000))
+    p = gcs.params(G=G, eps2=4e-4, dt=0.005, precision=gcs.PREC_COMPENSATED, use_graph=1)
+    with gcs.Engine(len(a) + len(b), 2) as e:
+        e.load(np.concatenate([a, b]), [len(a), len(b)])
+        say("checkpoint: 7 steps, checkpoint, 9 steps")
+        e.step(p, 7)
+        ck = e.checkpoint(1)
+        assert len(ck) == gcs.lib().gcs_checkpoint_bytes(len(b))
+        e.step(p, 9)
+        want = e.instance(1)
+        n1, off1 = e.info(1)
+        want_lo = e.position_lo()[off1:off1 + n1]
+    with gcs.Engine(len(b)) as e2:
+        say("resume: 9 steps")
+        iid = e2.restore(ck)
+        e2.step(p, 9)
+        got = e2.instance(iid)
+        got_lo = e2.position_lo()
+    assert np.array_equal(want.view(np.uint8), got.view(np.uint8))
+    assert np.array_equal(want_lo.view(np.uint32), got_lo.view(np.uint32))
+    assert np.any(got_lo != 0)
+
+
+def test_max_accel_and_adaptive_step(gcs):
+    stars = gcs.collision(4000, 45)
+    eps2 = float(np.float32(0.02) ** 2)
+    with gcs.Engine(len(stars)) as e:
+        e.load(stars)
+        fixed = gcs.params(G=G, eps2=eps2, dt=0.005, use_graph=0)
+        e.compute_accels(fixed)
+        acc = e.accels()
+        amax = float(np.sqrt((acc[:, :3].astype(np.float32) ** 2).sum(1)).max())
+        say("fixed step: max |a| and simulated time")
+        e.step(fixed, 1)
+        assert abs(e.max_accel() - amax) <= 1e-6 * amax
+        assert e.sim_time() == np.float64(np.float32(0.005))
+        # Adaptive: dt = min(dt_max, sqrt(2 eta eps / max|a|)) on that step's
+        # accelerations (positions of the first step's end).
+        e.compute_accels(fixed)
+        a2 = e.accels()
+        m2 = np.float32(np.sqrt((a2[:, :3].astype(np.float32) ** 2).sum(1)).max())
+        # eta chosen so the adaptive step is about half the maximum
+        eta = float(0.25 * 0.005 ** 2 * float(m2) / (2 * np.sqrt(eps2)))
+        want = min(np.float32(0.005), np.float32(np.sqrt(np.float32(2 * eta) *
+                                                         np.float32(np.sqrt(eps2)) / m2)))
+        say("adaptive step")
+        e.step(gcs.params(G=G, eps2=eps2, dt=0.005, dt_eta=eta, use_graph=0), 1)
+        got = e.sim_time() - np.float64(np.float32(0.005))
+        assert want < np.float32(0.005)
+        assert abs(got - float(want)) <= 1e-6 * float(want)
+        t0 = e.sim_time()
+        e.step(gcs.params(G=G, eps2=eps2, dt=0.005, dt_eta=eta, use_graph=1), 3)
+        assert e.sim_time() > t0
+
+
 def test_bake_driver_writes_reference_frames(gcs, tmp_path):
     exe = os.path.join(os.path.dirname(gcs.LIB_PATH), "gcs_bake")
     out = tmp_path / "simulation.bin"
