@@ -234,11 +234,18 @@ def served_run(window, rate, max_resident, leaf, seed=1005, budget=200e9, steps_
             "jobs_finished": int(st["finished"]), "jobs_admitted": int(st["admitted"]),
             "jobs_waiting": int(st["waiting"]), "mean_resident_particles": st["mean_resident"],
             "resident_cap": int(st["resident_cap"]), "rounds": int(st["rounds"]),
+            "stepping_fraction": st["stepping_s"] / st["run_s"],
+            "steady_value": (st["full_particle_steps"] / st["full_s"]) if st["full_s"] > 0
+            else None,
+            "steady_s": st["full_s"],
             "p50_latency_s": float(np.quantile(lat, 0.5)) if len(lat) else None,
             "p99_latency_s": float(np.quantile(lat, 0.99)) if len(lat) else None,
         }
         print("served leg %s: %s" % (name, json.dumps(out[name])), file=sys.stderr, flush=True)
     out["speedup_vs_one_at_a_time"] = out["served"]["value"] / out["one_at_a_time"]["value"]
+    if out["served"]["steady_value"] and out["one_at_a_time"]["steady_value"]:
+        out["steady_speedup_vs_one_at_a_time"] = (out["served"]["steady_value"] /
+                                                  out["one_at_a_time"]["steady_value"])
     return out
 
 

@@ -67,7 +67,9 @@ struct gcs_server {
     size_t next_release = 0, next_gen = 0, next_admit = 0, lookahead = 32;
     std::vector<gcs_job_result> results;
     std::map<int64_t, std::vector<gcs_star>> finals;
-    double particle_steps = 0.0, run_s = 0.0, resident_sum = 0.0;
+    double particle_steps = 0.0, run_s = 0.0, resident_sum = 0.0, gpu_s = 0.0;
+    double full_t0 = -1.0, full_steps = 0.0, full_s = 0.0;  // from the first full round
+    bool full = false;  // admission was held back by the cap in this run
     int64_t steps_run = 0, rounds = 0, admitted = 0;
 
     void worker()
@@ -117,8 +119,10 @@ int admit(gcs_server* s, double t)
             j = &s->jobs[s->next_admit];
         }
         if ((int32_t)s->resident.size() >= s->cap_jobs ||
-            s->resident_particles + j->spec.n > s->cap_particles)
+            s->resident_particles + j->spec.n > s->cap_particles) {
+            s->full = true;
             break;
+        }
         int32_t iid = -1;
         int rc = gcs_instance_add(s->eng, j->stars.data(), j->spec.n, &iid);
         if (rc != GCS_OK) return rc;
@@ -247,9 +251,13 @@ int gcs_server_run(gcs_server* s, double duration_s)
         }
         int32_t k = s->cfg.steps_per_round;
         for (size_t r : s->resident) k = std::min(k, s->jobs[r].remaining);
+        const double ts = now_s(t0);
+        if (s->full && s->full_t0 < 0.0) s->full_t0 = ts;
         rc = gcs_step(s->eng, &s->cfg.params, k);
         if (rc == GCS_OK) rc = gcs_sync(s->eng);
         if (rc != GCS_OK) break;
+        s->gpu_s += now_s(t0) - ts;
+        if (s->full_t0 >= 0.0) s->full_steps += (double)s->resident_particles * k;
         s->particle_steps += (double)s->resident_particles * k;
         s->resident_sum += (double)s->resident_particles * k;
         s->steps_run += k;
@@ -286,7 +294,11 @@ int gcs_server_run(gcs_server* s, double duration_s)
         if (rc != GCS_OK) break;
         s->resident.swap(keep);
     }
-    s->run_s += now_s(t0);
+    const double t_end = now_s(t0);
+    s->run_s += t_end;
+    if (s->full_t0 >= 0.0) s->full_s += t_end - s->full_t0;
+    s->full_t0 = -1.0;
+    s->full = false;
     return rc;
 }
 
@@ -299,19 +311,22 @@ int gcs_server_results(gcs_server* s, gcs_job_result* out, int32_t cap, int32_t*
     return GCS_OK;
 }
 
-int gcs_server_stats(gcs_server* s, double* out8)
+int gcs_server_stats(gcs_server* s, double* out, int32_t n)
 {
-    if (!s || !out8) return GCS_ERR_INVALID;
-    size_t waiting = 0;
-    for (size_t i = s->next_admit; i < s->next_release; ++i) ++waiting;
-    out8[0] = s->particle_steps;
-    out8[1] = s->run_s;
-    out8[2] = (double)s->results.size();
-    out8[3] = (double)s->admitted;
-    out8[4] = (double)waiting;
-    out8[5] = s->steps_run ? s->resident_sum / (double)s->steps_run : 0.0;
-    out8[6] = (double)s->rounds;
-    out8[7] = (double)s->cap_particles;
+    if (!s || (!out && n > 0) || n < 0) return GCS_ERR_INVALID;
+    const double v[11] = {s->particle_steps,
+                          s->run_s,
+                          (double)s->results.size(),
+                          (double)s->admitted,
+                          (double)(s->next_release - s->next_admit),
+                          s->steps_run ? s->resident_sum / (double)s->steps_run : 0.0,
+                          (double)s->rounds,
+                          (double)s->cap_particles,
+                          s->gpu_s,
+                          s->full_steps,
+                          s->full_s};
+    for (int32_t i = 0; i < n && i < 11; ++i) out[i] = v[i];
+    for (int32_t i = 11; i < n; ++i) out[i] = 0.0;
     return GCS_OK;
 }
 

@@ -128,7 +128,7 @@ SIGNATURES = {
     "gcs_server_submit": ([_P, _P, _I32], _I32),
     "gcs_server_run": ([_P, ctypes.c_double], _I32),
     "gcs_server_results": ([_P, _P, _I32, _P], _I32),
-    "gcs_server_stats": ([_P, _P], _I32),
+    "gcs_server_stats": ([_P, _P, _I32], _I32),
     "gcs_server_final": ([_P, ctypes.c_int64, _P, _I32], _I32),
 }
 

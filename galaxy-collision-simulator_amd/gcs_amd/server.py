@@ -69,10 +69,10 @@ class Server:
                      finish=a.finish_s) for a in arr[:n.value]]
 
     def stats(self):
-        out = np.zeros(8, np.float64)
-        L.check(L.lib().gcs_server_stats(self._h, L.ptr(out)))
+        out = np.zeros(11, np.float64)
+        L.check(L.lib().gcs_server_stats(self._h, L.ptr(out), len(out)))
         keys = ["particle_steps", "run_s", "finished", "admitted", "waiting", "mean_resident",
-                "rounds", "resident_cap"]
+                "rounds", "resident_cap", "stepping_s", "full_particle_steps", "full_s"]
         return dict(zip(keys, (float(x) for x in out)))
 
     def final(self, jid, n):

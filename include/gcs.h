@@ -291,10 +291,14 @@ int gcs_server_submit(gcs_server* s, const gcs_job* jobs, int32_t n);
 int gcs_server_run(gcs_server* s, double duration_s);
 /* Finished jobs in completion order. */
 int gcs_server_results(gcs_server* s, gcs_job_result* out, int32_t cap, int32_t* n_out);
-/* [0] particle-steps advanced, [1] run seconds, [2] jobs finished,
- * [3] jobs admitted, [4] jobs waiting at the end, [5] mean resident
- * particles per step, [6] rounds, [7] resident particle cap. */
-int gcs_server_stats(gcs_server* s, double* out8);
+/* Up to n of: [0] particle-steps advanced, [1] run seconds, [2] jobs
+ * finished, [3] jobs admitted, [4] jobs waiting at the end, [5] mean resident
+ * particles per step, [6] rounds, [7] resident particle cap, [8] seconds
+ * spent stepping (device work and its waits), [9] particle-steps and [10]
+ * seconds from the first round that found the resident set full (admission
+ * held back by the cap) to the end -- the steady state of an overloaded
+ * stream; 0 if it never filled. */
+int gcs_server_stats(gcs_server* s, double* out, int32_t n);
 /* Final state of a finished job (keep_final = 1), n records. */
 int gcs_server_final(gcs_server* s, int64_t id, gcs_star* out, int32_t n);
 
