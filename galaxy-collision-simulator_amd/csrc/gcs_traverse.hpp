@@ -44,16 +44,61 @@ constexpr int kPartList = 16384;   // particle-list entries per wave slot
 constexpr int kMaxLeaf = 64;       // largest direct-sum bucket
 constexpr int kSlotInts = kStackCap + kNodeList + kPartList;
 
+constexpr int kMaxSplit = 16;               // most sub-waves a list is split into
+constexpr int kMaxPer = 3;                  // most targets per lane
+constexpr int kChunk = kWave + kMaxSplit;   // a chunk of 64 sources and a zero pad
+
 struct WaveLds {
     int stk[kLdsStack];
-    float4 ca[kWave];
-    float4 cq0[kWave];
-    float4 cq1[kWave];  // Qxz, Qyz, (width), T
+    float4 ca[kChunk];   // COM and mass, or position and mass
+    float4 cq0[kChunk];  // Qxx, Qyy, Qzz, Qxy
+    float4 cq1[kChunk];  // Qxz, Qyz, (width), T
 };
 
 struct Acc {
     float ax, ay, az, ph;
 };
+
+// A wave's share of a group: its lanes form S sub-waves of W lanes; lane
+// (h, t) holds targets t, t + W, ..., t + (P-1) W and takes the list entries
+// h, h + S, h + 2S, ...  Lanes beyond S*W idle (they repeat sub-wave 0's
+// entries and are not summed).
+struct Share {
+    int S, W, P, h, t;
+};
+
+// Fewest lane-steps per list entry, P/S, for a group of n targets
+// (W = ceil(n/P), S = min(kMaxSplit, 64/W)), ties to the smaller P.
+__device__ __forceinline__ Share make_share(int n, int lane)
+{
+    Share sh;
+    sh.P = 1;
+    sh.W = n;
+    sh.S = min(kMaxSplit, kWave / n);
+    for (int p = 2; p <= kMaxPer; ++p) {
+        const int w = (n + p - 1) / p;
+        const int s = min(kMaxSplit, kWave / w);
+        if (p * sh.S < sh.P * s) {
+            sh.P = p;
+            sh.W = w;
+            sh.S = s;
+        }
+    }
+    const int h = lane / sh.W;
+    sh.h = h < sh.S ? h : 0;
+    sh.t = lane - h * sh.W;
+    return sh;
+}
+
+struct Targets {
+    float x[kMaxPer], y[kMaxPer], z[kMaxPer];
+};
+
+static_assert(offsetof(WaveLds, cq0) == offsetof(WaveLds, ca) + sizeof(float4) * kChunk &&
+                  offsetof(WaveLds, cq1) == offsetof(WaveLds, cq0) + sizeof(float4) * kChunk,
+              "the combine scratch spans the three chunk arrays");
+// S * n <= S * W * P <= 64 * kMaxPer partial sums fit the three 64-entry arrays.
+static_assert(kMaxPer <= 3, "the combine scratch holds 3 x 64 partial sums");
 
 // Particle source: G m d / (|d|^2 + eps^2)^{3/2} (G applied at the end).
 template <bool POT>
@@ -109,16 +154,15 @@ __device__ __forceinline__ void interact_n(Acc& a, float px, float py, float pz,
 }
 
 // Phase 2 over the node list nl[0..k): gathers of chunk c+1 are issued before
-// chunk c is evaluated, and the indices one chunk further ahead.  With ks > 1
-// the wave holds ks copies of the group's targets (sub-wave h = lane / (64/ks))
-// and sub-wave h takes chunk entries h, h + ks, h + 2*ks, ...; entries past the
-// list end were gathered as zero-mass sources and add nothing.
-template <bool POT, bool QUAD>
+// chunk c is evaluated, and the indices one chunk further ahead.  Sub-wave h
+// takes chunk entries h, h + S, h + 2S, ...; entries past the list end were
+// gathered as zero-mass sources and add nothing, and the pad after the 64
+// entries holds zero-mass sources too.
+template <bool POT, bool QUAD, int P>
 __device__ __forceinline__ void eval_node_list(WaveLds& L, const int* nl, int k,
                                                const float4* __restrict__ node,
-                                               const float4* __restrict__ nodeQ, Acc& a,
-                                               float px, float py, float pz, float eps2, int ks,
-                                               int h)
+                                               const float4* __restrict__ nodeQ, Acc* a,
+                                               const Targets& tg, float eps2, int S, int h)
 {
     const int l = lane_id();
     float4 A = make_float4(0.f, 0.f, 0.f, 0.f), Q0 = A, Q1 = A;
@@ -144,64 +188,96 @@ __device__ __forceinline__ void eval_node_list(WaveLds& L, const int* nl, int k,
         }
         const int j2 = base + 2 * kWave + l;
         idx_next = j2 < k ? nl[j2] : -1;
-        const int trips = uniform((min(kWave, k - base) + ks - 1) / ks);
+        const int trips = uniform((min(kWave, k - base) + S - 1) / S);
         int t = 0;
-        for (; t + 1 < trips; t += 2) {
-            const int e0 = t * ks + h, e1 = e0 + ks;
-            const float4 c0 = L.ca[e0], c1 = L.ca[e1];
-            float4 q00 = make_float4(0, 0, 0, 0), q01 = q00;
-            float4 q10 = q00, q11 = q00;
-            if (QUAD) {
-                q00 = L.cq0[e0]; q01 = L.cq0[e1];
-                q10 = L.cq1[e0]; q11 = L.cq1[e1];
+        // Two entries per iteration with one target per lane; with more
+        // targets the targets themselves give the independent work.
+        if (P == 1) {
+            for (; t + 1 < trips; t += 2) {
+                const int e0 = t * S + h, e1 = e0 + S;
+                const float4 c0 = L.ca[e0], c1 = L.ca[e1];
+                float4 q00 = make_float4(0, 0, 0, 0), q01 = q00, q10 = q00, q11 = q00;
+                if (QUAD) {
+                    q00 = L.cq0[e0]; q01 = L.cq0[e1];
+                    q10 = L.cq1[e0]; q11 = L.cq1[e1];
+                }
+                interact_n<POT, QUAD>(a[0], tg.x[0], tg.y[0], tg.z[0], c0, q00, q10, eps2);
+                interact_n<POT, QUAD>(a[0], tg.x[0], tg.y[0], tg.z[0], c1, q01, q11, eps2);
             }
-            interact_n<POT, QUAD>(a, px, py, pz, c0, q00, q10, eps2);
-            interact_n<POT, QUAD>(a, px, py, pz, c1, q01, q11, eps2);
         }
-        if (t < trips) {
-            const int e0 = t * ks + h;
-            float4 q0 = make_float4(0, 0, 0, 0);
-            float4 q1 = q0;
+        for (; t < trips; ++t) {
+            const int e0 = t * S + h;
+            const float4 c0 = L.ca[e0];
+            float4 q0 = make_float4(0, 0, 0, 0), q1 = q0;
             if (QUAD) { q0 = L.cq0[e0]; q1 = L.cq1[e0]; }
-            interact_n<POT, QUAD>(a, px, py, pz, L.ca[e0], q0, q1, eps2);
+#pragma unroll
+            for (int u = 0; u < P; ++u)
+                interact_n<POT, QUAD>(a[u], tg.x[u], tg.y[u], tg.z[u], c0, q0, q1, eps2);
         }
         __builtin_amdgcn_wave_barrier();
     }
 }
 
 // Phase 2 over the particle list pl[0..k), same pipeline and split.
-template <bool POT>
+template <bool POT, int P>
 __device__ __forceinline__ void eval_part_list(WaveLds& L, const int* pl, int k,
-                                               const float4* __restrict__ posm, Acc& a,
-                                               float px, float py, float pz, float eps2, int ks,
-                                               int h)
+                                               const float4* __restrict__ posm, Acc* a,
+                                               const Targets& tg, float eps2, int S, int h)
 {
     const int l = lane_id();
-    float4 S = make_float4(0.f, 0.f, 0.f, 0.f);
+    float4 X = make_float4(0.f, 0.f, 0.f, 0.f);
     const int idx = l < k ? pl[l] : -1;
     int idx_next = l + kWave < k ? pl[l + kWave] : -1;
-    if (idx >= 0) S = posm[idx];
+    if (idx >= 0) X = posm[idx];
     for (int base = 0; base < k; base += kWave) {
-        L.ca[l] = S;
+        L.ca[l] = X;
         __builtin_amdgcn_wave_barrier();
-        S = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (idx_next >= 0) S = posm[idx_next];
+        X = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (idx_next >= 0) X = posm[idx_next];
         const int j2 = base + 2 * kWave + l;
         idx_next = j2 < k ? pl[j2] : -1;
-        const int trips = uniform((min(kWave, k - base) + ks - 1) / ks);
+        const int trips = uniform((min(kWave, k - base) + S - 1) / S);
         int t = 0;
-        for (; t + 3 < trips; t += 4) {
-            const int e0 = t * ks + h;
-            const float4 s0 = L.ca[e0], s1 = L.ca[e0 + ks], s2 = L.ca[e0 + 2 * ks],
-                         s3 = L.ca[e0 + 3 * ks];
-            interact_p<POT>(a, px, py, pz, s0, eps2);
-            interact_p<POT>(a, px, py, pz, s1, eps2);
-            interact_p<POT>(a, px, py, pz, s2, eps2);
-            interact_p<POT>(a, px, py, pz, s3, eps2);
+        // About four interactions in flight per iteration.
+        constexpr int U = P == 1 ? 4 : 2;
+        for (; t + U - 1 < trips; t += U) {
+            const int e0 = t * S + h;
+            float4 sv[U];
+#pragma unroll
+            for (int v = 0; v < U; ++v) sv[v] = L.ca[e0 + v * S];
+#pragma unroll
+            for (int u = 0; u < P; ++u)
+#pragma unroll
+                for (int v = 0; v < U; ++v)
+                    interact_p<POT>(a[u], tg.x[u], tg.y[u], tg.z[u], sv[v], eps2);
         }
-        for (; t < trips; ++t) interact_p<POT>(a, px, py, pz, L.ca[t * ks + h], eps2);
+        for (; t < trips; ++t) {
+            const float4 s0 = L.ca[t * S + h];
+#pragma unroll
+            for (int u = 0; u < P; ++u) interact_p<POT>(a[u], tg.x[u], tg.y[u], tg.z[u], s0, eps2);
+        }
         __builtin_amdgcn_wave_barrier();
     }
+}
+
+// The evaluation of both lists for the share's P (uniform per group).
+template <bool POT, bool QUAD>
+__device__ __forceinline__ void eval_nodes(WaveLds& L, const int* nl, int k, const float4* node,
+                                           const float4* nodeQ, Acc* a, const Targets& tg,
+                                           float eps2, const Share& sh)
+{
+    if (sh.P == 1) eval_node_list<POT, QUAD, 1>(L, nl, k, node, nodeQ, a, tg, eps2, sh.S, sh.h);
+    else if (sh.P == 2) eval_node_list<POT, QUAD, 2>(L, nl, k, node, nodeQ, a, tg, eps2, sh.S, sh.h);
+    else eval_node_list<POT, QUAD, 3>(L, nl, k, node, nodeQ, a, tg, eps2, sh.S, sh.h);
+}
+
+template <bool POT>
+__device__ __forceinline__ void eval_parts(WaveLds& L, const int* pl, int k, const float4* posm,
+                                           Acc* a, const Targets& tg, float eps2, const Share& sh)
+{
+    if (sh.P == 1) eval_part_list<POT, 1>(L, pl, k, posm, a, tg, eps2, sh.S, sh.h);
+    else if (sh.P == 2) eval_part_list<POT, 2>(L, pl, k, posm, a, tg, eps2, sh.S, sh.h);
+    else eval_part_list<POT, 3>(L, pl, k, posm, a, tg, eps2, sh.S, sh.h);
 }
 
 __device__ __forceinline__ float box_dist2(float lx, float ly, float lz, float hx, float hy,
@@ -237,6 +313,14 @@ __global__ __launch_bounds__(WPB * 64) void k_traverse_group(
     const float self_inv = __builtin_amdgcn_rsqf(eps2);
     const uint64_t lt = (1ull << l) - 1ull;
     bool overflow = false;
+    // The pads after each chunk hold zero-mass sources for the whole launch.
+    if (l < kMaxSplit) {
+        const float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
+        L.ca[kWave + l] = z;
+        L.cq0[kWave + l] = z;
+        L.cq1[kWave + l] = z;
+    }
+    __builtin_amdgcn_wave_barrier();
 
     for (;;) {
         int g = -1;
@@ -256,15 +340,21 @@ __global__ __launch_bounds__(WPB * 64) void k_traverse_group(
 
         const GroupInfo gi = groups[g];
         const SegInfo sg = segs[gi.seg];
-        // Small groups: ks copies of the targets, each taking 1/ks of every list.
-        const int ks = gi.count <= 8 ? 8 : gi.count <= 16 ? 4 : gi.count <= 32 ? 2 : 1;
-        const int sub = kWave / ks;
-        const int h = l / sub, tl = l & (sub - 1);
-        const int p = gi.first + min(tl, gi.count - 1);
-        const float4 pp = posm[p];
-        const float lx = wave_min(pp.x), ly = wave_min(pp.y), lz = wave_min(pp.z);
-        const float hx = wave_max(pp.x), hy = wave_max(pp.y), hz = wave_max(pp.z);
-        Acc a = {0.f, 0.f, 0.f, 0.f};
+        // The group's bounding box, then this lane's share of the targets.
+        const float4 pb = posm[gi.first + min(l, gi.count - 1)];
+        const float lx = wave_min(pb.x), ly = wave_min(pb.y), lz = wave_min(pb.z);
+        const float hx = wave_max(pb.x), hy = wave_max(pb.y), hz = wave_max(pb.z);
+        const Share sh = make_share(gi.count, l);
+        Targets tg;
+        Acc a[kMaxPer];
+#pragma unroll
+        for (int u = 0; u < kMaxPer; ++u) {
+            const float4 pu = posm[gi.first + min(sh.t + u * sh.W, gi.count - 1)];
+            tg.x[u] = pu.x;
+            tg.y[u] = pu.y;
+            tg.z[u] = pu.z;
+            a[u] = Acc{0.f, 0.f, 0.f, 0.f};
+        }
         int nn = 0, np = 0, top = 0, spilled = 0;
         long long tot_nodes = 0, tot_parts = 0;
         if (sg.n == 1) {
@@ -305,7 +395,7 @@ __global__ __launch_bounds__(WPB * 64) void k_traverse_group(
             const uint64_t mA = ballot(accept);
             const int na = __popcll(mA);
             if (nn + na > kNodeList) {
-                eval_node_list<POT, QUAD>(L, nl, nn, node, nodeQ, a, pp.x, pp.y, pp.z, eps2, ks, h);
+                eval_nodes<POT, QUAD>(L, nl, nn, node, nodeQ, a, tg, eps2, sh);
                 tot_nodes += nn;
                 nn = 0;
             }
@@ -318,7 +408,7 @@ __global__ __launch_bounds__(WPB * 64) void k_traverse_group(
             const int incl = wave_scan_incl(cnt_p);
             const int tp = uniform(__shfl(incl, 63));
             if (np + tp > kPartList) {
-                eval_part_list<POT>(L, pl, np, posm, a, pp.x, pp.y, pp.z, eps2, ks, h);
+                eval_parts<POT>(L, pl, np, posm, a, tg, eps2, sh);
                 tot_parts += np;
                 np = 0;
             }
@@ -359,20 +449,37 @@ __global__ __launch_bounds__(WPB * 64) void k_traverse_group(
             if (l == 0) atomicOr(err, kErrStack);
             break;
         }
-        if (nn > 0) eval_node_list<POT, QUAD>(L, nl, nn, node, nodeQ, a, pp.x, pp.y, pp.z, eps2, ks, h);
-        if (np > 0) eval_part_list<POT>(L, pl, np, posm, a, pp.x, pp.y, pp.z, eps2, ks, h);
+        if (nn > 0) eval_nodes<POT, QUAD>(L, nl, nn, node, nodeQ, a, tg, eps2, sh);
+        if (np > 0) eval_parts<POT>(L, pl, np, posm, a, tg, eps2, sh);
         tot_nodes += nn;
         tot_parts += np;
-        // Combine the sub-wave partial sums (fixed order).
-        for (int o = sub; o < kWave; o <<= 1) {
-            a.ax += __shfl_xor(a.ax, o);
-            a.ay += __shfl_xor(a.ay, o);
-            a.az += __shfl_xor(a.az, o);
-            if (POT) a.ph += __shfl_xor(a.ph, o);
+        // Combine the sub-wave partial sums through LDS (the chunk arrays, 3 x 64
+        // entries), in sub-wave order.
+        float4* const scr = L.ca;
+        if (l < sh.S * sh.W) {
+#pragma unroll
+            for (int u = 0; u < kMaxPer; ++u) {
+                const int j = sh.t + u * sh.W;
+                if (u < sh.P && j < gi.count) {
+                    const int i = sh.h * gi.count + j;
+                    scr[(i >> 6) * kChunk + (i & 63)] =
+                        make_float4(a[u].ax, a[u].ay, a[u].az, a[u].ph);
+                }
+            }
         }
-        if (h == 0 && tl < gi.count) {
-            const float ph = POT ? -G * (a.ph - pp.w * self_inv) : 0.0f;
-            acc[p] = make_float4(G * a.ax, G * a.ay, G * a.az, ph);
+        __builtin_amdgcn_wave_barrier();
+        if (l < gi.count) {
+            float4 r = make_float4(0.f, 0.f, 0.f, 0.f);
+            for (int hh = 0; hh < sh.S; ++hh) {
+                const int i = hh * gi.count + l;
+                const float4 v = scr[(i >> 6) * kChunk + (i & 63)];
+                r.x += v.x;
+                r.y += v.y;
+                r.z += v.z;
+                r.w += v.w;
+            }
+            const float ph = POT ? -G * (r.w - pb.w * self_inv) : 0.0f;
+            acc[gi.first + l] = make_float4(G * r.x, G * r.y, G * r.z, ph);
         }
         n_node += (unsigned long long)tot_nodes * (unsigned long long)gi.count;
         n_part += (unsigned long long)(tot_parts - 1) * (unsigned long long)gi.count;
