@@ -18,12 +18,17 @@
 //            a full list is evaluated early, so no list overflows);
 //   phase 2  streams each list in chunks of 64 entries: every lane gathers one
 //            source (node COM, mass and quadrupole, or particle position and
-//            mass) into LDS while the next chunk's gathers are in flight, and
-//            every lane then accumulates its own particle against all 64
-//            entries read back as broadcasts.
+//            mass) into LDS while the next chunk's gathers are in flight.
+//            The lanes form S sub-waves of W lanes, each lane holding up to 3
+//            of the group's particles, and sub-wave h evaluates entries h,
+//            h + S, ... for its particles (LDS broadcasts); S, W and the
+//            particles per lane are chosen per group so that the fewest
+//            lane-steps cover the list (a group of 40 uses 3 sub-waves of 20
+//            lanes with 2 particles each, 94% of the lanes instead of 63%).
+//            The sub-wave sums are added through LDS in sub-wave order.
 // The interaction set is a function of the tree, the group and the
-// parameters only, and every sum runs in list order, so results do not depend
-// on scheduling.  Persistent grid: waves take groups from eight queues, one
+// parameters only, and every sum runs in a fixed order, so results do not
+// depend on scheduling.  Persistent grid: waves take groups from eight queues, one
 // per XCD, each holding a contiguous run of groups (neighbouring groups touch
 // the same nodes, which then stay in that XCD's L2), and steal from the other
 // queues when theirs is empty.  The particle itself is in its own particle
