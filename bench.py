@@ -3,7 +3,7 @@
 
 Workload (N=1): the plain GPU configuration -- one collision of 2 x 131,072
 particles (70/10/20 disc/bulge/halo), FP32, theta = 0.5, softening 0.02,
-dt = 0.005, leaf 8, quadrupole group traversal.  Inputs are generated on the
+dt = 0.005, leaf 4, quadrupole group traversal.  Inputs are generated on the
 host and resident in HBM before the timed region; each timed step is a full
 step (bounds, Morton, sort, permute, tree build, aggregation, traversal,
 kick-drift), replayed from a hipGraph.
@@ -33,7 +33,7 @@ TREE_BYTES_PER_PARTICLE = 376  # Morton 24 + sort 24 + permute 68 + init 88 + bu
 # WRITE_SIZE (x1.000) passes scaled by a 256 MiB copy (profiles/
 # k_traverse_group_counters_plain-2x131072.md); valid for the configuration
 # it was measured on.
-TRAVERSE_TRAFFIC = {("plain-2x131072", 8, 256, 3): 237532485}
+TRAVERSE_TRAFFIC = {("plain-2x131072", 4, 256, 3): 240034714}
 
 CONFIGS = {
     # name: (particles per galaxy, eps, dt, theta)
@@ -255,11 +255,14 @@ def main():
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--config", default="plain-2x131072", choices=sorted(CONFIGS))
-    ap.add_argument("--leaf", type=int, default=8)
+    ap.add_argument("--leaf", type=int, default=4)
     ap.add_argument("--block", type=int, default=256)
     ap.add_argument("--wps", type=int, default=3, help="traversal waves per SIMD")
-    ap.add_argument("--no-autotune", action="store_true",
-                    help="use --leaf/--block/--wps as given instead of the autotuner's choice")
+    ap.add_argument("--autotune", action="store_true",
+                    help="pick leaf/block/waves per SIMD with the autotuner instead of "
+                         "--leaf/--block/--wps (the batched, served and precision legs always "
+                         "autotune)")
+    ap.add_argument("--no-autotune", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--batched", type=int, default=256,
@@ -292,7 +295,7 @@ def main():
                        block_size=a.block, waves_per_simd=a.wps, use_graph=1)
     e = gcs_amd.Engine(n)
     e.load(stars)
-    if not a.no_autotune:
+    if a.autotune and not a.no_autotune:
         p, _ = e.autotune(p, reps=3)
         a.leaf, a.block, a.wps = p.leaf_size, p.block_size, p.waves_per_simd
 
@@ -346,7 +349,7 @@ def main():
         "data": "synthetic (seeded two-galaxy generator)",
         "config": {"workload": a.config, "particles": n, "theta": theta, "eps": eps, "dt": dt,
                    "leaf": a.leaf, "block": a.block, "waves_per_simd": a.wps,
-                   "autotuned": not a.no_autotune,
+                   "autotuned": bool(a.autotune and not a.no_autotune),
                    "multipole": "quadrupole",
                    "parallelism": "single GPU"},
         "interactions_per_step": n_inter,

@@ -76,26 +76,47 @@ __global__ __launch_bounds__(kTileThreads) void k_sort_scatter(
     const int l = lane_id(), w = threadIdx.x >> 6;
     for (int d = threadIdx.x; d < 4 * kRadix; d += kTileThreads) (&wcnt[0][0])[d] = 0u;
 
+    // The tile's keys, loaded first so they are in flight with the offsets.
+    uint64_t key[kTileItems];
+    uint32_t val[kTileItems];
+#pragma unroll
+    for (int r = 0; r < kTileItems; ++r) {
+        const int idx = w * (kWave * kTileItems) + r * kWave + l;
+        const bool valid = idx < t.count;
+        key[r] = valid ? kin[t.start + idx] : 0ull;
+        val[r] = valid ? vin[t.start + idx] : 0u;
+    }
+
     // Output offset of digit d (= this thread) for the tile.
     const int d = threadIdx.x;
     const SegInfo sg = segs[t.seg];
     const int nch = (sg.ntiles + kChunkTiles - 1) / kChunkTiles;
     const int myc = t.tile_in_seg / kChunkTiles;
+    // All loads of a batch are issued before any is used (one L2 round trip
+    // per 16 chunks, one for the tile rows).
     const uint32_t* ch = chist + (size_t)sg.chunk0 * kRadix + d;
     uint32_t seg_tot = 0u, before = 0u;
-#pragma unroll 4
-    for (int c = 0; c < nch; ++c) {
-        const uint32_t v = ch[(size_t)c * kRadix];
-        seg_tot += v;
-        before += c < myc ? v : 0u;
+    for (int c0 = 0; c0 < nch; c0 += 16) {
+        uint32_t v[16];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) v[u] = c0 + u < nch ? ch[(size_t)(c0 + u) * kRadix] : 0u;
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+            seg_tot += v[u];
+            before += c0 + u < myc ? v[u] : 0u;
+        }
     }
     const int tfirst = tile - t.tile_in_seg % kChunkTiles;
-#pragma unroll 4
-    for (int j = tfirst; j < tile; ++j) before += thist[(size_t)j * kRadix + d];
+    {
+        uint32_t v[kChunkTiles - 1];
+#pragma unroll
+        for (int j = 0; j < kChunkTiles - 1; ++j)
+            v[j] = tfirst + j < tile ? thist[(size_t)(tfirst + j) * kRadix + d] : 0u;
+#pragma unroll
+        for (int j = 0; j < kChunkTiles - 1; ++j) before += v[j];
+    }
     __syncthreads();
 
-    uint64_t key[kTileItems];
-    uint32_t val[kTileItems];
     uint32_t dig[kTileItems];
     uint32_t rank[kTileItems];
     const uint64_t lt = (1ull << l) - 1ull;
@@ -103,8 +124,6 @@ __global__ __launch_bounds__(kTileThreads) void k_sort_scatter(
     for (int r = 0; r < kTileItems; ++r) {
         const int idx = w * (kWave * kTileItems) + r * kWave + l;
         const bool valid = idx < t.count;
-        key[r] = valid ? kin[t.start + idx] : 0ull;
-        val[r] = valid ? vin[t.start + idx] : 0u;
         const uint32_t dg = (uint32_t)(key[r] >> shift) & (kRadix - 1);
         dig[r] = dg;
         uint64_t m = ballot(valid);
