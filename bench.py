@@ -3,7 +3,7 @@
 
 Workload (N=1): the plain GPU configuration -- one collision of 2 x 131,072
 particles (70/10/20 disc/bulge/halo), FP32, theta = 0.5, softening 0.02,
-dt = 0.005, leaf 4, quadrupole group traversal.  Inputs are generated on the
+dt = 0.005, leaf 4, groups of at most 48, quadrupole group traversal.  Inputs are generated on the
 host and resident in HBM before the timed region; each timed step is a full
 step (bounds, Morton, sort, permute, tree build, aggregation, traversal,
 kick-drift), replayed from a hipGraph.
@@ -33,7 +33,7 @@ TREE_BYTES_PER_PARTICLE = 376  # Morton 24 + sort 24 + permute 68 + init 88 + bu
 # WRITE_SIZE (x1.000) passes scaled by a 256 MiB copy (profiles/
 # k_traverse_group_counters_plain-2x131072.md); valid for the configuration
 # it was measured on.
-TRAVERSE_TRAFFIC = {("plain-2x131072", 4, 256, 3): 240034714}
+TRAVERSE_TRAFFIC = {("plain-2x131072", 4, 256, 3, 64): 240034714}
 
 CONFIGS = {
     # name: (particles per galaxy, eps, dt, theta)
@@ -258,6 +258,7 @@ def main():
     ap.add_argument("--leaf", type=int, default=4)
     ap.add_argument("--block", type=int, default=256)
     ap.add_argument("--wps", type=int, default=3, help="traversal waves per SIMD")
+    ap.add_argument("--group", type=int, default=48, help="largest traversal group")
     ap.add_argument("--autotune", action="store_true",
                     help="pick leaf/block/waves per SIMD with the autotuner instead of "
                          "--leaf/--block/--wps (the batched, served and precision legs always "
@@ -292,12 +293,13 @@ def main():
     n = len(stars)
     eps2 = float(np.float32(eps) ** 2)
     p = gcs_amd.params(G=1.0, eps2=eps2, dt=dt, theta=theta, leaf_size=a.leaf,
-                       block_size=a.block, waves_per_simd=a.wps, use_graph=1)
+                       block_size=a.block, waves_per_simd=a.wps, group_size=a.group, use_graph=1)
     e = gcs_amd.Engine(n)
     e.load(stars)
     if a.autotune and not a.no_autotune:
         p, _ = e.autotune(p, reps=3)
-        a.leaf, a.block, a.wps = p.leaf_size, p.block_size, p.waves_per_simd
+        a.leaf, a.block, a.wps, a.group = (p.leaf_size, p.block_size, p.waves_per_simd,
+                                           p.group_size)
 
     # Accelerations of the initial state (for the accuracy check).
     e.compute_accels(p)
@@ -315,7 +317,8 @@ def main():
 
     # Roofline of the traversal kernel: event-timed eager steps.
     pe = gcs_amd.params(G=1.0, eps2=eps2, dt=dt, theta=theta, leaf_size=a.leaf,
-                        block_size=a.block, waves_per_simd=a.wps, use_graph=0)
+                        block_size=a.block, waves_per_simd=a.wps, group_size=a.group,
+                        use_graph=0)
     trav = []
     stages = []
     inter = None
@@ -349,6 +352,7 @@ def main():
         "data": "synthetic (seeded two-galaxy generator)",
         "config": {"workload": a.config, "particles": n, "theta": theta, "eps": eps, "dt": dt,
                    "leaf": a.leaf, "block": a.block, "waves_per_simd": a.wps,
+                   "group": a.group,
                    "autotuned": bool(a.autotune and not a.no_autotune),
                    "multipole": "quadrupole",
                    "parallelism": "single GPU"},
@@ -358,7 +362,7 @@ def main():
                               "traverse", "integrate"], stage_ms)),
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS,
-                     "traffic": TRAVERSE_TRAFFIC.get((a.config, a.leaf, a.block, a.wps)),
+                     "traffic": TRAVERSE_TRAFFIC.get((a.config, a.leaf, a.block, a.wps, a.group)),
                      "kernel": "k_traverse_group", "kernel_ms": trav_ms,
                      "flop_per_interaction": FLOP_PER_INTERACTION},
         "roofline_tree": {"bound": "hbm", "unit": "GB/s", "peak": PEAK_HBM_GBS,

@@ -28,7 +28,8 @@ constexpr int kTileItems = kTile / kTileThreads;
 constexpr int kRadixBits = 8;
 constexpr int kRadix = 1 << kRadixBits;
 constexpr int kSortPasses = 8;       // 64 bits (bit 63 of a Morton key is 0)
-constexpr int kGroup = 64;           // particles per traversal group (one wave)
+constexpr int kGroup = 64;           // default largest traversal group (one wave)
+constexpr int kGroupMax = 192;       // largest group a wave takes (3 particles per lane)
 constexpr int kGroupChunk = 128;    // quarter tiles per group-numbering chunk
 constexpr int kStackCap = 8192;      // per-wave traversal stack entries
 constexpr int kNumXcd = 8;           // MI355X: 8 XCDs, blocks dealt round-robin
@@ -54,7 +55,8 @@ struct SegInfo {
     int chunk0;     // first sort-histogram chunk (32 tiles each)
 };
 
-// Traversal group: up to kGroup consecutive particles of one instance.
+// Traversal group: up to group_size (<= kGroupMax) consecutive particles of one
+// instance.
 struct GroupInfo {
     int first;
     int count;

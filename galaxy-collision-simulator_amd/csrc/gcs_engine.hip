@@ -260,6 +260,8 @@ int check_params(const gcs_params* p)
             return fail(GCS_ERR_INVALID, "block_size must be 64, 128 or 256");
         if (p->waves_per_simd < 1 || p->waves_per_simd > 4)
             return fail(GCS_ERR_INVALID, "waves_per_simd must be in [1, 4]");
+        if (p->group_size < 1 || p->group_size > kGroupMax)
+            return fail(GCS_ERR_INVALID, "group_size must be in [1, 192]");
     } else {
         if (p->stack_depth < 2 || p->stack_depth > 1024)
             return fail(GCS_ERR_INVALID, "stack_depth must be in [2, 1024]");
@@ -358,7 +360,8 @@ int enqueue_step(gcs_engine* e, const gcs_params& p, bool integrate)
     float4* posm = e->posm[e->cur];
     mark(e, 3);
     k_build<<<qblocks, kTileThreads, 0, e->stream>>>(e->tiles, e->segs, e->keys[0], e->node,
-                                                     e->parent, e->flag, e->gmark, e->gq, e->gcnt);
+                                                     e->parent, e->flag, e->gmark, e->gq, e->gcnt,
+                                                     group ? p.group_size : kGroup);
     if (group) {
         // Number the groups in particle order and write the group table.
         k_group_chunksum<<<ngchunk, kGroupChunk, 0, e->stream>>>(e->gq, qblocks, e->gchunk);
@@ -509,6 +512,7 @@ void gcs_params_default(gcs_params* p)
     p->diagnostics = 0;
     p->use_graph = 1;
     p->waves_per_simd = 3;
+    p->group_size = kGroup;
 }
 
 int64_t gcs_device_bytes(int64_t cap_n, int32_t cap_k)
@@ -964,15 +968,17 @@ int gcs_autotune(gcs_engine* e, const gcs_params* base, int32_t reps, gcs_params
     // relative RMS at leaf 4, 2.6e-4 at leaf 16).
     std::vector<gcs_params> cand;
     if (base->mac == GCS_MAC_GROUP) {
-        for (int leaf : {4, 8, 12, 16})
-            for (int block : {64, 128, 256})
-                for (int wps : {2, 3, 4}) {
-                    gcs_params q = *base;
-                    q.leaf_size = leaf;
-                    q.block_size = block;
-                    q.waves_per_simd = wps;
-                    cand.push_back(q);
-                }
+        for (int group : {48, 64, 96})
+            for (int leaf : {4, 8, 12, 16})
+                for (int block : {64, 128, 256})
+                    for (int wps : {2, 3, 4}) {
+                        gcs_params q = *base;
+                        q.group_size = group;
+                        q.leaf_size = leaf;
+                        q.block_size = block;
+                        q.waves_per_simd = wps;
+                        cand.push_back(q);
+                    }
     } else {
         for (int depth : {64, 128, 256, 1024}) {
             gcs_params q = *base;
@@ -1271,7 +1277,7 @@ int gcs_karras_tree(const uint64_t* sorted_keys, int32_t n, int32_t* left, int32
         const int qblocks = 4 * (int)e->h_tiles.size();
         k_build<<<qblocks, kTileThreads, 0, e->stream>>>(e->tiles, e->segs, e->keys[0], e->node,
                                                          e->parent, e->flag, e->gmark, e->gq,
-                                                         e->gcnt);
+                                                         e->gcnt, kGroup);
         hipError_t le = hipGetLastError();
         if (le != hipSuccess) rc = fail(GCS_ERR_HIP, hipGetErrorString(le));
     }

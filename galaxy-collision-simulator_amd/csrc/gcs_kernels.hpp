@@ -183,7 +183,7 @@ __device__ __forceinline__ void mark_group(const SegInfo& sg, int x, int count,
 // each instance, built on instance-local indices so every instance's tree is
 // the stand-alone tree.  Writes range/split, parents and clears the arrival
 // flag of the node it builds.  Also marks traversal groups: the maximal
-// subtrees of at most kGroup particles (a child of at most kGroup under a
+// subtrees of at most gmax particles (a child of at most gmax under a
 // parent of more, or the root of a small instance) are flagged at their first
 // particle (gmark = 1) with their size (gcnt), and counted per quarter tile
 // (gq) for the numbering.
@@ -192,7 +192,8 @@ __global__ __launch_bounds__(256) void k_build(const TileInfo* __restrict__ tile
                                                const uint64_t* __restrict__ skeys,
                                                float4* __restrict__ node, int* __restrict__ parent,
                                                int* __restrict__ flag, uint32_t* __restrict__ gmark,
-                                               uint32_t* __restrict__ gq, int* __restrict__ gcnt)
+                                               uint32_t* __restrict__ gq, int* __restrict__ gcnt,
+                                               int gmax)
 {
     const QuarterTile q = quarter_tile(tiles, segs);
     if (!q.valid) return;
@@ -200,7 +201,7 @@ __global__ __launch_bounds__(256) void k_build(const TileInfo* __restrict__ tile
     const int n = sg.n, i = q.local;
     if (i == 0) {
         parent[sg.root] = -1;
-        if (n <= kGroup) mark_group(sg, sg.start, n, gmark, gq, gcnt);
+        if (n <= gmax) mark_group(sg, sg.start, n, gmark, gq, gcnt);
     }
     if (i >= n - 1) return;
     const uint64_t* k = skeys + sg.start;
@@ -229,10 +230,10 @@ __global__ __launch_bounds__(256) void k_build(const TileInfo* __restrict__ tile
     parent[cl] = me;
     parent[cr] = me;
     flag[me] = 0;
-    if (hi - lo + 1 > kGroup) {
+    if (hi - lo + 1 > gmax) {
         const int nl = split - lo + 1, nr = hi - split;
-        if (nl <= kGroup) mark_group(sg, sg.start + lo, nl, gmark, gq, gcnt);
-        if (nr <= kGroup) mark_group(sg, gsplit + 1, nr, gmark, gq, gcnt);
+        if (nl <= gmax) mark_group(sg, sg.start + lo, nl, gmark, gq, gcnt);
+        if (nr <= gmax) mark_group(sg, gsplit + 1, nr, gmark, gq, gcnt);
     }
 }
 

@@ -77,10 +77,10 @@ struct Share {
 __device__ __forceinline__ Share make_share(int n, int lane)
 {
     Share sh;
-    sh.P = 1;
-    sh.W = n;
-    sh.S = min(kMaxSplit, kWave / n);
-    for (int p = 2; p <= kMaxPer; ++p) {
+    sh.P = (n + kWave - 1) / kWave;  // at least ceil(n/64) per lane
+    sh.W = (n + sh.P - 1) / sh.P;
+    sh.S = min(kMaxSplit, kWave / sh.W);
+    for (int p = sh.P + 1; p <= kMaxPer; ++p) {
         const int w = (n + p - 1) / p;
         const int s = min(kMaxSplit, kWave / w);
         if (p * sh.S < sh.P * s) {
@@ -102,8 +102,10 @@ struct Targets {
 static_assert(offsetof(WaveLds, cq0) == offsetof(WaveLds, ca) + sizeof(float4) * kChunk &&
                   offsetof(WaveLds, cq1) == offsetof(WaveLds, cq0) + sizeof(float4) * kChunk,
               "the combine scratch spans the three chunk arrays");
-// S * n <= S * W * P <= 64 * kMaxPer partial sums fit the three 64-entry arrays.
+// S * n <= S * W * P <= 64 * kMaxPer partial sums fit the three 64-entry arrays,
+// and a group of kGroupMax fits the wave at kMaxPer particles per lane.
 static_assert(kMaxPer <= 3, "the combine scratch holds 3 x 64 partial sums");
+static_assert(kGroupMax <= kWave * kMaxPer, "a group fits one wave");
 
 // Particle source: G m d / (|d|^2 + eps^2)^{3/2} (G applied at the end).
 template <bool POT>
@@ -346,9 +348,14 @@ __global__ __launch_bounds__(WPB * 64) void k_traverse_group(
         const GroupInfo gi = groups[g];
         const SegInfo sg = segs[gi.seg];
         // The group's bounding box, then this lane's share of the targets.
-        const float4 pb = posm[gi.first + min(l, gi.count - 1)];
-        const float lx = wave_min(pb.x), ly = wave_min(pb.y), lz = wave_min(pb.z);
-        const float hx = wave_max(pb.x), hy = wave_max(pb.y), hz = wave_max(pb.z);
+        float lx = __builtin_inff(), ly = lx, lz = lx, hx = -lx, hy = -lx, hz = -lx;
+        for (int j = l; j < gi.count; j += kWave) {
+            const float4 pj = posm[gi.first + j];
+            lx = fminf(lx, pj.x); ly = fminf(ly, pj.y); lz = fminf(lz, pj.z);
+            hx = fmaxf(hx, pj.x); hy = fmaxf(hy, pj.y); hz = fmaxf(hz, pj.z);
+        }
+        lx = wave_min(lx); ly = wave_min(ly); lz = wave_min(lz);
+        hx = wave_max(hx); hy = wave_max(hy); hz = wave_max(hz);
         const Share sh = make_share(gi.count, l);
         Targets tg;
         Acc a[kMaxPer];
@@ -473,18 +480,18 @@ __global__ __launch_bounds__(WPB * 64) void k_traverse_group(
             }
         }
         __builtin_amdgcn_wave_barrier();
-        if (l < gi.count) {
+        for (int j = l; j < gi.count; j += kWave) {
             float4 r = make_float4(0.f, 0.f, 0.f, 0.f);
             for (int hh = 0; hh < sh.S; ++hh) {
-                const int i = hh * gi.count + l;
+                const int i = hh * gi.count + j;
                 const float4 v = scr[(i >> 6) * kChunk + (i & 63)];
                 r.x += v.x;
                 r.y += v.y;
                 r.z += v.z;
                 r.w += v.w;
             }
-            const float ph = POT ? -G * (r.w - pb.w * self_inv) : 0.0f;
-            acc[gi.first + l] = make_float4(G * r.x, G * r.y, G * r.z, ph);
+            const float ph = POT ? -G * (r.w - posm[gi.first + j].w * self_inv) : 0.0f;
+            acc[gi.first + j] = make_float4(G * r.x, G * r.y, G * r.z, ph);
         }
         n_node += (unsigned long long)tot_nodes * (unsigned long long)gi.count;
         n_part += (unsigned long long)(tot_parts - 1) * (unsigned long long)gi.count;

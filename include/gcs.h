@@ -94,12 +94,15 @@ typedef struct gcs_params {
     int32_t waves_per_simd; /* resident traversal waves per SIMD, 1..4 (GCS_MAC_GROUP) */
     float dt_eta;       /* > 0: adaptive step per instance, min(dt, sqrt(2 dt_eta eps /
                            max|a|)) with eps = sqrt(eps2); 0: fixed dt */
-    int32_t reserved[3];
+    int32_t group_size; /* largest traversal group (maximal subtree), 1..192
+                           (GCS_MAC_GROUP) */
+    int32_t reserved[2];
 } gcs_params;
 
 /* Defaults: G=1, eps2=0.0025, dt=0.01, theta=0.5, GCS_MAC_GROUP,
  * GCS_PREC_PLAIN, leaf 8, block 256, stack 256, device bounds, colours on,
- * diagnostics off, graph on, 3 traversal waves per SIMD, fixed dt. */
+ * diagnostics off, graph on, 3 traversal waves per SIMD, fixed dt, groups of
+ * at most 64. */
 void gcs_params_default(gcs_params* p);
 
 typedef struct gcs_engine gcs_engine;
@@ -160,8 +163,9 @@ int gcs_compute_accels(gcs_engine* e, const gcs_params* p);
  * passes of the current state (positions are not advanced; the state is
  * re-sorted as every step does) for each candidate setting and writes the
  * fastest into *best (base with those fields replaced) and its mean time per
- * pass in ms into *best_ms.  Group traversal: leaf_size {4, 8, 12, 16} x
- * block_size {64, 128, 256} x waves_per_simd {2, 3, 4}; reference traversal:
+ * pass in ms into *best_ms.  Group traversal: group_size {48, 64, 96} x
+ * leaf_size {4, 8, 12, 16} x block_size {64, 128, 256} x waves_per_simd
+ * {2, 3, 4}; reference traversal:
  * stack_depth {64, 128, 256, 1024}, skipping depths that overflow.  reps
  * timed passes per candidate after one warm-up pass, 1 <= reps <= 100. */
 int gcs_autotune(gcs_engine* e, const gcs_params* base, int32_t reps,

@@ -130,19 +130,21 @@ def check_pipeline(gcs, orc, e, stars, p, iid=0, theta=0.5, bounds=None):
     return cpu
 
 
-def test_engine_c1_group_parity(gcs, orc):
+@pytest.mark.parametrize("gsize", [64, 24, 128, 192])
+def test_engine_c1_group_parity(gcs, orc, gsize):
     stars = gcs.collision(8192, 1001)
     n = len(stars)
     eps2 = np.float32(0.05) ** 2
-    p = gcs.params(G=G, eps2=float(eps2), dt=0.01, theta=0.5, leaf_size=16, diagnostics=1)
+    p = gcs.params(G=G, eps2=float(eps2), dt=0.01, theta=0.5, leaf_size=16, diagnostics=1,
+                   group_size=gsize)
     with gcs.Engine(n) as e:
         e.load(stars)
-        say("engine C1 compute")
+        say("engine C1 compute, groups of at most %d" % gsize)
         e.compute_accels(p)
         cpu = check_pipeline(gcs, orc, e, stars, p)
         acc = e.accels()
         s = cpu["sorted"]
-        grp = orc.tree_groups(cpu["left"], cpu["right"], cpu["first"], cpu["last"], 64)
+        grp = orc.tree_groups(cpu["left"], cpu["right"], cpu["first"], cpu["last"], gsize)
         want, msum, cnt, aabs = orc.traverse_group(cpu["left"], cpu["right"], cpu["first"],
                                                    cpu["last"], s["x"], s["y"], s["z"],
                                                    s["mass"], cpu["agg"], G, float(eps2), 64, 16,
@@ -389,6 +391,7 @@ def test_autotune_picks_a_candidate_and_keeps_the_state(gcs):
         assert ms > 0
         assert best.leaf_size in (4, 8, 12, 16) and best.block_size in (64, 128, 256)
         assert best.waves_per_simd in (2, 3, 4) and best.use_graph == 1
+        assert best.group_size in (48, 64, 96)
         assert best.theta == p.theta and best.eps2 == p.eps2
         # positions unchanged (re-sorted only)
         after = np.sort(e.stars(), order=["x", "y", "z"])
