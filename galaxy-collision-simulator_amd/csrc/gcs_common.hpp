@@ -31,6 +31,15 @@ constexpr int kSortPasses = 8;       // 64 bits (bit 63 of a Morton key is 0)
 constexpr int kGroup = 64;           // default largest traversal group (one wave)
 constexpr int kGroupMax = 192;       // largest group a wave takes (3 particles per lane)
 constexpr int kGroupChunk = 128;    // quarter tiles per group-numbering chunk
+constexpr int kChunkTiles = 32;      // tiles per sort-histogram and aggregation chunk
+constexpr int kTileShift = 10;       // kTile == 1 << kTileShift
+constexpr int kChunkShift = 5;       // kChunkTiles == 1 << kChunkShift
+static_assert(kTile == 1 << kTileShift && kChunkTiles == 1 << kChunkShift, "shifts");
+
+// Instances of more tiles than this get the chunk aggregation pass between
+// the tile and instance passes (below it the extra launch costs more than
+// the shorter instance pass saves).
+constexpr int kMidTiles = 512;
 constexpr int kStackCap = 8192;      // per-wave traversal stack entries
 constexpr int kNumXcd = 8;           // MI355X: 8 XCDs, blocks dealt round-robin
 
@@ -53,6 +62,15 @@ struct SegInfo {
     int tile0;      // global index of the first tile
     int ntiles;
     int chunk0;     // first sort-histogram chunk (32 tiles each)
+};
+
+// Aggregation chunk: kChunkTiles consecutive tiles of one instance (also the
+// sort's histogram chunk).
+struct ChunkInfo {
+    int start;  // first particle
+    int count;  // particles
+    int seg;
+    int pad;
 };
 
 // Traversal group: up to group_size (<= kGroupMax) consecutive particles of one

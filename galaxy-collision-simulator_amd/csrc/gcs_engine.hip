@@ -120,6 +120,9 @@ struct gcs_engine {
     int* ngroups = nullptr;
     int* front = nullptr;   // aggregation frontier: instance s uses [2*start_s, 2*(start_s+n_s))
     int* nfront = nullptr;
+    int* cfront = nullptr;  // chunk frontier: chunk c uses [2*start_c, 2*(start_c+n_c))
+    int* ncfront = nullptr;
+    ChunkInfo* chunks = nullptr;
     uint32_t* hist = nullptr;   // per-tile digit counts of the current pass
     uint32_t* chist = nullptr;  // per-chunk digit counts, one block per pass
     uint32_t* seg_bounds = nullptr;
@@ -139,9 +142,11 @@ struct gcs_engine {
     std::vector<Instance> inst;
     std::vector<SegInfo> h_segs;
     std::vector<TileInfo> h_tiles;
+    std::vector<ChunkInfo> h_chunks;
     int64_t n_total = 0;
     int32_t next_id = 0;
     int nchunks = 0;  // sort-histogram chunks in use
+    bool any_mid = false;  // an instance has more than kMidTiles tiles
     bool have_tree = false;
     bool capture_keys = false;
     int last_mac = GCS_MAC_GROUP;
@@ -189,7 +194,7 @@ void free_all(gcs_engine* e)
     dfree(e->acc); dfree(e->keys_raw);
     dfree(e->node); dfree(e->nodeQ); dfree(e->boxLo); dfree(e->boxHi);
     dfree(e->parent); dfree(e->flag);
-    dfree(e->tiles); dfree(e->segs); dfree(e->groups); dfree(e->hist); dfree(e->chist);
+    dfree(e->tiles); dfree(e->segs); dfree(e->chunks); dfree(e->cfront); dfree(e->ncfront); dfree(e->groups); dfree(e->hist); dfree(e->chist);
     dfree(e->gmark); dfree(e->gq); dfree(e->gchunk); dfree(e->gcnt); dfree(e->ngroups);
     dfree(e->front); dfree(e->nfront);
     dfree(e->seg_bounds); dfree(e->bpart); dfree(e->qctr); dfree(e->counters); dfree(e->err);
@@ -203,6 +208,7 @@ int rebuild_tables(gcs_engine* e)
 {
     e->h_segs.clear();
     e->h_tiles.clear();
+    e->h_chunks.clear();
     int start = 0, chunks = 0;
     for (size_t s = 0; s < e->inst.size(); ++s) {
         const int n = e->inst[s].n;
@@ -216,6 +222,14 @@ int rebuild_tables(gcs_engine* e)
         sg.ntiles = (n + kTile - 1) / kTile;
         sg.chunk0 = chunks;
         chunks += (sg.ntiles + kChunkTiles - 1) / kChunkTiles;
+        for (int c = 0; c < (sg.ntiles + kChunkTiles - 1) / kChunkTiles; ++c) {
+            ChunkInfo ch;
+            ch.start = start + c * kChunkTiles * kTile;
+            ch.count = std::min(kChunkTiles * kTile, n - c * kChunkTiles * kTile);
+            ch.seg = (int)s;
+            ch.pad = 0;
+            e->h_chunks.push_back(ch);
+        }
         for (int j = 0; j < sg.ntiles; ++j) {
             TileInfo t;
             t.start = start + j * kTile;
@@ -230,10 +244,14 @@ int rebuild_tables(gcs_engine* e)
     }
     e->n_total = start;
     e->nchunks = chunks;
+    e->any_mid = false;
+    for (const SegInfo& sg : e->h_segs) e->any_mid = e->any_mid || sg.ntiles > kMidTiles;
     if (!e->h_segs.empty()) {
         HIPCHK(hipMemcpy(e->segs, e->h_segs.data(), sizeof(SegInfo) * e->h_segs.size(),
                          hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(e->tiles, e->h_tiles.data(), sizeof(TileInfo) * e->h_tiles.size(),
+                         hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(e->chunks, e->h_chunks.data(), sizeof(ChunkInfo) * e->h_chunks.size(),
                          hipMemcpyHostToDevice));
     }
     e->graph_valid = false;
@@ -335,7 +353,7 @@ int enqueue_step(gcs_engine* e, const gcs_params& p, bool integrate)
     const int rblocks = (int)std::max<int64_t>((nseg + 255) / 256,
                                                std::min<int64_t>(1024, (cwords + 255) / 256));
     k_reset<<<rblocks, 256, 0, e->stream>>>(nseg, e->seg_bounds, e->qctr, e->counters, e->nfront,
-                                            e->chist, cwords);
+                                            e->chist, cwords, e->ncfront, e->nchunks);
     if (!p.use_bounds) {
         k_bounds<<<qblocks, kTileThreads, 0, e->stream>>>(e->tiles, e->segs, e->posm[c],
                                                            e->bpart);
@@ -373,7 +391,12 @@ int enqueue_step(gcs_engine* e, const gcs_params& p, bool integrate)
     const float inv_theta = p.theta > 0.0f ? 1.0f / p.theta : __builtin_inff();
     k_aggregate<<<ntiles, kTile, 0, e->stream>>>(e->tiles, e->segs, posm, e->node, e->nodeQ,
                                                  e->boxLo, e->boxHi, e->parent, e->front,
-                                                 e->nfront, inv_theta, group ? 1 : 0);
+                                                 e->nfront, e->cfront, e->ncfront, inv_theta,
+                                                 group ? 1 : 0);
+    if (e->any_mid)
+        k_aggregate_mid<<<e->nchunks, 1024, 0, e->stream>>>(
+        e->chunks, e->segs, posm, e->node, e->nodeQ, e->boxLo, e->boxHi, e->parent, e->cfront,
+        e->ncfront, e->front, e->nfront, inv_theta, group ? 1 : 0);
     k_aggregate_top<<<nseg, 1024, 0, e->stream>>>(e->segs, posm, e->node, e->nodeQ, e->boxLo,
                                                   e->boxHi, e->parent, e->flag, e->front,
                                                   e->nfront, inv_theta, group ? 1 : 0);
@@ -530,6 +553,7 @@ int64_t gcs_device_bytes(int64_t cap_n, int32_t cap_k)
                    4 * (size_t)kSortPasses * kRadix * max_chunks(cap_n, cap_k));
     b += (int64_t)(4 * 4 * max_tiles(cap_n, cap_k) + 4 * max_gchunks(cap_n, cap_k));
     b += 8 * cap_k + 64 + 5 * 8 * 4 * (int64_t)max_tiles(cap_n, cap_k) + 5 * 8 * cap_k;
+    b += 8 * n + 20 * (int64_t)max_chunks(cap_n, cap_k);  // chunk frontier and table
     b += 4 * 4 * (int64_t)max_tiles(cap_n, cap_k) + 12 * (int64_t)cap_k;  // amax, dt, time
     b += 32 * n;               // staging
     return b;
@@ -589,6 +613,9 @@ int gcs_create(int64_t cap_n, int32_t cap_k, int32_t device, gcs_engine** out)
     if (!rc) rc = dalloc(&e->ngroups, 1);
     if (!rc) rc = dalloc(&e->front, 2 * n + 2);
     if (!rc) rc = dalloc(&e->nfront, (size_t)cap_k);
+    if (!rc) rc = dalloc(&e->cfront, 2 * n + 2);
+    if (!rc) rc = dalloc(&e->ncfront, max_chunks(cap_n, cap_k));
+    if (!rc) rc = dalloc(&e->chunks, max_chunks(cap_n, cap_k));
     if (!rc) rc = dalloc(&e->hist, he);
     if (!rc) rc = dalloc(&e->chist, (size_t)kSortPasses * kRadix * max_chunks(cap_n, cap_k));
     if (!rc) rc = dalloc(&e->seg_bounds, 2 * (size_t)cap_k);

@@ -30,12 +30,12 @@ __device__ __forceinline__ QuarterTile quarter_tile(const TileInfo* __restrict__
 }
 
 // Start-of-step reset: cube bounds to empty; work-queue, interaction and
-// aggregation-frontier counters and the sort's chunk histograms (all passes)
-// to zero.  A kernel rather than
+// aggregation-frontier counters (instances and chunks) and the sort's chunk
+// histograms (all passes) to zero.  A kernel rather than
 // hipMemsetAsync so the step can be captured into a hipGraph.
 __global__ void k_reset(int nseg, uint32_t* seg_bounds, unsigned* qctr,
                         unsigned long long* counters, int* nfront, uint32_t* chist,
-                        int64_t chist_words)
+                        int64_t chist_words, int* ncfront, int nchunks)
 {
     const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
@@ -47,6 +47,7 @@ __global__ void k_reset(int nseg, uint32_t* seg_bounds, unsigned* qctr,
     if (tid < kNumXcd) qctr[tid] = 0u;
     if (tid < 2) counters[tid] = 0ull;
     for (int64_t i = tid; i < chist_words; i += stride) chist[i] = 0u;
+    for (int64_t i = tid; i < nchunks; i += stride) ncfront[i] = 0;
 }
 
 // main.cpp:152-166 -- per-instance cube [min over x,y,z, max over x,y,z], in
@@ -403,6 +404,20 @@ __device__ __forceinline__ void store_node(int v, const NodeData& d, float w, fl
 // node whose range crosses the tile appends the child it came from to its
 // instance's frontier list and stops.
 //
+// Slot of this lane in a list that the lanes with `pred` append to through
+// `counter` (uniform over the active lanes): one atomic per wave; lanes
+// without pred get -1.
+__device__ __forceinline__ int wave_append(int* counter, bool pred)
+{
+    const uint64_t m = ballot(pred);
+    if (m == 0ull) return -1;
+    const int leader = __ffsll((unsigned long long)m) - 1;
+    int base = 0;
+    if (lane_id() == leader) base = atomicAdd(counter, __popcll(m));
+    base = __shfl(base, leader);
+    return pred ? base + __popcll(m & ((1ull << lane_id()) - 1ull)) : -1;
+}
+
 // k_aggregate_top: one block per instance finishes the tile-crossing nodes,
 // climbing from the frontier with arrival marks at workgroup scope (bits in
 // LDS, or counters in global memory for very large instances) -- every
@@ -414,7 +429,7 @@ __global__ __launch_bounds__(kTile) void k_aggregate(
     const TileInfo* __restrict__ tiles, const SegInfo* __restrict__ segs,
     const float4* __restrict__ posm, float4* node, float4* nodeQ, float4* boxLo, float4* boxHi,
     const int* __restrict__ parent, int* __restrict__ front, int* __restrict__ nfront,
-    float inv_theta, int mode)
+    int* __restrict__ cfront, int* __restrict__ ncfront, float inv_theta, int mode)
 {
     __shared__ uint32_t lflag[kTile / 32];
     __shared__ float4 ldat[kTile][4];
@@ -438,13 +453,13 @@ __global__ __launch_bounds__(kTile) void k_aggregate(
         bb = node[2 * cur + 1];
         par = parent[cur];
     }
+    bool crossed = false;  // stopped below a node that crosses the tile
     while (cur != -1) {
         const int first = __float_as_int(bb.y), last = __float_as_int(bb.z),
                   split = __float_as_int(bb.w);
         if (first < t0 || last > t1) {
-            const int k = atomicAdd(&nfront[t.seg], 1);
-            front[2 * sg.start + k] = child;
-            return;
+            crossed = true;
+            break;
         }
         float4 bb_next = make_float4(0.f, 0.f, 0.f, 0.f);
         int par_next = -1;
@@ -489,6 +504,21 @@ __global__ __launch_bounds__(kTile) void k_aggregate(
         bb = bb_next;
         par = par_next;
     }
+    if (crossed) {
+        // The parent crosses the tile: it is finished by the chunk pass when
+        // the instance has one and the parent's range lies in the tile's
+        // chunk, else by the instance pass.
+        const int first = __float_as_int(bb.y), last = __float_as_int(bb.z);
+        const int cl = t.tile_in_seg / kChunkTiles;
+        const int cs = sg.start + cl * kChunkTiles * kTile;
+        const int ce = min(cs + kChunkTiles * kTile, sg.start + sg.n) - 1;
+        const bool to_chunk = sg.ntiles > kMidTiles && first >= cs && last <= ce;
+        // One atomic per wave and counter (both counters are uniform in the block).
+        const int kc = wave_append(&ncfront[sg.chunk0 + cl], to_chunk);
+        const int ki = wave_append(&nfront[t.seg], !to_chunk);
+        if (to_chunk) cfront[2 * cs + kc] = child;
+        else front[2 * sg.start + ki] = child;
+    }
 }
 
 // Arrival marks of the top pass: bits in LDS indexed by node (instances of
@@ -506,7 +536,21 @@ static_assert((1 << kTopHashBits) == kTopLdsNodes / 32, "hash and bitmap share t
 // goes on, for at most `levels` levels.  Returns the last node it completed
 // when it stopped because of the level limit (with more levels to go), else
 // -1.
+// The climb covers the nodes whose range lies in the span [lo, hi] (global
+// particle indices); a climber whose parent reaches outside it appends its
+// node to the outer frontier (out, nout) and stops.  A sibling whose range
+// lies in one unit (1 << unit_shift particles, counted from the instance start) was
+// finished by an earlier launch.  Bit marks are indexed by internal node
+// relative to `slot_base`.
+struct ClimbSpan {
+    int lo, hi, unit_shift, slot_base;  // unit = 1 << unit_shift particles
+    int* out;
+    int* nout;
+};
+
+template <bool BOUNDED>
 __device__ __forceinline__ int top_climb(const SegInfo& sg, int child, int levels,
+                                         const ClimbSpan& span,
                                          const float4* __restrict__ posm, float4* node,
                                          float4* nodeQ, float4* boxLo, float4* boxHi,
                                          const int* __restrict__ parent, int* flag,
@@ -523,16 +567,21 @@ __device__ __forceinline__ int top_climb(const SegInfo& sg, int child, int level
         if (lv == levels) return child;
         const int first = __float_as_int(bb.y), last = __float_as_int(bb.z),
                   split = __float_as_int(bb.w);
+        if (BOUNDED && (first < span.lo || last > span.hi)) {
+            span.out[atomicAdd(span.nout, 1)] = child;
+            return -1;
+        }
         const bool lleaf = first == split, rleaf = last == split + 1;
         const int lidx = lleaf ? split + sg.leaf_base : split + sg.int_base;
         const bool mine_left = child == lidx;
         const bool sleaf = mine_left ? rleaf : lleaf;
         const int sp = mine_left ? split + 1 : split;
-        // A sibling whose range lies in one tile (or a leaf) was completed by
-        // k_aggregate: load it before the arrival check.  A tile-crossing
-        // sibling is written in this launch and is read after the acquire.
+        // A sibling whose range lies in one unit (or a leaf) was finished by an
+        // earlier launch: load it before the arrival check.  Any other sibling
+        // is written in this launch and is read after the acquire.
         const int sf = mine_left ? split + 1 : first, sl = mine_left ? last : split;
-        const bool sfinal = sleaf || ((sf - sg.start) / kTile) == ((sl - sg.start) / kTile);
+        const bool sfinal =
+            sleaf || ((sf - sg.start) >> span.unit_shift) == ((sl - sg.start) >> span.unit_shift);
         NodeData sib;
         if (sfinal)
             sib = sleaf ? leaf_data(posm[sp])
@@ -549,7 +598,7 @@ __device__ __forceinline__ int top_climb(const SegInfo& sg, int child, int level
         // sibling's thread).
         bool second;
         if (marks == kMarksBits) {
-            const int slot = cur - sg.int_base - sg.start;
+            const int slot = cur - sg.int_base - span.slot_base;
             uint32_t* w = &lmem[slot >> 5];
             const uint32_t bit = 1u << (slot & 31);
             second =
@@ -609,6 +658,41 @@ __device__ __forceinline__ int top_climb(const SegInfo& sg, int child, int level
     }
 }
 
+// Runs the climbers of list la[0, nf) (with la[n, 2n) as the second list of
+// the rounds below) for the whole block; BOUNDED: the span does not hold the
+// whole instance.
+template <bool BOUNDED>
+__device__ __forceinline__ void climb_rounds(const SegInfo& sg, int* la, int n, int nf,
+                                             const ClimbSpan& span,
+                                             const float4* __restrict__ posm, float4* node,
+                                             float4* nodeQ, float4* boxLo, float4* boxHi,
+                                             const int* __restrict__ parent, int* flag,
+                                             uint32_t* lmem, int marks, float inv_theta,
+                                             int mode, int* lcount)
+{
+    int* lb = la + n;
+    int count = nf;
+    while (count > (int)blockDim.x) {
+        if (threadIdx.x == 0) *lcount = 0;
+        __syncthreads();
+        for (int k = threadIdx.x; k < count; k += blockDim.x) {
+            const int v = top_climb<BOUNDED>(sg, la[k], 1, span, posm, node, nodeQ, boxLo,
+                                             boxHi, parent, flag, lmem, marks, inv_theta, mode);
+            if (v >= 0) lb[atomicAdd(lcount, 1)] = v;
+        }
+        __syncthreads();
+        count = *lcount;
+        int* t = la;
+        la = lb;
+        lb = t;
+        __syncthreads();  // every thread has read the count before the next reset
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < count; k += blockDim.x)
+        top_climb<BOUNDED>(sg, la[k], 1 << 30, span, posm, node, nodeQ, boxLo, boxHi, parent,
+                           flag, lmem, marks, inv_theta, mode);
+}
+
 // While the frontier holds more climbers than the block has threads, every
 // climber advances one level per round and the ones that go on (second
 // arrivals) form the next, much shorter list; then each remaining climber
@@ -634,28 +718,43 @@ __global__ __launch_bounds__(1024) void k_aggregate_top(
         for (int k = threadIdx.x; k < (sg.n + 31) / 32; k += blockDim.x) lmem[k] = 0u;
     else if (marks == kMarksHash)
         for (int k = threadIdx.x; k < (1 << kTopHashBits); k += blockDim.x) lmem[k] = 0xFFFFFFFFu;
-    int* la = front + 2 * (size_t)sg.start;
-    int* lb = la + sg.n;
-    int count = nf;
-    while (count > (int)blockDim.x) {
-        if (threadIdx.x == 0) lcount = 0;
-        __syncthreads();
-        for (int k = threadIdx.x; k < count; k += blockDim.x) {
-            const int v = top_climb(sg, la[k], 1, posm, node, nodeQ, boxLo, boxHi, parent, flag,
-                                    lmem, marks, inv_theta, mode);
-            if (v >= 0) lb[atomicAdd(&lcount, 1)] = v;
-        }
-        __syncthreads();
-        count = lcount;
-        int* t = la;
-        la = lb;
-        lb = t;
-        __syncthreads();  // every thread has read lcount before the next reset
-    }
-    __syncthreads();
-    for (int k = threadIdx.x; k < count; k += blockDim.x)
-        top_climb(sg, la[k], 1 << 30, posm, node, nodeQ, boxLo, boxHi, parent, flag, lmem, marks,
-                  inv_theta, mode);
+    ClimbSpan span;
+    span.lo = sg.start;
+    span.hi = sg.start + sg.n - 1;
+    span.unit_shift = sg.ntiles > kMidTiles ? kChunkShift + kTileShift : kTileShift;
+    span.slot_base = sg.start;
+    span.out = nullptr;  // the instance span contains every node
+    span.nout = nullptr;
+    climb_rounds<false>(sg, front + 2 * (size_t)sg.start, sg.n, nf, span, posm, node, nodeQ,
+                        boxLo, boxHi, parent, flag, lmem, marks, inv_theta, mode, &lcount);
+}
+
+// k_aggregate_mid: one block per chunk finishes the nodes that cross its
+// tiles but not its bounds (arrival bits in LDS), and hands the chunk's
+// outermost nodes to the instance pass.
+__global__ __launch_bounds__(1024) void k_aggregate_mid(
+    const ChunkInfo* __restrict__ chunks, const SegInfo* __restrict__ segs,
+    const float4* __restrict__ posm, float4* node, float4* nodeQ, float4* boxLo, float4* boxHi,
+    const int* __restrict__ parent, int* cfront, const int* __restrict__ ncfront, int* front,
+    int* nfront, float inv_theta, int mode)
+{
+    __shared__ uint32_t lbits[kChunkTiles * kTile / 32];
+    __shared__ int lcount;
+    const ChunkInfo c = chunks[blockIdx.x];
+    const int nf = ncfront[blockIdx.x];
+    if (nf == 0) return;
+    const SegInfo sg = segs[c.seg];
+    for (int k = threadIdx.x; k < (c.count + 31) / 32; k += blockDim.x) lbits[k] = 0u;
+    ClimbSpan span;
+    span.lo = c.start;
+    span.hi = c.start + c.count - 1;
+    span.unit_shift = kTileShift;
+    span.slot_base = c.start;
+    span.out = front + 2 * (size_t)sg.start;
+    span.nout = &nfront[c.seg];
+    climb_rounds<true>(sg, cfront + 2 * (size_t)c.start, c.count, nf, span, posm, node, nodeQ,
+                       boxLo, boxHi, parent, nullptr, lbits, kMarksBits, inv_theta, mode,
+                       &lcount);
 }
 
 __device__ __forceinline__ uint8_t to_u8(float f) { return (uint8_t)f2u_sat(f); }

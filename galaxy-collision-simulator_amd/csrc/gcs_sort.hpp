@@ -21,8 +21,6 @@
 
 namespace gcs {
 
-constexpr int kChunkTiles = 32;  // tiles per histogram chunk
-
 __global__ __launch_bounds__(kTileThreads) void k_sort_count(const TileInfo* __restrict__ tiles,
                                                              const SegInfo* __restrict__ segs,
                                                              const uint64_t* __restrict__ keys,
