@@ -52,6 +52,20 @@ __device__ __forceinline__ T wave_sum(T v)
 }
 
 // Inclusive prefix sum over the wave.
+// Exclusive prefix sum over the wave of a value below 2^bits (bits uniform),
+// from one ballot per bit: no cross-lane data movement.  *total = wave sum.
+__device__ __forceinline__ int wave_scan_excl_bits(int v, int bits, int* total)
+{
+    int ex = 0, tot = 0;
+    for (int b = 0; b < bits; ++b) {
+        const uint64_t m = ballot((v >> b) & 1);
+        ex += mbcnt(m) << b;
+        tot += __popcll(m) << b;
+    }
+    *total = uniform(tot);
+    return ex;
+}
+
 __device__ __forceinline__ int wave_scan_incl(int v)
 {
     const int l = lane_id();

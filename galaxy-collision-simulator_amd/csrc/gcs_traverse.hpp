@@ -318,6 +318,7 @@ __global__ __launch_bounds__(WPB * 64) void k_traverse_group(
     unsigned long long n_node = 0, n_part = 0;
     int q = 0;
     const float self_inv = __builtin_amdgcn_rsqf(eps2);
+    const int pbits = 32 - __builtin_clz((unsigned)(leaf + 2));  // bits of a round's count
     const uint64_t lt = (1ull << l) - 1ull;
     bool overflow = false;
     // The pads after each chunk hold zero-mass sources for the whole launch.
@@ -369,6 +370,11 @@ __global__ __launch_bounds__(WPB * 64) void k_traverse_group(
         }
         int nn = 0, np = 0, top = 0, spilled = 0;
         long long tot_nodes = 0, tot_parts = 0;
+        // List stores of one round are issued in the next round after its node
+        // loads, so that the wait for those loads does not also wait for them
+        // (vector memory operations complete in issue order).
+        int dn_pos = -1, dn_val = 0;          // node-list entry
+        int dp_pos = 0, dp_self = 0, dp_first = 0, dp_lr = 0, dp_split = 0;  // particles
         if (sg.n == 1) {
             if (l == 0) pl[0] = sg.start;
             np = 1;
@@ -394,9 +400,20 @@ __global__ __launch_bounds__(WPB * 64) void k_traverse_group(
             float4 A = make_float4(0.f, 0.f, 0.f, 0.f);
             int4 B = make_int4(0, 0, 0, 0);
             bool accept = false, self_direct = false, open = false;
+            float4 bb = make_float4(0.f, 0.f, 0.f, 0.f);
             if (have) {
                 A = node[2 * v];
-                const float4 bb = node[2 * v + 1];
+                bb = node[2 * v + 1];
+            }
+            // The previous round's list entries.
+            if (dn_pos >= 0) nl[dn_pos] = dn_val;
+            for (int k = 0; k < dp_self; ++k) pl[dp_pos + k] = dp_first + k;
+            if (dp_lr & 1) pl[dp_pos + dp_self] = dp_split;
+            if (dp_lr & 2) pl[dp_pos + dp_self + (dp_lr & 1)] = dp_split + 1;
+            dn_pos = -1;
+            dp_self = 0;
+            dp_lr = 0;
+            if (have) {
                 B = make_int4(__float_as_int(bb.x), __float_as_int(bb.y), __float_as_int(bb.z),
                               __float_as_int(bb.w));
                 accept = box_dist2(lx, ly, lz, hx, hy, hz, A.x, A.y, A.z) > bb.x;
@@ -411,24 +428,27 @@ __global__ __launch_bounds__(WPB * 64) void k_traverse_group(
                 tot_nodes += nn;
                 nn = 0;
             }
-            if (accept) nl[nn + __popcll(mA & lt)] = v;
+            if (accept) {
+                dn_pos = nn + mbcnt(mA);
+                dn_val = v;
+            }
             nn = uniform(nn + na);
             // particles: the node's own (summed) or its single-particle children
             const bool lleaf = open && B.y == B.w, rleaf = open && B.z == B.w + 1;
             const int c_self = self_direct ? B.z - B.y + 1 : 0;
             const int cnt_p = c_self + (lleaf ? 1 : 0) + (rleaf ? 1 : 0);
-            const int incl = wave_scan_incl(cnt_p);
-            const int tp = uniform(__shfl(incl, 63));
+            int tp;
+            const int ex = wave_scan_excl_bits(cnt_p, pbits, &tp);
             if (np + tp > kPartList) {
                 eval_parts<POT>(L, pl, np, posm, a, tg, eps2, sh);
                 tot_parts += np;
                 np = 0;
             }
-            int o = np + incl - cnt_p;
-            for (int k = 0; k < c_self; ++k) pl[o + k] = B.y + k;
-            o += c_self;
-            if (lleaf) pl[o++] = B.w;
-            if (rleaf) pl[o] = B.w + 1;
+            dp_pos = np + ex;
+            dp_self = c_self;
+            dp_first = B.y;
+            dp_lr = (lleaf ? 1 : 0) | (rleaf ? 2 : 0);
+            dp_split = B.w;
             np = uniform(np + tp);
             // larger children -> stack
             const bool lp = open && !lleaf, rp = open && !rleaf;
@@ -461,6 +481,10 @@ __global__ __launch_bounds__(WPB * 64) void k_traverse_group(
             if (l == 0) atomicOr(err, kErrStack);
             break;
         }
+        if (dn_pos >= 0) nl[dn_pos] = dn_val;
+        for (int k = 0; k < dp_self; ++k) pl[dp_pos + k] = dp_first + k;
+        if (dp_lr & 1) pl[dp_pos + dp_self] = dp_split;
+        if (dp_lr & 2) pl[dp_pos + dp_self + (dp_lr & 1)] = dp_split + 1;
         if (nn > 0) eval_nodes<POT, QUAD>(L, nl, nn, node, nodeQ, a, tg, eps2, sh);
         if (np > 0) eval_parts<POT>(L, pl, np, posm, a, tg, eps2, sh);
         tot_nodes += nn;
