@@ -33,7 +33,7 @@ TREE_BYTES_PER_PARTICLE = 376  # Morton 24 + sort 24 + permute 68 + init 88 + bu
 # WRITE_SIZE (x1.000) passes scaled by a 256 MiB copy (profiles/
 # k_traverse_group_counters_plain-2x131072.md); valid for the configuration
 # it was measured on.
-TRAVERSE_TRAFFIC = {("plain-2x131072", 4, 256, 3, 64): 240034714}
+TRAVERSE_TRAFFIC = {("plain-2x131072", 4, 256, 3, 48): 262947564}
 
 CONFIGS = {
     # name: (particles per galaxy, eps, dt, theta)

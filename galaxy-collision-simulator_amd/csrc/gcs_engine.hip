@@ -91,7 +91,8 @@ struct gcs_engine {
     int64_t cap_n = 0;
     int32_t cap_k = 0;
     int num_cu = 256;
-    int trav_blocks_cap = 0;  // persistent traversal blocks the stack memory covers
+    int trav_blocks_cap = 0;  // persistent traversal blocks resident at once
+    int trav_slots = 0;       // traversal waves the per-wave memory covers
 
     // state (double-buffered by the permute)
     float4* posm[2] = {nullptr, nullptr};
@@ -407,10 +408,10 @@ int enqueue_step(gcs_engine* e, const gcs_params& p, bool integrate)
         // Persistent: as many blocks as the per-wave memory covers; waves that
         // find no group left exit at once.
         // waves_per_simd * 4 SIMDs * CUs waves, within the per-wave memory.
-        const int waves = std::min(e->trav_blocks_cap * 4, e->num_cu * 4 * p.waves_per_simd);
+        const int waves = std::min(e->trav_slots, e->num_cu * 4 * p.waves_per_simd);
         const int blocks = std::max(1, std::min(waves / wpb, (int)((e->n_total + wpb - 1) / wpb)));
         const bool pot = p.diagnostics != 0;
-        // Stack slots: blocks * wpb waves <= trav_blocks_cap * 4.
+        // Stack slots: blocks * wpb waves <= trav_slots.
         if (wpb == 4) {
             if (pot) launch_group<true, true, 4>(e, blocks, p, posm);
             else launch_group<false, true, 4>(e, blocks, p, posm);
@@ -628,7 +629,11 @@ int gcs_create(int64_t cap_n, int32_t cap_k, int32_t device, gcs_engine** out)
     if (!rc) rc = dalloc(&e->amax_part, 4 * nt);
     if (!rc) rc = dalloc(&e->dt_inst, (size_t)cap_k);
     if (!rc) rc = dalloc(&e->tsim, (size_t)cap_k);
-    if (!rc) rc = dalloc(&e->slot_mem, (size_t)e->trav_blocks_cap * 4 * kSlotInts);
+    // One slot per resident traversal wave, and no more than the groups of
+    // cap_n particles can use.
+    e->trav_slots = (int)std::max<int64_t>(4, std::min<int64_t>((int64_t)e->trav_blocks_cap * 4,
+                                                              (cap_n + 3) / 4 * 4));
+    if (!rc) rc = dalloc(&e->slot_mem, (size_t)e->trav_slots * kSlotInts);
     if (!rc) rc = dalloc(&e->staging, 32 * n);
     if (rc) {
         free_all(e);
@@ -1194,8 +1199,8 @@ int gcs_engine_info(gcs_engine* e, int64_t* out8)
     if (!e || !out8) return fail(GCS_ERR_INVALID, "null argument");
     out8[0] = e->num_cu;
     out8[1] = e->trav_blocks_cap;
-    out8[2] = (int64_t)e->trav_blocks_cap * 4;
-    out8[3] = (int64_t)e->trav_blocks_cap * 4 * kSlotInts * 4;
+    out8[2] = (int64_t)e->trav_slots;
+    out8[3] = (int64_t)e->trav_slots * kSlotInts * 4;
     out8[4] = e->cap_n;
     out8[5] = e->cap_k;
     int ng = 0;
