@@ -33,13 +33,10 @@ constexpr int kGroupMax = 192;       // largest group a wave takes (3 particles 
 constexpr int kGroupChunk = 128;    // quarter tiles per group-numbering chunk
 constexpr int kChunkTiles = 32;      // tiles per sort-histogram and aggregation chunk
 constexpr int kTileShift = 10;       // kTile == 1 << kTileShift
-constexpr int kChunkShift = 5;       // kChunkTiles == 1 << kChunkShift
-static_assert(kTile == 1 << kTileShift && kChunkTiles == 1 << kChunkShift, "shifts");
-
-// Instances of more tiles than this get the chunk aggregation pass between
-// the tile and instance passes (below it the extra launch costs more than
-// the shorter instance pass saves).
-constexpr int kMidTiles = 512;
+static_assert(kTile == 1 << kTileShift, "shift");
+// Aggregation subtrees: the maximal subtrees of at most kATile particles are
+// finished one per block, the nodes above them by the instance pass.
+constexpr int kATile = kTile;
 constexpr int kStackCap = 8192;      // per-wave traversal stack entries
 constexpr int kNumXcd = 8;           // MI355X: 8 XCDs, blocks dealt round-robin
 
@@ -66,11 +63,12 @@ struct SegInfo {
 
 // Aggregation chunk: kChunkTiles consecutive tiles of one instance (also the
 // sort's histogram chunk).
-struct ChunkInfo {
-    int start;  // first particle
+// Aggregation subtree: a maximal subtree of at most kATile particles.
+struct ATileInfo {
+    int first;  // first particle (global index)
     int count;  // particles
-    int seg;
-    int pad;
+    int seg;    // instance slot
+    int root;   // its root node (a leaf when count == 1)
 };
 
 // Traversal group: up to group_size (<= kGroupMax) consecutive particles of one

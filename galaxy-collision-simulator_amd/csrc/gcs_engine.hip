@@ -121,9 +121,14 @@ struct gcs_engine {
     int* ngroups = nullptr;
     int* front = nullptr;   // aggregation frontier: instance s uses [2*start_s, 2*(start_s+n_s))
     int* nfront = nullptr;
-    int* cfront = nullptr;  // chunk frontier: chunk c uses [2*start_c, 2*(start_c+n_c))
-    int* ncfront = nullptr;
-    ChunkInfo* chunks = nullptr;
+    // aggregation subtrees (maximal subtrees of at most kATile particles)
+    uint32_t* amark = nullptr;
+    uint32_t* aq = nullptr;      // subtrees per quarter tile
+    uint32_t* achunk = nullptr;  // subtrees per kGroupChunk quarter tiles
+    int* acnt = nullptr;
+    int* aroot = nullptr;
+    ATileInfo* atiles = nullptr;
+    int* natiles = nullptr;
     uint32_t* hist = nullptr;   // per-tile digit counts of the current pass
     uint32_t* chist = nullptr;  // per-chunk digit counts, one block per pass
     uint32_t* seg_bounds = nullptr;
@@ -143,11 +148,9 @@ struct gcs_engine {
     std::vector<Instance> inst;
     std::vector<SegInfo> h_segs;
     std::vector<TileInfo> h_tiles;
-    std::vector<ChunkInfo> h_chunks;
     int64_t n_total = 0;
     int32_t next_id = 0;
     int nchunks = 0;  // sort-histogram chunks in use
-    bool any_mid = false;  // an instance has more than kMidTiles tiles
     bool have_tree = false;
     bool capture_keys = false;
     int last_mac = GCS_MAC_GROUP;
@@ -195,7 +198,7 @@ void free_all(gcs_engine* e)
     dfree(e->acc); dfree(e->keys_raw);
     dfree(e->node); dfree(e->nodeQ); dfree(e->boxLo); dfree(e->boxHi);
     dfree(e->parent); dfree(e->flag);
-    dfree(e->tiles); dfree(e->segs); dfree(e->chunks); dfree(e->cfront); dfree(e->ncfront); dfree(e->groups); dfree(e->hist); dfree(e->chist);
+    dfree(e->tiles); dfree(e->segs); dfree(e->amark); dfree(e->aq); dfree(e->achunk); dfree(e->acnt); dfree(e->aroot); dfree(e->atiles); dfree(e->natiles); dfree(e->groups); dfree(e->hist); dfree(e->chist);
     dfree(e->gmark); dfree(e->gq); dfree(e->gchunk); dfree(e->gcnt); dfree(e->ngroups);
     dfree(e->front); dfree(e->nfront);
     dfree(e->seg_bounds); dfree(e->bpart); dfree(e->qctr); dfree(e->counters); dfree(e->err);
@@ -209,7 +212,6 @@ int rebuild_tables(gcs_engine* e)
 {
     e->h_segs.clear();
     e->h_tiles.clear();
-    e->h_chunks.clear();
     int start = 0, chunks = 0;
     for (size_t s = 0; s < e->inst.size(); ++s) {
         const int n = e->inst[s].n;
@@ -223,14 +225,6 @@ int rebuild_tables(gcs_engine* e)
         sg.ntiles = (n + kTile - 1) / kTile;
         sg.chunk0 = chunks;
         chunks += (sg.ntiles + kChunkTiles - 1) / kChunkTiles;
-        for (int c = 0; c < (sg.ntiles + kChunkTiles - 1) / kChunkTiles; ++c) {
-            ChunkInfo ch;
-            ch.start = start + c * kChunkTiles * kTile;
-            ch.count = std::min(kChunkTiles * kTile, n - c * kChunkTiles * kTile);
-            ch.seg = (int)s;
-            ch.pad = 0;
-            e->h_chunks.push_back(ch);
-        }
         for (int j = 0; j < sg.ntiles; ++j) {
             TileInfo t;
             t.start = start + j * kTile;
@@ -245,14 +239,10 @@ int rebuild_tables(gcs_engine* e)
     }
     e->n_total = start;
     e->nchunks = chunks;
-    e->any_mid = false;
-    for (const SegInfo& sg : e->h_segs) e->any_mid = e->any_mid || sg.ntiles > kMidTiles;
     if (!e->h_segs.empty()) {
         HIPCHK(hipMemcpy(e->segs, e->h_segs.data(), sizeof(SegInfo) * e->h_segs.size(),
                          hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(e->tiles, e->h_tiles.data(), sizeof(TileInfo) * e->h_tiles.size(),
-                         hipMemcpyHostToDevice));
-        HIPCHK(hipMemcpy(e->chunks, e->h_chunks.data(), sizeof(ChunkInfo) * e->h_chunks.size(),
                          hipMemcpyHostToDevice));
     }
     e->graph_valid = false;
@@ -317,6 +307,9 @@ void mark(gcs_engine* e, int k)
     if (e->timed) (void)hipEventRecord(e->ev[k], e->stream);
 }
 
+Marks group_marks(gcs_engine* e) { return Marks{e->gmark, e->gq, e->gcnt, nullptr}; }
+Marks atile_marks(gcs_engine* e) { return Marks{e->amark, e->aq, e->acnt, e->aroot}; }
+
 // Queues one step on the stream.  Reads state[cur], leaves the result in
 // state[cur ^ 1] and flips cur.
 // The eight sort passes over keys[0]/vals[0] (sorted result back in
@@ -354,7 +347,7 @@ int enqueue_step(gcs_engine* e, const gcs_params& p, bool integrate)
     const int rblocks = (int)std::max<int64_t>((nseg + 255) / 256,
                                                std::min<int64_t>(1024, (cwords + 255) / 256));
     k_reset<<<rblocks, 256, 0, e->stream>>>(nseg, e->seg_bounds, e->qctr, e->counters, e->nfront,
-                                            e->chist, cwords, e->ncfront, e->nchunks);
+                                            e->chist, cwords);
     if (!p.use_bounds) {
         k_bounds<<<qblocks, kTileThreads, 0, e->stream>>>(e->tiles, e->segs, e->posm[c],
                                                            e->bpart);
@@ -373,31 +366,30 @@ int enqueue_step(gcs_engine* e, const gcs_params& p, bool integrate)
     mark(e, 2);
     k_permute<<<qblocks, kTileThreads, 0, e->stream>>>(
         e->tiles, e->segs, e->vals[0], e->posm[c], e->vel[c], e->plo[c], e->posm[c ^ 1],
-        e->vel[c ^ 1], e->plo[c ^ 1], comp ? 1 : 0, group ? 0 : 1, e->node, e->nodeQ, e->gmark,
-        e->gq);
+        e->vel[c ^ 1], e->plo[c ^ 1], comp ? 1 : 0, group ? 0 : 1, e->node, e->nodeQ,
+        group_marks(e), atile_marks(e));
     e->cur = c ^ 1;
     float4* posm = e->posm[e->cur];
     mark(e, 3);
     k_build<<<qblocks, kTileThreads, 0, e->stream>>>(e->tiles, e->segs, e->keys[0], e->node,
-                                                     e->parent, e->flag, e->gmark, e->gq, e->gcnt,
+                                                     e->parent, e->flag, group_marks(e),
+                                                     atile_marks(e),
                                                      group ? p.group_size : kGroup);
-    if (group) {
-        // Number the groups in particle order and write the group table.
-        k_group_chunksum<<<ngchunk, kGroupChunk, 0, e->stream>>>(e->gq, qblocks, e->gchunk);
-        k_group_emit<<<qblocks, kTileThreads, 0, e->stream>>>(
-            e->tiles, e->segs, e->gmark, e->gq, e->gchunk, e->gcnt, e->groups, e->ngroups,
-            (int)e->n_total);
-    }
+    // Number the traversal groups and the aggregation subtrees in particle
+    // order and write their tables.
+    k_group_chunksum<<<ngchunk, kGroupChunk, 0, e->stream>>>(e->gq, e->aq, qblocks, e->gchunk,
+                                                             e->achunk);
+    k_group_emit<<<qblocks, kTileThreads, 0, e->stream>>>(
+        e->tiles, e->segs, group_marks(e), e->gchunk, atile_marks(e), e->achunk, e->groups,
+        e->ngroups, e->atiles, e->natiles, (int)e->n_total);
     mark(e, 4);
     const float inv_theta = p.theta > 0.0f ? 1.0f / p.theta : __builtin_inff();
-    k_aggregate<<<ntiles, kTile, 0, e->stream>>>(e->tiles, e->segs, posm, e->node, e->nodeQ,
-                                                 e->boxLo, e->boxHi, e->parent, e->front,
-                                                 e->nfront, e->cfront, e->ncfront, inv_theta,
-                                                 group ? 1 : 0);
-    if (e->any_mid)
-        k_aggregate_mid<<<e->nchunks, 1024, 0, e->stream>>>(
-        e->chunks, e->segs, posm, e->node, e->nodeQ, e->boxLo, e->boxHi, e->parent, e->cfront,
-        e->ncfront, e->front, e->nfront, inv_theta, group ? 1 : 0);
+    // Persistent: two blocks per CU (their LDS) take the subtrees in turn.
+    const int ablocks = (int)std::max<int64_t>(1, std::min<int64_t>(2 * e->num_cu, e->n_total));
+    k_aggregate<<<ablocks, kATile, 0, e->stream>>>(e->atiles, e->natiles, e->segs, posm, e->node,
+                                                   e->nodeQ, e->boxLo, e->boxHi, e->parent,
+                                                   e->front, e->nfront, inv_theta,
+                                                   group ? 1 : 0);
     k_aggregate_top<<<nseg, 1024, 0, e->stream>>>(e->segs, posm, e->node, e->nodeQ, e->boxLo,
                                                   e->boxHi, e->parent, e->flag, e->front,
                                                   e->nfront, inv_theta, group ? 1 : 0);
@@ -554,7 +546,8 @@ int64_t gcs_device_bytes(int64_t cap_n, int32_t cap_k)
                    4 * (size_t)kSortPasses * kRadix * max_chunks(cap_n, cap_k));
     b += (int64_t)(4 * 4 * max_tiles(cap_n, cap_k) + 4 * max_gchunks(cap_n, cap_k));
     b += 8 * cap_k + 64 + 5 * 8 * 4 * (int64_t)max_tiles(cap_n, cap_k) + 5 * 8 * cap_k;
-    b += 8 * n + 20 * (int64_t)max_chunks(cap_n, cap_k);  // chunk frontier and table
+    b += 12 * n + 16 * (n + 1) + 4 * 4 * (int64_t)max_tiles(cap_n, cap_k) +
+         4 * (int64_t)max_gchunks(cap_n, cap_k) + 4;  // aggregation subtrees
     b += 4 * 4 * (int64_t)max_tiles(cap_n, cap_k) + 12 * (int64_t)cap_k;  // amax, dt, time
     b += 32 * n;               // staging
     return b;
@@ -614,9 +607,13 @@ int gcs_create(int64_t cap_n, int32_t cap_k, int32_t device, gcs_engine** out)
     if (!rc) rc = dalloc(&e->ngroups, 1);
     if (!rc) rc = dalloc(&e->front, 2 * n + 2);
     if (!rc) rc = dalloc(&e->nfront, (size_t)cap_k);
-    if (!rc) rc = dalloc(&e->cfront, 2 * n + 2);
-    if (!rc) rc = dalloc(&e->ncfront, max_chunks(cap_n, cap_k));
-    if (!rc) rc = dalloc(&e->chunks, max_chunks(cap_n, cap_k));
+    if (!rc) rc = dalloc(&e->amark, n);
+    if (!rc) rc = dalloc(&e->aq, 4 * nt);
+    if (!rc) rc = dalloc(&e->achunk, max_gchunks(cap_n, cap_k));
+    if (!rc) rc = dalloc(&e->acnt, n);
+    if (!rc) rc = dalloc(&e->aroot, n);
+    if (!rc) rc = dalloc(&e->atiles, n + 1);
+    if (!rc) rc = dalloc(&e->natiles, 1);
     if (!rc) rc = dalloc(&e->hist, he);
     if (!rc) rc = dalloc(&e->chist, (size_t)kSortPasses * kRadix * max_chunks(cap_n, cap_k));
     if (!rc) rc = dalloc(&e->seg_bounds, 2 * (size_t)cap_k);
@@ -1308,8 +1305,8 @@ int gcs_karras_tree(const uint64_t* sorted_keys, int32_t n, int32_t* left, int32
     if (!rc) {
         const int qblocks = 4 * (int)e->h_tiles.size();
         k_build<<<qblocks, kTileThreads, 0, e->stream>>>(e->tiles, e->segs, e->keys[0], e->node,
-                                                         e->parent, e->flag, e->gmark, e->gq,
-                                                         e->gcnt, kGroup);
+                                                         e->parent, e->flag, group_marks(e),
+                                                         atile_marks(e), kGroup);
         hipError_t le = hipGetLastError();
         if (le != hipSuccess) rc = fail(GCS_ERR_HIP, hipGetErrorString(le));
     }

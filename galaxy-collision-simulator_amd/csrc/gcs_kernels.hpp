@@ -30,12 +30,12 @@ __device__ __forceinline__ QuarterTile quarter_tile(const TileInfo* __restrict__
 }
 
 // Start-of-step reset: cube bounds to empty; work-queue, interaction and
-// aggregation-frontier counters (instances and chunks) and the sort's chunk
-// histograms (all passes) to zero.  A kernel rather than
-// hipMemsetAsync so the step can be captured into a hipGraph.
+// aggregation-frontier counters and the sort's chunk histograms (all passes)
+// to zero.  A kernel rather than hipMemsetAsync so the step can be captured
+// into a hipGraph.
 __global__ void k_reset(int nseg, uint32_t* seg_bounds, unsigned* qctr,
                         unsigned long long* counters, int* nfront, uint32_t* chist,
-                        int64_t chist_words, int* ncfront, int nchunks)
+                        int64_t chist_words)
 {
     const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
@@ -47,7 +47,6 @@ __global__ void k_reset(int nseg, uint32_t* seg_bounds, unsigned* qctr,
     if (tid < kNumXcd) qctr[tid] = 0u;
     if (tid < 2) counters[tid] = 0ull;
     for (int64_t i = tid; i < chist_words; i += stride) chist[i] = 0u;
-    for (int64_t i = tid; i < nchunks; i += stride) ncfront[i] = 0;
 }
 
 // main.cpp:152-166 -- per-instance cube [min over x,y,z, max over x,y,z], in
@@ -129,20 +128,35 @@ __global__ __launch_bounds__(256) void k_morton(const TileInfo* __restrict__ til
     vals[q.i] = (uint32_t)q.i;
 }
 
+// Subtree marks of one kind (traversal groups, aggregation subtrees): per
+// particle a flag at a subtree's first particle, its size and root node, and
+// the number of subtrees starting in each quarter tile.
+struct Marks {
+    uint32_t* mark;
+    uint32_t* perq;
+    int* cnt;
+    int* root;
+};
+
 // karras_bvh.cuh:44-70 -- gather the state into sorted order (and, for the
-// reference traversal, karras_bvh.cuh:134-164's leaf records).
+// reference traversal, karras_bvh.cuh:134-164's leaf records); clears the
+// subtree marks of the coming build.
 __global__ __launch_bounds__(256) void k_permute(
     const TileInfo* __restrict__ tiles, const SegInfo* __restrict__ segs,
     const uint32_t* __restrict__ perm, const float4* __restrict__ posm_in,
     const float4* __restrict__ vel_in, const float4* __restrict__ plo_in,
     float4* __restrict__ posm_out, float4* __restrict__ vel_out, float4* __restrict__ plo_out,
     int compensated, int write_leaves, float4* __restrict__ node, float4* __restrict__ nodeQ,
-    uint32_t* __restrict__ gmark, uint32_t* __restrict__ gq)
+    Marks gm, Marks am)
 {
     const QuarterTile q = quarter_tile(tiles, segs);
-    if (threadIdx.x == 0) gq[blockIdx.x] = 0u;
+    if (threadIdx.x == 0) {
+        gm.perq[blockIdx.x] = 0u;
+        am.perq[blockIdx.x] = 0u;
+    }
     if (!q.valid) return;
-    gmark[q.i] = 0u;
+    gm.mark[q.i] = 0u;
+    am.mark[q.i] = 0u;
     const int src = (int)perm[q.i];
     const float4 p = posm_in[src];
     posm_out[q.i] = p;
@@ -167,34 +181,32 @@ __device__ __forceinline__ int delta(int i, int j, int n, const uint64_t* __rest
     return clz64s(a ^ b);
 }
 
-// Marks a traversal group starting at global particle x with `count`
-// particles and counts it in its quarter tile.
-__device__ __forceinline__ void mark_group(const SegInfo& sg, int x, int count,
-                                           uint32_t* __restrict__ gmark, uint32_t* __restrict__ gq,
-                                           int* __restrict__ gcnt)
+// Marks a subtree starting at global particle x with `count` particles and
+// root node `root`, and counts it in its quarter tile.
+__device__ __forceinline__ void mark_subtree(const SegInfo& sg, int x, int count, int root,
+                                             const Marks& mk)
 {
-    gmark[x] = 1u;
-    gcnt[x] = count;
+    mk.mark[x] = 1u;
+    mk.cnt[x] = count;
+    if (mk.root) mk.root[x] = root;
     const int off = x - sg.start;
     const int qt = 4 * (sg.tile0 + off / kTile) + (off % kTile) / kTileThreads;
-    atomicAdd(&gq[qt], 1u);
+    atomicAdd(&mk.perq[qt], 1u);
 }
 
 // karras_bvh.cuh:73-131 -- one internal node per particle but the last of
 // each instance, built on instance-local indices so every instance's tree is
 // the stand-alone tree.  Writes range/split, parents and clears the arrival
-// flag of the node it builds.  Also marks traversal groups: the maximal
-// subtrees of at most gmax particles (a child of at most gmax under a
-// parent of more, or the root of a small instance) are flagged at their first
-// particle (gmark = 1) with their size (gcnt), and counted per quarter tile
-// (gq) for the numbering.
+// flag of the node it builds.  Also marks the maximal subtrees of at most
+// gmax particles (traversal groups) and of at most kATile particles
+// (aggregation subtrees) -- a child of at most that many under a parent of
+// more, or the root of a small instance -- at their first particle, with
+// their size (and root node), counted per quarter tile for the numbering.
 __global__ __launch_bounds__(256) void k_build(const TileInfo* __restrict__ tiles,
                                                const SegInfo* __restrict__ segs,
                                                const uint64_t* __restrict__ skeys,
                                                float4* __restrict__ node, int* __restrict__ parent,
-                                               int* __restrict__ flag, uint32_t* __restrict__ gmark,
-                                               uint32_t* __restrict__ gq, int* __restrict__ gcnt,
-                                               int gmax)
+                                               int* __restrict__ flag, Marks gm, Marks am, int gmax)
 {
     const QuarterTile q = quarter_tile(tiles, segs);
     if (!q.valid) return;
@@ -202,7 +214,8 @@ __global__ __launch_bounds__(256) void k_build(const TileInfo* __restrict__ tile
     const int n = sg.n, i = q.local;
     if (i == 0) {
         parent[sg.root] = -1;
-        if (n <= gmax) mark_group(sg, sg.start, n, gmark, gq, gcnt);
+        if (n <= gmax) mark_subtree(sg, sg.start, n, sg.root, gm);
+        if (n <= kATile) mark_subtree(sg, sg.start, n, sg.root, am);
     }
     if (i >= n - 1) return;
     const uint64_t* k = skeys + sg.start;
@@ -231,60 +244,98 @@ __global__ __launch_bounds__(256) void k_build(const TileInfo* __restrict__ tile
     parent[cl] = me;
     parent[cr] = me;
     flag[me] = 0;
+    const int nl = split - lo + 1, nr = hi - split;
     if (hi - lo + 1 > gmax) {
-        const int nl = split - lo + 1, nr = hi - split;
-        if (nl <= gmax) mark_group(sg, sg.start + lo, nl, gmark, gq, gcnt);
-        if (nr <= gmax) mark_group(sg, gsplit + 1, nr, gmark, gq, gcnt);
+        if (nl <= gmax) mark_subtree(sg, sg.start + lo, nl, cl, gm);
+        if (nr <= gmax) mark_subtree(sg, gsplit + 1, nr, cr, gm);
+    }
+    if (hi - lo + 1 > kATile) {
+        if (nl <= kATile) mark_subtree(sg, sg.start + lo, nl, cl, am);
+        if (nr <= kATile) mark_subtree(sg, gsplit + 1, nr, cr, am);
     }
 }
 
-// Groups per chunk of kGroupChunk quarter tiles.
+// Groups and aggregation subtrees per chunk of kGroupChunk quarter tiles.
 __global__ __launch_bounds__(kGroupChunk) void k_group_chunksum(const uint32_t* __restrict__ gq,
+                                                                 const uint32_t* __restrict__ aq,
                                                                  int nq,
-                                                                 uint32_t* __restrict__ gchunk)
+                                                                 uint32_t* __restrict__ gchunk,
+                                                                 uint32_t* __restrict__ achunk)
 {
     static_assert(kGroupChunk == 2 * kWave, "two waves per chunk");
-    __shared__ uint32_t w1;
+    __shared__ uint32_t w1[2];
     const int q = blockIdx.x * kGroupChunk + (int)threadIdx.x;
     const uint32_t v = (uint32_t)wave_sum(q < nq ? (int)gq[q] : 0);
-    if (threadIdx.x == kWave) w1 = v;
+    const uint32_t u = (uint32_t)wave_sum(q < nq ? (int)aq[q] : 0);
+    if (threadIdx.x == kWave) {
+        w1[0] = v;
+        w1[1] = u;
+    }
     __syncthreads();
-    if (threadIdx.x == 0) gchunk[blockIdx.x] = v + w1;
+    if (threadIdx.x == 0) {
+        gchunk[blockIdx.x] = v + w1[0];
+        achunk[blockIdx.x] = u + w1[1];
+    }
 }
 
-// Writes the group table in particle (Morton) order, and the number of
-// groups.  A quarter tile's first group number is the groups of earlier
-// chunks, plus those of earlier quarter tiles of its chunk, plus the marks
-// before each particle within the quarter tile (block scan).
+// First number of this block's quarter tile among subtrees of one kind: the
+// subtrees of earlier chunks plus those of earlier quarter tiles of its chunk.
+__device__ __forceinline__ uint32_t quarter_base(const uint32_t* __restrict__ perq,
+                                                 const uint32_t* __restrict__ chunk, int b,
+                                                 uint32_t* ws)
+{
+    const int c = b / kGroupChunk;
+    uint32_t part = 0u;
+    for (int k = threadIdx.x; k < c; k += 256) part += chunk[k];
+    for (int k = c * kGroupChunk + (int)threadIdx.x; k < b; k += 256) part += perq[k];
+    uint32_t base;
+    block_exclusive_scan(part, &base, ws);
+    return base;
+}
+
+// Writes the group table and the aggregation-subtree table in particle
+// (Morton) order, and their lengths.  A subtree's number is its quarter
+// tile's first number plus the marks before its first particle within the
+// quarter tile (block scan).
 __global__ __launch_bounds__(256) void k_group_emit(const TileInfo* __restrict__ tiles,
-                                                    const SegInfo* __restrict__ segs,
-                                                    const uint32_t* __restrict__ gmark,
-                                                    const uint32_t* __restrict__ gq,
-                                                    const uint32_t* __restrict__ gchunk,
-                                                    const int* __restrict__ gcnt,
+                                                    const SegInfo* __restrict__ segs, Marks gm,
+                                                    const uint32_t* __restrict__ gchunk, Marks am,
+                                                    const uint32_t* __restrict__ achunk,
                                                     GroupInfo* __restrict__ groups,
-                                                    int* __restrict__ ngroups, int n_total)
+                                                    int* __restrict__ ngroups,
+                                                    ATileInfo* __restrict__ atiles,
+                                                    int* __restrict__ natiles, int n_total)
 {
     __shared__ uint32_t ws[4];
     const QuarterTile q = quarter_tile(tiles, segs);
-    const int b = blockIdx.x, c = b / kGroupChunk;
-    uint32_t part = 0u;
-    for (int k = threadIdx.x; k < c; k += 256) part += gchunk[k];
-    for (int k = c * kGroupChunk + (int)threadIdx.x; k < b; k += 256) part += gq[k];
-    uint32_t base;
-    block_exclusive_scan(part, &base, ws);
-    const uint32_t m = q.valid ? gmark[q.i] : 0u;
-    uint32_t nb;
+    const int b = blockIdx.x;
+    const uint32_t gbase = quarter_base(gm.perq, gchunk, b, ws);
+    const uint32_t abase = quarter_base(am.perq, achunk, b, ws);
+    const uint32_t m = q.valid ? gm.mark[q.i] : 0u;
+    const uint32_t ma = q.valid ? am.mark[q.i] : 0u;
+    uint32_t nb, na;
     const uint32_t ex = block_exclusive_scan(m, &nb, ws);
+    const uint32_t exa = block_exclusive_scan(ma, &na, ws);
     if (m) {
         GroupInfo g;
         g.first = q.i;
-        g.count = gcnt[q.i];
+        g.count = gm.cnt[q.i];
         g.seg = q.seg;
         g.pad = 0;
-        groups[base + ex] = g;
+        groups[gbase + ex] = g;
     }
-    if (q.valid && q.i == n_total - 1) *ngroups = (int)(base + nb);
+    if (ma) {
+        ATileInfo a;
+        a.first = q.i;
+        a.count = am.cnt[q.i];
+        a.seg = q.seg;
+        a.root = am.root[q.i];
+        atiles[abase + exa] = a;
+    }
+    if (q.valid && q.i == n_total - 1) {
+        *ngroups = (int)(gbase + nb);
+        *natiles = (int)(abase + na);
+    }
 }
 
 // Aggregate of one node: COM, mass, AABB, quadrupole (xx, yy, zz, xy, xz, yz).
@@ -395,136 +446,101 @@ __device__ __forceinline__ void store_node(int v, const NodeData& d, float w, fl
 // node the first of the two arrivals stops, the second loads only the sibling
 // and merges.
 //
-// k_aggregate: one block per tile, one thread per leaf.  Nodes whose particle
-// range lies inside the tile are only reached by this block: their arrival
-// flags are bits of an LDS word (workgroup-scope acquire-release), their
-// aggregates are kept in LDS (slot = the end of the range that numbers the
-// node, minus the tile start) besides global memory, and the tile's particles
-// are staged in LDS, so sibling reads stay on the CU.  A thread that reaches a
-// node whose range crosses the tile appends the child it came from to its
-// instance's frontier list and stops.
-//
-// Slot of this lane in a list that the lanes with `pred` append to through
-// `counter` (uniform over the active lanes): one atomic per wave; lanes
-// without pred get -1.
-__device__ __forceinline__ int wave_append(int* counter, bool pred)
+// k_aggregate: one aggregation subtree (a maximal subtree of at most kATile
+// particles) per block at a time, one thread per leaf.  Every node of the
+// subtree is reached only by this block: its arrival flag is a bit of an LDS
+// word (workgroup-scope acquire-release) and its aggregate is kept in LDS
+// (slot = the end of the range that numbers the node, minus the subtree's
+// first particle) besides global memory, so sibling reads stay on the CU.
+// The thread that completes the subtree's root appends it to its instance's
+// frontier for the instance pass.
+__device__ __forceinline__ NodeData lds_node(const float4 (*ldat)[4], const float* lqt, int s)
 {
-    const uint64_t m = ballot(pred);
-    if (m == 0ull) return -1;
-    const int leader = __ffsll((unsigned long long)m) - 1;
-    int base = 0;
-    if (lane_id() == leader) base = atomicAdd(counter, __popcll(m));
-    base = __shfl(base, leader);
-    return pred ? base + __popcll(m & ((1ull << lane_id()) - 1ull)) : -1;
+    const float4 a4 = ldat[s][0], b4 = ldat[s][1], c4 = ldat[s][2], d4 = ldat[s][3];
+    NodeData d;
+    d.qt = lqt[s];
+    d.cx = a4.x; d.cy = a4.y; d.cz = a4.z; d.m = a4.w;
+    d.lx = b4.x; d.ly = b4.y; d.lz = b4.z; d.hx = b4.w;
+    d.hy = c4.x; d.hz = c4.y; d.q0 = c4.z; d.q1 = c4.w;
+    d.q2 = d4.x; d.q3 = d4.y; d.q4 = d4.z; d.q5 = d4.w;
+    return d;
 }
 
-// k_aggregate_top: one block per instance finishes the tile-crossing nodes,
-// climbing from the frontier with arrival marks at workgroup scope (bits in
-// LDS, or counters in global memory for very large instances) -- every
-// arrival at a node of an instance is in that instance's block, so no
-// write-back or invalidate across XCDs is needed (the ordering the
-// reference's plain atomicAdd, with no fence, leaves out); the children
-// written by k_aggregate are visible because it completed before this launch.
-__global__ __launch_bounds__(kTile) void k_aggregate(
-    const TileInfo* __restrict__ tiles, const SegInfo* __restrict__ segs,
-    const float4* __restrict__ posm, float4* node, float4* nodeQ, float4* boxLo, float4* boxHi,
-    const int* __restrict__ parent, int* __restrict__ front, int* __restrict__ nfront,
-    int* __restrict__ cfront, int* __restrict__ ncfront, float inv_theta, int mode)
+__global__ __launch_bounds__(kATile) void k_aggregate(
+    const ATileInfo* __restrict__ atiles, const int* __restrict__ natiles_dev,
+    const SegInfo* __restrict__ segs, const float4* __restrict__ posm, float4* node,
+    float4* nodeQ, float4* boxLo, float4* boxHi, const int* __restrict__ parent,
+    int* __restrict__ front, int* __restrict__ nfront, float inv_theta, int mode)
 {
-    __shared__ uint32_t lflag[kTile / 32];
-    __shared__ float4 ldat[kTile][4];
-    __shared__ float lqt[kTile];
-    __shared__ float4 lpos[kTile];
-    const TileInfo t = tiles[blockIdx.x];
+    __shared__ uint32_t lflag[kATile / 32];
+    __shared__ float4 ldat[kATile][4];
+    __shared__ float lqt[kATile];
     const int tid = threadIdx.x;
-    if (tid < kTile / 32) lflag[tid] = 0u;
-    if (tid < t.count) lpos[tid] = posm[t.start + tid];
-    __syncthreads();
-    if (tid >= t.count) return;
-    const SegInfo sg = segs[t.seg];
-    const int t0 = t.start, t1 = t.start + t.count - 1;
-    int child = t.start + tid + sg.leaf_base;
-    NodeData mine = leaf_data(lpos[tid]);
-    int cur = parent[child];
-    // The node record and parent of the next level are loaded one level ahead.
-    float4 bb = make_float4(0.f, 0.f, 0.f, 0.f);
-    int par = -1;
-    if (cur != -1) {
-        bb = node[2 * cur + 1];
-        par = parent[cur];
-    }
-    bool crossed = false;  // stopped below a node that crosses the tile
-    while (cur != -1) {
-        const int first = __float_as_int(bb.y), last = __float_as_int(bb.z),
-                  split = __float_as_int(bb.w);
-        if (first < t0 || last > t1) {
-            crossed = true;
-            break;
+    const int natiles = *natiles_dev;
+    for (int a = blockIdx.x; a < natiles; a += gridDim.x) {
+        const ATileInfo t = atiles[a];
+        __syncthreads();  // the previous subtree's LDS reads are done
+        if (tid < kATile / 32) lflag[tid] = 0u;
+        __syncthreads();
+        if (tid >= t.count) continue;
+        const SegInfo sg = segs[t.seg];
+        const int t0 = t.first;
+        int child = t0 + tid + sg.leaf_base;
+        if (t.count == 1) {
+            front[2 * sg.start + atomicAdd(&nfront[t.seg], 1)] = child;
+            continue;
         }
-        float4 bb_next = make_float4(0.f, 0.f, 0.f, 0.f);
-        int par_next = -1;
-        if (par != -1) {
-            bb_next = node[2 * par + 1];
-            par_next = parent[par];
+        NodeData mine = leaf_data(posm[t0 + tid]);
+        int cur = parent[child];
+        // The node record and parent of the next level are loaded one level ahead.
+        float4 bb = node[2 * cur + 1];
+        int par = cur == t.root ? -1 : parent[cur];
+        for (;;) {
+            const int first = __float_as_int(bb.y), last = __float_as_int(bb.z),
+                      split = __float_as_int(bb.w);
+            float4 bb_next = make_float4(0.f, 0.f, 0.f, 0.f);
+            int par_next = -1;
+            if (par != -1) {
+                bb_next = node[2 * par + 1];
+                par_next = par == t.root ? -1 : parent[par];
+            }
+            const int slot = cur - sg.int_base - t0;
+            const uint32_t bit = 1u << (slot & 31);
+            const uint32_t old = __hip_atomic_fetch_or(&lflag[slot >> 5], bit, __ATOMIC_ACQ_REL,
+                                                       __HIP_MEMORY_SCOPE_WORKGROUP);
+            if (!(old & bit)) break;
+            const bool lleaf = first == split, rleaf = last == split + 1;
+            const int lidx = lleaf ? split + sg.leaf_base : split + sg.int_base;
+            const bool mine_left = child == lidx;
+            // The sibling: a leaf of this subtree, or one of its nodes in LDS.
+            const bool sleaf = mine_left ? rleaf : lleaf;
+            const int sp = mine_left ? split + 1 : split;  // its particle / numbering end
+            const NodeData sib = sleaf ? leaf_data(posm[sp]) : lds_node(ldat, lqt, sp - t0);
+            float w, open;
+            mine = mine_left ? merge(mine, sib, inv_theta, mode, w, open)
+                             : merge(sib, mine, inv_theta, mode, w, open);
+            ldat[slot][0] = make_float4(mine.cx, mine.cy, mine.cz, mine.m);
+            ldat[slot][1] = make_float4(mine.lx, mine.ly, mine.lz, mine.hx);
+            ldat[slot][2] = make_float4(mine.hy, mine.hz, mine.q0, mine.q1);
+            ldat[slot][3] = make_float4(mine.q2, mine.q3, mine.q4, mine.q5);
+            lqt[slot] = mine.qt;
+            store_node(cur, mine, w, open, node, nodeQ, boxLo, boxHi);
+            if (cur == t.root) {
+                front[2 * sg.start + atomicAdd(&nfront[t.seg], 1)] = cur;
+                break;
+            }
+            child = cur;
+            cur = par;
+            bb = bb_next;
+            par = par_next;
         }
-        const int slot = cur - sg.int_base - t0;
-        const uint32_t bit = 1u << (slot & 31);
-        const uint32_t old = __hip_atomic_fetch_or(&lflag[slot >> 5], bit, __ATOMIC_ACQ_REL,
-                                                   __HIP_MEMORY_SCOPE_WORKGROUP);
-        if (!(old & bit)) return;
-        const bool lleaf = first == split, rleaf = last == split + 1;
-        const int lidx = lleaf ? split + sg.leaf_base : split + sg.int_base;
-        const bool mine_left = child == lidx;
-        // The sibling: a leaf of this tile, or an intra-tile node in LDS.
-        const bool sleaf = mine_left ? rleaf : lleaf;
-        const int sp = mine_left ? split + 1 : split;  // its particle / numbering end
-        NodeData sib;
-        if (sleaf) {
-            sib = leaf_data(lpos[sp - t0]);
-        } else {
-            const int ss = sp - t0;
-            const float4 a = ldat[ss][0], b = ldat[ss][1], c = ldat[ss][2], d = ldat[ss][3];
-            sib.qt = lqt[ss];
-            sib.cx = a.x; sib.cy = a.y; sib.cz = a.z; sib.m = a.w;
-            sib.lx = b.x; sib.ly = b.y; sib.lz = b.z; sib.hx = b.w;
-            sib.hy = c.x; sib.hz = c.y; sib.q0 = c.z; sib.q1 = c.w;
-            sib.q2 = d.x; sib.q3 = d.y; sib.q4 = d.z; sib.q5 = d.w;
-        }
-        float w, open;
-        mine = mine_left ? merge(mine, sib, inv_theta, mode, w, open)
-                         : merge(sib, mine, inv_theta, mode, w, open);
-        ldat[slot][0] = make_float4(mine.cx, mine.cy, mine.cz, mine.m);
-        ldat[slot][1] = make_float4(mine.lx, mine.ly, mine.lz, mine.hx);
-        ldat[slot][2] = make_float4(mine.hy, mine.hz, mine.q0, mine.q1);
-        ldat[slot][3] = make_float4(mine.q2, mine.q3, mine.q4, mine.q5);
-        lqt[slot] = mine.qt;
-        store_node(cur, mine, w, open, node, nodeQ, boxLo, boxHi);
-        child = cur;
-        cur = par;
-        bb = bb_next;
-        par = par_next;
-    }
-    if (crossed) {
-        // The parent crosses the tile: it is finished by the chunk pass when
-        // the instance has one and the parent's range lies in the tile's
-        // chunk, else by the instance pass.
-        const int first = __float_as_int(bb.y), last = __float_as_int(bb.z);
-        const int cl = t.tile_in_seg / kChunkTiles;
-        const int cs = sg.start + cl * kChunkTiles * kTile;
-        const int ce = min(cs + kChunkTiles * kTile, sg.start + sg.n) - 1;
-        const bool to_chunk = sg.ntiles > kMidTiles && first >= cs && last <= ce;
-        // One atomic per wave and counter (both counters are uniform in the block).
-        const int kc = wave_append(&ncfront[sg.chunk0 + cl], to_chunk);
-        const int ki = wave_append(&nfront[t.seg], !to_chunk);
-        if (to_chunk) cfront[2 * cs + kc] = child;
-        else front[2 * sg.start + ki] = child;
     }
 }
 
 // Arrival marks of the top pass: bits in LDS indexed by node (instances of
-// up to kTopLdsNodes particles), an LDS hash of the tile-crossing nodes (at
-// most kTopHashMax of them: the table stays at most 3/4 full), or counters in
-// global memory.
+// up to kTopLdsNodes particles), an LDS hash of the nodes above the
+// aggregation subtrees (at most kTopHashMax of them: the table stays at most
+// 3/4 full), or counters in global memory.
 constexpr int kTopLdsNodes = 1 << 20;
 constexpr int kTopHashBits = 15;
 constexpr int kTopHashMax = 3 << (kTopHashBits - 2);
@@ -535,22 +551,10 @@ static_assert((1 << kTopHashBits) == kTopLdsNodes / 32, "hash and bitmap share t
 // parent the first of the two arrivals stops and the second merges and
 // goes on, for at most `levels` levels.  Returns the last node it completed
 // when it stopped because of the level limit (with more levels to go), else
-// -1.
-// The climb covers the nodes whose range lies in the span [lo, hi] (global
-// particle indices); a climber whose parent reaches outside it appends its
-// node to the outer frontier (out, nout) and stops.  A sibling whose range
-// lies in one unit (1 << unit_shift particles, counted from the instance start) was
-// finished by an earlier launch.  Bit marks are indexed by internal node
-// relative to `slot_base`.
-struct ClimbSpan {
-    int lo, hi, unit_shift, slot_base;  // unit = 1 << unit_shift particles
-    int* out;
-    int* nout;
-};
-
-template <bool BOUNDED>
+// -1.  A sibling of at most kATile particles (or a leaf) was finished by the
+// subtree pass; larger ones are finished in this launch.  Bit marks are
+// indexed by internal node relative to the instance start.
 __device__ __forceinline__ int top_climb(const SegInfo& sg, int child, int levels,
-                                         const ClimbSpan& span,
                                          const float4* __restrict__ posm, float4* node,
                                          float4* nodeQ, float4* boxLo, float4* boxHi,
                                          const int* __restrict__ parent, int* flag,
@@ -567,21 +571,16 @@ __device__ __forceinline__ int top_climb(const SegInfo& sg, int child, int level
         if (lv == levels) return child;
         const int first = __float_as_int(bb.y), last = __float_as_int(bb.z),
                   split = __float_as_int(bb.w);
-        if (BOUNDED && (first < span.lo || last > span.hi)) {
-            span.out[atomicAdd(span.nout, 1)] = child;
-            return -1;
-        }
         const bool lleaf = first == split, rleaf = last == split + 1;
         const int lidx = lleaf ? split + sg.leaf_base : split + sg.int_base;
         const bool mine_left = child == lidx;
         const bool sleaf = mine_left ? rleaf : lleaf;
         const int sp = mine_left ? split + 1 : split;
-        // A sibling whose range lies in one unit (or a leaf) was finished by an
-        // earlier launch: load it before the arrival check.  Any other sibling
+        // A sibling of at most kATile particles (or a leaf) was finished by the
+        // subtree pass: load it before the arrival check.  Any other sibling
         // is written in this launch and is read after the acquire.
         const int sf = mine_left ? split + 1 : first, sl = mine_left ? last : split;
-        const bool sfinal =
-            sleaf || ((sf - sg.start) >> span.unit_shift) == ((sl - sg.start) >> span.unit_shift);
+        const bool sfinal = sleaf || sl - sf + 1 <= kATile;
         NodeData sib;
         if (sfinal)
             sib = sleaf ? leaf_data(posm[sp])
@@ -598,7 +597,7 @@ __device__ __forceinline__ int top_climb(const SegInfo& sg, int child, int level
         // sibling's thread).
         bool second;
         if (marks == kMarksBits) {
-            const int slot = cur - sg.int_base - span.slot_base;
+            const int slot = cur - sg.int_base - sg.start;
             uint32_t* w = &lmem[slot >> 5];
             const uint32_t bit = 1u << (slot & 31);
             second =
@@ -659,11 +658,8 @@ __device__ __forceinline__ int top_climb(const SegInfo& sg, int child, int level
 }
 
 // Runs the climbers of list la[0, nf) (with la[n, 2n) as the second list of
-// the rounds below) for the whole block; BOUNDED: the span does not hold the
-// whole instance.
-template <bool BOUNDED>
+// the rounds below) for the whole block.
 __device__ __forceinline__ void climb_rounds(const SegInfo& sg, int* la, int n, int nf,
-                                             const ClimbSpan& span,
                                              const float4* __restrict__ posm, float4* node,
                                              float4* nodeQ, float4* boxLo, float4* boxHi,
                                              const int* __restrict__ parent, int* flag,
@@ -676,8 +672,8 @@ __device__ __forceinline__ void climb_rounds(const SegInfo& sg, int* la, int n, 
         if (threadIdx.x == 0) *lcount = 0;
         __syncthreads();
         for (int k = threadIdx.x; k < count; k += blockDim.x) {
-            const int v = top_climb<BOUNDED>(sg, la[k], 1, span, posm, node, nodeQ, boxLo,
-                                             boxHi, parent, flag, lmem, marks, inv_theta, mode);
+            const int v = top_climb(sg, la[k], 1, posm, node, nodeQ, boxLo, boxHi, parent, flag,
+                                    lmem, marks, inv_theta, mode);
             if (v >= 0) lb[atomicAdd(lcount, 1)] = v;
         }
         __syncthreads();
@@ -689,10 +685,18 @@ __device__ __forceinline__ void climb_rounds(const SegInfo& sg, int* la, int n, 
     }
     __syncthreads();
     for (int k = threadIdx.x; k < count; k += blockDim.x)
-        top_climb<BOUNDED>(sg, la[k], 1 << 30, span, posm, node, nodeQ, boxLo, boxHi, parent,
-                           flag, lmem, marks, inv_theta, mode);
+        top_climb(sg, la[k], 1 << 30, posm, node, nodeQ, boxLo, boxHi, parent, flag, lmem,
+                  marks, inv_theta, mode);
 }
 
+// k_aggregate_top: one block per instance finishes the nodes of more than
+// kATile particles, climbing from the roots of the aggregation subtrees with
+// arrival marks at workgroup scope (bits in LDS, an LDS hash, or counters in
+// global memory for very large instances) -- every arrival at a node of an
+// instance is in that instance's block, so no write-back or invalidate across
+// XCDs is needed (the ordering the reference's plain atomicAdd, with no
+// fence, leaves out); the subtrees written by k_aggregate are visible because
+// it completed before this launch.
 // While the frontier holds more climbers than the block has threads, every
 // climber advances one level per round and the ones that go on (second
 // arrivals) form the next, much shorter list; then each remaining climber
@@ -710,7 +714,7 @@ __global__ __launch_bounds__(1024) void k_aggregate_top(
     const int nf = nfront[s];
     if (nf == 0) return;
     const SegInfo sg = segs[s];
-    // nf - 1 tile-crossing nodes get a mark.
+    // nf - 1 nodes above the aggregation subtrees get a mark.
     const int marks = sg.n <= kTopLdsNodes ? kMarksBits
                       : nf <= kTopHashMax  ? kMarksHash
                                            : kMarksGlobal;
@@ -718,43 +722,8 @@ __global__ __launch_bounds__(1024) void k_aggregate_top(
         for (int k = threadIdx.x; k < (sg.n + 31) / 32; k += blockDim.x) lmem[k] = 0u;
     else if (marks == kMarksHash)
         for (int k = threadIdx.x; k < (1 << kTopHashBits); k += blockDim.x) lmem[k] = 0xFFFFFFFFu;
-    ClimbSpan span;
-    span.lo = sg.start;
-    span.hi = sg.start + sg.n - 1;
-    span.unit_shift = sg.ntiles > kMidTiles ? kChunkShift + kTileShift : kTileShift;
-    span.slot_base = sg.start;
-    span.out = nullptr;  // the instance span contains every node
-    span.nout = nullptr;
-    climb_rounds<false>(sg, front + 2 * (size_t)sg.start, sg.n, nf, span, posm, node, nodeQ,
-                        boxLo, boxHi, parent, flag, lmem, marks, inv_theta, mode, &lcount);
-}
-
-// k_aggregate_mid: one block per chunk finishes the nodes that cross its
-// tiles but not its bounds (arrival bits in LDS), and hands the chunk's
-// outermost nodes to the instance pass.
-__global__ __launch_bounds__(1024) void k_aggregate_mid(
-    const ChunkInfo* __restrict__ chunks, const SegInfo* __restrict__ segs,
-    const float4* __restrict__ posm, float4* node, float4* nodeQ, float4* boxLo, float4* boxHi,
-    const int* __restrict__ parent, int* cfront, const int* __restrict__ ncfront, int* front,
-    int* nfront, float inv_theta, int mode)
-{
-    __shared__ uint32_t lbits[kChunkTiles * kTile / 32];
-    __shared__ int lcount;
-    const ChunkInfo c = chunks[blockIdx.x];
-    const int nf = ncfront[blockIdx.x];
-    if (nf == 0) return;
-    const SegInfo sg = segs[c.seg];
-    for (int k = threadIdx.x; k < (c.count + 31) / 32; k += blockDim.x) lbits[k] = 0u;
-    ClimbSpan span;
-    span.lo = c.start;
-    span.hi = c.start + c.count - 1;
-    span.unit_shift = kTileShift;
-    span.slot_base = c.start;
-    span.out = front + 2 * (size_t)sg.start;
-    span.nout = &nfront[c.seg];
-    climb_rounds<true>(sg, cfront + 2 * (size_t)c.start, c.count, nf, span, posm, node, nodeQ,
-                       boxLo, boxHi, parent, nullptr, lbits, kMarksBits, inv_theta, mode,
-                       &lcount);
+    climb_rounds(sg, front + 2 * (size_t)sg.start, sg.n, nf, posm, node, nodeQ, boxLo, boxHi,
+                 parent, flag, lmem, marks, inv_theta, mode, &lcount);
 }
 
 __device__ __forceinline__ uint8_t to_u8(float f) { return (uint8_t)f2u_sat(f); }
