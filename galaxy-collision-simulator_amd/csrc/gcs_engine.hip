@@ -271,6 +271,8 @@ int check_params(const gcs_params* p)
             return fail(GCS_ERR_INVALID, "waves_per_simd must be in [1, 4]");
         if (p->group_size < 1 || p->group_size > kGroupMax)
             return fail(GCS_ERR_INVALID, "group_size must be in [1, 192]");
+        if (p->open_offset != 0 && p->open_offset != 1)
+            return fail(GCS_ERR_INVALID, "open_offset must be 0 or 1");
     } else {
         if (p->stack_depth < 2 || p->stack_depth > 1024)
             return fail(GCS_ERR_INVALID, "stack_depth must be in [2, 1024]");
@@ -384,15 +386,18 @@ int enqueue_step(gcs_engine* e, const gcs_params& p, bool integrate)
         e->ngroups, e->atiles, e->natiles, (int)e->n_total);
     mark(e, 4);
     const float inv_theta = p.theta > 0.0f ? 1.0f / p.theta : __builtin_inff();
+    // Opening value of the merge: width (reference traversal), (w/theta)^2 or
+    // (w/theta + |COM - box centre|)^2 (group traversal).
+    const int amode = group ? (p.open_offset ? 2 : 1) : 0;
     // Persistent: two blocks per CU (their LDS) take the subtrees in turn.
     const int ablocks = (int)std::max<int64_t>(1, std::min<int64_t>(2 * e->num_cu, e->n_total));
     k_aggregate<<<ablocks, kATile, 0, e->stream>>>(e->atiles, e->natiles, e->segs, posm, e->node,
                                                    e->nodeQ, e->boxLo, e->boxHi, e->parent,
                                                    e->front, e->nfront, inv_theta,
-                                                   group ? 1 : 0);
+                                                   amode);
     k_aggregate_top<<<nseg, 1024, 0, e->stream>>>(e->segs, posm, e->node, e->nodeQ, e->boxLo,
                                                   e->boxHi, e->parent, e->flag, e->front,
-                                                  e->nfront, inv_theta, group ? 1 : 0);
+                                                  e->nfront, inv_theta, amode);
     LAUNCHCHK();
     mark(e, 5);
     if (group) {

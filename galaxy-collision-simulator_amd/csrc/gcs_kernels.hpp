@@ -392,9 +392,10 @@ __device__ __forceinline__ void quad_shift(NodeData& o, const NodeData& c)
 // Merge of one internal node from its children (karras_bvh.cuh:190-221):
 //   AABB = union; width = max extent; m = ml + mr;
 //   com = (cl*ml + cr*mr)/m, or the AABB centre when m == 0.
-// Extension: opening value = (width*inv_theta + |com - AABB centre|)^2 for the
-// group traversal (mode 1), or width for the reference traversal (mode 0);
-// quadrupole moments by the parallel-axis shift.
+// Extension: opening value = width for the reference traversal (mode 0); for
+// the group traversal (width*inv_theta)^2 (mode 1) or
+// (width*inv_theta + |com - AABB centre|)^2 (mode 2); quadrupole moments by the
+// parallel-axis shift.
 __device__ __forceinline__ NodeData merge(const NodeData& l, const NodeData& r, float inv_theta,
                                           int mode, float& w, float& open)
 {
@@ -413,12 +414,15 @@ __device__ __forceinline__ NodeData merge(const NodeData& l, const NodeData& r, 
         o.cy = (o.ly + o.hy) * 0.5f;
         o.cz = (o.lz + o.hz) * 0.5f;
     }
-    if (mode == 1) {
+    if (mode == 2) {
         const float dx = o.cx - (o.lx + o.hx) * 0.5f;
         const float dy = o.cy - (o.ly + o.hy) * 0.5f;
         const float dz = o.cz - (o.lz + o.hz) * 0.5f;
         const float off = sqrtf(dx * dx + dy * dy + dz * dz);
         const float ro = w * inv_theta + off;
+        open = ro * ro;
+    } else if (mode == 1) {
+        const float ro = w * inv_theta;
         open = ro * ro;
     } else {
         open = w;

@@ -44,7 +44,8 @@ class Params(ctypes.Structure):
                 ("global_max", ctypes.c_float), ("colours", ctypes.c_int32),
                 ("diagnostics", ctypes.c_int32), ("use_graph", ctypes.c_int32),
                 ("waves_per_simd", ctypes.c_int32), ("dt_eta", ctypes.c_float),
-                ("group_size", ctypes.c_int32), ("reserved", ctypes.c_int32 * 2)]
+                ("group_size", ctypes.c_int32), ("open_offset", ctypes.c_int32),
+                ("reserved", ctypes.c_int32)]
 
 
 class CollisionSpec(ctypes.Structure):

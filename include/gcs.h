@@ -96,7 +96,10 @@ typedef struct gcs_params {
                            max|a|)) with eps = sqrt(eps2); 0: fixed dt */
     int32_t group_size; /* largest traversal group (maximal subtree), 1..192
                            (GCS_MAC_GROUP) */
-    int32_t reserved[2];
+    int32_t open_offset;/* GCS_MAC_GROUP opening radius: 0 = w/theta, the
+                           reference's theta test taken from the group's box;
+                           1 = w/theta + |COM - box centre| (stricter) */
+    int32_t reserved;
 } gcs_params;
 
 /* Defaults: G=1, eps2=0.0025, dt=0.01, theta=0.5, GCS_MAC_GROUP,

@@ -323,12 +323,12 @@ static void quad_shift(float* q, const float* qc, const float* cc, const float* 
  * the atomic climb exactly.  Children read: AABB, mass, COM.  Written:
  *   AABB = union; width = max extent (:206); m = ml + mr;
  *   com = (cl*ml + cr*mr)/m, or the AABB centre when m == 0 (:208-221).
- * Extension (not in the reference): crit = (width*inv_theta + |com - c|)^2
- * with c the AABB centre, the opening radius squared used by the group
- * traversal. */
+ * Extension (not in the reference): crit = (width*inv_theta)^2, or with
+ * offset != 0 (width*inv_theta + |com - c|)^2 with c the AABB centre, the
+ * opening radius squared used by the group traversal. */
 void orc_aggregate(int32_t n, const int32_t* left, const int32_t* right,
                    const float* x, const float* y, const float* z,
-                   const float* m, float inv_theta, float* com4, float* width,
+                   const float* m, float inv_theta, int32_t offset, float* com4, float* width,
                    float* bmin, float* bmax, float* crit, float* quad)
 {
     for (int32_t i = 0; i < n; ++i) {
@@ -371,7 +371,7 @@ void orc_aggregate(int32_t n, const int32_t* left, const int32_t* right,
         float dy = com4[4 * v + 1] - (lo[1] + hi[1]) * 0.5f;
         float dz = com4[4 * v + 2] - (lo[2] + hi[2]) * 0.5f;
         float off = sqrtf(dx * dx + dy * dy + dz * dz);
-        float ro = w * inv_theta + off;
+        float ro = offset ? w * inv_theta + off : w * inv_theta;
         crit[v] = ro * ro;
         if (quad) {
             float q[7] = {0, 0, 0, 0, 0, 0, 0};

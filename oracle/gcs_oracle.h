@@ -45,14 +45,14 @@ void orc_node_ranges(int32_t n, const int32_t* left, const int32_t* right,
 
 /* ---- Bottom-up aggregation (karras_bvh.cuh:134-225) ---------------------
  * com4: (2n-1)*4 floats (com x,y,z, mass); width: 2n-1; bmin/bmax: (2n-1)*3;
- * crit: (2n-1) opening radius squared (w*inv_theta + |com - box centre|)^2,
- * 0 for leaves.  quad (optional, (2n-1)*7): traceless quadrupole moment
+ * crit: (2n-1) opening radius squared, (w*inv_theta)^2, or with offset != 0
+ * (w*inv_theta + |com - box centre|)^2; 0 for leaves.  quad (optional, (2n-1)*7): traceless quadrupole moment
  * about the COM, components xx, yy, zz, xy, xz, yz, then the trace T of the
  * raw second moment sum m |x - COM|^2 (0 for leaves). */
 void orc_aggregate(int32_t n, const int32_t* left, const int32_t* right,
                    const float* x, const float* y, const float* z,
-                   const float* m, float inv_theta, float* com4, float* width,
-                   float* bmin, float* bmax, float* crit, float* quad);
+                   const float* m, float inv_theta, int32_t offset, float* com4,
+                   float* width, float* bmin, float* bmax, float* crit, float* quad);
 
 /* ---- Reference GPU traversal semantics (cuda_engine.cu:91-161) ---------
  * Per-particle DFS from root 0; softened distance; internal accepted when

@@ -34,7 +34,7 @@ _SIG = {
     "orc_karras_build": ([_P, _I32, _P, _P, _P], None),
     "orc_radix_tree_definition": ([_P, _I32, _P, _P, _P], ctypes.c_int),
     "orc_node_ranges": ([_I32, _P, _P, _P, _P], None),
-    "orc_aggregate": ([_I32, _P, _P, _P, _P, _P, _P, _F, _P, _P, _P, _P, _P, _P], None),
+    "orc_aggregate": ([_I32, _P, _P, _P, _P, _P, _P, _F, _I32, _P, _P, _P, _P, _P, _P], None),
     "orc_traverse_reference": ([_I32, _P, _P, _P, _P, _P, _P, _P, _F, _F, _P, _P, _P], _I64),
     "orc_traverse_group": ([_I32, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _I32, _I32, _F,
                             _F, _P, _P, _P, _P, _P, _I32, _P], None),
@@ -127,7 +127,7 @@ def node_ranges(n, left, right):
     return f, la
 
 
-def aggregate(left, right, x, y, z, m, theta):
+def aggregate(left, right, x, y, z, m, theta, offset=False):
     n = len(x)
     x, y, z, m = c32(x), c32(y), c32(z), c32(m)
     k = 2 * n - 1
@@ -138,7 +138,8 @@ def aggregate(left, right, x, y, z, m, theta):
     crit = np.zeros(k, np.float32)
     quad = np.zeros((k, 7), np.float32)
     inv_theta = np.float32(np.inf) if theta == 0 else np.float32(1.0) / np.float32(theta)
-    lib().orc_aggregate(n, P(left), P(right), P(x), P(y), P(z), P(m), float(inv_theta), P(com4),
+    lib().orc_aggregate(n, P(left), P(right), P(x), P(y), P(z), P(m), float(inv_theta),
+                        1 if offset else 0, P(com4),
                         P(width), P(bmin), P(bmax), P(crit), P(quad))
     return dict(com4=com4, width=width, bmin=bmin, bmax=bmax, crit=crit, quad=quad)
 
@@ -263,7 +264,7 @@ def ref_octree_step(stars, G, eps2, dt, integrate=True, want_acc=False, fast=Fal
     return s, acc, ph
 
 
-def build_tree_pipeline(stars, theta, mn=None, mx=None):
+def build_tree_pipeline(stars, theta, mn=None, mx=None, offset=False):
     """Bounds, Morton, stable sort, Karras build, ranges and aggregation on the
     CPU for one instance.  Returns a dict with every intermediate."""
     x, y, z = c32(stars["x"]), c32(stars["y"]), c32(stars["z"])
@@ -275,6 +276,6 @@ def build_tree_pipeline(stars, theta, mn=None, mx=None):
     left, right, parent = karras_build(skeys)
     n = len(s)
     first, last = node_ranges(n, left, right) if n > 0 else (None, None)
-    agg = aggregate(left, right, s["x"], s["y"], s["z"], s["mass"], theta)
+    agg = aggregate(left, right, s["x"], s["y"], s["z"], s["mass"], theta, offset)
     return dict(mn=mn, mx=mx, keys=keys, skeys=skeys, perm=perm, sorted=s, left=left,
                 right=right, parent=parent, first=first, last=last, agg=agg)

@@ -114,7 +114,8 @@ def test_karras_tree_matches_oracle(gcs, orc, case):
 
 def check_pipeline(gcs, orc, e, stars, p, iid=0, theta=0.5, bounds=None):
     raw, srt, perm = e.keys()
-    cpu = orc.build_tree_pipeline(stars, theta, *(bounds or (None, None)))
+    cpu = orc.build_tree_pipeline(stars, theta, *(bounds or (None, None)),
+                                  offset=bool(p.open_offset))
     if bounds is None:
         mn, mx = e.bounds(iid)
         assert (mn, mx) == (np.float32(cpu["mn"]), np.float32(cpu["mx"]))
@@ -130,16 +131,16 @@ def check_pipeline(gcs, orc, e, stars, p, iid=0, theta=0.5, bounds=None):
     return cpu
 
 
-@pytest.mark.parametrize("gsize", [64, 24, 128, 192])
-def test_engine_c1_group_parity(gcs, orc, gsize):
+@pytest.mark.parametrize("gsize,offset", [(64, 0), (24, 0), (128, 0), (192, 0), (64, 1)])
+def test_engine_c1_group_parity(gcs, orc, gsize, offset):
     stars = gcs.collision(8192, 1001)
     n = len(stars)
     eps2 = np.float32(0.05) ** 2
     p = gcs.params(G=G, eps2=float(eps2), dt=0.01, theta=0.5, leaf_size=16, diagnostics=1,
-                   group_size=gsize)
+                   group_size=gsize, open_offset=offset)
     with gcs.Engine(n) as e:
         e.load(stars)
-        say("engine C1 compute, groups of at most %d" % gsize)
+        say("engine C1 compute, groups of at most %d, opening offset %d" % (gsize, offset))
         e.compute_accels(p)
         cpu = check_pipeline(gcs, orc, e, stars, p)
         acc = e.accels()

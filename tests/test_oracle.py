@@ -227,6 +227,40 @@ def rel_rms(a, b):
     return np.sqrt(((a - b) ** 2).sum() / (b ** 2).sum())
 
 
+@pytest.mark.parametrize("dist,classic,offset", [(1.9, 0, 0), (2.1, 1, 0), (2.4, 1, 1)])
+def test_group_opening_threshold(orc, dist, classic, offset):
+    # Particles of mass 3 and 1 at x = 0 and 1 form a node of width 1 whose
+    # COM (x = 0.25) sits 0.25 from its box centre; a third particle at
+    # distance `dist` from that COM is the only target that can accept it.
+    # At theta = 0.5 the node is taken whole iff dist > w/theta = 2 (the
+    # reference's s/d < theta), or with the offset form iff dist > 2.25.
+    x = np.array([0.0, 1.0, 0.25 + dist], np.float32)
+    st = np.zeros(3, orc.STAR)
+    st["x"], st["mass"] = x, np.array([3.0, 1.0, 1.0], np.float32)
+    for off, want in ((False, classic), (True, offset)):
+        t = orc.build_tree_pipeline(st, 0.5, offset=off)
+        s = t["sorted"]
+        acc, msum, cnt = orc.traverse_group(t["left"], t["right"], t["first"], t["last"],
+                                            s["x"], s["y"], s["z"], s["mass"], t["agg"], G, EPS2,
+                                            1, 1)
+        assert cnt[0] == want, (dist, off)
+        assert np.allclose(msum + s["mass"], 5.0)
+
+
+def test_opening_offset_only_adds(orc):
+    t0 = pipeline(orc, 3000, 12)
+    s = t0["sorted"]
+    a1 = orc.aggregate(t0["left"], t0["right"], s["x"], s["y"], s["z"], s["mass"], 0.5, True)
+    a0 = t0["agg"]
+    assert np.all(a1["crit"] >= a0["crit"])
+    assert np.array_equal(a1["com4"], a0["com4"]) and np.array_equal(a1["quad"], a0["quad"])
+    c0 = orc.traverse_group(t0["left"], t0["right"], t0["first"], t0["last"], s["x"], s["y"],
+                            s["z"], s["mass"], a0, G, EPS2, 64, 8)[2]
+    c1 = orc.traverse_group(t0["left"], t0["right"], t0["first"], t0["last"], s["x"], s["y"],
+                            s["z"], s["mass"], a1, G, EPS2, 64, 8)[2]
+    assert c0[0] + c0[1] < c1[0] + c1[1]
+
+
 def test_group_traversal_theta0_is_direct_sum(orc):
     t = pipeline(orc, 700, 5, theta=0.0)
     s = t["sorted"]
