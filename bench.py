@@ -3,7 +3,7 @@
 
 Workload (N=1): the plain GPU configuration -- one collision of 2 x 131,072
 particles (70/10/20 disc/bulge/halo), FP32, theta = 0.5, softening 0.02,
-dt = 0.005, leaf 4, groups of at most 48, quadrupole group traversal.  Inputs are generated on the
+dt = 0.005, leaf 4, groups of at most 96, quadrupole group traversal.  Inputs are generated on the
 host and resident in HBM before the timed region; each timed step is a full
 step (bounds, Morton, sort, permute, tree build, aggregation, traversal,
 kick-drift), replayed from a hipGraph.
@@ -33,7 +33,7 @@ TREE_BYTES_PER_PARTICLE = 376  # Morton 24 + sort 24 + permute 68 + init 88 + bu
 # WRITE_SIZE (x1.000) passes scaled by a 256 MiB copy (profiles/
 # k_traverse_group_counters_plain-2x131072.md); valid for the configuration
 # it was measured on.
-TRAVERSE_TRAFFIC = {("plain-2x131072", 4, 256, 3, 48): 262947564}
+TRAVERSE_TRAFFIC = {("plain-2x131072", 4, 256, 4, 96): 205134126}
 
 CONFIGS = {
     # name: (particles per galaxy, eps, dt, theta)
@@ -123,13 +123,13 @@ def precision_study(steps, targets, cpu_seconds, seed=1003):
             acc = e.accels()
             s0 = e.stars()
         run = {"theta": theta, "leaf": base.leaf_size, "block": base.block_size,
-               "waves_per_simd": base.waves_per_simd}
+               "waves_per_simd": base.waves_per_simd, "group": base.group_size}
         run.update(accuracy_check(s0, base, acc, targets))
         for prec, name in ((gcs_amd.PREC_PLAIN, "plain"),
                            (gcs_amd.PREC_COMPENSATED, "compensated")):
             kw = dict(G=1.0, eps2=eps2, dt=0.005, theta=theta, leaf_size=base.leaf_size,
                       block_size=base.block_size, waves_per_simd=base.waves_per_simd,
-                      precision=prec, use_graph=1)
+                      group_size=base.group_size, precision=prec, use_graph=1)
             pd = gcs_amd.params(diagnostics=1, **kw)
             pn = gcs_amd.params(diagnostics=0, **kw)
             with gcs_amd.Engine(n) as e:
@@ -216,14 +216,33 @@ def served_run(window, rate, max_resident, leaf, seed=1005, budget=200e9, steps_
 
     jobs = gcs_amd.poisson_jobs(seed, rate, window, 8192, 524288, 200, 2000)
     eps2 = float(np.float32(0.02) ** 2)
+    # Traversal settings for both legs: the autotuner on a forest of about
+    # 10M particles drawn from the same job-size distribution.
+    specs, tot = [], 0
+    while tot < 10_000_000:
+        sp = gcs_amd.random_spec(seed + 1, len(specs), 8192, 524288)
+        specs.append(sp)
+        tot += sp.n_a + sp.n_b
+    parts = [gcs_amd.generate(sp) for sp in specs]
+    with gcs_amd.Engine(tot, len(parts)) as e:
+        e.load(np.concatenate(parts), [len(a) for a in parts])
+        tp, _ = e.autotune(gcs_amd.params(G=1.0, eps2=eps2, dt=0.005, theta=0.5,
+                                          leaf_size=leaf, precision=gcs_amd.PREC_COMPENSATED,
+                                          use_graph=1), reps=2)
+    del parts
+    tuned = dict(leaf_size=tp.leaf_size, group_size=tp.group_size, block_size=tp.block_size,
+                 waves_per_simd=tp.waves_per_simd)
     out = {"window_s": window, "rate_jobs_per_s": rate, "jobs_offered": len(jobs),
            "offered_particle_steps_per_s": sum(n * k for _, n, k, _ in jobs) / window,
-           "hbm_budget_bytes": budget}
+           "hbm_budget_bytes": budget,
+           "autotuned": {"leaf": tp.leaf_size, "group": tp.group_size, "block": tp.block_size,
+                         "waves_per_simd": tp.waves_per_simd,
+                         "tuning_forest": {"jobs": len(specs), "particles": tot}}}
     for name, kw in (("served", dict(max_resident=max_resident)),
                      ("one_at_a_time", dict(max_jobs=1, max_resident=524288))):
         with gcs_amd.Server(hbm_budget_bytes=budget, steps_per_round=steps_per_round,
                             gen_threads=8, G=1.0, eps2=eps2, dt=0.005, theta=0.5,
-                            leaf_size=leaf, precision=gcs_amd.PREC_COMPENSATED, use_graph=1,
+                            precision=gcs_amd.PREC_COMPENSATED, use_graph=1, **tuned,
                             **kw) as srv:
             srv.submit(jobs, seed)
             srv.run(window)
@@ -257,8 +276,8 @@ def main():
     ap.add_argument("--config", default="plain-2x131072", choices=sorted(CONFIGS))
     ap.add_argument("--leaf", type=int, default=4)
     ap.add_argument("--block", type=int, default=256)
-    ap.add_argument("--wps", type=int, default=3, help="traversal waves per SIMD")
-    ap.add_argument("--group", type=int, default=48, help="largest traversal group")
+    ap.add_argument("--wps", type=int, default=4, help="traversal waves per SIMD")
+    ap.add_argument("--group", type=int, default=96, help="largest traversal group")
     ap.add_argument("--autotune", action="store_true",
                     help="pick leaf/block/waves per SIMD with the autotuner instead of "
                          "--leaf/--block/--wps (the batched, served and precision legs always "
