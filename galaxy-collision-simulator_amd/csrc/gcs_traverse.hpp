@@ -7,7 +7,9 @@
 //            round from an LDS stack (spilling its oldest half to global
 //            memory when full).  A popped node is
 //              accepted  when the squared distance from its COM to the group's
-//                        bounding box exceeds its opening value
+//                        bounding box exceeds its opening value, (width/theta)^2
+//                        (gravity_aggregator.cpp:37-43's s^2 < theta^2 d^2 with d
+//                        taken from the group's box) or, with the offset form,
 //                        (width/theta + |COM - box centre|)^2: its index goes
 //                        to the node list;
 //              summed    otherwise, when it holds <= leaf particles: its
