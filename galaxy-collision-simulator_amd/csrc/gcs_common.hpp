@@ -38,6 +38,7 @@ static_assert(kTile == 1 << kTileShift, "shift");
 // finished one per block, the nodes above them by the instance pass.
 constexpr int kATile = kTile;
 constexpr int kStackCap = 8192;      // per-wave traversal stack entries
+constexpr int kOrderClasses = 8;     // traversal dispatch classes by group size
 constexpr int kNumXcd = 8;           // MI355X: 8 XCDs, blocks dealt round-robin
 
 // Sort tile: a run of at most kTile particles of one instance.

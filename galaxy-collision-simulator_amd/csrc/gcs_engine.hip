@@ -114,6 +114,7 @@ struct gcs_engine {
     TileInfo* tiles = nullptr;
     SegInfo* segs = nullptr;
     GroupInfo* groups = nullptr;  // traversal groups, rebuilt on the device each step
+    int* gorder = nullptr;        // their dispatch order (size classes within each XCD queue)
     uint32_t* gmark = nullptr;
     uint32_t* gq = nullptr;      // groups per quarter tile
     uint32_t* gchunk = nullptr;  // groups per kGroupChunk quarter tiles
@@ -199,7 +200,7 @@ void free_all(gcs_engine* e)
     dfree(e->node); dfree(e->nodeQ); dfree(e->boxLo); dfree(e->boxHi);
     dfree(e->parent); dfree(e->flag);
     dfree(e->tiles); dfree(e->segs); dfree(e->amark); dfree(e->aq); dfree(e->achunk); dfree(e->acnt); dfree(e->aroot); dfree(e->atiles); dfree(e->natiles); dfree(e->groups); dfree(e->hist); dfree(e->chist);
-    dfree(e->gmark); dfree(e->gq); dfree(e->gchunk); dfree(e->gcnt); dfree(e->ngroups);
+    dfree(e->gmark); dfree(e->gq); dfree(e->gorder); dfree(e->gchunk); dfree(e->gcnt); dfree(e->ngroups);
     dfree(e->front); dfree(e->nfront);
     dfree(e->seg_bounds); dfree(e->bpart); dfree(e->qctr); dfree(e->counters); dfree(e->err);
     dfree(e->diag_part); dfree(e->diag_out); dfree(e->amax_part); dfree(e->dt_inst); dfree(e->tsim); dfree(e->slot_mem); dfree(e->staging);
@@ -288,11 +289,20 @@ int check_params(const gcs_params* p)
     return GCS_OK;
 }
 
+// Waves of the persistent group traversal: waves_per_simd x 4 SIMDs x CUs,
+// within the per-wave memory, and no more blocks than particles need.
+int group_waves(const gcs_engine* e, const gcs_params& p)
+{
+    const int wpb = p.block_size / kWave;
+    const int waves = std::min(e->trav_slots, e->num_cu * 4 * p.waves_per_simd);
+    return std::max(1, std::min(waves / wpb, (int)((e->n_total + wpb - 1) / wpb))) * wpb;
+}
+
 template <bool POT, bool QUAD, int WPB>
-void launch_group(gcs_engine* e, int blocks, const gcs_params& p, float4* posm)
+void launch_group(gcs_engine* e, int blocks, const gcs_params& p, float4* posm, bool order)
 {
     k_traverse_group<POT, QUAD, WPB><<<blocks, WPB * kWave, 0, e->stream>>>(
-        e->groups, e->ngroups, e->segs, posm, e->node, e->nodeQ, e->acc,
+        e->groups, order ? e->gorder : nullptr, e->ngroups, e->segs, posm, e->node, e->nodeQ, e->acc,
         e->slot_mem, e->qctr, e->counters, e->err, p.G, p.eps2, p.leaf_size);
 }
 
@@ -384,6 +394,13 @@ int enqueue_step(gcs_engine* e, const gcs_params& p, bool integrate)
     k_group_emit<<<qblocks, kTileThreads, 0, e->stream>>>(
         e->tiles, e->segs, group_marks(e), e->gchunk, atile_marks(e), e->achunk, e->groups,
         e->ngroups, e->atiles, e->natiles, (int)e->n_total);
+    // Size-class dispatch order, when the groups (at least n / group_size)
+    // outnumber the traversal's waves four times over; with about one group per
+    // wave the extra launch costs more than the order saves.
+    const bool order = group && e->n_total >= 4 * (int64_t)p.group_size * group_waves(e, p);
+    if (order)
+        k_group_order<<<kNumXcd, 1024, 0, e->stream>>>(e->groups, e->ngroups, p.group_size,
+                                                        e->gorder);
     mark(e, 4);
     const float inv_theta = p.theta > 0.0f ? 1.0f / p.theta : __builtin_inff();
     // Opening value of the merge: width (reference traversal), (w/theta)^2 or
@@ -404,20 +421,18 @@ int enqueue_step(gcs_engine* e, const gcs_params& p, bool integrate)
         const int wpb = p.block_size / kWave;
         // Persistent: as many blocks as the per-wave memory covers; waves that
         // find no group left exit at once.
-        // waves_per_simd * 4 SIMDs * CUs waves, within the per-wave memory.
-        const int waves = std::min(e->trav_slots, e->num_cu * 4 * p.waves_per_simd);
-        const int blocks = std::max(1, std::min(waves / wpb, (int)((e->n_total + wpb - 1) / wpb)));
+        const int blocks = group_waves(e, p) / wpb;
         const bool pot = p.diagnostics != 0;
         // Stack slots: blocks * wpb waves <= trav_slots.
         if (wpb == 4) {
-            if (pot) launch_group<true, true, 4>(e, blocks, p, posm);
-            else launch_group<false, true, 4>(e, blocks, p, posm);
+            if (pot) launch_group<true, true, 4>(e, blocks, p, posm, order);
+            else launch_group<false, true, 4>(e, blocks, p, posm, order);
         } else if (wpb == 2) {
-            if (pot) launch_group<true, true, 2>(e, blocks, p, posm);
-            else launch_group<false, true, 2>(e, blocks, p, posm);
+            if (pot) launch_group<true, true, 2>(e, blocks, p, posm, order);
+            else launch_group<false, true, 2>(e, blocks, p, posm, order);
         } else {
-            if (pot) launch_group<true, true, 1>(e, blocks, p, posm);
-            else launch_group<false, true, 1>(e, blocks, p, posm);
+            if (pot) launch_group<true, true, 1>(e, blocks, p, posm, order);
+            else launch_group<false, true, 1>(e, blocks, p, posm, order);
         }
     } else {
         if (p.stack_depth <= 64) launch_reference<64>(e, p, posm, qblocks);
@@ -606,6 +621,7 @@ int gcs_create(int64_t cap_n, int32_t cap_k, int32_t device, gcs_engine** out)
     if (!rc) rc = dalloc(&e->segs, (size_t)cap_k);
     if (!rc) rc = dalloc(&e->groups, n + 1);
     if (!rc) rc = dalloc(&e->gmark, n);
+    if (!rc) rc = dalloc(&e->gorder, n + 1);
     if (!rc) rc = dalloc(&e->gq, 4 * nt);
     if (!rc) rc = dalloc(&e->gchunk, max_gchunks(cap_n, cap_k));
     if (!rc) rc = dalloc(&e->gcnt, n);

@@ -338,6 +338,74 @@ __global__ __launch_bounds__(256) void k_group_emit(const TileInfo* __restrict__
     }
 }
 
+// Dispatch order of the traversal.  The groups are dealt to kNumXcd queues
+// in contiguous (Morton) runs; within its run each queue hands out the
+// groups by decreasing size class (eighths of the largest group size) and
+// in Morton order within a class, so the groups with the most targets start
+// first and neighbouring groups still run close together.  The order
+// changes scheduling only: every group's sums are the same whenever it runs.
+__device__ __forceinline__ uint32_t size_class(int count, int gmax)
+{
+    return (uint32_t)min(kOrderClasses - 1, max(0, (gmax - count) * kOrderClasses / gmax));
+}
+
+__global__ __launch_bounds__(1024) void k_group_order(const GroupInfo* __restrict__ groups,
+                                                      const int* __restrict__ ngroups_dev,
+                                                      int gmax, int* __restrict__ gorder)
+{
+    __shared__ uint32_t base[kOrderClasses];
+    __shared__ uint32_t wc[16][kOrderClasses];
+    const int ng = *ngroups_dev;
+    const int per = (ng + kNumXcd - 1) / kNumXcd;
+    const int lo = blockIdx.x * per, hi = min(ng, lo + per);
+    const int l = lane_id(), w = threadIdx.x >> 6;
+    if (threadIdx.x < kOrderClasses) base[threadIdx.x] = 0u;
+    __syncthreads();
+    for (int i = lo + (int)threadIdx.x; i < hi; i += 1024)
+        atomicAdd(&base[size_class(groups[i].count, gmax)], 1u);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t run = 0;
+        for (int b = 0; b < kOrderClasses; ++b) {
+            const uint32_t c = base[b];
+            base[b] = run;
+            run += c;
+        }
+    }
+    __syncthreads();
+    // Stable scatter, 1024 groups at a time: lanes of one class are ranked by
+    // lane order (ballots per class bit), waves by wave order.
+    const uint64_t lt = (1ull << l) - 1ull;
+    for (int c0 = lo; c0 < hi; c0 += 1024) {
+        const int i = c0 + (int)threadIdx.x;
+        const bool valid = i < hi;
+        const uint32_t k = valid ? size_class(groups[i].count, gmax) : 0u;
+        uint64_t m = ballot(valid);
+#pragma unroll
+        for (int b = 0; (1 << b) < kOrderClasses; ++b) {
+            const uint64_t bb = ballot((k >> b) & 1u);
+            m &= ((k >> b) & 1u) ? bb : ~bb;
+        }
+        if (l < kOrderClasses) wc[w][l] = 0u;
+        __builtin_amdgcn_wave_barrier();
+        if (valid && (m & lt) == 0ull) wc[w][k] = (uint32_t)__popcll(m);
+        __syncthreads();
+        // Per class: exclusive offsets over the waves; thread t owns class t.
+        if (threadIdx.x < kOrderClasses) {
+            uint32_t run = base[threadIdx.x];
+            for (int v = 0; v < 16; ++v) {
+                const uint32_t c = wc[v][threadIdx.x];
+                wc[v][threadIdx.x] = run;
+                run += c;
+            }
+            base[threadIdx.x] = run;
+        }
+        __syncthreads();
+        if (valid) gorder[lo + (int)(wc[w][k] + (uint32_t)__popcll(m & lt))] = i;
+        __syncthreads();
+    }
+}
+
 // Aggregate of one node: COM, mass, AABB, quadrupole (xx, yy, zz, xy, xz, yz).
 struct NodeData {
     float cx, cy, cz, m;

@@ -301,9 +301,9 @@ __device__ __forceinline__ float box_dist2(float lx, float ly, float lz, float h
 
 template <bool POT, bool QUAD, int WPB>
 __global__ __launch_bounds__(WPB * 64) void k_traverse_group(
-    const GroupInfo* __restrict__ groups, const int* __restrict__ ngroups_dev,
-    const SegInfo* __restrict__ segs, const float4* __restrict__ posm,
-    const float4* __restrict__ node,
+    const GroupInfo* __restrict__ groups, const int* __restrict__ gorder,
+    const int* __restrict__ ngroups_dev, const SegInfo* __restrict__ segs,
+    const float4* __restrict__ posm, const float4* __restrict__ node,
     const float4* __restrict__ nodeQ, float4* __restrict__ acc, int* __restrict__ slot_mem,
     unsigned* qctr, unsigned long long* counters, unsigned* err, float G, float eps2, int leaf)
 {
@@ -341,7 +341,7 @@ __global__ __launch_bounds__(WPB * 64) void k_traverse_group(
             if (l == 0) k = atomicAdd(&qctr[c], 1u);
             k = (unsigned)uniform((int)k);
             if (lo + (int)k < hi) {
-                g = lo + (int)k;
+                g = gorder ? uniform(gorder[lo + (int)k]) : lo + (int)k;
                 break;
             }
             ++q;
