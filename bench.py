@@ -3,7 +3,8 @@
 
 Workload (N=1): the plain GPU configuration -- one collision of 2 x 131,072
 particles (70/10/20 disc/bulge/halo), FP32, theta = 0.5, softening 0.02,
-dt = 0.005, leaf 4, groups of at most 96, quadrupole group traversal.  Inputs are generated on the
+dt = 0.005, leaf 4, groups of at most 96 each walked and evaluated by a block of
+two waves, quadrupole group traversal.  Inputs are generated on the
 host and resident in HBM before the timed region; each timed step is a full
 step (bounds, Morton, sort, permute, tree build, aggregation, traversal,
 kick-drift), replayed from a hipGraph.
@@ -29,11 +30,11 @@ FLOP_PER_INTERACTION = 20
 PEAK_HBM_GBS = 8000.0
 TREE_BYTES_PER_PARTICLE = 376  # Morton 24 + sort 24 + permute 68 + init 88 + build 24 + agg 148
 
-# HBM bytes per k_traverse_group launch, from rocprofv3 FETCH_SIZE (x2.000) and
+# HBM bytes per traversal launch, from rocprofv3 FETCH_SIZE (x2.000) and
 # WRITE_SIZE (x1.000) passes scaled by a 256 MiB copy (profiles/
-# k_traverse_group_counters_plain-2x131072.md); valid for the configuration
-# it was measured on.
-TRAVERSE_TRAFFIC = {("plain-2x131072", 4, 256, 4, 96): 205134126}
+# traversal_counters_plain-2x131072.md); valid for the configuration it was
+# measured on: (workload, leaf, block, waves per SIMD, group, team).
+TRAVERSE_TRAFFIC = {}
 
 CONFIGS = {
     # name: (particles per galaxy, eps, dt, theta)
@@ -275,9 +276,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--config", default="plain-2x131072", choices=sorted(CONFIGS))
     ap.add_argument("--leaf", type=int, default=4)
-    ap.add_argument("--block", type=int, default=256)
+    ap.add_argument("--block", type=int, default=128)
     ap.add_argument("--wps", type=int, default=4, help="traversal waves per SIMD")
     ap.add_argument("--group", type=int, default=96, help="largest traversal group")
+    ap.add_argument("--team", type=int, default=2,
+                    help="waves per traversal group: 1, or 2 (a two-wave block per group)")
     ap.add_argument("--autotune", action="store_true",
                     help="pick leaf/block/waves per SIMD with the autotuner instead of "
                          "--leaf/--block/--wps (the batched, served and precision legs always "
@@ -312,13 +315,14 @@ def main():
     n = len(stars)
     eps2 = float(np.float32(eps) ** 2)
     p = gcs_amd.params(G=1.0, eps2=eps2, dt=dt, theta=theta, leaf_size=a.leaf,
-                       block_size=a.block, waves_per_simd=a.wps, group_size=a.group, use_graph=1)
+                       block_size=a.block, waves_per_simd=a.wps, group_size=a.group, team=a.team,
+                       use_graph=1)
     e = gcs_amd.Engine(n)
     e.load(stars)
     if a.autotune and not a.no_autotune:
         p, _ = e.autotune(p, reps=3)
-        a.leaf, a.block, a.wps, a.group = (p.leaf_size, p.block_size, p.waves_per_simd,
-                                           p.group_size)
+        a.leaf, a.block, a.wps, a.group, a.team = (p.leaf_size, p.block_size,
+                                                   p.waves_per_simd, p.group_size, p.team)
 
     # Accelerations of the initial state (for the accuracy check).
     e.compute_accels(p)
@@ -337,7 +341,7 @@ def main():
     # Roofline of the traversal kernel: event-timed eager steps.
     pe = gcs_amd.params(G=1.0, eps2=eps2, dt=dt, theta=theta, leaf_size=a.leaf,
                         block_size=a.block, waves_per_simd=a.wps, group_size=a.group,
-                        use_graph=0)
+                        team=a.team, use_graph=0)
     trav = []
     stages = []
     inter = None
@@ -371,7 +375,7 @@ def main():
         "data": "synthetic (seeded two-galaxy generator)",
         "config": {"workload": a.config, "particles": n, "theta": theta, "eps": eps, "dt": dt,
                    "leaf": a.leaf, "block": a.block, "waves_per_simd": a.wps,
-                   "group": a.group,
+                   "group": a.group, "team": a.team,
                    "autotuned": bool(a.autotune and not a.no_autotune),
                    "multipole": "quadrupole",
                    "parallelism": "single GPU"},
@@ -381,8 +385,10 @@ def main():
                               "traverse", "integrate"], stage_ms)),
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS,
-                     "traffic": TRAVERSE_TRAFFIC.get((a.config, a.leaf, a.block, a.wps, a.group)),
-                     "kernel": "k_traverse_group", "kernel_ms": trav_ms,
+                     "traffic": TRAVERSE_TRAFFIC.get((a.config, a.leaf, a.block, a.wps, a.group,
+                                                      a.team)),
+                     "kernel": "k_traverse_team" if a.team == 2 else "k_traverse_group",
+                     "kernel_ms": trav_ms,
                      "flop_per_interaction": FLOP_PER_INTERACTION},
         "roofline_tree": {"bound": "hbm", "unit": "GB/s", "peak": PEAK_HBM_GBS,
                           "achieved": tree_gbs, "frac": tree_gbs / PEAK_HBM_GBS,

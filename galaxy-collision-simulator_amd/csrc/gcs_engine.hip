@@ -274,6 +274,8 @@ int check_params(const gcs_params* p)
             return fail(GCS_ERR_INVALID, "group_size must be in [1, 192]");
         if (p->open_offset != 0 && p->open_offset != 1)
             return fail(GCS_ERR_INVALID, "open_offset must be 0 or 1");
+        if (p->team < 0 || p->team > kTeam)
+            return fail(GCS_ERR_INVALID, "team must be 0, 1 or 2");
     } else {
         if (p->stack_depth < 2 || p->stack_depth > 1024)
             return fail(GCS_ERR_INVALID, "stack_depth must be in [2, 1024]");
@@ -293,7 +295,7 @@ int check_params(const gcs_params* p)
 // within the per-wave memory, and no more blocks than particles need.
 int group_waves(const gcs_engine* e, const gcs_params& p)
 {
-    const int wpb = p.block_size / kWave;
+    const int wpb = p.team == kTeam ? kTeam : p.block_size / kWave;
     const int waves = std::min(e->trav_slots, e->num_cu * 4 * p.waves_per_simd);
     return std::max(1, std::min(waves / wpb, (int)((e->n_total + wpb - 1) / wpb))) * wpb;
 }
@@ -304,6 +306,14 @@ void launch_group(gcs_engine* e, int blocks, const gcs_params& p, float4* posm, 
     k_traverse_group<POT, QUAD, WPB><<<blocks, WPB * kWave, 0, e->stream>>>(
         e->groups, order ? e->gorder : nullptr, e->ngroups, e->segs, posm, e->node, e->nodeQ, e->acc,
         e->slot_mem, e->qctr, e->counters, e->err, p.G, p.eps2, p.leaf_size);
+}
+
+template <bool POT>
+void launch_team(gcs_engine* e, int blocks, const gcs_params& p, float4* posm, bool order)
+{
+    k_traverse_team<POT, true><<<blocks, kTeam * kWave, 0, e->stream>>>(
+        e->groups, order ? e->gorder : nullptr, e->ngroups, e->segs, posm, e->node, e->nodeQ,
+        e->acc, e->slot_mem, e->qctr, e->counters, e->err, p.G, p.eps2, p.leaf_size);
 }
 
 template <int STACK>
@@ -418,13 +428,16 @@ int enqueue_step(gcs_engine* e, const gcs_params& p, bool integrate)
     LAUNCHCHK();
     mark(e, 5);
     if (group) {
-        const int wpb = p.block_size / kWave;
-        // Persistent: as many blocks as the per-wave memory covers; waves that
-        // find no group left exit at once.
+        // Persistent: as many blocks as the per-wave memory covers (blocks *
+        // waves per block <= trav_slots); waves that find no group left exit
+        // at once.
+        const int wpb = p.team == kTeam ? kTeam : p.block_size / kWave;
         const int blocks = group_waves(e, p) / wpb;
         const bool pot = p.diagnostics != 0;
-        // Stack slots: blocks * wpb waves <= trav_slots.
-        if (wpb == 4) {
+        if (p.team == kTeam) {
+            if (pot) launch_team<true>(e, blocks, p, posm, order);
+            else launch_team<false>(e, blocks, p, posm, order);
+        } else if (wpb == 4) {
             if (pot) launch_group<true, true, 4>(e, blocks, p, posm, order);
             else launch_group<false, true, 4>(e, blocks, p, posm, order);
         } else if (wpb == 2) {
@@ -1012,20 +1025,22 @@ int gcs_autotune(gcs_engine* e, const gcs_params* base, int32_t reps, gcs_params
     HIPCHK(hipSetDevice(e->device));
     rc = gcs_sync(e);
     if (rc) return rc;
-    // Candidate settings: the group traversal's leaf size, block size and
-    // waves per SIMD; the reference traversal's stack depth.  The leaf size
-    // barely moves the error (2 x 1,048,576 particles at theta = 0.5: 2.7e-4
-    // relative RMS at leaf 4, 2.6e-4 at leaf 16).
+    // Candidate settings: the group traversal's group size, leaf size, block
+    // size (or teams of two waves per group) and waves per SIMD; the reference
+    // traversal's stack depth.  The leaf size barely moves the error (2 x
+    // 1,048,576 particles at theta = 0.5: 2.7e-4 relative RMS at leaf 4,
+    // 2.6e-4 at leaf 16).
     std::vector<gcs_params> cand;
     if (base->mac == GCS_MAC_GROUP) {
         for (int group : {48, 64, 96})
             for (int leaf : {4, 8, 12, 16})
-                for (int block : {64, 128, 256})
+                for (int block : {0, 64, 128, 256})  // 0: teams of two waves
                     for (int wps : {2, 3, 4}) {
                         gcs_params q = *base;
                         q.group_size = group;
                         q.leaf_size = leaf;
-                        q.block_size = block;
+                        q.team = block == 0 ? kTeam : 1;
+                        q.block_size = block == 0 ? kTeam * kWave : block;
                         q.waves_per_simd = wps;
                         cand.push_back(q);
                     }

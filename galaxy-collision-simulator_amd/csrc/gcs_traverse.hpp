@@ -529,6 +529,299 @@ __global__ __launch_bounds__(WPB * 64) void k_traverse_group(
     }
 }
 
+// Sub-wave partial sums of a wave's targets, added through LDS (the chunk
+// arrays, 3 x 64 entries) in sub-wave order: r[u] is the sum for target
+// l + 64 u (targets < n).
+__device__ __forceinline__ void combine_share(WaveLds& L, const Acc* a, const Share& sh, int n,
+                                              float4* r)
+{
+    const int l = lane_id();
+    float4* const scr = L.ca;
+    __builtin_amdgcn_wave_barrier();
+    if (l < sh.S * sh.W) {
+#pragma unroll
+        for (int u = 0; u < kMaxPer; ++u) {
+            const int j = sh.t + u * sh.W;
+            if (u < sh.P && j < n) {
+                const int i = sh.h * n + j;
+                scr[(i >> 6) * kChunk + (i & 63)] = make_float4(a[u].ax, a[u].ay, a[u].az, a[u].ph);
+            }
+        }
+    }
+    __builtin_amdgcn_wave_barrier();
+#pragma unroll
+    for (int u = 0; u < kMaxPer; ++u) {
+        const int j = l + u * kWave;
+        float4 v4 = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (j < n) {
+            for (int hh = 0; hh < sh.S; ++hh) {
+                const int i = hh * n + j;
+                const float4 v = scr[(i >> 6) * kChunk + (i & 63)];
+                v4.x += v.x;
+                v4.y += v.y;
+                v4.z += v.z;
+                v4.w += v.w;
+            }
+        }
+        r[u] = v4;
+    }
+    __builtin_amdgcn_wave_barrier();
+}
+
+// k_traverse_team: the group traversal with a team of kTeam waves (one block)
+// per group.  The team walks the tree together from one LDS stack, kTeam x 64
+// nodes per round; each round's list entries and pushes are placed in wave
+// order (wave 0's first), so the lists are a function of the tree and the
+// group only.  Each wave then evaluates its share of the group's targets
+// (wave w takes targets [w*ceil(n/kTeam), ...)) over the whole lists; the
+// targets are disjoint, so no sums cross waves.  A group's wall time is about
+// 1/kTeam of the one-wave kernel's, which evens out the work when there are
+// few groups per wave.  Lists that would overflow are evaluated early into
+// partial sums (kept unscaled in acc until the final evaluation).
+constexpr int kTeam = 2;
+
+template <bool POT, bool QUAD>
+__global__ __launch_bounds__(kTeam * 64) void k_traverse_team(
+    const GroupInfo* __restrict__ groups, const int* __restrict__ gorder,
+    const int* __restrict__ ngroups_dev, const SegInfo* __restrict__ segs,
+    const float4* __restrict__ posm, const float4* __restrict__ node,
+    const float4* __restrict__ nodeQ, float4* __restrict__ acc, int* __restrict__ slot_mem,
+    unsigned* qctr, unsigned long long* counters, unsigned* err, float G, float eps2, int leaf)
+{
+    __shared__ WaveLds lds[kTeam];  // the team's stack is lds[0].stk
+    __shared__ int xch[kTeam][4];   // per wave and round: accepted, particles, left, right pushes
+    __shared__ int ctl[4];          // claimed group; stack top; spilled entries; overflow
+    const int l = lane_id();
+    const int w = threadIdx.x >> 6;
+    const int tid = threadIdx.x;
+    WaveLds& L = lds[w];
+    int* const stk = lds[0].stk;
+    int* const spill = slot_mem + (size_t)blockIdx.x * kTeam * kSlotInts;
+    int* const nl = spill + kStackCap;
+    int* const pl = nl + kNodeList;
+    const int xcd = blockIdx.x % kNumXcd;
+    const int ngroups = uniform(*ngroups_dev);
+    const int per = (ngroups + kNumXcd - 1) / kNumXcd;
+    unsigned long long n_node = 0, n_part = 0;
+    int q = 0;
+    const float self_inv = __builtin_amdgcn_rsqf(eps2);
+    const int pbits = 32 - __builtin_clz((unsigned)(leaf + 2));
+    const uint64_t lt = (1ull << l) - 1ull;
+    // A round adds at most kTeam*64 node entries and kTeam*64*(leaf+2)
+    // particle entries; lists are evaluated before a round could overflow them.
+    const int node_room = kNodeList - kTeam * kWave;
+    const int part_room = kPartList - kTeam * kWave * (leaf + 2);
+    if (l < kMaxSplit) {
+        const float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
+        L.ca[kWave + l] = z;
+        L.cq0[kWave + l] = z;
+        L.cq1[kWave + l] = z;
+    }
+    bool overflow = false;
+    for (;;) {
+        __syncthreads();  // the previous group's use of ctl is over
+        if (tid == 0) {
+            int g = -1;
+            while (q < kNumXcd) {
+                const int c = (xcd + q) % kNumXcd;
+                const int lo = c * per, hi = min(ngroups, lo + per);
+                const unsigned k = atomicAdd(&qctr[c], 1u);
+                if (lo + (int)k < hi) {
+                    g = gorder ? gorder[lo + (int)k] : lo + (int)k;
+                    break;
+                }
+                ++q;
+            }
+            ctl[0] = g;
+        }
+        __syncthreads();
+        const int g = ctl[0];
+        if (g < 0) break;
+        const GroupInfo gi = groups[g];
+        const SegInfo sg = segs[gi.seg];
+        // The group's bounding box (each wave computes it).
+        float lx = __builtin_inff(), ly = lx, lz = lx, hx = -lx, hy = -lx, hz = -lx;
+        for (int j = l; j < gi.count; j += kWave) {
+            const float4 pj = posm[gi.first + j];
+            lx = fminf(lx, pj.x); ly = fminf(ly, pj.y); lz = fminf(lz, pj.z);
+            hx = fmaxf(hx, pj.x); hy = fmaxf(hy, pj.y); hz = fmaxf(hz, pj.z);
+        }
+        lx = wave_min(lx); ly = wave_min(ly); lz = wave_min(lz);
+        hx = wave_max(hx); hy = wave_max(hy); hz = wave_max(hz);
+        // This wave's targets.
+        const int half = (gi.count + kTeam - 1) / kTeam;
+        const int t0 = w * half;
+        const int nw = max(0, min(gi.count - t0, half));
+        int nn = 0, np = 0;
+        int top = 0, spilled = 0;
+        bool flushed = false;
+        if (sg.n == 1) {
+            if (tid == 0) pl[0] = sg.start;
+            np = 1;
+        } else if (tid == 0) {
+            stk[0] = sg.root;
+        }
+        if (sg.n != 1) top = 1;
+        for (;;) {
+            // Walk until the stack is empty or a list could overflow.
+            bool need = false;
+            while (top > 0 || spilled > 0) {
+                if (nn > node_room || np > part_room) {
+                    need = true;
+                    break;
+                }
+                __syncthreads();  // the previous round's stack and list writes are done
+                if (top == 0) {
+                    // Reload the most recently spilled half.
+                    const int c = min(spilled, kLdsStack / 2);
+                    for (int k = tid; k < c; k += kTeam * kWave) stk[k] = spill[spilled - c + k];
+                    spilled -= c;
+                    top = c;
+                    __syncthreads();
+                }
+                const int cnt = min(top, kTeam * kWave);
+                const int e = w * kWave + l;
+                const bool have = e < cnt;
+                int v = 0;
+                if (have) v = stk[top - cnt + e];
+                float4 A = make_float4(0.f, 0.f, 0.f, 0.f);
+                float4 bb = A;
+                if (have) {
+                    A = node[2 * v];
+                    bb = node[2 * v + 1];
+                }
+                const int4 B = make_int4(__float_as_int(bb.x), __float_as_int(bb.y),
+                                         __float_as_int(bb.z), __float_as_int(bb.w));
+                const bool accept = have && box_dist2(lx, ly, lz, hx, hy, hz, A.x, A.y, A.z) > bb.x;
+                const bool self_direct = have && !accept && (B.z - B.y + 1) <= leaf;
+                const bool open = have && !accept && !self_direct;
+                const bool lleaf = open && B.y == B.w, rleaf = open && B.z == B.w + 1;
+                const int c_self = self_direct ? B.z - B.y + 1 : 0;
+                const int cnt_p = c_self + (lleaf ? 1 : 0) + (rleaf ? 1 : 0);
+                const bool lp = open && !lleaf, rp = open && !rleaf;
+                const uint64_t mA = ballot(accept), m3 = ballot(lp), m4 = ballot(rp);
+                int tp;
+                const int ex = wave_scan_excl_bits(cnt_p, pbits, &tp);
+                if (l == 0) {
+                    xch[w][0] = __popcll(mA);
+                    xch[w][1] = tp;
+                    xch[w][2] = __popcll(m3);
+                    xch[w][3] = __popcll(m4);
+                }
+                __syncthreads();  // every entry of the round has been read; counts exchanged
+                int na_before = 0, tp_before = 0, nl_before = 0, nr_before = 0;
+                int na_all = 0, tp_all = 0, nl_all = 0, nr_all = 0;
+#pragma unroll
+                for (int k = 0; k < kTeam; ++k) {
+                    const int a0 = xch[k][0], a1 = xch[k][1], a2 = xch[k][2], a3 = xch[k][3];
+                    if (k < w) {
+                        na_before += a0;
+                        tp_before += a1;
+                        nl_before += a2;
+                        nr_before += a3;
+                    }
+                    na_all += a0;
+                    tp_all += a1;
+                    nl_all += a2;
+                    nr_all += a3;
+                }
+                // accepted nodes -> node list; particles -> particle list
+                if (accept) nl[nn + na_before + mbcnt(mA)] = v;
+                int o = np + tp_before + ex;
+                for (int k = 0; k < c_self; ++k) pl[o + k] = B.y + k;
+                o += c_self;
+                if (lleaf) pl[o++] = B.w;
+                if (rleaf) pl[o] = B.w + 1;
+                nn += na_all;
+                np += tp_all;
+                // larger children -> stack, in place of the popped entries
+                int base = top - cnt;
+                const int npush = nl_all + nr_all;
+                if (base + npush > kLdsStack) {
+                    // Spill the oldest half of the LDS stack to global memory.
+                    const int h = kLdsStack / 2;
+                    if (spilled + h > kStackCap) {
+                        overflow = true;
+                        break;
+                    }
+                    for (int k = tid; k < h; k += kTeam * kWave) spill[spilled + k] = stk[k];
+                    int keep[kLdsStack / 2 / (kTeam * kWave)];
+#pragma unroll
+                    for (int r = 0; r < kLdsStack / 2 / (kTeam * kWave); ++r) {
+                        const int k = tid + r * kTeam * kWave;
+                        keep[r] = k < base - h ? stk[k + h] : 0;
+                    }
+                    __syncthreads();
+#pragma unroll
+                    for (int r = 0; r < kLdsStack / 2 / (kTeam * kWave); ++r) {
+                        const int k = tid + r * kTeam * kWave;
+                        if (k < base - h) stk[k] = keep[r];
+                    }
+                    spilled += h;
+                    base -= h;
+                }
+                if (lp) stk[base + nl_before + mbcnt(m3)] = B.w + sg.int_base;
+                if (rp) stk[base + nl_all + nr_before + mbcnt(m4)] = B.w + 1 + sg.int_base;
+                top = base + npush;
+            }
+            if (overflow) break;
+            __syncthreads();  // the lists are complete in memory
+            // Evaluate this wave's targets over the lists: partial sums, or
+            // the final sums once the walk is done.
+            if (nw > 0) {
+                const Share sh = make_share(nw, l);
+                Targets tg;
+                Acc a[kMaxPer];
+#pragma unroll
+                for (int u = 0; u < kMaxPer; ++u) {
+                    const float4 pu = posm[gi.first + t0 + min(sh.t + u * sh.W, nw - 1)];
+                    tg.x[u] = pu.x;
+                    tg.y[u] = pu.y;
+                    tg.z[u] = pu.z;
+                    a[u] = Acc{0.f, 0.f, 0.f, 0.f};
+                }
+                if (nn > 0) eval_nodes<POT, QUAD>(L, nl, nn, node, nodeQ, a, tg, eps2, sh);
+                if (np > 0) eval_parts<POT>(L, pl, np, posm, a, tg, eps2, sh);
+                float4 r[kMaxPer];
+                combine_share(L, a, sh, nw, r);
+#pragma unroll
+                for (int u = 0; u < kMaxPer; ++u) {
+                    const int t = l + u * kWave;
+                    if (t < nw) {
+                        const int i = gi.first + t0 + t;
+                        float4 v4 = r[u];
+                        if (flushed) {
+                            const float4 p4 = acc[i];
+                            v4 = make_float4(p4.x + v4.x, p4.y + v4.y, p4.z + v4.z, p4.w + v4.w);
+                        }
+                        if (!need) {
+                            const float ph = POT ? -G * (v4.w - posm[i].w * self_inv) : 0.0f;
+                            v4 = make_float4(G * v4.x, G * v4.y, G * v4.z, ph);
+                        }
+                        acc[i] = v4;
+                    }
+                }
+            }
+            n_node += (unsigned long long)nn * (unsigned long long)nw;
+            n_part += (unsigned long long)np * (unsigned long long)nw;
+            nn = 0;
+            np = 0;
+            flushed = true;
+            if (!need) break;
+            __syncthreads();  // both waves are done reading the lists before they refill
+        }
+        if (overflow) {
+            if (tid == 0) atomicOr(err, kErrStack);
+            break;
+        }
+        n_part -= (unsigned long long)nw;  // the particle itself, in its own list
+    }
+    if (l == 0) {
+        atomicAdd(&counters[0], n_node);
+        atomicAdd(&counters[1], n_part);
+    }
+}
+
 // cuda_engine.cu:91-161 -- per-particle DFS from the instance root; left child
 // pushed before right; leaf taken when d^2+eps^2 > 1.01*eps^2; internal node
 // taken when width^2 < d^2+eps^2; force G*m*rsqrt(d2)^3 * d.  `limit` (at most

@@ -45,7 +45,7 @@ class Params(ctypes.Structure):
                 ("diagnostics", ctypes.c_int32), ("use_graph", ctypes.c_int32),
                 ("waves_per_simd", ctypes.c_int32), ("dt_eta", ctypes.c_float),
                 ("group_size", ctypes.c_int32), ("open_offset", ctypes.c_int32),
-                ("reserved", ctypes.c_int32)]
+                ("team", ctypes.c_int32)]
 
 
 class CollisionSpec(ctypes.Structure):

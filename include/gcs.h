@@ -99,7 +99,10 @@ typedef struct gcs_params {
     int32_t open_offset;/* GCS_MAC_GROUP opening radius: 0 = w/theta, the
                            reference's theta test taken from the group's box;
                            1 = w/theta + |COM - box centre| (stricter) */
-    int32_t reserved;
+    int32_t team;       /* GCS_MAC_GROUP waves per group: 0 or 1 = one wave walks
+                           and evaluates a group; 2 = a block of two waves walks
+                           it together and splits its targets (block_size is then
+                           128) */
 } gcs_params;
 
 /* Defaults: G=1, eps2=0.0025, dt=0.01, theta=0.5, GCS_MAC_GROUP,
