@@ -4,7 +4,7 @@
 Workload (N=1): the plain GPU configuration -- one collision of 2 x 131,072
 particles (70/10/20 disc/bulge/halo), FP32, theta = 0.5, softening 0.02,
 dt = 0.005, leaf 4, groups of at most 96 each walked and evaluated by a block of
-two waves, quadrupole group traversal.  Inputs are generated on the
+four waves, quadrupole group traversal.  Inputs are generated on the
 host and resident in HBM before the timed region; each timed step is a full
 step (bounds, Morton, sort, permute, tree build, aggregation, traversal,
 kick-drift), replayed from a hipGraph.
@@ -276,11 +276,12 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--config", default="plain-2x131072", choices=sorted(CONFIGS))
     ap.add_argument("--leaf", type=int, default=4)
-    ap.add_argument("--block", type=int, default=128)
+    ap.add_argument("--block", type=int, default=256)
     ap.add_argument("--wps", type=int, default=4, help="traversal waves per SIMD")
     ap.add_argument("--group", type=int, default=96, help="largest traversal group")
-    ap.add_argument("--team", type=int, default=2,
-                    help="waves per traversal group: 1, or 2 (a two-wave block per group)")
+    ap.add_argument("--team", type=int, default=4,
+                    help="waves per traversal group: 1, or 2, 4, 8 (a block of that many "
+                         "waves per group; --block is then 64 x team)")
     ap.add_argument("--autotune", action="store_true",
                     help="pick leaf/block/waves per SIMD with the autotuner instead of "
                          "--leaf/--block/--wps (the batched, served and precision legs always "
@@ -387,7 +388,7 @@ def main():
                      "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS,
                      "traffic": TRAVERSE_TRAFFIC.get((a.config, a.leaf, a.block, a.wps, a.group,
                                                       a.team)),
-                     "kernel": "k_traverse_team" if a.team == 2 else "k_traverse_group",
+                     "kernel": "k_traverse_team" if a.team > 1 else "k_traverse_group",
                      "kernel_ms": trav_ms,
                      "flop_per_interaction": FLOP_PER_INTERACTION},
         "roofline_tree": {"bound": "hbm", "unit": "GB/s", "peak": PEAK_HBM_GBS,

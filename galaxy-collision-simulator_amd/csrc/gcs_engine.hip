@@ -266,16 +266,19 @@ int check_params(const gcs_params* p)
             return fail(GCS_ERR_INVALID, "theta must be in [0, 1]");
         if (p->leaf_size < 1 || p->leaf_size > kMaxLeaf)
             return fail(GCS_ERR_INVALID, "leaf_size must be in [1, 64]");
-        if (p->block_size != 64 && p->block_size != 128 && p->block_size != 256)
-            return fail(GCS_ERR_INVALID, "block_size must be 64, 128 or 256");
+        if (p->block_size != 64 && p->block_size != 128 && p->block_size != 256 &&
+            !(p->team == 8 && p->block_size == 512))
+            return fail(GCS_ERR_INVALID, "block_size must be 64, 128 or 256 (512 for a team of 8)");
         if (p->waves_per_simd < 1 || p->waves_per_simd > 4)
             return fail(GCS_ERR_INVALID, "waves_per_simd must be in [1, 4]");
         if (p->group_size < 1 || p->group_size > kGroupMax)
             return fail(GCS_ERR_INVALID, "group_size must be in [1, 192]");
         if (p->open_offset != 0 && p->open_offset != 1)
             return fail(GCS_ERR_INVALID, "open_offset must be 0 or 1");
-        if (p->team < 0 || p->team > kTeam)
-            return fail(GCS_ERR_INVALID, "team must be 0, 1 or 2");
+        if (p->team != 0 && p->team != 1 && p->team != 2 && p->team != 4 && p->team != 8)
+            return fail(GCS_ERR_INVALID, "team must be 0, 1, 2, 4 or 8");
+        if (p->team > 1 && p->block_size != p->team * kWave)
+            return fail(GCS_ERR_INVALID, "a team of w waves takes block_size 64 w");
     } else {
         if (p->stack_depth < 2 || p->stack_depth > 1024)
             return fail(GCS_ERR_INVALID, "stack_depth must be in [2, 1024]");
@@ -295,7 +298,7 @@ int check_params(const gcs_params* p)
 // within the per-wave memory, and no more blocks than particles need.
 int group_waves(const gcs_engine* e, const gcs_params& p)
 {
-    const int wpb = p.team == kTeam ? kTeam : p.block_size / kWave;
+    const int wpb = p.team > 1 ? p.team : p.block_size / kWave;
     const int waves = std::min(e->trav_slots, e->num_cu * 4 * p.waves_per_simd);
     return std::max(1, std::min(waves / wpb, (int)((e->n_total + wpb - 1) / wpb))) * wpb;
 }
@@ -308,10 +311,10 @@ void launch_group(gcs_engine* e, int blocks, const gcs_params& p, float4* posm, 
         e->slot_mem, e->qctr, e->counters, e->err, p.G, p.eps2, p.leaf_size);
 }
 
-template <bool POT>
+template <bool POT, int TEAM>
 void launch_team(gcs_engine* e, int blocks, const gcs_params& p, float4* posm, bool order)
 {
-    k_traverse_team<POT, true><<<blocks, kTeam * kWave, 0, e->stream>>>(
+    k_traverse_team<POT, true, TEAM><<<blocks, TEAM * kWave, 0, e->stream>>>(
         e->groups, order ? e->gorder : nullptr, e->ngroups, e->segs, posm, e->node, e->nodeQ,
         e->acc, e->slot_mem, e->qctr, e->counters, e->err, p.G, p.eps2, p.leaf_size);
 }
@@ -431,12 +434,18 @@ int enqueue_step(gcs_engine* e, const gcs_params& p, bool integrate)
         // Persistent: as many blocks as the per-wave memory covers (blocks *
         // waves per block <= trav_slots); waves that find no group left exit
         // at once.
-        const int wpb = p.team == kTeam ? kTeam : p.block_size / kWave;
+        const int wpb = p.team > 1 ? p.team : p.block_size / kWave;
         const int blocks = group_waves(e, p) / wpb;
         const bool pot = p.diagnostics != 0;
-        if (p.team == kTeam) {
-            if (pot) launch_team<true>(e, blocks, p, posm, order);
-            else launch_team<false>(e, blocks, p, posm, order);
+        if (p.team == 2) {
+            if (pot) launch_team<true, 2>(e, blocks, p, posm, order);
+            else launch_team<false, 2>(e, blocks, p, posm, order);
+        } else if (p.team == 4) {
+            if (pot) launch_team<true, 4>(e, blocks, p, posm, order);
+            else launch_team<false, 4>(e, blocks, p, posm, order);
+        } else if (p.team == 8) {
+            if (pot) launch_team<true, 8>(e, blocks, p, posm, order);
+            else launch_team<false, 8>(e, blocks, p, posm, order);
         } else if (wpb == 4) {
             if (pot) launch_group<true, true, 4>(e, blocks, p, posm, order);
             else launch_group<false, true, 4>(e, blocks, p, posm, order);
@@ -1026,7 +1035,7 @@ int gcs_autotune(gcs_engine* e, const gcs_params* base, int32_t reps, gcs_params
     rc = gcs_sync(e);
     if (rc) return rc;
     // Candidate settings: the group traversal's group size, leaf size, block
-    // size (or teams of two waves per group) and waves per SIMD; the reference
+    // size (or teams of two or four waves per group) and waves per SIMD; the reference
     // traversal's stack depth.  The leaf size barely moves the error (2 x
     // 1,048,576 particles at theta = 0.5: 2.7e-4 relative RMS at leaf 4,
     // 2.6e-4 at leaf 16).
@@ -1034,13 +1043,13 @@ int gcs_autotune(gcs_engine* e, const gcs_params* base, int32_t reps, gcs_params
     if (base->mac == GCS_MAC_GROUP) {
         for (int group : {48, 64, 96})
             for (int leaf : {4, 8, 12, 16})
-                for (int block : {0, 64, 128, 256})  // 0: teams of two waves
+                for (int block : {-4, -2, 64, 128, 256})  // -t: teams of t waves
                     for (int wps : {2, 3, 4}) {
                         gcs_params q = *base;
                         q.group_size = group;
                         q.leaf_size = leaf;
-                        q.team = block == 0 ? kTeam : 1;
-                        q.block_size = block == 0 ? kTeam * kWave : block;
+                        q.team = block < 0 ? -block : 1;
+                        q.block_size = block < 0 ? -block * kWave : block;
                         q.waves_per_simd = wps;
                         cand.push_back(q);
                     }

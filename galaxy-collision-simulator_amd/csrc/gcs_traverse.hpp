@@ -568,35 +568,34 @@ __device__ __forceinline__ void combine_share(WaveLds& L, const Acc* a, const Sh
     __builtin_amdgcn_wave_barrier();
 }
 
-// k_traverse_team: the group traversal with a team of kTeam waves (one block)
-// per group.  The team walks the tree together from one LDS stack, kTeam x 64
+// k_traverse_team: the group traversal with a team of TEAM waves (one block)
+// per group.  The team walks the tree together from one LDS stack, TEAM x 64
 // nodes per round; each round's list entries and pushes are placed in wave
 // order (wave 0's first), so the lists are a function of the tree and the
 // group only.  Each wave then evaluates its share of the group's targets
-// (wave w takes targets [w*ceil(n/kTeam), ...)) over the whole lists; the
+// (wave w takes targets [w*ceil(n/TEAM), ...)) over the whole lists; the
 // targets are disjoint, so no sums cross waves.  A group's wall time is about
-// 1/kTeam of the one-wave kernel's, which evens out the work when there are
+// 1/TEAM of the one-wave kernel's, which evens out the work when there are
 // few groups per wave.  Lists that would overflow are evaluated early into
 // partial sums (kept unscaled in acc until the final evaluation).
-constexpr int kTeam = 2;
 
-template <bool POT, bool QUAD>
-__global__ __launch_bounds__(kTeam * 64) void k_traverse_team(
+template <bool POT, bool QUAD, int TEAM>
+__global__ __launch_bounds__(TEAM * 64) void k_traverse_team(
     const GroupInfo* __restrict__ groups, const int* __restrict__ gorder,
     const int* __restrict__ ngroups_dev, const SegInfo* __restrict__ segs,
     const float4* __restrict__ posm, const float4* __restrict__ node,
     const float4* __restrict__ nodeQ, float4* __restrict__ acc, int* __restrict__ slot_mem,
     unsigned* qctr, unsigned long long* counters, unsigned* err, float G, float eps2, int leaf)
 {
-    __shared__ WaveLds lds[kTeam];  // the team's stack is lds[0].stk
-    __shared__ int xch[kTeam][4];   // per wave and round: accepted, particles, left, right pushes
+    __shared__ WaveLds lds[TEAM];  // the team's stack is lds[0].stk
+    __shared__ int xch[TEAM][4];   // per wave and round: accepted, particles, left, right pushes
     __shared__ int ctl[4];          // claimed group; stack top; spilled entries; overflow
     const int l = lane_id();
     const int w = threadIdx.x >> 6;
     const int tid = threadIdx.x;
     WaveLds& L = lds[w];
     int* const stk = lds[0].stk;
-    int* const spill = slot_mem + (size_t)blockIdx.x * kTeam * kSlotInts;
+    int* const spill = slot_mem + (size_t)blockIdx.x * TEAM * kSlotInts;
     int* const nl = spill + kStackCap;
     int* const pl = nl + kNodeList;
     const int xcd = blockIdx.x % kNumXcd;
@@ -607,10 +606,10 @@ __global__ __launch_bounds__(kTeam * 64) void k_traverse_team(
     const float self_inv = __builtin_amdgcn_rsqf(eps2);
     const int pbits = 32 - __builtin_clz((unsigned)(leaf + 2));
     const uint64_t lt = (1ull << l) - 1ull;
-    // A round adds at most kTeam*64 node entries and kTeam*64*(leaf+2)
+    // A round adds at most TEAM*64 node entries and TEAM*64*(leaf+2)
     // particle entries; lists are evaluated before a round could overflow them.
-    const int node_room = kNodeList - kTeam * kWave;
-    const int part_room = kPartList - kTeam * kWave * (leaf + 2);
+    const int node_room = kNodeList - TEAM * kWave;
+    const int part_room = kPartList - TEAM * kWave * (leaf + 2);
     if (l < kMaxSplit) {
         const float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
         L.ca[kWave + l] = z;
@@ -649,7 +648,7 @@ __global__ __launch_bounds__(kTeam * 64) void k_traverse_team(
         lx = wave_min(lx); ly = wave_min(ly); lz = wave_min(lz);
         hx = wave_max(hx); hy = wave_max(hy); hz = wave_max(hz);
         // This wave's targets.
-        const int half = (gi.count + kTeam - 1) / kTeam;
+        const int half = (gi.count + TEAM - 1) / TEAM;
         const int t0 = w * half;
         const int nw = max(0, min(gi.count - t0, half));
         int nn = 0, np = 0;
@@ -674,12 +673,12 @@ __global__ __launch_bounds__(kTeam * 64) void k_traverse_team(
                 if (top == 0) {
                     // Reload the most recently spilled half.
                     const int c = min(spilled, kLdsStack / 2);
-                    for (int k = tid; k < c; k += kTeam * kWave) stk[k] = spill[spilled - c + k];
+                    for (int k = tid; k < c; k += TEAM * kWave) stk[k] = spill[spilled - c + k];
                     spilled -= c;
                     top = c;
                     __syncthreads();
                 }
-                const int cnt = min(top, kTeam * kWave);
+                const int cnt = min(top, TEAM * kWave);
                 const int e = w * kWave + l;
                 const bool have = e < cnt;
                 int v = 0;
@@ -712,7 +711,7 @@ __global__ __launch_bounds__(kTeam * 64) void k_traverse_team(
                 int na_before = 0, tp_before = 0, nl_before = 0, nr_before = 0;
                 int na_all = 0, tp_all = 0, nl_all = 0, nr_all = 0;
 #pragma unroll
-                for (int k = 0; k < kTeam; ++k) {
+                for (int k = 0; k < TEAM; ++k) {
                     const int a0 = xch[k][0], a1 = xch[k][1], a2 = xch[k][2], a3 = xch[k][3];
                     if (k < w) {
                         na_before += a0;
@@ -744,17 +743,17 @@ __global__ __launch_bounds__(kTeam * 64) void k_traverse_team(
                         overflow = true;
                         break;
                     }
-                    for (int k = tid; k < h; k += kTeam * kWave) spill[spilled + k] = stk[k];
-                    int keep[kLdsStack / 2 / (kTeam * kWave)];
+                    for (int k = tid; k < h; k += TEAM * kWave) spill[spilled + k] = stk[k];
+                    int keep[kLdsStack / 2 / (TEAM * kWave)];
 #pragma unroll
-                    for (int r = 0; r < kLdsStack / 2 / (kTeam * kWave); ++r) {
-                        const int k = tid + r * kTeam * kWave;
+                    for (int r = 0; r < kLdsStack / 2 / (TEAM * kWave); ++r) {
+                        const int k = tid + r * TEAM * kWave;
                         keep[r] = k < base - h ? stk[k + h] : 0;
                     }
                     __syncthreads();
 #pragma unroll
-                    for (int r = 0; r < kLdsStack / 2 / (kTeam * kWave); ++r) {
-                        const int k = tid + r * kTeam * kWave;
+                    for (int r = 0; r < kLdsStack / 2 / (TEAM * kWave); ++r) {
+                        const int k = tid + r * TEAM * kWave;
                         if (k < base - h) stk[k] = keep[r];
                     }
                     spilled += h;

@@ -100,9 +100,9 @@ typedef struct gcs_params {
                            reference's theta test taken from the group's box;
                            1 = w/theta + |COM - box centre| (stricter) */
     int32_t team;       /* GCS_MAC_GROUP waves per group: 0 or 1 = one wave walks
-                           and evaluates a group; 2 = a block of two waves walks
-                           it together and splits its targets (block_size is then
-                           128) */
+                           and evaluates a group; 2, 4 or 8 = a block of that many
+                           waves walks it together and splits its targets
+                           (block_size is then 64 x team) */
 } gcs_params;
 
 /* Defaults: G=1, eps2=0.0025, dt=0.01, theta=0.5, GCS_MAC_GROUP,

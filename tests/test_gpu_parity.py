@@ -132,14 +132,15 @@ def check_pipeline(gcs, orc, e, stars, p, iid=0, theta=0.5, bounds=None):
 
 
 @pytest.mark.parametrize("gsize,offset,team", [(64, 0, 1), (24, 0, 1), (128, 0, 1), (192, 0, 1),
-                                               (64, 1, 1), (64, 0, 2), (24, 0, 2), (192, 0, 2)])
+                                               (64, 1, 1), (64, 0, 2), (24, 0, 2), (192, 0, 2),
+                                               (96, 0, 4), (5, 0, 4), (192, 0, 8)])
 def test_engine_c1_group_parity(gcs, orc, gsize, offset, team):
     stars = gcs.collision(8192, 1001)
     n = len(stars)
     eps2 = np.float32(0.05) ** 2
     p = gcs.params(G=G, eps2=float(eps2), dt=0.01, theta=0.5, leaf_size=16, diagnostics=1,
                    group_size=gsize, open_offset=offset, team=team,
-                   block_size=128 if team == 2 else 256)
+                   block_size=64 * team if team > 1 else 256)
     with gcs.Engine(n) as e:
         e.load(stars)
         say("engine C1 compute, groups of at most %d, opening offset %d, team %d"
@@ -245,14 +246,14 @@ def test_large_instance_aggregates(gcs, orc):
 
 # ---- forests, graphs, composition ----------------------------------------------------
 
-@pytest.mark.parametrize("team", [1, 2])
+@pytest.mark.parametrize("team", [1, 2, 4, 8])
 def test_forest_equals_standalone(gcs, orc, team):
     specs = [gcs.random_spec(11, k, 600, 9000) for k in range(6)]
     parts = [gcs.generate(s) for s in specs]
     parts.append(cloud(gcs, 1, 3))      # single particle instance
     parts.append(cloud(gcs, 1025, 4))   # crosses a tile boundary
     p = gcs.params(G=G, eps2=0.0025, dt=0.01, theta=0.5, leaf_size=8, team=team,
-                   block_size=128 if team == 2 else 256)
+                   block_size=64 * team if team > 1 else 256)
     counts = [len(a) for a in parts]
     with gcs.Engine(sum(counts), len(parts)) as e:
         e.load(np.concatenate(parts), counts)
