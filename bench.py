@@ -124,13 +124,15 @@ def precision_study(steps, targets, cpu_seconds, seed=1003):
             acc = e.accels()
             s0 = e.stars()
         run = {"theta": theta, "leaf": base.leaf_size, "block": base.block_size,
-               "waves_per_simd": base.waves_per_simd, "group": base.group_size}
+               "waves_per_simd": base.waves_per_simd, "group": base.group_size,
+               "team": base.team}
         run.update(accuracy_check(s0, base, acc, targets))
         for prec, name in ((gcs_amd.PREC_PLAIN, "plain"),
                            (gcs_amd.PREC_COMPENSATED, "compensated")):
             kw = dict(G=1.0, eps2=eps2, dt=0.005, theta=theta, leaf_size=base.leaf_size,
                       block_size=base.block_size, waves_per_simd=base.waves_per_simd,
-                      group_size=base.group_size, precision=prec, use_graph=1)
+                      group_size=base.group_size, team=base.team, precision=prec,
+                      use_graph=1)
             pd = gcs_amd.params(diagnostics=1, **kw)
             pn = gcs_amd.params(diagnostics=0, **kw)
             with gcs_amd.Engine(n) as e:
@@ -201,7 +203,8 @@ def batched_run(n_inst, steps, warmup, leaf, seed=1004, plain_steps=10, plain_sa
     return {"instances": n_inst, "particles": total, "steps": steps,
             "ms_per_step": 1e3 * el / steps, "value": batched,
             "autotuned": {"leaf": p.leaf_size, "block": p.block_size,
-                          "waves_per_simd": p.waves_per_simd},
+                           "waves_per_simd": p.waves_per_simd, "group": p.group_size,
+                           "team": p.team},
             "plain_one_at_a_time": plain, "speedup_vs_plain": batched / plain,
             "plain_sample": "%d collisions x %d steps" % (min(plain_sample, n_inst), plain_steps)}
 
@@ -232,12 +235,12 @@ def served_run(window, rate, max_resident, leaf, seed=1005, budget=200e9, steps_
                                           use_graph=1), reps=2)
     del parts
     tuned = dict(leaf_size=tp.leaf_size, group_size=tp.group_size, block_size=tp.block_size,
-                 waves_per_simd=tp.waves_per_simd)
+                 waves_per_simd=tp.waves_per_simd, team=tp.team)
     out = {"window_s": window, "rate_jobs_per_s": rate, "jobs_offered": len(jobs),
            "offered_particle_steps_per_s": sum(n * k for _, n, k, _ in jobs) / window,
            "hbm_budget_bytes": budget,
            "autotuned": {"leaf": tp.leaf_size, "group": tp.group_size, "block": tp.block_size,
-                         "waves_per_simd": tp.waves_per_simd,
+                         "waves_per_simd": tp.waves_per_simd, "team": tp.team,
                          "tuning_forest": {"jobs": len(specs), "particles": tot}}}
     for name, kw in (("served", dict(max_resident=max_resident)),
                      ("one_at_a_time", dict(max_jobs=1, max_resident=524288))):
