@@ -162,26 +162,28 @@ __device__ __forceinline__ void interact_n(Acc& a, float px, float py, float pz,
     if (POT) a.ph += __builtin_fmaf(0.5f, t, minv);
 }
 
-// Phase 2 over the node list nl[0..k): gathers of chunk c+1 are issued before
-// chunk c is evaluated, and the indices one chunk further ahead.  Sub-wave h
-// takes chunk entries h, h + S, h + 2S, ...; entries past the list end were
-// gathered as zero-mass sources and add nothing, and the pad after the 64
-// entries holds zero-mass sources too.
+// Phase 2 over the 64-entry chunks of the node list nl[0..k) that start at
+// first, first + step, ... (step a multiple of 64): gathers of one chunk are
+// issued before the previous one is evaluated, and the indices one chunk
+// further ahead.  Sub-wave h takes chunk entries h, h + S, h + 2S, ...;
+// entries past the list end were gathered as zero-mass sources and add
+// nothing, and the pad after the 64 entries holds zero-mass sources too.
 template <bool POT, bool QUAD, int P>
 __device__ __forceinline__ void eval_node_list(WaveLds& L, const int* nl, int k,
                                                const float4* __restrict__ node,
                                                const float4* __restrict__ nodeQ, Acc* a,
-                                               const Targets& tg, float eps2, int S, int h)
+                                               const Targets& tg, float eps2, int S, int h,
+                                               int first, int step)
 {
     const int l = lane_id();
     float4 A = make_float4(0.f, 0.f, 0.f, 0.f), Q0 = A, Q1 = A;
-    const int idx = l < k ? nl[l] : -1;
-    int idx_next = l + kWave < k ? nl[l + kWave] : -1;
+    const int idx = first + l < k ? nl[first + l] : -1;
+    int idx_next = first + step + l < k ? nl[first + step + l] : -1;
     if (idx >= 0) {
         A = node[2 * idx];
         if (QUAD) { Q0 = nodeQ[2 * idx]; Q1 = nodeQ[2 * idx + 1]; }
     }
-    for (int base = 0; base < k; base += kWave) {
+    for (int base = first; base < k; base += step) {
         L.ca[l] = A;
         if (QUAD) {
             L.cq0[l] = Q0;
@@ -195,7 +197,7 @@ __device__ __forceinline__ void eval_node_list(WaveLds& L, const int* nl, int k,
             A = node[2 * idx_next];
             if (QUAD) { Q0 = nodeQ[2 * idx_next]; Q1 = nodeQ[2 * idx_next + 1]; }
         }
-        const int j2 = base + 2 * kWave + l;
+        const int j2 = base + 2 * step + l;
         idx_next = j2 < k ? nl[j2] : -1;
         const int trips = uniform((min(kWave, k - base) + S - 1) / S);
         int t = 0;
@@ -227,23 +229,24 @@ __device__ __forceinline__ void eval_node_list(WaveLds& L, const int* nl, int k,
     }
 }
 
-// Phase 2 over the particle list pl[0..k), same pipeline and split.
+// Phase 2 over the particle list pl[0..k), same chunks, pipeline and split.
 template <bool POT, int P>
 __device__ __forceinline__ void eval_part_list(WaveLds& L, const int* pl, int k,
                                                const float4* __restrict__ posm, Acc* a,
-                                               const Targets& tg, float eps2, int S, int h)
+                                               const Targets& tg, float eps2, int S, int h,
+                                               int first, int step)
 {
     const int l = lane_id();
     float4 X = make_float4(0.f, 0.f, 0.f, 0.f);
-    const int idx = l < k ? pl[l] : -1;
-    int idx_next = l + kWave < k ? pl[l + kWave] : -1;
+    const int idx = first + l < k ? pl[first + l] : -1;
+    int idx_next = first + step + l < k ? pl[first + step + l] : -1;
     if (idx >= 0) X = posm[idx];
-    for (int base = 0; base < k; base += kWave) {
+    for (int base = first; base < k; base += step) {
         L.ca[l] = X;
         __builtin_amdgcn_wave_barrier();
         X = make_float4(0.f, 0.f, 0.f, 0.f);
         if (idx_next >= 0) X = posm[idx_next];
-        const int j2 = base + 2 * kWave + l;
+        const int j2 = base + 2 * step + l;
         idx_next = j2 < k ? pl[j2] : -1;
         const int trips = uniform((min(kWave, k - base) + S - 1) / S);
         int t = 0;
@@ -270,23 +273,29 @@ __device__ __forceinline__ void eval_part_list(WaveLds& L, const int* pl, int k,
 }
 
 // The evaluation of both lists for the share's P (uniform per group).
+// first/step: the chunks this wave takes (all of them: 0 and 64).
 template <bool POT, bool QUAD>
 __device__ __forceinline__ void eval_nodes(WaveLds& L, const int* nl, int k, const float4* node,
                                            const float4* nodeQ, Acc* a, const Targets& tg,
-                                           float eps2, const Share& sh)
+                                           float eps2, const Share& sh, int first = 0,
+                                           int step = kWave)
 {
-    if (sh.P == 1) eval_node_list<POT, QUAD, 1>(L, nl, k, node, nodeQ, a, tg, eps2, sh.S, sh.h);
-    else if (sh.P == 2) eval_node_list<POT, QUAD, 2>(L, nl, k, node, nodeQ, a, tg, eps2, sh.S, sh.h);
-    else eval_node_list<POT, QUAD, 3>(L, nl, k, node, nodeQ, a, tg, eps2, sh.S, sh.h);
+    if (sh.P == 1)
+        eval_node_list<POT, QUAD, 1>(L, nl, k, node, nodeQ, a, tg, eps2, sh.S, sh.h, first, step);
+    else if (sh.P == 2)
+        eval_node_list<POT, QUAD, 2>(L, nl, k, node, nodeQ, a, tg, eps2, sh.S, sh.h, first, step);
+    else
+        eval_node_list<POT, QUAD, 3>(L, nl, k, node, nodeQ, a, tg, eps2, sh.S, sh.h, first, step);
 }
 
 template <bool POT>
 __device__ __forceinline__ void eval_parts(WaveLds& L, const int* pl, int k, const float4* posm,
-                                           Acc* a, const Targets& tg, float eps2, const Share& sh)
+                                           Acc* a, const Targets& tg, float eps2, const Share& sh,
+                                           int first = 0, int step = kWave)
 {
-    if (sh.P == 1) eval_part_list<POT, 1>(L, pl, k, posm, a, tg, eps2, sh.S, sh.h);
-    else if (sh.P == 2) eval_part_list<POT, 2>(L, pl, k, posm, a, tg, eps2, sh.S, sh.h);
-    else eval_part_list<POT, 3>(L, pl, k, posm, a, tg, eps2, sh.S, sh.h);
+    if (sh.P == 1) eval_part_list<POT, 1>(L, pl, k, posm, a, tg, eps2, sh.S, sh.h, first, step);
+    else if (sh.P == 2) eval_part_list<POT, 2>(L, pl, k, posm, a, tg, eps2, sh.S, sh.h, first, step);
+    else eval_part_list<POT, 3>(L, pl, k, posm, a, tg, eps2, sh.S, sh.h, first, step);
 }
 
 __device__ __forceinline__ float box_dist2(float lx, float ly, float lz, float hx, float hy,
@@ -572,9 +581,9 @@ __device__ __forceinline__ void combine_share(WaveLds& L, const Acc* a, const Sh
 // per group.  The team walks the tree together from one LDS stack, TEAM x 64
 // nodes per round; each round's list entries and pushes are placed in wave
 // order (wave 0's first), so the lists are a function of the tree and the
-// group only.  Each wave then evaluates its share of the group's targets
-// (wave w takes targets [w*ceil(n/TEAM), ...)) over the whole lists; the
-// targets are disjoint, so no sums cross waves.  A group's wall time is about
+// group only.  Each wave then evaluates the 64-entry chunks w, w + TEAM, ...
+// of the lists for all of the group's targets, and the TEAM partial sums of a
+// target are added through LDS in wave order.  A group's wall time is about
 // 1/TEAM of the one-wave kernel's, which evens out the work when there are
 // few groups per wave.  Lists that would overflow are evaluated early into
 // partial sums (kept unscaled in acc until the final evaluation).
@@ -647,10 +656,7 @@ __global__ __launch_bounds__(TEAM * 64) void k_traverse_team(
         }
         lx = wave_min(lx); ly = wave_min(ly); lz = wave_min(lz);
         hx = wave_max(hx); hy = wave_max(hy); hz = wave_max(hz);
-        // This wave's targets.
-        const int half = (gi.count + TEAM - 1) / TEAM;
-        const int t0 = w * half;
-        const int nw = max(0, min(gi.count - t0, half));
+        const int n = gi.count;
         int nn = 0, np = 0;
         int top = 0, spilled = 0;
         bool flushed = false;
@@ -767,28 +773,51 @@ __global__ __launch_bounds__(TEAM * 64) void k_traverse_team(
             __syncthreads();  // the lists are complete in memory
             // Evaluate this wave's targets over the lists: partial sums, or
             // the final sums once the walk is done.
-            if (nw > 0) {
-                const Share sh = make_share(nw, l);
+            // This wave's chunks of the lists, for all targets; then the
+            // waves' sums of each target in wave order.
+            {
+                const Share sh = make_share(n, l);
                 Targets tg;
                 Acc a[kMaxPer];
 #pragma unroll
                 for (int u = 0; u < kMaxPer; ++u) {
-                    const float4 pu = posm[gi.first + t0 + min(sh.t + u * sh.W, nw - 1)];
+                    const float4 pu = posm[gi.first + min(sh.t + u * sh.W, n - 1)];
                     tg.x[u] = pu.x;
                     tg.y[u] = pu.y;
                     tg.z[u] = pu.z;
                     a[u] = Acc{0.f, 0.f, 0.f, 0.f};
                 }
-                if (nn > 0) eval_nodes<POT, QUAD>(L, nl, nn, node, nodeQ, a, tg, eps2, sh);
-                if (np > 0) eval_parts<POT>(L, pl, np, posm, a, tg, eps2, sh);
+                if (nn > w * kWave)
+                    eval_nodes<POT, QUAD>(L, nl, nn, node, nodeQ, a, tg, eps2, sh, w * kWave,
+                                          TEAM * kWave);
+                if (np > w * kWave)
+                    eval_parts<POT>(L, pl, np, posm, a, tg, eps2, sh, w * kWave, TEAM * kWave);
                 float4 r[kMaxPer];
-                combine_share(L, a, sh, nw, r);
+                combine_share(L, a, sh, n, r);
+                float4* const scr = L.ca;
 #pragma unroll
                 for (int u = 0; u < kMaxPer; ++u) {
                     const int t = l + u * kWave;
-                    if (t < nw) {
-                        const int i = gi.first + t0 + t;
-                        float4 v4 = r[u];
+                    if (t < n) scr[(t >> 6) * kChunk + (t & 63)] = r[u];
+                }
+            }
+            __syncthreads();  // every wave's sums are in LDS
+            if (w == 0) {
+#pragma unroll
+                for (int u = 0; u < kMaxPer; ++u) {
+                    const int t = l + u * kWave;
+                    if (t < n) {
+                        const int slot = (t >> 6) * kChunk + (t & 63);
+                        float4 v4 = lds[0].ca[slot];
+#pragma unroll
+                        for (int ww = 1; ww < TEAM; ++ww) {
+                            const float4 o4 = lds[ww].ca[slot];
+                            v4.x += o4.x;
+                            v4.y += o4.y;
+                            v4.z += o4.z;
+                            v4.w += o4.w;
+                        }
+                        const int i = gi.first + t;
                         if (flushed) {
                             const float4 p4 = acc[i];
                             v4 = make_float4(p4.x + v4.x, p4.y + v4.y, p4.z + v4.z, p4.w + v4.w);
@@ -800,20 +829,20 @@ __global__ __launch_bounds__(TEAM * 64) void k_traverse_team(
                         acc[i] = v4;
                     }
                 }
+                n_node += (unsigned long long)nn * (unsigned long long)n;
+                n_part += (unsigned long long)np * (unsigned long long)n;
             }
-            n_node += (unsigned long long)nn * (unsigned long long)nw;
-            n_part += (unsigned long long)np * (unsigned long long)nw;
             nn = 0;
             np = 0;
             flushed = true;
             if (!need) break;
-            __syncthreads();  // both waves are done reading the lists before they refill
+            __syncthreads();  // the waves are done with the lists and LDS sums before reuse
         }
         if (overflow) {
             if (tid == 0) atomicOr(err, kErrStack);
             break;
         }
-        n_part -= (unsigned long long)nw;  // the particle itself, in its own list
+        if (w == 0) n_part -= (unsigned long long)n;  // the particle itself, in its own list
     }
     if (l == 0) {
         atomicAdd(&counters[0], n_node);
