@@ -369,18 +369,23 @@ int enqueue_step(gcs_engine* e, const gcs_params& p, bool integrate)
     mark(e, 0);
     const int64_t cwords = (int64_t)kSortPasses * e->nchunks * kRadix;
     const int ngchunk = (4 * ntiles + kGroupChunk - 1) / kGroupChunk;
-    const int rblocks = (int)std::max<int64_t>((nseg + 255) / 256,
-                                               std::min<int64_t>(1024, (cwords + 255) / 256));
-    k_reset<<<rblocks, 256, 0, e->stream>>>(nseg, e->seg_bounds, e->qctr, e->counters, e->nfront,
-                                            e->chist, cwords);
     if (!p.use_bounds) {
         k_bounds<<<qblocks, kTileThreads, 0, e->stream>>>(e->tiles, e->segs, e->posm[c],
                                                            e->bpart);
         k_bounds_reduce<<<nseg, 256, 0, e->stream>>>(e->segs, e->bpart, e->seg_bounds);
     }
+    StepReset rs;
+    rs.nseg = nseg;
+    rs.nfront = e->nfront;
+    rs.qctr = e->qctr;
+    rs.counters = e->counters;
+    rs.chist = e->chist;
+    rs.chist_words = cwords;
+    rs.gm = group_marks(e);
+    rs.am = atile_marks(e);
     k_morton<<<qblocks, kTileThreads, 0, e->stream>>>(e->tiles, e->segs, e->posm[c],
                                                        e->seg_bounds, p.use_bounds, p.global_min,
-                                                       p.global_max, e->keys[0], e->vals[0]);
+                                                       p.global_max, e->keys[0], e->vals[0], rs);
     LAUNCHCHK();
     if (e->capture_keys)
         HIPCHK(hipMemcpyAsync(e->keys_raw, e->keys[0], sizeof(uint64_t) * e->n_total,
@@ -389,17 +394,23 @@ int enqueue_step(gcs_engine* e, const gcs_params& p, bool integrate)
     enqueue_sort(e);
     LAUNCHCHK();
     mark(e, 2);
-    k_permute<<<qblocks, kTileThreads, 0, e->stream>>>(
-        e->tiles, e->segs, e->vals[0], e->posm[c], e->vel[c], e->plo[c], e->posm[c ^ 1],
-        e->vel[c ^ 1], e->plo[c ^ 1], comp ? 1 : 0, group ? 0 : 1, e->node, e->nodeQ,
-        group_marks(e), atile_marks(e));
+    // (The permute is timed with the build: one launch does both.)
+    mark(e, 3);
+    PermuteArgs pa;
+    pa.perm = e->vals[0];
+    pa.posm_in = e->posm[c];
+    pa.vel_in = e->vel[c];
+    pa.plo_in = e->plo[c];
+    pa.posm_out = e->posm[c ^ 1];
+    pa.vel_out = e->vel[c ^ 1];
+    pa.plo_out = e->plo[c ^ 1];
+    pa.compensated = comp ? 1 : 0;
+    pa.write_leaves = group ? 0 : 1;
+    k_permute_build<<<qblocks, kTileThreads, 0, e->stream>>>(
+        e->tiles, e->segs, pa, e->keys[0], e->node, e->nodeQ, e->parent, e->flag, group_marks(e),
+        atile_marks(e), group ? p.group_size : kGroup);
     e->cur = c ^ 1;
     float4* posm = e->posm[e->cur];
-    mark(e, 3);
-    k_build<<<qblocks, kTileThreads, 0, e->stream>>>(e->tiles, e->segs, e->keys[0], e->node,
-                                                     e->parent, e->flag, group_marks(e),
-                                                     atile_marks(e),
-                                                     group ? p.group_size : kGroup);
     // Number the traversal groups and the aggregation subtrees in particle
     // order and write their tables.
     k_group_chunksum<<<ngchunk, kGroupChunk, 0, e->stream>>>(e->gq, e->aq, qblocks, e->gchunk,
@@ -419,8 +430,10 @@ int enqueue_step(gcs_engine* e, const gcs_params& p, bool integrate)
     // Opening value of the merge: width (reference traversal), (w/theta)^2 or
     // (w/theta + |COM - box centre|)^2 (group traversal).
     const int amode = group ? (p.open_offset ? 2 : 1) : 0;
-    // Persistent: two blocks per CU (their LDS) take the subtrees in turn.
-    const int ablocks = (int)std::max<int64_t>(1, std::min<int64_t>(2 * e->num_cu, e->n_total));
+    // Persistent: as many blocks as fit on the CUs at once (2048 threads per
+    // CU; the LDS of two 1024-thread blocks) take the subtrees in turn.
+    const int ablocks = (int)std::max<int64_t>(
+        1, std::min<int64_t>((int64_t)e->num_cu * (2048 / kATile), e->n_total));
     k_aggregate<<<ablocks, kATile, 0, e->stream>>>(e->atiles, e->natiles, e->segs, posm, e->node,
                                                    e->nodeQ, e->boxLo, e->boxHi, e->parent,
                                                    e->front, e->nfront, inv_theta,
@@ -1288,9 +1301,11 @@ int gcs_morton_keys(const float* x, const float* y, const float* z, int64_t n, f
     rc = gcs_load(e, st.data(), &cnt, 1);
     if (!rc) {
         const int qblocks = 4 * (int)e->h_tiles.size();
+        StepReset none;
+        std::memset(&none, 0, sizeof(none));
         k_morton<<<qblocks, kTileThreads, 0, e->stream>>>(e->tiles, e->segs, e->posm[e->cur],
                                                            e->seg_bounds, 1, mn, mx, e->keys[0],
-                                                           e->vals[0]);
+                                                           e->vals[0], none);
         hipError_t le = hipGetLastError();
         if (le != hipSuccess) rc = fail(GCS_ERR_HIP, hipGetErrorString(le));
         else if (hipMemcpyAsync(keys, e->keys[0], 8 * (size_t)n, hipMemcpyDeviceToHost, e->stream) != hipSuccess ||

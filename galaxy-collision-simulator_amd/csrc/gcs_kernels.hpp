@@ -29,26 +29,6 @@ __device__ __forceinline__ QuarterTile quarter_tile(const TileInfo* __restrict__
     return q;
 }
 
-// Start-of-step reset: cube bounds to empty; work-queue, interaction and
-// aggregation-frontier counters and the sort's chunk histograms (all passes)
-// to zero.  A kernel rather than hipMemsetAsync so the step can be captured
-// into a hipGraph.
-__global__ void k_reset(int nseg, uint32_t* seg_bounds, unsigned* qctr,
-                        unsigned long long* counters, int* nfront, uint32_t* chist,
-                        int64_t chist_words)
-{
-    const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    if (tid < nseg) {
-        nfront[tid] = 0;
-        seg_bounds[2 * tid] = 0xFFFFFFFFu;
-        seg_bounds[2 * tid + 1] = 0u;
-    }
-    if (tid < kNumXcd) qctr[tid] = 0u;
-    if (tid < 2) counters[tid] = 0ull;
-    for (int64_t i = tid; i < chist_words; i += stride) chist[i] = 0u;
-}
-
 // main.cpp:152-166 -- per-instance cube [min over x,y,z, max over x,y,z], in
 // two kernels: k_bounds writes each quarter tile's (min, max) as
 // order-preserving integer encodings, k_bounds_reduce folds an instance's
@@ -107,16 +87,66 @@ __global__ __launch_bounds__(256) void k_bounds_reduce(const SegInfo* __restrict
     }
 }
 
+// Subtree marks of one kind (traversal groups, aggregation subtrees): per
+// particle a flag at a subtree's first particle, its size and root node, and
+// the number of subtrees starting in each quarter tile.
+struct Marks {
+    uint32_t* mark;
+    uint32_t* perq;
+    int* cnt;
+    int* root;
+};
+
+// Start-of-step clearing, done by the Morton kernel (no launch of its own):
+// aggregation-frontier, work-queue and interaction counters, the sort's
+// chunk histograms (all passes) and the subtree marks of the coming build.
+// A null chist means no reset (a Morton-only call).
+// Kernels rather than hipMemsetAsync, so the step can be captured into a
+// hipGraph.
+struct StepReset {
+    int nseg;
+    int* nfront;
+    unsigned* qctr;
+    unsigned long long* counters;
+    uint32_t* chist;
+    int64_t chist_words;
+    Marks gm, am;
+};
+
 // karras_bvh.cuh:34-41 -- keys of the current order, payload = global index.
 __global__ __launch_bounds__(256) void k_morton(const TileInfo* __restrict__ tiles,
                                                 const SegInfo* __restrict__ segs,
                                                 const float4* __restrict__ posm,
-                                                const uint32_t* __restrict__ seg_bounds,
+                                                uint32_t* __restrict__ seg_bounds,
                                                 int use_given, float gmin, float gmax,
                                                 uint64_t* __restrict__ keys,
-                                                uint32_t* __restrict__ vals)
+                                                uint32_t* __restrict__ vals, StepReset rs)
 {
     const QuarterTile q = quarter_tile(tiles, segs);
+    if (rs.chist) {
+        const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+        const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+        if (tid < rs.nseg) {
+            rs.nfront[tid] = 0;
+            // Given bounds are recorded as the ones used (read back per
+            // instance); computed ones were written by the bounds kernels.
+            if (use_given) {
+                seg_bounds[2 * tid] = fenc(gmin);
+                seg_bounds[2 * tid + 1] = fenc(gmax);
+            }
+        }
+        if (tid < kNumXcd) rs.qctr[tid] = 0u;
+        if (tid < 2) rs.counters[tid] = 0ull;
+        for (int64_t k = tid; k < rs.chist_words; k += stride) rs.chist[k] = 0u;
+        if (threadIdx.x == 0) {
+            rs.gm.perq[blockIdx.x] = 0u;
+            rs.am.perq[blockIdx.x] = 0u;
+        }
+        if (q.valid) {
+            rs.gm.mark[q.i] = 0u;
+            rs.am.mark[q.i] = 0u;
+        }
+    }
     if (!q.valid) return;
     float mn = gmin, mx = gmax;
     if (!use_given) {
@@ -128,35 +158,31 @@ __global__ __launch_bounds__(256) void k_morton(const TileInfo* __restrict__ til
     vals[q.i] = (uint32_t)q.i;
 }
 
-// Subtree marks of one kind (traversal groups, aggregation subtrees): per
-// particle a flag at a subtree's first particle, its size and root node, and
-// the number of subtrees starting in each quarter tile.
-struct Marks {
-    uint32_t* mark;
-    uint32_t* perq;
-    int* cnt;
-    int* root;
+// karras_bvh.cuh:44-70 -- gather the state into sorted order (and, for the
+// reference traversal, karras_bvh.cuh:134-164's leaf records).
+struct PermuteArgs {
+    const uint32_t* perm;
+    const float4* posm_in;
+    const float4* vel_in;
+    const float4* plo_in;
+    float4* posm_out;
+    float4* vel_out;
+    float4* plo_out;
+    int compensated, write_leaves;
 };
 
-// karras_bvh.cuh:44-70 -- gather the state into sorted order (and, for the
-// reference traversal, karras_bvh.cuh:134-164's leaf records); clears the
-// subtree marks of the coming build.
-__global__ __launch_bounds__(256) void k_permute(
-    const TileInfo* __restrict__ tiles, const SegInfo* __restrict__ segs,
-    const uint32_t* __restrict__ perm, const float4* __restrict__ posm_in,
-    const float4* __restrict__ vel_in, const float4* __restrict__ plo_in,
-    float4* __restrict__ posm_out, float4* __restrict__ vel_out, float4* __restrict__ plo_out,
-    int compensated, int write_leaves, float4* __restrict__ node, float4* __restrict__ nodeQ,
-    Marks gm, Marks am)
+__device__ __forceinline__ void permute_one(const QuarterTile& q, const SegInfo* __restrict__ segs,
+                                            const PermuteArgs& pa, float4* __restrict__ node,
+                                            float4* __restrict__ nodeQ)
 {
-    const QuarterTile q = quarter_tile(tiles, segs);
-    if (threadIdx.x == 0) {
-        gm.perq[blockIdx.x] = 0u;
-        am.perq[blockIdx.x] = 0u;
-    }
-    if (!q.valid) return;
-    gm.mark[q.i] = 0u;
-    am.mark[q.i] = 0u;
+    const uint32_t* __restrict__ perm = pa.perm;
+    const float4* __restrict__ posm_in = pa.posm_in;
+    const float4* __restrict__ vel_in = pa.vel_in;
+    const float4* __restrict__ plo_in = pa.plo_in;
+    float4* __restrict__ posm_out = pa.posm_out;
+    float4* __restrict__ vel_out = pa.vel_out;
+    float4* __restrict__ plo_out = pa.plo_out;
+    const int compensated = pa.compensated, write_leaves = pa.write_leaves;
     const int src = (int)perm[q.i];
     const float4 p = posm_in[src];
     posm_out[q.i] = p;
@@ -202,14 +228,12 @@ __device__ __forceinline__ void mark_subtree(const SegInfo& sg, int x, int count
 // (aggregation subtrees) -- a child of at most that many under a parent of
 // more, or the root of a small instance -- at their first particle, with
 // their size (and root node), counted per quarter tile for the numbering.
-__global__ __launch_bounds__(256) void k_build(const TileInfo* __restrict__ tiles,
-                                               const SegInfo* __restrict__ segs,
-                                               const uint64_t* __restrict__ skeys,
-                                               float4* __restrict__ node, int* __restrict__ parent,
-                                               int* __restrict__ flag, Marks gm, Marks am, int gmax)
+__device__ __forceinline__ void build_one(const QuarterTile& q, const SegInfo* __restrict__ segs,
+                                          const uint64_t* __restrict__ skeys,
+                                          float4* __restrict__ node, int* __restrict__ parent,
+                                          int* __restrict__ flag, const Marks& gm, const Marks& am,
+                                          int gmax)
 {
-    const QuarterTile q = quarter_tile(tiles, segs);
-    if (!q.valid) return;
     const SegInfo sg = segs[q.seg];
     const int n = sg.n, i = q.local;
     if (i == 0) {
@@ -253,6 +277,29 @@ __global__ __launch_bounds__(256) void k_build(const TileInfo* __restrict__ tile
         if (nl <= kATile) mark_subtree(sg, sg.start + lo, nl, cl, am);
         if (nr <= kATile) mark_subtree(sg, gsplit + 1, nr, cr, am);
     }
+}
+
+__global__ __launch_bounds__(256) void k_build(const TileInfo* __restrict__ tiles,
+                                               const SegInfo* __restrict__ segs,
+                                               const uint64_t* __restrict__ skeys,
+                                               float4* __restrict__ node, int* __restrict__ parent,
+                                               int* __restrict__ flag, Marks gm, Marks am, int gmax)
+{
+    const QuarterTile q = quarter_tile(tiles, segs);
+    if (q.valid) build_one(q, segs, skeys, node, parent, flag, gm, am, gmax);
+}
+
+// The gather and the tree build of a step in one launch: the two only read
+// the sorted keys and permutation, and write disjoint data.
+__global__ __launch_bounds__(256) void k_permute_build(
+    const TileInfo* __restrict__ tiles, const SegInfo* __restrict__ segs, PermuteArgs pa,
+    const uint64_t* __restrict__ skeys, float4* __restrict__ node, float4* __restrict__ nodeQ,
+    int* __restrict__ parent, int* __restrict__ flag, Marks gm, Marks am, int gmax)
+{
+    const QuarterTile q = quarter_tile(tiles, segs);
+    if (!q.valid) return;
+    permute_one(q, segs, pa, node, nodeQ);
+    build_one(q, segs, skeys, node, parent, flag, gm, am, gmax);
 }
 
 // Groups and aggregation subtrees per chunk of kGroupChunk quarter tiles.

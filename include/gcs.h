@@ -214,7 +214,8 @@ int gcs_read_sim_time(gcs_engine* e, int32_t id, double* t);
 int gcs_engine_info(gcs_engine* e, int64_t* out8);
 /* Milliseconds of the last step's stages on the device (needs
  * use_graph = 0): [0] bounds+morton, [1] sort, [2] permute, [3] build,
- * [4] aggregate, [5] traverse, [6] integrate. */
+ * [4] aggregate, [5] traverse, [6] integrate.  The permute runs in the
+ * build's launch, so [3] holds both and [2] only the event overhead. */
 int gcs_read_stage_ms(gcs_engine* e, float* out7);
 
 /* ---- Stand-alone stage entry points (host buffers in and out) ----------- */
