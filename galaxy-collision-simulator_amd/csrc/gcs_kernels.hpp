@@ -511,24 +511,12 @@ __device__ __forceinline__ void quad_shift(NodeData& o, const NodeData& c)
 // the group traversal (width*inv_theta)^2 (mode 1) or
 // (width*inv_theta + |com - AABB centre|)^2 (mode 2); quadrupole moments by the
 // parallel-axis shift.
-__device__ __forceinline__ NodeData merge(const NodeData& l, const NodeData& r, float inv_theta,
-                                          int mode, float& w, float& open)
+// Width and opening value of a merged node (from its AABB and COM only).
+__device__ __forceinline__ void node_open(const NodeData& o, float inv_theta, int mode, float& w,
+                                          float& open)
 {
 #pragma clang fp contract(off)
-    NodeData o;
-    o.lx = fminf(l.lx, r.lx); o.ly = fminf(l.ly, r.ly); o.lz = fminf(l.lz, r.lz);
-    o.hx = fmaxf(l.hx, r.hx); o.hy = fmaxf(l.hy, r.hy); o.hz = fmaxf(l.hz, r.hz);
     w = fmaxf(o.hx - o.lx, fmaxf(o.hy - o.ly, o.hz - o.lz));
-    o.m = l.m + r.m;
-    if (o.m > 0.0f) {
-        o.cx = (l.cx * l.m + r.cx * r.m) / o.m;
-        o.cy = (l.cy * l.m + r.cy * r.m) / o.m;
-        o.cz = (l.cz * l.m + r.cz * r.m) / o.m;
-    } else {
-        o.cx = (o.lx + o.hx) * 0.5f;
-        o.cy = (o.ly + o.hy) * 0.5f;
-        o.cz = (o.lz + o.hz) * 0.5f;
-    }
     if (mode == 2) {
         const float dx = o.cx - (o.lx + o.hx) * 0.5f;
         const float dy = o.cy - (o.ly + o.hy) * 0.5f;
@@ -542,9 +530,35 @@ __device__ __forceinline__ NodeData merge(const NodeData& l, const NodeData& r, 
     } else {
         open = w;
     }
+}
+
+__device__ __forceinline__ NodeData merge(const NodeData& l, const NodeData& r)
+{
+#pragma clang fp contract(off)
+    NodeData o;
+    o.lx = fminf(l.lx, r.lx); o.ly = fminf(l.ly, r.ly); o.lz = fminf(l.lz, r.lz);
+    o.hx = fmaxf(l.hx, r.hx); o.hy = fmaxf(l.hy, r.hy); o.hz = fmaxf(l.hz, r.hz);
+    o.m = l.m + r.m;
+    if (o.m > 0.0f) {
+        o.cx = (l.cx * l.m + r.cx * r.m) / o.m;
+        o.cy = (l.cy * l.m + r.cy * r.m) / o.m;
+        o.cz = (l.cz * l.m + r.cz * r.m) / o.m;
+    } else {
+        o.cx = (o.lx + o.hx) * 0.5f;
+        o.cy = (o.ly + o.hy) * 0.5f;
+        o.cz = (o.lz + o.hz) * 0.5f;
+    }
     o.q0 = o.q1 = o.q2 = o.q3 = o.q4 = o.q5 = o.qt = 0.0f;
     quad_shift(o, l);
     quad_shift(o, r);
+    return o;
+}
+
+__device__ __forceinline__ NodeData merge(const NodeData& l, const NodeData& r, float inv_theta,
+                                          int mode, float& w, float& open)
+{
+    const NodeData o = merge(l, r);
+    node_open(o, inv_theta, mode, w, open);
     return o;
 }
 
@@ -560,19 +574,27 @@ __device__ __forceinline__ void store_node(int v, const NodeData& d, float w, fl
     boxHi[v] = make_float4(d.hx, d.hy, d.hz, 0.0f);
 }
 
-// karras_bvh.cuh:175-225 -- bottom-up aggregation, in two kernels.  A thread
-// climbs from a leaf carrying its subtree's aggregate in registers; at each
-// node the first of the two arrivals stops, the second loads only the sibling
-// and merges.
+// karras_bvh.cuh:175-225 -- bottom-up aggregation, in two kernels.  Each
+// node is merged once from its two children, by the second of the two
+// climbs that reach it (the first stops there).
 //
 // k_aggregate: one aggregation subtree (a maximal subtree of at most kATile
-// particles) per block at a time, one thread per leaf.  Every node of the
-// subtree is reached only by this block: its arrival flag is a bit of an LDS
-// word (workgroup-scope acquire-release) and its aggregate is kept in LDS
-// (slot = the end of the range that numbers the node, minus the subtree's
-// first particle) besides global memory, so sibling reads stay on the CU.
-// The thread that completes the subtree's root appends it to its instance's
-// frontier for the instance pass.
+// particles) per block at a time.  Its c - 1 internal nodes have indices
+// in t0 .. t0 + c - 1 (Karras numbering: a node's index is one end of its
+// range), one of those c indices belonging to a node outside the subtree;
+// slot = index - t0.  The whole climb runs in LDS:
+//   1. thread k reads the node of slot k and, if its range lies inside the
+//      subtree's, loads its split, leaf flags and parent slot into an LDS
+//      word, and the particle data of its leaf children: a node with two
+//      leaf children is merged at once and starts a climb; the one leaf child
+//      of a node with one is parked in the node's LDS record;
+//   2. climbs go up: at a node whose other child is a leaf, the parked leaf
+//      is the sibling; at one with two internal children, an LDS arrival bit
+//      (workgroup-scope acquire-release) lets the second climb through;
+//   3. every node record is written to global memory once, at the end.
+// No global memory access sits between two levels of a climb.  The thread
+// that completes the subtree's root appends it to its instance's frontier
+// for the instance pass.
 __device__ __forceinline__ NodeData lds_node(const float4 (*ldat)[4], const float* lqt, int s)
 {
     const float4 a4 = ldat[s][0], b4 = ldat[s][1], c4 = ldat[s][2], d4 = ldat[s][3];
@@ -585,73 +607,105 @@ __device__ __forceinline__ NodeData lds_node(const float4 (*ldat)[4], const floa
     return d;
 }
 
-__global__ __launch_bounds__(kATile) void k_aggregate(
+__device__ __forceinline__ void lds_put(float4 (*ldat)[4], float* lqt, int s, const NodeData& d)
+{
+    ldat[s][0] = make_float4(d.cx, d.cy, d.cz, d.m);
+    ldat[s][1] = make_float4(d.lx, d.ly, d.lz, d.hx);
+    ldat[s][2] = make_float4(d.hy, d.hz, d.q0, d.q1);
+    ldat[s][3] = make_float4(d.q2, d.q3, d.q4, d.q5);
+    lqt[s] = d.qt;
+}
+
+// LDS word of an internal node of a subtree: split (relative to the
+// subtree's first particle, 10 bits), left child is a leaf (bit 10), right
+// child is a leaf (bit 11), parent slot + 1 (bits 12..22; 0 at the root).
+static_assert(kATile <= 1024, "splits fit 10 bits, parent slots + 1 fit 11");
+__device__ __forceinline__ uint32_t topo_word(int split, bool lleaf, bool rleaf, int pslot)
+{
+    return (uint32_t)split | (lleaf ? 1u << 10 : 0u) | (rleaf ? 1u << 11 : 0u) |
+           ((uint32_t)(pslot + 1) << 12);
+}
+
+__global__ __launch_bounds__(kATile) __attribute__((amdgpu_waves_per_eu(8))) void k_aggregate(
     const ATileInfo* __restrict__ atiles, const int* __restrict__ natiles_dev,
     const SegInfo* __restrict__ segs, const float4* __restrict__ posm, float4* node,
     float4* nodeQ, float4* boxLo, float4* boxHi, const int* __restrict__ parent,
     int* __restrict__ front, int* __restrict__ nfront, float inv_theta, int mode)
 {
     __shared__ uint32_t lflag[kATile / 32];
+    __shared__ uint32_t ltopo[kATile];
     __shared__ float4 ldat[kATile][4];
     __shared__ float lqt[kATile];
     const int tid = threadIdx.x;
     const int natiles = *natiles_dev;
     for (int a = blockIdx.x; a < natiles; a += gridDim.x) {
         const ATileInfo t = atiles[a];
-        __syncthreads();  // the previous subtree's LDS reads are done
-        if (tid < kATile / 32) lflag[tid] = 0u;
-        __syncthreads();
-        if (tid >= t.count) continue;
         const SegInfo sg = segs[t.seg];
-        const int t0 = t.first;
-        int child = t0 + tid + sg.leaf_base;
-        if (t.count == 1) {
-            front[2 * sg.start + atomicAdd(&nfront[t.seg], 1)] = child;
+        const int t0 = t.first, c = t.count;
+        __syncthreads();  // the previous subtree's LDS use is over
+        if (c == 1) {
+            if (tid == 0) front[2 * sg.start + atomicAdd(&nfront[t.seg], 1)] = t0 + sg.leaf_base;
             continue;
         }
-        NodeData mine = leaf_data(posm[t0 + tid]);
-        int cur = parent[child];
-        // The node record and parent of the next level are loaded one level ahead.
-        float4 bb = node[2 * cur + 1];
-        int par = cur == t.root ? -1 : parent[cur];
-        for (;;) {
-            const int first = __float_as_int(bb.y), last = __float_as_int(bb.z),
-                      split = __float_as_int(bb.w);
-            float4 bb_next = make_float4(0.f, 0.f, 0.f, 0.f);
-            int par_next = -1;
-            if (par != -1) {
-                bb_next = node[2 * par + 1];
-                par_next = par == t.root ? -1 : parent[par];
-            }
-            const int slot = cur - sg.int_base - t0;
-            const uint32_t bit = 1u << (slot & 31);
-            const uint32_t old = __hip_atomic_fetch_or(&lflag[slot >> 5], bit, __ATOMIC_ACQ_REL,
-                                                       __HIP_MEMORY_SCOPE_WORKGROUP);
-            if (!(old & bit)) break;
+        if (tid < kATile / 32) lflag[tid] = 0u;
+        const int ibase = t0 + sg.int_base;  // global index of slot 0
+        bool climb = false;
+        // Slot tid holds a node of the subtree iff the index exists in the
+        // instance (internal nodes 0 .. n-2) and its range lies inside.
+        const int v = ibase + tid;
+        bool member = false;
+        int first = 0, last = 0, split = 0, pv = -1;
+        if (tid < c && t0 - sg.start + tid <= sg.n - 2) {
+            const float4 bb = node[2 * v + 1];
+            first = __float_as_int(bb.y);
+            last = __float_as_int(bb.z);
+            split = __float_as_int(bb.w);
+            member = first >= t0 && last <= t0 + c - 1;
+            if (member && v != t.root) pv = parent[v];
+        }
+        if (member) {
             const bool lleaf = first == split, rleaf = last == split + 1;
-            const int lidx = lleaf ? split + sg.leaf_base : split + sg.int_base;
-            const bool mine_left = child == lidx;
-            // The sibling: a leaf of this subtree, or one of its nodes in LDS.
-            const bool sleaf = mine_left ? rleaf : lleaf;
-            const int sp = mine_left ? split + 1 : split;  // its particle / numbering end
-            const NodeData sib = sleaf ? leaf_data(posm[sp]) : lds_node(ldat, lqt, sp - t0);
-            float w, open;
-            mine = mine_left ? merge(mine, sib, inv_theta, mode, w, open)
-                             : merge(sib, mine, inv_theta, mode, w, open);
-            ldat[slot][0] = make_float4(mine.cx, mine.cy, mine.cz, mine.m);
-            ldat[slot][1] = make_float4(mine.lx, mine.ly, mine.lz, mine.hx);
-            ldat[slot][2] = make_float4(mine.hy, mine.hz, mine.q0, mine.q1);
-            ldat[slot][3] = make_float4(mine.q2, mine.q3, mine.q4, mine.q5);
-            lqt[slot] = mine.qt;
-            store_node(cur, mine, w, open, node, nodeQ, boxLo, boxHi);
-            if (cur == t.root) {
-                front[2 * sg.start + atomicAdd(&nfront[t.seg], 1)] = cur;
+            ltopo[tid] = topo_word(split - t0, lleaf, rleaf, pv < 0 ? -1 : pv - ibase);
+            if (lleaf && rleaf) {
+                lds_put(ldat, lqt, tid, merge(leaf_data(posm[split]), leaf_data(posm[split + 1])));
+                climb = true;
+            } else if (lleaf || rleaf) {
+                lds_put(ldat, lqt, tid, leaf_data(posm[lleaf ? split : split + 1]));
+            }
+        }
+        __syncthreads();  // topology, parked leaves and the first merges are in LDS
+        int sl = tid;
+        while (climb) {
+            const uint32_t tw = ltopo[sl];
+            const int ps = (int)(tw >> 12) - 1;
+            if (ps < 0) {
+                front[2 * sg.start + atomicAdd(&nfront[t.seg], 1)] = ibase + sl;
                 break;
             }
-            child = cur;
-            cur = par;
-            bb = bb_next;
-            par = par_next;
+            const uint32_t pw = ltopo[ps];
+            const int psplit = (int)(pw & 1023u);
+            const bool plleaf = (pw >> 10) & 1u, prleaf = (pw >> 11) & 1u;
+            const bool mine_left = !plleaf && sl == psplit;
+            const bool sleaf = mine_left ? prleaf : plleaf;
+            int ss = ps;  // a parked leaf sits in the parent's own record
+            if (!sleaf) {
+                const uint32_t bit = 1u << (ps & 31);
+                const uint32_t old = __hip_atomic_fetch_or(&lflag[ps >> 5], bit, __ATOMIC_ACQ_REL,
+                                                           __HIP_MEMORY_SCOPE_WORKGROUP);
+                if (!(old & bit)) break;
+                ss = mine_left ? psplit + 1 : psplit;
+            }
+            // Both children from LDS (this climb's own record included).
+            const int ls = mine_left ? sl : ss, rs = mine_left ? ss : sl;
+            lds_put(ldat, lqt, ps, merge(lds_node(ldat, lqt, ls), lds_node(ldat, lqt, rs)));
+            sl = ps;
+        }
+        __syncthreads();  // every node of the subtree is complete in LDS
+        if (member) {
+            const NodeData d = lds_node(ldat, lqt, tid);
+            float w, open;
+            node_open(d, inv_theta, mode, w, open);
+            store_node(v, d, w, open, node, nodeQ, boxLo, boxHi);
         }
     }
 }
