@@ -122,6 +122,11 @@ struct gcs_engine {
     int* ngroups = nullptr;
     int* front = nullptr;   // aggregation frontier: instance s uses [2*start_s, 2*(start_s+n_s))
     int* nfront = nullptr;
+    // Nodes above the aggregation subtrees, listed by the build: instance s
+    // uses tlist[start_s, start_s + ntop[s]); tslot[node] = its list slot.
+    int* tlist = nullptr;
+    int* ntop = nullptr;
+    int* tslot = nullptr;
     // aggregation subtrees (maximal subtrees of at most kATile particles)
     uint32_t* amark = nullptr;
     uint32_t* aq = nullptr;      // subtrees per quarter tile
@@ -202,6 +207,7 @@ void free_all(gcs_engine* e)
     dfree(e->tiles); dfree(e->segs); dfree(e->amark); dfree(e->aq); dfree(e->achunk); dfree(e->acnt); dfree(e->aroot); dfree(e->atiles); dfree(e->natiles); dfree(e->groups); dfree(e->hist); dfree(e->chist);
     dfree(e->gmark); dfree(e->gq); dfree(e->gorder); dfree(e->gchunk); dfree(e->gcnt); dfree(e->ngroups);
     dfree(e->front); dfree(e->nfront);
+    dfree(e->tlist); dfree(e->ntop); dfree(e->tslot);
     dfree(e->seg_bounds); dfree(e->bpart); dfree(e->qctr); dfree(e->counters); dfree(e->err);
     dfree(e->diag_part); dfree(e->diag_out); dfree(e->amax_part); dfree(e->dt_inst); dfree(e->tsim); dfree(e->slot_mem); dfree(e->staging);
     for (auto& v : e->ev)
@@ -377,6 +383,7 @@ int enqueue_step(gcs_engine* e, const gcs_params& p, bool integrate)
     StepReset rs;
     rs.nseg = nseg;
     rs.nfront = e->nfront;
+    rs.ntop = e->ntop;
     rs.qctr = e->qctr;
     rs.counters = e->counters;
     rs.chist = e->chist;
@@ -408,7 +415,7 @@ int enqueue_step(gcs_engine* e, const gcs_params& p, bool integrate)
     pa.write_leaves = group ? 0 : 1;
     k_permute_build<<<qblocks, kTileThreads, 0, e->stream>>>(
         e->tiles, e->segs, pa, e->keys[0], e->node, e->nodeQ, e->parent, e->flag, group_marks(e),
-        atile_marks(e), group ? p.group_size : kGroup);
+        atile_marks(e), group ? p.group_size : kGroup, TopList{e->tlist, e->ntop, e->tslot});
     e->cur = c ^ 1;
     float4* posm = e->posm[e->cur];
     // Number the traversal groups and the aggregation subtrees in particle
@@ -440,7 +447,8 @@ int enqueue_step(gcs_engine* e, const gcs_params& p, bool integrate)
                                                    amode);
     k_aggregate_top<<<nseg, 1024, 0, e->stream>>>(e->segs, posm, e->node, e->nodeQ, e->boxLo,
                                                   e->boxHi, e->parent, e->flag, e->front,
-                                                  e->nfront, inv_theta, amode);
+                                                  e->nfront, e->tlist, e->ntop, e->tslot,
+                                                  inv_theta, amode);
     LAUNCHCHK();
     mark(e, 5);
     if (group) {
@@ -663,6 +671,9 @@ int gcs_create(int64_t cap_n, int32_t cap_k, int32_t device, gcs_engine** out)
     if (!rc) rc = dalloc(&e->ngroups, 1);
     if (!rc) rc = dalloc(&e->front, 2 * n + 2);
     if (!rc) rc = dalloc(&e->nfront, (size_t)cap_k);
+    if (!rc) rc = dalloc(&e->tlist, n);
+    if (!rc) rc = dalloc(&e->ntop, (size_t)cap_k);
+    if (!rc) rc = dalloc(&e->tslot, 2 * n);
     if (!rc) rc = dalloc(&e->amark, n);
     if (!rc) rc = dalloc(&e->aq, 4 * nt);
     if (!rc) rc = dalloc(&e->achunk, max_gchunks(cap_n, cap_k));
@@ -1366,7 +1377,8 @@ int gcs_karras_tree(const uint64_t* sorted_keys, int32_t n, int32_t* left, int32
         const int qblocks = 4 * (int)e->h_tiles.size();
         k_build<<<qblocks, kTileThreads, 0, e->stream>>>(e->tiles, e->segs, e->keys[0], e->node,
                                                          e->parent, e->flag, group_marks(e),
-                                                         atile_marks(e), kGroup);
+                                                         atile_marks(e), kGroup,
+                                                         TopList{nullptr, nullptr, nullptr});
         hipError_t le = hipGetLastError();
         if (le != hipSuccess) rc = fail(GCS_ERR_HIP, hipGetErrorString(le));
     }

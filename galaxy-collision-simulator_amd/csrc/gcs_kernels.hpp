@@ -106,6 +106,7 @@ struct Marks {
 struct StepReset {
     int nseg;
     int* nfront;
+    int* ntop;
     unsigned* qctr;
     unsigned long long* counters;
     uint32_t* chist;
@@ -128,6 +129,7 @@ __global__ __launch_bounds__(256) void k_morton(const TileInfo* __restrict__ til
         const int64_t stride = (int64_t)gridDim.x * blockDim.x;
         if (tid < rs.nseg) {
             rs.nfront[tid] = 0;
+            rs.ntop[tid] = 0;
             // Given bounds are recorded as the ones used (read back per
             // instance); computed ones were written by the bounds kernels.
             if (use_given) {
@@ -228,11 +230,21 @@ __device__ __forceinline__ void mark_subtree(const SegInfo& sg, int x, int count
 // (aggregation subtrees) -- a child of at most that many under a parent of
 // more, or the root of a small instance -- at their first particle, with
 // their size (and root node), counted per quarter tile for the numbering.
+// The nodes of more than kATile particles of each instance (the ones above
+// the aggregation subtrees), listed from the build in any order: list[start +
+// k] = node, slot[node] = k, count[instance] = number listed.  list == nullptr:
+// not listed.
+struct TopList {
+    int* list;
+    int* count;
+    int* slot;
+};
+
 __device__ __forceinline__ void build_one(const QuarterTile& q, const SegInfo* __restrict__ segs,
                                           const uint64_t* __restrict__ skeys,
                                           float4* __restrict__ node, int* __restrict__ parent,
                                           int* __restrict__ flag, const Marks& gm, const Marks& am,
-                                          int gmax)
+                                          int gmax, const TopList& tl)
 {
     const SegInfo sg = segs[q.seg];
     const int n = sg.n, i = q.local;
@@ -276,6 +288,11 @@ __device__ __forceinline__ void build_one(const QuarterTile& q, const SegInfo* _
     if (hi - lo + 1 > kATile) {
         if (nl <= kATile) mark_subtree(sg, sg.start + lo, nl, cl, am);
         if (nr <= kATile) mark_subtree(sg, gsplit + 1, nr, cr, am);
+        if (tl.list) {
+            const int k = atomicAdd(&tl.count[q.seg], 1);
+            tl.list[sg.start + k] = me;
+            tl.slot[me] = k;
+        }
     }
 }
 
@@ -283,10 +300,11 @@ __global__ __launch_bounds__(256) void k_build(const TileInfo* __restrict__ tile
                                                const SegInfo* __restrict__ segs,
                                                const uint64_t* __restrict__ skeys,
                                                float4* __restrict__ node, int* __restrict__ parent,
-                                               int* __restrict__ flag, Marks gm, Marks am, int gmax)
+                                               int* __restrict__ flag, Marks gm, Marks am, int gmax,
+                                               TopList tl)
 {
     const QuarterTile q = quarter_tile(tiles, segs);
-    if (q.valid) build_one(q, segs, skeys, node, parent, flag, gm, am, gmax);
+    if (q.valid) build_one(q, segs, skeys, node, parent, flag, gm, am, gmax, tl);
 }
 
 // The gather and the tree build of a step in one launch: the two only read
@@ -294,12 +312,12 @@ __global__ __launch_bounds__(256) void k_build(const TileInfo* __restrict__ tile
 __global__ __launch_bounds__(256) void k_permute_build(
     const TileInfo* __restrict__ tiles, const SegInfo* __restrict__ segs, PermuteArgs pa,
     const uint64_t* __restrict__ skeys, float4* __restrict__ node, float4* __restrict__ nodeQ,
-    int* __restrict__ parent, int* __restrict__ flag, Marks gm, Marks am, int gmax)
+    int* __restrict__ parent, int* __restrict__ flag, Marks gm, Marks am, int gmax, TopList tl)
 {
     const QuarterTile q = quarter_tile(tiles, segs);
     if (!q.valid) return;
     permute_one(q, segs, pa, node, nodeQ);
-    build_one(q, segs, skeys, node, parent, flag, gm, am, gmax);
+    build_one(q, segs, skeys, node, parent, flag, gm, am, gmax, tl);
 }
 
 // Groups and aggregation subtrees per chunk of kGroupChunk quarter tiles.
@@ -862,8 +880,133 @@ __device__ __forceinline__ void climb_rounds(const SegInfo& sg, int* la, int n, 
                   marks, inv_theta, mode);
 }
 
+// The top pass in LDS, for an instance with at most kTopCap nodes above its
+// aggregation subtrees (listed by the build): the same scheme as
+// k_aggregate with list slots instead of index slots.  Thread k loads the
+// node of slot k (and of k + blockDim): its parent's and its larger
+// children's slots, and the records of its other children (aggregation
+// subtree roots, finished by k_aggregate, or leaves) -- merged at once when
+// both are such, else parked in the node's own LDS record.  Climbs then run
+// in LDS, and every node record is written once at the end.
+constexpr int kTopCap = 2048;
+struct TopLds {
+    float4 dat[kTopCap][4];
+    float qt[kTopCap];
+    uint16_t par[kTopCap], cl[kTopCap], cr[kTopCap];  // slot + 1; 0: none (root) / not listed
+    uint32_t flag[kTopCap / 32];
+};
+
+__device__ __forceinline__ NodeData child_data(int idx, bool leaf, const SegInfo& sg,
+                                               const float4* __restrict__ posm,
+                                               const float4* node, const float4* nodeQ,
+                                               const float4* boxLo, const float4* boxHi)
+{
+    // The five vectors of either record, then one conversion (a choice
+    // between two whole records would go through private memory).
+    float4 A, lo, hi, qa, qb;
+    if (leaf) {
+        A = posm[idx - sg.leaf_base];
+        lo = hi = make_float4(A.x, A.y, A.z, 0.0f);
+        qa = qb = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    } else {
+        A = node[2 * idx];
+        lo = boxLo[idx];
+        hi = boxHi[idx];
+        qa = nodeQ[2 * idx];
+        qb = nodeQ[2 * idx + 1];
+    }
+    NodeData d;
+    d.cx = A.x; d.cy = A.y; d.cz = A.z; d.m = A.w;
+    d.lx = lo.x; d.ly = lo.y; d.lz = lo.z;
+    d.hx = hi.x; d.hy = hi.y; d.hz = hi.z;
+    d.q0 = qa.x; d.q1 = qa.y; d.q2 = qa.z; d.q3 = qa.w;
+    d.q4 = qb.x; d.q5 = qb.y; d.qt = qb.w;
+    return d;
+}
+
+__device__ __forceinline__ void top_lds(TopLds& T, const SegInfo& sg, int nt,
+                                        const int* __restrict__ tlist,
+                                        const int* __restrict__ tslot,
+                                        const float4* __restrict__ posm, float4* node,
+                                        float4* nodeQ, float4* boxLo, float4* boxHi,
+                                        const int* __restrict__ parent, float inv_theta, int mode)
+{
+    constexpr int kPer = kTopCap / 1024;
+    const int tid = threadIdx.x;
+    for (int k = tid; k < kTopCap / 32; k += blockDim.x) T.flag[k] = 0u;
+    int vk[kPer];
+    bool start[kPer];
+#pragma unroll
+    for (int r = 0; r < kPer; ++r) {
+        const int k = tid + r * (int)blockDim.x;
+        vk[r] = -1;
+        start[r] = false;
+        if (k >= nt) continue;
+        const int v = tlist[sg.start + k];
+        vk[r] = v;
+        const float4 bb = node[2 * v + 1];
+        const int first = __float_as_int(bb.y), last = __float_as_int(bb.z),
+                  split = __float_as_int(bb.w);
+        const int pv = parent[v];
+        const bool lleaf = first == split, rleaf = last == split + 1;
+        const int lidx = lleaf ? split + sg.leaf_base : split + sg.int_base;
+        const int ridx = rleaf ? split + 1 + sg.leaf_base : split + 1 + sg.int_base;
+        const bool ltop = split - first + 1 > kATile, rtop = last - split > kATile;
+        T.par[k] = (uint16_t)(pv < 0 ? 0 : tslot[pv] + 1);
+        T.cl[k] = (uint16_t)(ltop ? tslot[lidx] + 1 : 0);
+        T.cr[k] = (uint16_t)(rtop ? tslot[ridx] + 1 : 0);
+        if (!ltop || !rtop) {
+            // The child (or the left of two) that is not listed: merged with
+            // the other when both are not listed, else parked.
+            NodeData a = child_data(ltop ? ridx : lidx, ltop ? rleaf : lleaf, sg, posm, node, nodeQ,
+                                    boxLo, boxHi);
+            if (!ltop && !rtop) {
+                a = merge(a, child_data(ridx, rleaf, sg, posm, node, nodeQ, boxLo, boxHi));
+                start[r] = true;
+            }
+            lds_put(T.dat, T.qt, k, a);
+        }
+    }
+    __syncthreads();  // topology, parked children and the first merges are in LDS
+#pragma unroll
+    for (int r = 0; r < kPer; ++r) {
+        int sl = tid + r * (int)blockDim.x;
+        bool climb = start[r];
+        while (climb) {
+            const int ps = (int)T.par[sl] - 1;
+            if (ps < 0) break;
+            const int lc = (int)T.cl[ps] - 1, rc = (int)T.cr[ps] - 1;
+            const bool mine_left = lc == sl;
+            const int other = mine_left ? rc : lc;
+            int ss = ps;  // a parked child sits in the parent's own record
+            if (other >= 0) {
+                const uint32_t bit = 1u << (ps & 31);
+                const uint32_t old = __hip_atomic_fetch_or(&T.flag[ps >> 5], bit,
+                                                           __ATOMIC_ACQ_REL,
+                                                           __HIP_MEMORY_SCOPE_WORKGROUP);
+                if (!(old & bit)) break;
+                ss = other;
+            }
+            const int ls = mine_left ? sl : ss, rs = mine_left ? ss : sl;
+            lds_put(T.dat, T.qt, ps, merge(lds_node(T.dat, T.qt, ls), lds_node(T.dat, T.qt, rs)));
+            sl = ps;
+        }
+    }
+    __syncthreads();  // every listed node is complete in LDS
+#pragma unroll
+    for (int r = 0; r < kPer; ++r) {
+        const int k = tid + r * (int)blockDim.x;
+        if (k >= nt) continue;
+        const NodeData d = lds_node(T.dat, T.qt, k);
+        float w, open;
+        node_open(d, inv_theta, mode, w, open);
+        store_node(vk[r], d, w, open, node, nodeQ, boxLo, boxHi);
+    }
+}
+
 // k_aggregate_top: one block per instance finishes the nodes of more than
-// kATile particles, climbing from the roots of the aggregation subtrees with
+// kATile particles: in LDS (top_lds) when the build listed at most kTopCap of
+// them, else by climbing from the roots of the aggregation subtrees with
 // arrival marks at workgroup scope (bits in LDS, an LDS hash, or counters in
 // global memory for very large instances) -- every arrival at a node of an
 // instance is in that instance's block, so no write-back or invalidate across
@@ -876,17 +1019,31 @@ __device__ __forceinline__ void climb_rounds(const SegInfo& sg, int* la, int n, 
 // runs to the end.  A thread thus never holds back climbers that have not
 // started behind one long climb.  The lists live in the instance's 2n-entry
 // frontier area: the current one in the first half, the next in the second.
+constexpr int kTopSmemWords =
+    (int)(sizeof(TopLds) > 4 * (kTopLdsNodes / 32) ? sizeof(TopLds) / 4 : kTopLdsNodes / 32);
+static_assert(sizeof(TopLds) % 16 == 0, "LDS words");
+static_assert(kTopCap % 1024 == 0 && kTopCap <= 65535, "slots + 1 fit 16 bits");
+
 __global__ __launch_bounds__(1024) void k_aggregate_top(
     const SegInfo* __restrict__ segs, const float4* __restrict__ posm, float4* node,
     float4* nodeQ, float4* boxLo, float4* boxHi, const int* __restrict__ parent, int* flag,
-    int* front, const int* __restrict__ nfront, float inv_theta, int mode)
+    int* front, const int* __restrict__ nfront, const int* __restrict__ tlist,
+    const int* __restrict__ ntop, const int* __restrict__ tslot, float inv_theta, int mode)
 {
-    __shared__ uint32_t lmem[kTopLdsNodes / 32];
+    __shared__ __align__(16) uint32_t smem[kTopSmemWords];
     __shared__ int lcount;
+    uint32_t* const lmem = smem;
     const int s = blockIdx.x;
+    const SegInfo sg = segs[s];
+    const int nt = tlist ? ntop[s] : kTopCap + 1;
+    if (nt <= kTopCap) {
+        if (nt > 0)
+            top_lds(*reinterpret_cast<TopLds*>(smem), sg, nt, tlist, tslot, posm, node, nodeQ,
+                    boxLo, boxHi, parent, inv_theta, mode);
+        return;
+    }
     const int nf = nfront[s];
     if (nf == 0) return;
-    const SegInfo sg = segs[s];
     // nf - 1 nodes above the aggregation subtrees get a mark.
     const int marks = sg.n <= kTopLdsNodes ? kMarksBits
                       : nf <= kTopHashMax  ? kMarksHash
