@@ -443,8 +443,8 @@ int enqueue_step(gcs_engine* e, const gcs_params& p, bool integrate)
         1, std::min<int64_t>((int64_t)e->num_cu * (2048 / kATile), e->n_total));
     k_aggregate<<<ablocks, kATile, 0, e->stream>>>(e->atiles, e->natiles, e->segs, posm, e->node,
                                                    e->nodeQ, e->boxLo, e->boxHi, e->parent,
-                                                   e->front, e->nfront, inv_theta,
-                                                   amode);
+                                                   e->front, e->nfront, e->qctr + kNumXcd,
+                                                   inv_theta, amode);
     k_aggregate_top<<<nseg, 1024, 0, e->stream>>>(e->segs, posm, e->node, e->nodeQ, e->boxLo,
                                                   e->boxHi, e->parent, e->flag, e->front,
                                                   e->nfront, e->tlist, e->ntop, e->tslot,
@@ -685,7 +685,7 @@ int gcs_create(int64_t cap_n, int32_t cap_k, int32_t device, gcs_engine** out)
     if (!rc) rc = dalloc(&e->chist, (size_t)kSortPasses * kRadix * max_chunks(cap_n, cap_k));
     if (!rc) rc = dalloc(&e->seg_bounds, 2 * (size_t)cap_k);
     if (!rc) rc = dalloc(&e->bpart, 8 * nt);
-    if (!rc) rc = dalloc(&e->qctr, kNumXcd);
+    if (!rc) rc = dalloc(&e->qctr, kNumXcd + 1);  // traversal queues; aggregation items
     if (!rc) rc = dalloc(&e->counters, 2);
     if (!rc) rc = dalloc(&e->err, 1);
     if (!rc) rc = dalloc(&e->diag_part, 5 * 4 * nt);

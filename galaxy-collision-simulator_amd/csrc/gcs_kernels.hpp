@@ -137,7 +137,7 @@ __global__ __launch_bounds__(256) void k_morton(const TileInfo* __restrict__ til
                 seg_bounds[2 * tid + 1] = fenc(gmax);
             }
         }
-        if (tid < kNumXcd) rs.qctr[tid] = 0u;
+        if (tid <= kNumXcd) rs.qctr[tid] = 0u;
         if (tid < 2) rs.counters[tid] = 0ull;
         for (int64_t k = tid; k < rs.chist_words; k += stride) rs.chist[k] = 0u;
         if (threadIdx.x == 0) {
@@ -596,11 +596,15 @@ __device__ __forceinline__ void store_node(int v, const NodeData& d, float w, fl
 // node is merged once from its two children, by the second of the two
 // climbs that reach it (the first stops there).
 //
-// k_aggregate: one aggregation subtree (a maximal subtree of at most kATile
-// particles) per block at a time.  Its c - 1 internal nodes have indices
-// in t0 .. t0 + c - 1 (Karras numbering: a node's index is one end of its
-// range), one of those c indices belonging to a node outside the subtree;
-// slot = index - t0.  The whole climb runs in LDS:
+// k_aggregate: aggregation subtrees (maximal subtrees of at most kATile
+// particles) are taken in work items from a counter: item 2j is subtrees 2j
+// and 2j+1 together when they are of one instance and hold at most kATile
+// particles between them (item 2j+1 is then empty), else subtree 2j; item
+// 2j+1 is subtree 2j+1.  Consecutive subtrees cover consecutive particles,
+// so an item covers particles t0 .. t0 + c - 1 and its internal nodes have
+// indices in that range (Karras numbering: a node's index is one end of its
+// range); an index there is a node of the item iff that node's range lies
+// inside one of its subtrees.  slot = index - t0.  The climb runs in LDS:
 //   1. thread k reads the node of slot k and, if its range lies inside the
 //      subtree's, loads its split, leaf flags and parent slot into an LDS
 //      word, and the particle data of its leaf children: a node with two
@@ -611,7 +615,7 @@ __device__ __forceinline__ void store_node(int v, const NodeData& d, float w, fl
 //      (workgroup-scope acquire-release) lets the second climb through;
 //   3. every node record is written to global memory once, at the end.
 // No global memory access sits between two levels of a climb.  The thread
-// that completes the subtree's root appends it to its instance's frontier
+// that completes a subtree's root appends it to its instance's frontier
 // for the instance pass.
 __device__ __forceinline__ NodeData lds_node(const float4 (*ldat)[4], const float* lqt, int s)
 {
@@ -648,28 +652,46 @@ __global__ __launch_bounds__(kATile) __attribute__((amdgpu_waves_per_eu(8))) voi
     const ATileInfo* __restrict__ atiles, const int* __restrict__ natiles_dev,
     const SegInfo* __restrict__ segs, const float4* __restrict__ posm, float4* node,
     float4* nodeQ, float4* boxLo, float4* boxHi, const int* __restrict__ parent,
-    int* __restrict__ front, int* __restrict__ nfront, float inv_theta, int mode)
+    int* __restrict__ front, int* __restrict__ nfront, unsigned* __restrict__ ictr,
+    float inv_theta, int mode)
 {
     __shared__ uint32_t lflag[kATile / 32];
     __shared__ uint32_t ltopo[kATile];
     __shared__ float4 ldat[kATile][4];
     __shared__ float lqt[kATile];
+    __shared__ int litem;
     const int tid = threadIdx.x;
     const int natiles = *natiles_dev;
-    for (int a = blockIdx.x; a < natiles; a += gridDim.x) {
-        const ATileInfo t = atiles[a];
+    const int nitems = 2 * ((natiles + 1) / 2);
+    for (;;) {
+        __syncthreads();  // the previous item's LDS use is over
+        if (tid == 0) litem = (int)atomicAdd(ictr, 1u);
+        __syncthreads();
+        const int item = litem;
+        if (item >= nitems) break;
+        const int j0 = item & ~1;
+        const ATileInfo a0 = atiles[j0];
+        const bool has1 = j0 + 1 < natiles;
+        const ATileInfo a1 = has1 ? atiles[j0 + 1] : a0;
+        const bool pair = has1 && a1.seg == a0.seg && a0.count + a1.count <= kATile;
+        if ((item & 1) && (pair || !has1)) continue;  // an empty item
+        const ATileInfo t = (item & 1) ? a1 : a0;
+        const bool two = pair && !(item & 1);
         const SegInfo sg = segs[t.seg];
-        const int t0 = t.first, c = t.count;
-        __syncthreads();  // the previous subtree's LDS use is over
-        if (c == 1) {
-            if (tid == 0) front[2 * sg.start + atomicAdd(&nfront[t.seg], 1)] = t0 + sg.leaf_base;
-            continue;
+        const int t0 = t.first, e0 = t0 + t.count - 1;  // the first subtree's range
+        const int c = two ? t.count + a1.count : t.count;
+        if (tid == 0) {
+            // A subtree of one particle: its root is the leaf.
+            if (t.count == 1) front[2 * sg.start + atomicAdd(&nfront[t.seg], 1)] = t0 + sg.leaf_base;
+            if (two && a1.count == 1)
+                front[2 * sg.start + atomicAdd(&nfront[t.seg], 1)] = a1.first + sg.leaf_base;
         }
         if (tid < kATile / 32) lflag[tid] = 0u;
         const int ibase = t0 + sg.int_base;  // global index of slot 0
         bool climb = false;
-        // Slot tid holds a node of the subtree iff the index exists in the
-        // instance (internal nodes 0 .. n-2) and its range lies inside.
+        // Slot tid holds a node of the item iff the index exists in the
+        // instance (internal nodes 0 .. n-2) and its range lies inside one
+        // of the item's subtrees.
         const int v = ibase + tid;
         bool member = false;
         int first = 0, last = 0, split = 0, pv = -1;
@@ -678,8 +700,9 @@ __global__ __launch_bounds__(kATile) __attribute__((amdgpu_waves_per_eu(8))) voi
             first = __float_as_int(bb.y);
             last = __float_as_int(bb.z);
             split = __float_as_int(bb.w);
-            member = first >= t0 && last <= t0 + c - 1;
-            if (member && v != t.root) pv = parent[v];
+            member = (first >= t0 && last <= e0) || (two && first > e0 && last <= t0 + c - 1);
+            const bool root = (t.count > 1 && v == t.root) || (two && a1.count > 1 && v == a1.root);
+            if (member && !root) pv = parent[v];
         }
         if (member) {
             const bool lleaf = first == split, rleaf = last == split + 1;
