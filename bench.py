@@ -306,8 +306,8 @@ def main():
                     help="resident-particle admission target of the served form")
     a = ap.parse_args()
     if a.gpus != 1:
-        print("bench.py runs on one GPU: this build has no multi-GPU path "
-              "(independent collisions shard across GPUs without exchange; see DESIGN.md)",
+        print("bench.py runs on one GPU: the library has no multi-GPU path "
+              "(independent collisions shard across GPUs without exchange)",
               file=sys.stderr)
         sys.exit(2)
 
