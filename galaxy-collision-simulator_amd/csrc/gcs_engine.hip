@@ -21,6 +21,7 @@
 #include "gcs_kernels.hpp"
 #include "gcs_sort.hpp"
 #include "gcs_traverse.hpp"
+#include "gcs_gen_device.hpp"
 
 using namespace gcs;
 
@@ -149,6 +150,7 @@ struct gcs_engine {
     double* tsim = nullptr;      // per instance: simulated time
     int* slot_mem = nullptr;  // per traversal wave: stack spill, node and particle lists
     uint8_t* staging = nullptr;
+    double* mn_dev = nullptr;  // disc sampler table on the device (generated loads)
 
     // host mirror of the composition
     std::vector<Instance> inst;
@@ -209,7 +211,7 @@ void free_all(gcs_engine* e)
     dfree(e->front); dfree(e->nfront);
     dfree(e->tlist); dfree(e->ntop); dfree(e->tslot);
     dfree(e->seg_bounds); dfree(e->bpart); dfree(e->qctr); dfree(e->counters); dfree(e->err);
-    dfree(e->diag_part); dfree(e->diag_out); dfree(e->amax_part); dfree(e->dt_inst); dfree(e->tsim); dfree(e->slot_mem); dfree(e->staging);
+    dfree(e->diag_part); dfree(e->diag_out); dfree(e->amax_part); dfree(e->dt_inst); dfree(e->tsim); dfree(e->slot_mem); dfree(e->staging); dfree(e->mn_dev);
     for (auto& v : e->ev)
         if (v) (void)hipEventDestroy(v), v = nullptr;
 }
@@ -753,6 +755,90 @@ int gcs_set_stream(gcs_engine* e, void* s)
         e->own_stream = true;
     }
     e->graph_valid = false;
+    return GCS_OK;
+}
+
+int gcs_load_generated(gcs_engine* e, const gcs_collision_spec* specs, int32_t k)
+{
+    if (!e || (!specs && k > 0)) return fail(GCS_ERR_INVALID, "null argument");
+    if (k < 0 || k > e->cap_k) return fail(GCS_ERR_CAPACITY, "instance count exceeds capacity");
+    int64_t total = 0;
+    for (int32_t s = 0; s < k; ++s) {
+        if (specs[s].n_a < 0 || specs[s].n_b < 0 || (int64_t)specs[s].n_a + specs[s].n_b < 1)
+            return fail(GCS_ERR_INVALID, "every collision needs at least 1 particle");
+        total += (int64_t)specs[s].n_a + specs[s].n_b;
+    }
+    if (total > e->cap_n) return fail(GCS_ERR_CAPACITY, "particles exceed capacity");
+    HIPCHK(hipSetDevice(e->device));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    // The disc sampler table, uploaded once per engine.
+    constexpr int kMnWords = (gcs_gen::kNR + 1) + (gcs_gen::kNZ + 1) + 2 * gcs_gen::kNR * gcs_gen::kNZ;
+    const gcs_gen::MnTableView& ht = gcs_gen::host_mn_table();
+    if (!e->mn_dev) {
+        int rc = dalloc(&e->mn_dev, (size_t)kMnWords);
+        if (rc) return rc;
+        std::vector<double> w;
+        w.reserve(kMnWords);
+        w.insert(w.end(), ht.re, ht.re + gcs_gen::kNR + 1);
+        w.insert(w.end(), ht.ze, ht.ze + gcs_gen::kNZ + 1);
+        w.insert(w.end(), ht.bound, ht.bound + gcs_gen::kNR * gcs_gen::kNZ);
+        w.insert(w.end(), ht.cdf, ht.cdf + gcs_gen::kNR * gcs_gen::kNZ);
+        HIPCHK(hipMemcpy(e->mn_dev, w.data(), sizeof(double) * w.size(), hipMemcpyHostToDevice));
+    }
+    gcs_gen::MnTableView mt;
+    mt.re = e->mn_dev;
+    mt.ze = mt.re + gcs_gen::kNR + 1;
+    mt.bound = mt.ze + gcs_gen::kNZ + 1;
+    mt.cdf = mt.bound + gcs_gen::kNR * gcs_gen::kNZ;
+    e->inst.clear();
+    for (int32_t s = 0; s < k; ++s) e->inst.push_back({s, specs[s].n_a + specs[s].n_b});
+    e->next_id = k;
+    e->cur = 0;
+    int rc = rebuild_tables(e);
+    if (rc || k == 0) return rc;
+    HIPCHK(hipMemset(e->tsim, 0, sizeof(double) * (size_t)k));
+    std::vector<GenInst> hi((size_t)k);
+    for (int32_t s = 0; s < k; ++s) {
+        GenInst& g = hi[(size_t)s];
+        g.seed = specs[s].seed;
+        g.n_a = specs[s].n_a;
+        g.n_b = specs[s].n_b;
+        for (int d = 0; d < 3; ++d) {
+            g.off[d] = specs[s].offset[d];
+            g.vel[d] = specs[s].velocity[d];
+        }
+        const double t = specs[s].tilt_deg * gcs_gen::kPi / 180.0;
+        g.ct = std::cos(t);
+        g.st = std::sin(t);
+    }
+    GenInst* di = nullptr;
+    rc = dalloc(&di, (size_t)k);
+    if (rc) return rc;
+    const int ntiles = (int)e->h_tiles.size(), nseg = (int)e->h_segs.size();
+    const int qblocks = 4 * ntiles;
+    double4* xyz = reinterpret_cast<double4*>(e->staging);  // 32 B per particle
+    const int c = e->cur;
+    hipError_t le = hipMemcpyAsync(di, hi.data(), sizeof(GenInst) * hi.size(),
+                                   hipMemcpyHostToDevice, e->stream);
+    if (le == hipSuccess)
+        le = hipMemsetAsync(e->chist, 0, sizeof(uint32_t) * kSortPasses * kRadix * e->nchunks,
+                            e->stream);
+    if (le == hipSuccess) {
+        k_gen_pos<<<qblocks, kTileThreads, 0, e->stream>>>(e->tiles, e->segs, di, mt, xyz,
+                                                            e->keys[0], e->vals[0]);
+        enqueue_sort(e);
+        k_gen_count<<<ntiles, kTileThreads, 0, e->stream>>>(e->tiles, e->segs, di, e->vals[0],
+                                                             e->hist);
+        k_gen_scan<<<(nseg + 255) / 256, 256, 0, e->stream>>>(e->segs, nseg, e->hist);
+        k_gen_menc<<<ntiles, kTileThreads, 0, e->stream>>>(e->tiles, e->segs, di, e->vals[0],
+                                                            e->hist, xyz);
+        k_gen_state<<<qblocks, kTileThreads, 0, e->stream>>>(e->tiles, e->segs, di, xyz,
+                                                              e->posm[c], e->vel[c], e->plo[c]);
+        le = hipGetLastError();
+    }
+    if (le == hipSuccess) le = hipStreamSynchronize(e->stream);
+    dfree(di);
+    if (le != hipSuccess) return fail(GCS_ERR_HIP, std::string("generation: ") + hipGetErrorString(le));
     return GCS_OK;
 }
 

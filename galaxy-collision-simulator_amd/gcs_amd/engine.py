@@ -79,6 +79,12 @@ class Engine:
         c = np.ascontiguousarray(counts, dtype=np.int32)
         L.check(L.lib().gcs_load(self._h, L.ptr(stars), L.ptr(c), len(c)))
 
+    def load_generated(self, specs):
+        """Generates the collisions of `specs` (CollisionSpec list) on the
+        device, straight into the engine (gcs_load_generated)."""
+        arr = (L.CollisionSpec * len(specs))(*specs)
+        L.check(L.lib().gcs_load_generated(self._h, arr, len(specs)))
+
     def add(self, stars):
         stars = np.ascontiguousarray(stars, dtype=L.STAR)
         i = ctypes.c_int32()

@@ -256,6 +256,13 @@ void gcs_spec_random(uint64_t seed, int64_t k, int32_t n_min, int32_t n_max,
                      gcs_collision_spec* s);
 /* Writes n_a + n_b records (galaxy A then B; each halo, disc, bulge). */
 int gcs_generate(const gcs_collision_spec* s, gcs_star* out);
+/* Generates k collisions on the device straight into the engine's state, as
+ * gcs_load would load gcs_generate's records (instance ids 0..k-1, instance s
+ * = specs[s] with n_a + n_b particles).  The records equal gcs_generate's
+ * except where the device's log, cos, sin or pow rounds differently in the
+ * last bit (about one float value in 10^8).  No counterpart in the reference
+ * (its generator is host code, generator.cpp:3-196). */
+int gcs_load_generated(gcs_engine* e, const gcs_collision_spec* specs, int32_t k);
 
 /* ---- Served form: continuous batching -----------------------------------
  * A server owns one engine.  Jobs (a randomised collision of n particles
