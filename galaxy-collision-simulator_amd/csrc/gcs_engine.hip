@@ -347,21 +347,106 @@ Marks atile_marks(gcs_engine* e) { return Marks{e->amark, e->aq, e->acnt, e->aro
 // state[cur ^ 1] and flips cur.
 // The eight sort passes over keys[0]/vals[0] (sorted result back in
 // buffer 0).  The chunk histograms of all passes must be zero on entry.
-void enqueue_sort(gcs_engine* e)
+// Sorts the instances whose tiles are tile0 .. tile0 + ntiles - 1 (whole
+// instances); the other instances' keys are left alone.
+void enqueue_sort_range(gcs_engine* e, int tile0, int ntiles)
 {
-    const int ntiles = (int)e->h_tiles.size();
     const size_t cstride = (size_t)e->nchunks * kRadix;
+    const TileInfo* tiles = e->tiles + tile0;
     for (int pass = 0; pass < kSortPasses; ++pass) {
         const int shift = pass * kRadixBits;
         const int src = pass & 1, dst = src ^ 1;
         uint32_t* ch = e->chist + (size_t)pass * cstride;
-        k_sort_count<<<ntiles, kTileThreads, 0, e->stream>>>(e->tiles, e->segs, e->keys[src],
+        k_sort_count<<<ntiles, kTileThreads, 0, e->stream>>>(tiles, e->segs, e->keys[src],
                                                               e->hist, ch, shift);
         k_sort_scatter<<<ntiles, kTileThreads, 0, e->stream>>>(
-            e->tiles, e->segs, e->keys[src], e->vals[src], e->keys[dst], e->vals[dst], e->hist,
+            tiles, e->segs, e->keys[src], e->vals[src], e->keys[dst], e->vals[dst], e->hist,
             ch, shift);
     }
     static_assert(kSortPasses % 2 == 0, "sorted keys end in buffer 0");
+}
+
+void enqueue_sort(gcs_engine* e) { enqueue_sort_range(e, 0, (int)e->h_tiles.size()); }
+
+// The disc sampler table on the device (uploaded on first use).
+int mn_table_dev(gcs_engine* e, gcs_gen::MnTableView* mt)
+{
+    constexpr int kMnWords =
+        (gcs_gen::kNR + 1) + (gcs_gen::kNZ + 1) + 2 * gcs_gen::kNR * gcs_gen::kNZ;
+    if (!e->mn_dev) {
+        const gcs_gen::MnTableView& ht = gcs_gen::host_mn_table();
+        int rc = dalloc(&e->mn_dev, (size_t)kMnWords);
+        if (rc) return rc;
+        std::vector<double> w;
+        w.reserve(kMnWords);
+        w.insert(w.end(), ht.re, ht.re + gcs_gen::kNR + 1);
+        w.insert(w.end(), ht.ze, ht.ze + gcs_gen::kNZ + 1);
+        w.insert(w.end(), ht.bound, ht.bound + gcs_gen::kNR * gcs_gen::kNZ);
+        w.insert(w.end(), ht.cdf, ht.cdf + gcs_gen::kNR * gcs_gen::kNZ);
+        HIPCHK(hipMemcpy(e->mn_dev, w.data(), sizeof(double) * w.size(), hipMemcpyHostToDevice));
+    }
+    mt->re = e->mn_dev;
+    mt->ze = mt->re + gcs_gen::kNR + 1;
+    mt->bound = mt->ze + gcs_gen::kNZ + 1;
+    mt->cdf = mt->bound + gcs_gen::kNR * gcs_gen::kNZ;
+    return GCS_OK;
+}
+
+GenInst gen_inst(const gcs_collision_spec& sp)
+{
+    GenInst g;
+    g.seed = sp.seed;
+    g.n_a = sp.n_a;
+    g.n_b = sp.n_b;
+    for (int d = 0; d < 3; ++d) {
+        g.off[d] = sp.offset[d];
+        g.vel[d] = sp.velocity[d];
+    }
+    const double t = sp.tilt_deg * gcs_gen::kPi / 180.0;
+    g.ct = std::cos(t);
+    g.st = std::sin(t);
+    return g;
+}
+
+// Generates instances seg0 .. seg0 + k - 1 (the tables already list them)
+// into the current state, from their collisions insts[0..k).
+int generate_into(gcs_engine* e, const GenInst* insts, int seg0, int k)
+{
+    gcs_gen::MnTableView mt;
+    int rc = mn_table_dev(e, &mt);
+    if (rc) return rc;
+    GenInst* di = nullptr;
+    rc = dalloc(&di, (size_t)k);
+    if (rc) return rc;
+    const int tile0 = e->h_segs[(size_t)seg0].tile0;
+    const int ntiles = e->h_segs[(size_t)(seg0 + k - 1)].tile0 +
+                       e->h_segs[(size_t)(seg0 + k - 1)].ntiles - tile0;
+    const TileInfo* tiles = e->tiles + tile0;
+    double4* xyz = reinterpret_cast<double4*>(e->staging);  // 32 B per particle
+    const int c = e->cur;
+    hipError_t le = hipMemcpyAsync(di, insts, sizeof(GenInst) * (size_t)k, hipMemcpyHostToDevice,
+                                   e->stream);
+    if (le == hipSuccess)
+        le = hipMemsetAsync(e->chist, 0, sizeof(uint32_t) * kSortPasses * kRadix * e->nchunks,
+                            e->stream);
+    if (le == hipSuccess) {
+        k_gen_pos<<<4 * ntiles, kTileThreads, 0, e->stream>>>(tiles, e->segs, di, seg0, mt, xyz,
+                                                               e->keys[0], e->vals[0]);
+        enqueue_sort_range(e, tile0, ntiles);
+        k_gen_count<<<ntiles, kTileThreads, 0, e->stream>>>(tiles, e->segs, di, seg0, e->vals[0],
+                                                             e->hist);
+        k_gen_scan<<<(k + 255) / 256, 256, 0, e->stream>>>(e->segs, seg0, k, tile0, e->hist);
+        k_gen_menc<<<ntiles, kTileThreads, 0, e->stream>>>(tiles, e->segs, di, seg0, e->vals[0],
+                                                            e->hist, xyz);
+        k_gen_state<<<4 * ntiles, kTileThreads, 0, e->stream>>>(tiles, e->segs, di, seg0, xyz,
+                                                                 e->posm[c], e->vel[c],
+                                                                 e->plo[c]);
+        le = hipGetLastError();
+    }
+    if (le == hipSuccess) le = hipStreamSynchronize(e->stream);
+    dfree(di);
+    if (le != hipSuccess) return fail(GCS_ERR_HIP, std::string("generation: ") + hipGetErrorString(le));
+    return GCS_OK;
 }
 
 int enqueue_step(gcs_engine* e, const gcs_params& p, bool integrate)
@@ -771,25 +856,6 @@ int gcs_load_generated(gcs_engine* e, const gcs_collision_spec* specs, int32_t k
     if (total > e->cap_n) return fail(GCS_ERR_CAPACITY, "particles exceed capacity");
     HIPCHK(hipSetDevice(e->device));
     HIPCHK(hipStreamSynchronize(e->stream));
-    // The disc sampler table, uploaded once per engine.
-    constexpr int kMnWords = (gcs_gen::kNR + 1) + (gcs_gen::kNZ + 1) + 2 * gcs_gen::kNR * gcs_gen::kNZ;
-    const gcs_gen::MnTableView& ht = gcs_gen::host_mn_table();
-    if (!e->mn_dev) {
-        int rc = dalloc(&e->mn_dev, (size_t)kMnWords);
-        if (rc) return rc;
-        std::vector<double> w;
-        w.reserve(kMnWords);
-        w.insert(w.end(), ht.re, ht.re + gcs_gen::kNR + 1);
-        w.insert(w.end(), ht.ze, ht.ze + gcs_gen::kNZ + 1);
-        w.insert(w.end(), ht.bound, ht.bound + gcs_gen::kNR * gcs_gen::kNZ);
-        w.insert(w.end(), ht.cdf, ht.cdf + gcs_gen::kNR * gcs_gen::kNZ);
-        HIPCHK(hipMemcpy(e->mn_dev, w.data(), sizeof(double) * w.size(), hipMemcpyHostToDevice));
-    }
-    gcs_gen::MnTableView mt;
-    mt.re = e->mn_dev;
-    mt.ze = mt.re + gcs_gen::kNR + 1;
-    mt.bound = mt.ze + gcs_gen::kNZ + 1;
-    mt.cdf = mt.bound + gcs_gen::kNR * gcs_gen::kNZ;
     e->inst.clear();
     for (int32_t s = 0; s < k; ++s) e->inst.push_back({s, specs[s].n_a + specs[s].n_b});
     e->next_id = k;
@@ -798,47 +864,36 @@ int gcs_load_generated(gcs_engine* e, const gcs_collision_spec* specs, int32_t k
     if (rc || k == 0) return rc;
     HIPCHK(hipMemset(e->tsim, 0, sizeof(double) * (size_t)k));
     std::vector<GenInst> hi((size_t)k);
-    for (int32_t s = 0; s < k; ++s) {
-        GenInst& g = hi[(size_t)s];
-        g.seed = specs[s].seed;
-        g.n_a = specs[s].n_a;
-        g.n_b = specs[s].n_b;
-        for (int d = 0; d < 3; ++d) {
-            g.off[d] = specs[s].offset[d];
-            g.vel[d] = specs[s].velocity[d];
-        }
-        const double t = specs[s].tilt_deg * gcs_gen::kPi / 180.0;
-        g.ct = std::cos(t);
-        g.st = std::sin(t);
+    for (int32_t s = 0; s < k; ++s) hi[(size_t)s] = gen_inst(specs[s]);
+    return generate_into(e, hi.data(), 0, k);
+}
+
+int gcs_instance_add_generated(gcs_engine* e, const gcs_collision_spec* spec, int32_t* id)
+{
+    if (!e || !spec) return fail(GCS_ERR_INVALID, "bad argument");
+    const int64_t n = (int64_t)spec->n_a + spec->n_b;
+    if (spec->n_a < 0 || spec->n_b < 0 || n < 1)
+        return fail(GCS_ERR_INVALID, "a collision needs at least 1 particle");
+    if ((int32_t)e->inst.size() + 1 > e->cap_k) return fail(GCS_ERR_CAPACITY, "instance slots full");
+    if (e->n_total + n > e->cap_n) return fail(GCS_ERR_CAPACITY, "particle capacity full");
+    HIPCHK(hipSetDevice(e->device));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    const int slot = (int)e->inst.size();
+    HIPCHK(hipMemset(e->tsim + slot, 0, sizeof(double)));
+    const int32_t nid = e->next_id++;
+    e->inst.push_back({nid, (int32_t)n});
+    int rc = rebuild_tables(e);
+    if (!rc) {
+        const GenInst g = gen_inst(*spec);
+        rc = generate_into(e, &g, slot, 1);
     }
-    GenInst* di = nullptr;
-    rc = dalloc(&di, (size_t)k);
-    if (rc) return rc;
-    const int ntiles = (int)e->h_tiles.size(), nseg = (int)e->h_segs.size();
-    const int qblocks = 4 * ntiles;
-    double4* xyz = reinterpret_cast<double4*>(e->staging);  // 32 B per particle
-    const int c = e->cur;
-    hipError_t le = hipMemcpyAsync(di, hi.data(), sizeof(GenInst) * hi.size(),
-                                   hipMemcpyHostToDevice, e->stream);
-    if (le == hipSuccess)
-        le = hipMemsetAsync(e->chist, 0, sizeof(uint32_t) * kSortPasses * kRadix * e->nchunks,
-                            e->stream);
-    if (le == hipSuccess) {
-        k_gen_pos<<<qblocks, kTileThreads, 0, e->stream>>>(e->tiles, e->segs, di, mt, xyz,
-                                                            e->keys[0], e->vals[0]);
-        enqueue_sort(e);
-        k_gen_count<<<ntiles, kTileThreads, 0, e->stream>>>(e->tiles, e->segs, di, e->vals[0],
-                                                             e->hist);
-        k_gen_scan<<<(nseg + 255) / 256, 256, 0, e->stream>>>(e->segs, nseg, e->hist);
-        k_gen_menc<<<ntiles, kTileThreads, 0, e->stream>>>(e->tiles, e->segs, di, e->vals[0],
-                                                            e->hist, xyz);
-        k_gen_state<<<qblocks, kTileThreads, 0, e->stream>>>(e->tiles, e->segs, di, xyz,
-                                                              e->posm[c], e->vel[c], e->plo[c]);
-        le = hipGetLastError();
+    if (rc) {
+        // Leave the engine as it was.
+        e->inst.pop_back();
+        (void)rebuild_tables(e);
+        return rc;
     }
-    if (le == hipSuccess) le = hipStreamSynchronize(e->stream);
-    dfree(di);
-    if (le != hipSuccess) return fail(GCS_ERR_HIP, std::string("generation: ") + hipGetErrorString(le));
+    if (id) *id = nid;
     return GCS_OK;
 }
 

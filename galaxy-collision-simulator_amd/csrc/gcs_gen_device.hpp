@@ -46,9 +46,11 @@ __device__ __forceinline__ GenWho gen_who(const GenInst& I, int local)
     return w;
 }
 
+// Every kernel takes the tiles of the instances generated (from the first
+// tile of the first of them) and their collisions as insts[slot - seg_base].
 __global__ __launch_bounds__(256) void k_gen_pos(const TileInfo* __restrict__ tiles,
                                                  const SegInfo* __restrict__ segs,
-                                                 const GenInst* __restrict__ insts,
+                                                 const GenInst* __restrict__ insts, int seg_base,
                                                  gcs_gen::MnTableView mt,
                                                  double4* __restrict__ xyz,
                                                  uint64_t* __restrict__ keys,
@@ -56,7 +58,7 @@ __global__ __launch_bounds__(256) void k_gen_pos(const TileInfo* __restrict__ ti
 {
     const QuarterTile q = quarter_tile(tiles, segs);
     if (!q.valid) return;
-    const GenInst I = insts[q.seg];
+    const GenInst I = insts[q.seg - seg_base];
     const GenWho w = gen_who(I, q.local);
     const gcs_gen::GenParticle p = gcs_gen::gen_position(I.seed, w.g, w.i, w.gs, mt);
     const double r = gcs_gen::gen_radius(p);
@@ -70,6 +72,7 @@ __global__ __launch_bounds__(256) void k_gen_pos(const TileInfo* __restrict__ ti
 __global__ __launch_bounds__(kTileThreads) void k_gen_count(const TileInfo* __restrict__ tiles,
                                                             const SegInfo* __restrict__ segs,
                                                             const GenInst* __restrict__ insts,
+                                                            int seg_base,
                                                             const uint32_t* __restrict__ vals,
                                                             uint32_t* __restrict__ cnt)
 {
@@ -77,7 +80,7 @@ __global__ __launch_bounds__(kTileThreads) void k_gen_count(const TileInfo* __re
     const TileInfo t = tiles[blockIdx.x];
     if (threadIdx.x < 6) c[threadIdx.x] = 0u;
     __syncthreads();
-    const GenInst I = insts[t.seg];
+    const GenInst I = insts[t.seg - seg_base];
     const int start = segs[t.seg].start;
     for (int k = threadIdx.x; k < t.count; k += kTileThreads) {
         const GenWho w = gen_who(I, (int)vals[t.start + k] - start);
@@ -87,16 +90,18 @@ __global__ __launch_bounds__(kTileThreads) void k_gen_count(const TileInfo* __re
     if (threadIdx.x < 6) cnt[(size_t)blockIdx.x * kRadix + threadIdx.x] = c[threadIdx.x];
 }
 
-// Per instance (one thread each): exclusive prefix of the tile counts over
-// its tiles, written to entries 8..13 of each tile's row.
-__global__ void k_gen_scan(const SegInfo* __restrict__ segs, int nseg, uint32_t* __restrict__ cnt)
+// Per instance (one thread each, instances seg_base .. seg_base + nseg - 1):
+// exclusive prefix of the tile counts over its tiles, written to entries
+// 8..13 of each tile's row (rows from the first generated tile, tile_base).
+__global__ void k_gen_scan(const SegInfo* __restrict__ segs, int seg_base, int nseg,
+                           int tile_base, uint32_t* __restrict__ cnt)
 {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= nseg) return;
-    const SegInfo sg = segs[s];
+    const SegInfo sg = segs[seg_base + s];
     uint32_t run[6] = {0u, 0u, 0u, 0u, 0u, 0u};
     for (int t = sg.tile0; t < sg.tile0 + sg.ntiles; ++t) {
-        uint32_t* row = cnt + (size_t)t * kRadix;
+        uint32_t* row = cnt + (size_t)(t - tile_base) * kRadix;
         for (int k = 0; k < 6; ++k) {
             const uint32_t v = row[k];
             row[8 + k] = run[k];
@@ -110,13 +115,14 @@ __global__ void k_gen_scan(const SegInfo* __restrict__ segs, int nseg, uint32_t*
 __global__ __launch_bounds__(kTileThreads) void k_gen_menc(const TileInfo* __restrict__ tiles,
                                                            const SegInfo* __restrict__ segs,
                                                            const GenInst* __restrict__ insts,
+                                                           int seg_base,
                                                            const uint32_t* __restrict__ vals,
                                                            const uint32_t* __restrict__ cnt,
                                                            double4* __restrict__ xyz)
 {
     __shared__ uint32_t ws[kTileThreads / kWave];
     const TileInfo t = tiles[blockIdx.x];
-    const GenInst I = insts[t.seg];
+    const GenInst I = insts[t.seg - seg_base];
     const int start = segs[t.seg].start;
     // Thread k holds sorted items 4k .. 4k+3 of the tile.
     GenWho who[kTileItems];
@@ -152,7 +158,7 @@ __global__ __launch_bounds__(kTileThreads) void k_gen_menc(const TileInfo* __res
 // generation order, galaxy A then B, each halo, disc, bulge).
 __global__ __launch_bounds__(256) void k_gen_state(const TileInfo* __restrict__ tiles,
                                                    const SegInfo* __restrict__ segs,
-                                                   const GenInst* __restrict__ insts,
+                                                   const GenInst* __restrict__ insts, int seg_base,
                                                    const double4* __restrict__ xyz,
                                                    float4* __restrict__ posm,
                                                    float4* __restrict__ vel,
@@ -160,7 +166,7 @@ __global__ __launch_bounds__(256) void k_gen_state(const TileInfo* __restrict__ 
 {
     const QuarterTile q = quarter_tile(tiles, segs);
     if (!q.valid) return;
-    const GenInst I = insts[q.seg];
+    const GenInst I = insts[q.seg - seg_base];
     const GenWho w = gen_who(I, q.local);
     const double4 d = xyz[q.i];
     gcs_gen::GenParticle p;

@@ -2,8 +2,10 @@
 // jobs on one engine (host C++ over the gcs_* C ABI).
 //
 // Jobs arrive on a schedule; worker threads generate each job's initial
-// conditions when it arrives; the stepping loop admits generated jobs in
-// arrival order while their particles fit the HBM budget (and the job cap),
+// conditions when it arrives (or, with no worker threads, the engine
+// generates them on the device when the job is admitted); the stepping loop
+// admits generated jobs in arrival order while their particles fit the HBM
+// budget (and the job cap),
 // advances every resident job together -- one launch sequence per step over
 // the whole forest -- in rounds that end when the next job finishes or after
 // steps_per_round steps, and retires finished jobs.  Each job's result does
@@ -65,6 +67,7 @@ struct gcs_server {
     // generation (a bounded look-ahead past the admission head keeps host
     // memory bounded under overload); jobs[0, next_admit) are admitted.
     size_t next_release = 0, next_gen = 0, next_admit = 0, lookahead = 32;
+    bool device_gen = false;  // no generator threads: jobs are generated on the device
     std::vector<gcs_job_result> results;
     std::map<int64_t, std::vector<gcs_star>> finals;
     double particle_steps = 0.0, run_s = 0.0, resident_sum = 0.0, gpu_s = 0.0;
@@ -124,7 +127,14 @@ int admit(gcs_server* s, double t)
             break;
         }
         int32_t iid = -1;
-        int rc = gcs_instance_add(s->eng, j->stars.data(), j->spec.n, &iid);
+        int rc;
+        if (s->device_gen) {
+            gcs_collision_spec spec;
+            gcs_spec_random(j->spec.seed, j->spec.id, j->spec.n, j->spec.n, &spec);
+            rc = gcs_instance_add_generated(s->eng, &spec, &iid);
+        } else {
+            rc = gcs_instance_add(s->eng, j->stars.data(), j->spec.n, &iid);
+        }
         if (rc != GCS_OK) return rc;
         std::vector<gcs_star>().swap(j->stars);
         j->iid = iid;
@@ -174,7 +184,8 @@ int gcs_server_create(const gcs_server_config* c, gcs_server** out)
         delete s;
         return rc;
     }
-    const int nt = std::max(1, std::min(32, c->gen_threads));
+    s->device_gen = c->gen_threads <= 0;
+    const int nt = s->device_gen ? 0 : std::min(32, c->gen_threads);
     s->lookahead = (size_t)std::max(32, 4 * nt);
     for (int t = 0; t < nt; ++t) s->workers.emplace_back([s] { s->worker(); });
     *out = s;
@@ -229,8 +240,9 @@ int gcs_server_run(gcs_server* s, double duration_s)
                    s->jobs[s->next_release].spec.arrival_s <= t)
                 ++s->next_release;
             while (s->next_gen < s->next_release && s->next_gen < s->next_admit + s->lookahead) {
-                s->gen_queue.push_back(s->next_gen);
                 s->jobs[s->next_gen].queued = true;
+                if (s->device_gen) s->jobs[s->next_gen].ready = true;  // generated at admission
+                else s->gen_queue.push_back(s->next_gen);
                 ++s->next_gen;
             }
         }

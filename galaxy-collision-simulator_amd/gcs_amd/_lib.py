@@ -119,6 +119,7 @@ SIGNATURES = {
     "gcs_spec_random": ([ctypes.c_uint64, _I64, _I32, _I32, _P], None),
     "gcs_generate": ([_P, _P], _I32),
     "gcs_load_generated": ([_P, _P, _I32], _I32),
+    "gcs_instance_add_generated": ([_P, _P, _P], _I32),
     "gcs_autotune": ([_P, _P, _I32, _P, _P], _I32),
     "gcs_read_max_accel": ([_P, _I32, _P], _I32),
     "gcs_read_sim_time": ([_P, _I32, _P], _I32),

@@ -85,6 +85,12 @@ class Engine:
         arr = (L.CollisionSpec * len(specs))(*specs)
         L.check(L.lib().gcs_load_generated(self._h, arr, len(specs)))
 
+    def add_generated(self, spec):
+        """Adds one collision generated on the device (gcs_instance_add_generated)."""
+        i = ctypes.c_int32()
+        L.check(L.lib().gcs_instance_add_generated(self._h, ctypes.byref(spec), ctypes.byref(i)))
+        return i.value
+
     def add(self, stars):
         stars = np.ascontiguousarray(stars, dtype=L.STAR)
         i = ctypes.c_int32()

@@ -263,6 +263,11 @@ int gcs_generate(const gcs_collision_spec* s, gcs_star* out);
  * last bit (about one float value in 10^8).  No counterpart in the reference
  * (its generator is host code, generator.cpp:3-196). */
 int gcs_load_generated(gcs_engine* e, const gcs_collision_spec* specs, int32_t k);
+/* Adds one generated collision at the end, as gcs_instance_add would add
+ * gcs_generate's records; only the new instance is generated and sorted.  No
+ * counterpart in the reference; the job server admits jobs this way when it
+ * has no generator threads. */
+int gcs_instance_add_generated(gcs_engine* e, const gcs_collision_spec* spec, int32_t* id);
 
 /* ---- Served form: continuous batching -----------------------------------
  * A server owns one engine.  Jobs (a randomised collision of n particles
@@ -293,7 +298,8 @@ typedef struct gcs_server_config {
     int64_t max_resident;         /* resident-particle cap, 0 = what the budget holds */
     int32_t max_jobs;             /* resident-job cap (1 = one at a time) */
     int32_t steps_per_round;      /* steps between admission checks */
-    int32_t gen_threads;          /* initial-condition worker threads */
+    int32_t gen_threads;          /* initial-condition worker threads; 0: each job is
+                                     generated on the device when admitted */
     int32_t keep_final;           /* 1: keep every finished job's final state */
     gcs_params params;
 } gcs_server_config;

@@ -102,3 +102,20 @@ def test_bad_specs_rejected(gcs):
         s.n_a = 2048
         with pytest.raises(Exception):
             e.load_generated([s])  # over capacity
+
+
+def test_added_instance_matches_host(gcs):
+    """gcs_instance_add_generated: only the new instance is generated; the
+    resident ones keep their records."""
+    first = [gcs.random_spec(21, k, 3000, 9000) for k in range(2)]
+    extra = gcs.random_spec(22, 0, 5000, 20000)
+    n0 = sum(s.n_a + s.n_b for s in first)
+    say("forest of 2 + 1 added collision")
+    with gcs.Engine(n0 + extra.n_a + extra.n_b, 3) as e:
+        e.load_generated(first)
+        before = e.stars()
+        iid = e.add_generated(extra)
+        assert iid == 2
+        after = e.stars()
+    assert after[:n0].tobytes() == before.tobytes()
+    compare(after[n0:], host_records(gcs, [extra]), "added")

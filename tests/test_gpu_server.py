@@ -23,7 +23,8 @@ def say(*a):
 
 
 def run_server(gcs, jobs, **cfg):
-    with gcs.Server(hbm_budget_bytes=4 << 30, keep_final=True, gen_threads=4, **cfg,
+    cfg.setdefault("gen_threads", 4)
+    with gcs.Server(hbm_budget_bytes=4 << 30, keep_final=True, **cfg,
                     precision=gcs.PREC_COMPENSATED, **KW) as s:
         s.submit(jobs, SEED)
         s.run(0.0)
@@ -80,3 +81,26 @@ def test_admission_respects_caps(gcs):
         fin = sorted(r["finish"] for r in order[:i])
         assert order[i]["admit"] >= fin[i - 2]
     assert st["particle_steps"] == sum(n * k for _, n, k, _ in jobs)
+
+
+def test_device_generated_jobs(gcs):
+    """gen_threads = 0: each job is generated on the device at admission.  A
+    job in the mix equals the same collision generated on the device and
+    stepped by a plain engine, next to another instance."""
+    say("server with device generation")
+    res, finals, st = run_server(gcs, MIX[:4], steps_per_round=8, gen_threads=0)
+    assert sorted(r["id"] for r in res) == [j[0] for j in MIX[:4]]
+    assert st["particle_steps"] == sum(n * k for _, n, k, _ in MIX[:4])
+    p = gcs.params(precision=gcs.PREC_COMPENSATED, **KW)
+    for jid, n, k, _ in MIX[:4]:
+        say("  job %d n=%d steps=%d, engine" % (jid, n, k))
+        other = gcs.random_spec(SEED + 1, jid, 3000, 3000)
+        with gcs.Engine(n + 3000, 2) as e:
+            e.add_generated(other)
+            before = e.instance(0).copy()
+            iid = e.add_generated(gcs.random_spec(SEED, jid, n, n))
+            assert e.instance(0).tobytes() == before.tobytes()  # the other is untouched
+            e.step(p, k)
+            ref = e.instance(iid)
+        for f in ("x", "y", "z", "vx", "vy", "vz", "mass", "r", "g", "b", "type"):
+            assert np.array_equal(ref[f], finals[jid][f]), (jid, f)
