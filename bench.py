@@ -171,13 +171,12 @@ def batched_run(n_inst, steps, warmup, leaf, seed=1004, plain_steps=10, plain_sa
     import gcs_amd
 
     specs = [gcs_amd.random_spec(seed, k, 4096, 65536) for k in range(n_inst)]
-    parts = [gcs_amd.generate(sp) for sp in specs]
-    counts = [len(a) for a in parts]
+    counts = [sp.n_a + sp.n_b for sp in specs]
     total = int(sum(counts))
     p = gcs_amd.params(G=1.0, eps2=float(np.float32(0.02) ** 2), dt=0.005, theta=0.5,
                        leaf_size=leaf, precision=gcs_amd.PREC_COMPENSATED, use_graph=1)
     e = gcs_amd.Engine(total, n_inst)
-    e.load(np.concatenate(parts), counts)
+    e.load_generated(specs)  # initial conditions generated on the device
     p, _ = e.autotune(p, reps=2)
     e.step(p, warmup)
     t0 = time.perf_counter()
@@ -189,15 +188,15 @@ def batched_run(n_inst, steps, warmup, leaf, seed=1004, plain_steps=10, plain_sa
     # One at a time: each collision alone (fresh engine, graph-replayed steps).
     ps = 0
     pt = 0.0
-    for a in parts[:plain_sample]:
-        e1 = gcs_amd.Engine(len(a))
-        e1.load(a)
+    for sp, n in list(zip(specs, counts))[:plain_sample]:
+        e1 = gcs_amd.Engine(n)
+        e1.load_generated([sp])
         e1.step(p, 2)
         t1 = time.perf_counter()
         e1.step(p, plain_steps, sync=False)
         e1.sync()
         pt += time.perf_counter() - t1
-        ps += len(a) * plain_steps
+        ps += n * plain_steps
         e1.close()
     plain = ps / pt
     return {"instances": n_inst, "particles": total, "steps": steps,
@@ -209,12 +208,15 @@ def batched_run(n_inst, steps, warmup, leaf, seed=1004, plain_steps=10, plain_sa
             "plain_sample": "%d collisions x %d steps" % (min(plain_sample, n_inst), plain_steps)}
 
 
-def served_run(window, rate, max_resident, leaf, seed=1005, budget=200e9, steps_per_round=25):
+def served_run(window, rate, max_resident, leaf, seed=1005, budget=200e9, steps_per_round=25,
+               gen_threads=0):
     """Served form: Poisson arrivals at `rate` jobs/s over `window` seconds
     (N log-uniform in [8192, 524288], steps uniform in [200, 2000]), admitted
     under the HBM budget and a resident-particle target, continuously batched.
     The comparator runs the same job list one job at a time (max_jobs = 1)
-    over the same window.  Latency = finish - arrival, over finished jobs."""
+    over the same window.  Latency = finish - arrival, over finished jobs.
+    gen_threads host threads generate the jobs' initial conditions (0: each
+    job is generated on the device at admission)."""
     import numpy as np
     import gcs_amd
 
@@ -227,25 +229,25 @@ def served_run(window, rate, max_resident, leaf, seed=1005, budget=200e9, steps_
         sp = gcs_amd.random_spec(seed + 1, len(specs), 8192, 524288)
         specs.append(sp)
         tot += sp.n_a + sp.n_b
-    parts = [gcs_amd.generate(sp) for sp in specs]
-    with gcs_amd.Engine(tot, len(parts)) as e:
-        e.load(np.concatenate(parts), [len(a) for a in parts])
+    with gcs_amd.Engine(tot, len(specs)) as e:
+        e.load_generated(specs)
         tp, _ = e.autotune(gcs_amd.params(G=1.0, eps2=eps2, dt=0.005, theta=0.5,
                                           leaf_size=leaf, precision=gcs_amd.PREC_COMPENSATED,
                                           use_graph=1), reps=2)
-    del parts
     tuned = dict(leaf_size=tp.leaf_size, group_size=tp.group_size, block_size=tp.block_size,
                  waves_per_simd=tp.waves_per_simd, team=tp.team)
     out = {"window_s": window, "rate_jobs_per_s": rate, "jobs_offered": len(jobs),
            "offered_particle_steps_per_s": sum(n * k for _, n, k, _ in jobs) / window,
            "hbm_budget_bytes": budget,
+           "generation": "device, at admission" if gen_threads <= 0 else
+           "%d host threads" % gen_threads,
            "autotuned": {"leaf": tp.leaf_size, "group": tp.group_size, "block": tp.block_size,
                          "waves_per_simd": tp.waves_per_simd, "team": tp.team,
                          "tuning_forest": {"jobs": len(specs), "particles": tot}}}
     for name, kw in (("served", dict(max_resident=max_resident)),
                      ("one_at_a_time", dict(max_jobs=1, max_resident=524288))):
         with gcs_amd.Server(hbm_budget_bytes=budget, steps_per_round=steps_per_round,
-                            gen_threads=8, G=1.0, eps2=eps2, dt=0.005, theta=0.5,
+                            gen_threads=gen_threads, G=1.0, eps2=eps2, dt=0.005, theta=0.5,
                             precision=gcs_amd.PREC_COMPENSATED, use_graph=1, **tuned,
                             **kw) as srv:
             srv.submit(jobs, seed)
@@ -302,6 +304,9 @@ def main():
     ap.add_argument("--served", type=float, default=20.0,
                     help="seconds of the served form (each of two legs; 0 = skip)")
     ap.add_argument("--served-rate", type=float, default=40.0)
+    ap.add_argument("--served-gen-threads", type=int, default=0,
+                    help="host threads generating the served jobs (0 = on the device at "
+                         "admission)")
     ap.add_argument("--served-resident", type=int, default=1 << 24,
                     help="resident-particle admission target of the served form")
     a = ap.parse_args()
@@ -407,7 +412,8 @@ def main():
     if a.precision > 0:
         line["precision"] = precision_study(a.precision, a.precision_targets, a.cpu_seconds)
     if a.served > 0:
-        line["served"] = served_run(a.served, a.served_rate, a.served_resident, a.leaf)
+        line["served"] = served_run(a.served, a.served_rate, a.served_resident, a.leaf,
+                                    gen_threads=a.served_gen_threads)
     if not a.no_cpu:
         cb, acc_check = cpu_baseline(sorted0, p, acc0, a.cpu_seconds)
         line["cpu_baseline"] = cb
