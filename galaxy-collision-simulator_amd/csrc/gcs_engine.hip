@@ -699,7 +699,9 @@ int64_t gcs_device_bytes(int64_t cap_n, int32_t cap_k)
     b += 12 * n + 16 * (n + 1) + 4 * 4 * (int64_t)max_tiles(cap_n, cap_k) +
          4 * (int64_t)max_gchunks(cap_n, cap_k) + 4;  // aggregation subtrees
     b += 4 * 4 * (int64_t)max_tiles(cap_n, cap_k) + 12 * (int64_t)cap_k;  // amax, dt, time
-    b += 32 * n;               // staging
+    b += 4 * n + 8 * n + 4 * (int64_t)cap_k;  // nodes above the aggregation subtrees
+    b += 32 * n;               // staging (also the device generator's scratch)
+    b += 8 * ((gcs_gen::kNR + 1) + (gcs_gen::kNZ + 1) + 2 * gcs_gen::kNR * gcs_gen::kNZ);
     return b;
 }
 
