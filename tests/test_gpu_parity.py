@@ -228,11 +228,14 @@ def test_c2_full_size(gcs, orc):
         say("C2 5 steps ok")
 
 
-def test_large_instance_aggregates(gcs, orc):
-    """An instance of more than 2^20 particles: the top aggregation pass keeps
-    its arrival marks in an LDS hash instead of a bitmap.  Topology and
-    aggregates bit-exact against the oracle."""
-    stars = gcs.collision(600000, 1006)
+@pytest.mark.parametrize("half", [450000, 600000])
+def test_large_instance_aggregates(gcs, orc, half):
+    """Large instances: 2 x 450,000 particles has between 1024 and 2048 nodes
+    above the aggregation subtrees (the LDS top pass with two list slots per
+    thread); 2 x 600,000 has more than 2048 (the global top pass, arrival
+    marks in an LDS hash since the instance exceeds 2^20 particles).  Topology
+    and aggregates bit-exact against the oracle."""
+    stars = gcs.collision(half, 1006)
     n = len(stars)
     eps2 = float(np.float32(0.02) ** 2)
     p = gcs.params(G=G, eps2=eps2, dt=0.005, theta=0.5, leaf_size=8)
