@@ -203,7 +203,9 @@ def batched_run(n_inst, steps, warmup, leaf, seed=1004, plain_steps=10, plain_sa
     """Batched form: n_inst collisions (N log-uniform in [4096, 65536]) advanced
     together as one forest, against the same collisions run one at a time
     through the plain path (a sample of plain_sample of them, plain_steps steps
-    each, extrapolated per particle-step).  Returns a dict."""
+    each, extrapolated per particle-step).  Each form runs at the settings the
+    autotuner picks for it: the forest's for the batched form, a single
+    collision of the median size's for the plain path.  Returns a dict."""
     import numpy as np
     import gcs_amd
 
@@ -222,15 +224,23 @@ def batched_run(n_inst, steps, warmup, leaf, seed=1004, plain_steps=10, plain_sa
     el = time.perf_counter() - t0
     e.close()
     batched = total * steps / el
-    # One at a time: each collision alone (fresh engine, graph-replayed steps).
+    # One at a time: each collision alone (fresh engine, graph-replayed steps),
+    # at the settings tuned on the median-size collision.
+    med = sorted(zip(counts, range(n_inst)))[n_inst // 2][1]
+    with gcs_amd.Engine(counts[med]) as e1:
+        e1.load_generated([specs[med]])
+        p1, _ = e1.autotune(gcs_amd.params(G=1.0, eps2=float(np.float32(0.02) ** 2), dt=0.005,
+                                           theta=0.5, leaf_size=leaf,
+                                           precision=gcs_amd.PREC_COMPENSATED, use_graph=1),
+                            reps=2)
     ps = 0
     pt = 0.0
     for sp, n in list(zip(specs, counts))[:plain_sample]:
         e1 = gcs_amd.Engine(n)
         e1.load_generated([sp])
-        e1.step(p, 2)
+        e1.step(p1, 2)
         t1 = time.perf_counter()
-        e1.step(p, plain_steps, sync=False)
+        e1.step(p1, plain_steps, sync=False)
         e1.sync()
         pt += time.perf_counter() - t1
         ps += n * plain_steps
@@ -242,6 +252,9 @@ def batched_run(n_inst, steps, warmup, leaf, seed=1004, plain_steps=10, plain_sa
                            "waves_per_simd": p.waves_per_simd, "group": p.group_size,
                            "team": p.team},
             "plain_one_at_a_time": plain, "speedup_vs_plain": batched / plain,
+            "plain_autotuned": {"leaf": p1.leaf_size, "block": p1.block_size,
+                                "waves_per_simd": p1.waves_per_simd, "group": p1.group_size,
+                                "team": p1.team, "tuned_on": int(counts[med])},
             "plain_sample": "%d collisions x %d steps" % (min(plain_sample, n_inst), plain_steps)}
 
 
@@ -251,7 +264,10 @@ def served_run(window, rate, max_resident, leaf, seed=1005, budget=200e9, steps_
     (N log-uniform in [8192, 524288], steps uniform in [200, 2000]), admitted
     under the HBM budget and a resident-particle target, continuously batched.
     The comparator runs the same job list one job at a time (max_jobs = 1)
-    over the same window.  Latency = finish - arrival, over finished jobs.
+    over the same window.  Each leg runs at the settings the autotuner picks
+    for it: a forest of about 10M particles drawn from the job distribution
+    for the served leg, one job of the mean size (124,095) for the
+    comparator.  Latency = finish - arrival, over finished jobs.
     gen_threads host threads generate the jobs' initial conditions (0: each
     job is generated on the device at admission)."""
     import numpy as np
@@ -273,6 +289,14 @@ def served_run(window, rate, max_resident, leaf, seed=1005, budget=200e9, steps_
                                           use_graph=1), reps=2)
     tuned = dict(leaf_size=tp.leaf_size, group_size=tp.group_size, block_size=tp.block_size,
                  waves_per_simd=tp.waves_per_simd, team=tp.team)
+    one = gcs_amd.random_spec(seed + 2, 0, 124095, 124095)
+    with gcs_amd.Engine(one.n_a + one.n_b) as e:
+        e.load_generated([one])
+        t1, _ = e.autotune(gcs_amd.params(G=1.0, eps2=eps2, dt=0.005, theta=0.5,
+                                          leaf_size=leaf, precision=gcs_amd.PREC_COMPENSATED,
+                                          use_graph=1), reps=2)
+    tuned_one = dict(leaf_size=t1.leaf_size, group_size=t1.group_size,
+                     block_size=t1.block_size, waves_per_simd=t1.waves_per_simd, team=t1.team)
     out = {"window_s": window, "rate_jobs_per_s": rate, "jobs_offered": len(jobs),
            "offered_particle_steps_per_s": sum(n * k for _, n, k, _ in jobs) / window,
            "hbm_budget_bytes": budget,
@@ -280,12 +304,16 @@ def served_run(window, rate, max_resident, leaf, seed=1005, budget=200e9, steps_
            "%d host threads" % gen_threads,
            "autotuned": {"leaf": tp.leaf_size, "group": tp.group_size, "block": tp.block_size,
                          "waves_per_simd": tp.waves_per_simd, "team": tp.team,
-                         "tuning_forest": {"jobs": len(specs), "particles": tot}}}
-    for name, kw in (("served", dict(max_resident=max_resident)),
-                     ("one_at_a_time", dict(max_jobs=1, max_resident=524288))):
+                         "tuning_forest": {"jobs": len(specs), "particles": tot}},
+           "autotuned_one_at_a_time": {"leaf": t1.leaf_size, "group": t1.group_size,
+                                       "block": t1.block_size,
+                                       "waves_per_simd": t1.waves_per_simd, "team": t1.team,
+                                       "tuned_on": one.n_a + one.n_b}}
+    for name, kw, tn in (("served", dict(max_resident=max_resident), tuned),
+                         ("one_at_a_time", dict(max_jobs=1, max_resident=524288), tuned_one)):
         with gcs_amd.Server(hbm_budget_bytes=budget, steps_per_round=steps_per_round,
                             gen_threads=gen_threads, G=1.0, eps2=eps2, dt=0.005, theta=0.5,
-                            precision=gcs_amd.PREC_COMPENSATED, use_graph=1, **tuned,
+                            precision=gcs_amd.PREC_COMPENSATED, use_graph=1, **tn,
                             **kw) as srv:
             srv.submit(jobs, seed)
             srv.run(window)
