@@ -1208,7 +1208,7 @@ int gcs_autotune(gcs_engine* e, const gcs_params* base, int32_t reps, gcs_params
     // 2.6e-4 at leaf 16).
     std::vector<gcs_params> cand;
     if (base->mac == GCS_MAC_GROUP) {
-        for (int group : {48, 64, 96})
+        for (int group : {48, 64, 96, 128})
             for (int leaf : {4, 8, 12, 16})
                 for (int block : {-4, -2, 64, 128, 256})  // -t: teams of t waves
                     for (int wps : {2, 3, 4}) {

@@ -169,9 +169,10 @@ int gcs_compute_accels(gcs_engine* e, const gcs_params* p);
  * passes of the current state (positions are not advanced; the state is
  * re-sorted as every step does) for each candidate setting and writes the
  * fastest into *best (base with those fields replaced) and its mean time per
- * pass in ms into *best_ms.  Group traversal: group_size {48, 64, 96} x
- * leaf_size {4, 8, 12, 16} x block_size {64, 128, 256} x waves_per_simd
- * {2, 3, 4}; reference traversal:
+ * pass in ms into *best_ms.  Group traversal: group_size {48, 64, 96, 128} x
+ * leaf_size {4, 8, 12, 16} x (block_size {64, 128, 256} with one wave per
+ * group, or teams of 2 or 4 waves) x waves_per_simd {2, 3, 4}; reference
+ * traversal:
  * stack_depth {64, 128, 256, 1024}, skipping depths that overflow.  reps
  * timed passes per candidate after one warm-up pass, 1 <= reps <= 100. */
 int gcs_autotune(gcs_engine* e, const gcs_params* base, int32_t reps,
