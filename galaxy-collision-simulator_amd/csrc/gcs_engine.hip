@@ -525,7 +525,8 @@ int enqueue_step(gcs_engine* e, const gcs_params& p, bool integrate)
     // (w/theta + |COM - box centre|)^2 (group traversal).
     const int amode = group ? (p.open_offset ? 2 : 1) : 0;
     // Persistent: as many blocks as fit on the CUs at once (2048 threads per
-    // CU; the LDS of two 1024-thread blocks) take the subtrees in turn.
+    // CU; the LDS of two 1024-thread blocks) take the work items (one or two
+    // aggregation subtrees each) from a counter.
     const int ablocks = (int)std::max<int64_t>(
         1, std::min<int64_t>((int64_t)e->num_cu * (2048 / kATile), e->n_total));
     k_aggregate<<<ablocks, kATile, 0, e->stream>>>(e->atiles, e->natiles, e->segs, posm, e->node,
