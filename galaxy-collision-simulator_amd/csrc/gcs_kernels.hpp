@@ -200,13 +200,51 @@ __device__ __forceinline__ void permute_one(const QuarterTile& q, const SegInfo*
     }
 }
 
-// karras_bvh.cuh:23-31 -- common prefix of the 96-bit key (key, local index).
-__device__ __forceinline__ int delta(int i, int j, int n, const uint64_t* __restrict__ k)
+// The sorted keys of an instance as a build block sees them: its window --
+// the block's 256 nodes and 384 keys either side -- staged in LDS, the rest
+// read from global memory.  Most nodes' searches stay inside the window.
+constexpr int kKeyWin = 1024;
+struct KeyWin {
+    const uint64_t* g;  // the instance's keys (local index)
+    const uint64_t* w;  // w[j - lo] for j in [lo, lo + kKeyWin) and [0, n)
+    int lo, n;
+};
+
+__device__ __forceinline__ uint64_t key_at(const KeyWin& kw, int j)
 {
-    if (j < 0 || j >= n) return -1;
-    const uint64_t a = k[i], b = k[j];
-    if (a == b) return 64 + clz32s((uint32_t)(i ^ j));
-    return clz64s(a ^ b);
+    const unsigned r = (unsigned)(j - kw.lo);
+    return r < (unsigned)kKeyWin ? kw.w[r] : kw.g[j];
+}
+
+// karras_bvh.cuh:23-31 -- common prefix of the 96-bit key (key, local index),
+// with key i (ki) already loaded.
+__device__ __forceinline__ int delta(const KeyWin& kw, uint64_t ki, int i, int j)
+{
+    if (j < 0 || j >= kw.n) return -1;
+    const uint64_t b = key_at(kw, j);
+    if (ki == b) return 64 + clz32s((uint32_t)(i ^ j));
+    return clz64s(ki ^ b);
+}
+
+// Stages a build block's key window (all threads; ends with a barrier).
+__device__ __forceinline__ KeyWin stage_keys(const TileInfo* __restrict__ tiles,
+                                             const SegInfo* __restrict__ segs,
+                                             const uint64_t* __restrict__ skeys, uint64_t* wk)
+{
+    const TileInfo t = tiles[blockIdx.x >> 2];
+    const SegInfo sg = segs[t.seg];
+    KeyWin kw;
+    kw.g = skeys + sg.start;
+    kw.w = wk;
+    kw.n = sg.n;
+    kw.lo = t.start - sg.start + (int)(blockIdx.x & 3) * kTileThreads -
+            (kKeyWin - kTileThreads) / 2;
+    for (int k = threadIdx.x; k < kKeyWin; k += blockDim.x) {
+        const int j = kw.lo + k;
+        if (j >= 0 && j < kw.n) wk[k] = kw.g[j];
+    }
+    __syncthreads();
+    return kw;
 }
 
 // Marks a subtree starting at global particle x with `count` particles and
@@ -241,7 +279,7 @@ struct TopList {
 };
 
 __device__ __forceinline__ void build_one(const QuarterTile& q, const SegInfo* __restrict__ segs,
-                                          const uint64_t* __restrict__ skeys,
+                                          const KeyWin& kw,
                                           float4* __restrict__ node, int* __restrict__ parent,
                                           int* __restrict__ flag, const Marks& gm, const Marks& am,
                                           int gmax, const TopList& tl)
@@ -254,20 +292,20 @@ __device__ __forceinline__ void build_one(const QuarterTile& q, const SegInfo* _
         if (n <= kATile) mark_subtree(sg, sg.start, n, sg.root, am);
     }
     if (i >= n - 1) return;
-    const uint64_t* k = skeys + sg.start;
-    const int dir = (delta(i, i + 1, n, k) - delta(i, i - 1, n, k)) > 0 ? 1 : -1;
-    const int dmin = delta(i, i - dir, n, k);
+    const uint64_t ki = key_at(kw, i);
+    const int dir = (delta(kw, ki, i, i + 1) - delta(kw, ki, i, i - 1)) > 0 ? 1 : -1;
+    const int dmin = delta(kw, ki, i, i - dir);
     int lmax = 2;
-    while (delta(i, i + lmax * dir, n, k) > dmin) lmax *= 2;
+    while (delta(kw, ki, i, i + lmax * dir) > dmin) lmax *= 2;
     int len = 0;
     for (int t = lmax >> 1; t >= 1; t >>= 1)
-        if (delta(i, i + (len + t) * dir, n, k) > dmin) len += t;
+        if (delta(kw, ki, i, i + (len + t) * dir) > dmin) len += t;
     const int j = i + len * dir;
-    const int dnode = delta(i, j, n, k);
+    const int dnode = delta(kw, ki, i, j);
     int s = 0, step = len;
     do {
         step = (step + 1) >> 1;
-        if (delta(i, i + (s + step) * dir, n, k) > dnode) s += step;
+        if (delta(kw, ki, i, i + (s + step) * dir) > dnode) s += step;
     } while (step > 1);
     const int split = i + s * dir + (dir > 0 ? 0 : -1);
     const int lo = min(i, j), hi = max(i, j);
@@ -303,8 +341,10 @@ __global__ __launch_bounds__(256) void k_build(const TileInfo* __restrict__ tile
                                                int* __restrict__ flag, Marks gm, Marks am, int gmax,
                                                TopList tl)
 {
+    __shared__ uint64_t wk[kKeyWin];
+    const KeyWin kw = stage_keys(tiles, segs, skeys, wk);
     const QuarterTile q = quarter_tile(tiles, segs);
-    if (q.valid) build_one(q, segs, skeys, node, parent, flag, gm, am, gmax, tl);
+    if (q.valid) build_one(q, segs, kw, node, parent, flag, gm, am, gmax, tl);
 }
 
 // The gather and the tree build of a step in one launch: the two only read
@@ -314,10 +354,12 @@ __global__ __launch_bounds__(256) void k_permute_build(
     const uint64_t* __restrict__ skeys, float4* __restrict__ node, float4* __restrict__ nodeQ,
     int* __restrict__ parent, int* __restrict__ flag, Marks gm, Marks am, int gmax, TopList tl)
 {
+    __shared__ uint64_t wk[kKeyWin];
+    const KeyWin kw = stage_keys(tiles, segs, skeys, wk);
     const QuarterTile q = quarter_tile(tiles, segs);
     if (!q.valid) return;
     permute_one(q, segs, pa, node, nodeQ);
-    build_one(q, segs, skeys, node, parent, flag, gm, am, gmax, tl);
+    build_one(q, segs, kw, node, parent, flag, gm, am, gmax, tl);
 }
 
 // Groups and aggregation subtrees per chunk of kGroupChunk quarter tiles.
