@@ -34,7 +34,7 @@ TREE_BYTES_PER_PARTICLE = 376  # Morton 24 + sort 24 + permute 68 + init 88 + bu
 # WRITE_SIZE (x1.000) passes scaled by a 256 MiB copy (profiles/
 # traversal_counters_plain-2x131072.md); valid for the configuration it was
 # measured on: (workload, leaf, block, waves per SIMD, group, team).
-TRAVERSE_TRAFFIC = {("plain-2x131072", 4, 256, 4, 96, 4): 137563909}
+TRAVERSE_TRAFFIC = {("plain-2x131072", 4, 256, 4, 96, 4): 137057843}
 
 CONFIGS = {
     # name: (particles per galaxy, eps, dt, theta)
