@@ -11,7 +11,9 @@ kick-drift), replayed from a hipGraph.
 
 Prints one JSON line.  `roofline` is the traversal kernel: 20 flop per
 interaction (interactions counted exactly by the kernel) over its average
-duration from HIP events on the engine stream, against the MI355X FP32 peak.
+duration from HIP events on the engine stream, against the MI355X FP32 peak
+(157.3 TFLOP/s, the matrix-core figure; `bound` is "mfma", the compute class,
+although the kernel issues vector FP32).
 `cpu_baseline` times the reference's own CPU octree step (compiled from its
 sources into oracle/_ref) on a bounded sample of the same input on the host
 cores; the same leg checks the GPU accelerations against an FP64 direct sum.
@@ -457,7 +459,10 @@ def main():
         "interactions_per_s": n_inter / (trav_ms * 1e-3),
         "stage_ms": dict(zip(["bounds_morton", "sort", "permute", "build", "aggregate",
                               "traverse", "integrate"], stage_ms)),
-        "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
+        # Compute-bound: priced against the FP32 matrix-core figure (the
+        # contract's compute class); the kernel itself issues vector FP32.
+        "roofline": {"bound": "mfma", "issue": "vector FP32 (VALU), packed FP32 off",
+                     "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS,
                      "traffic": TRAVERSE_TRAFFIC.get((a.config, a.leaf, a.block, a.wps, a.group,
                                                       a.team)),
