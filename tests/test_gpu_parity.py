@@ -228,36 +228,41 @@ def test_c2_full_size(gcs, orc):
         say("C2 5 steps ok")
 
 
-def test_c2_pipeline_over_500_steps(gcs, orc):
-    """The C2 collision stepped 500 times at the bench's settings.  At steps
-    0, 125, 250, 375 and 500 the current state is read back and the oracle
-    restates the frame from it: bounds, Morton keys, the sort's permutation,
-    the topology and the node aggregates are bit-exact.  At steps 0, 250 and
-    500 the accelerations are within 1e-3 relative RMS of the FP64 direct
-    sum (4,096 targets)."""
-    stars = gcs.collision(131072, 1002)
+@pytest.mark.parametrize("per,eps,dt,seed,checks", [
+    (8192, 0.05, 0.01, 1001, (0, 50, 100)),              # C1, 100 steps
+    (131072, 0.02, 0.005, 1002, (0, 125, 250, 375, 500)),  # C2, 500 steps
+])
+def test_pipeline_over_many_steps(gcs, orc, per, eps, dt, seed, checks):
+    """A collision stepped at the bench's settings.  At each checkpoint the
+    current state is read back and the oracle restates the frame from it:
+    bounds, Morton keys, the sort's permutation, the topology and the node
+    aggregates are bit-exact.  At the first, middle and last checkpoints the
+    accelerations are within 1e-3 relative RMS of the FP64 direct sum (4,096
+    targets)."""
+    stars = gcs.collision(per, seed)
     n = len(stars)
-    eps2 = float(np.float32(0.02) ** 2)
-    p = gcs.params(G=G, eps2=eps2, dt=0.005, theta=0.5, leaf_size=4, group_size=96, team=4,
+    eps2 = float(np.float32(eps) ** 2)
+    p = gcs.params(G=G, eps2=eps2, dt=dt, theta=0.5, leaf_size=4, group_size=96, team=4,
                    block_size=256, waves_per_simd=4)
     tgt = np.linspace(0, n - 1, 4096).astype(np.int64)
+    direct = (checks[0], checks[len(checks) // 2], checks[-1])
     with gcs.Engine(n) as e:
         e.load(stars)
         done = 0
-        for at in (0, 125, 250, 375, 500):
+        for at in checks:
             if at > done:
                 e.step(p, at - done)
                 done = at
             cur = e.stars()
             e.compute_accels(p)
             cpu = check_pipeline(gcs, orc, e, cur, p)
-            say("C2 step %d: keys, permutation, topology and aggregates bit-exact" % at)
-            if at % 250 == 0:
+            say("n=%d step %d: keys, permutation, topology and aggregates bit-exact" % (n, at))
+            if at in direct:
                 acc = e.accels()
                 s = cpu["sorted"]
                 ref = orc.direct_sum_f64(s["x"], s["y"], s["z"], s["mass"], G, eps2, tgt)
                 err = rel_rms(acc[tgt, :3], ref[:, :3])
-                say("C2 step %d: rel rms vs FP64 direct sum %.3g" % (at, err))
+                say("n=%d step %d: rel rms vs FP64 direct sum %.3g" % (n, at, err))
                 assert err < 1e-3
 
 
