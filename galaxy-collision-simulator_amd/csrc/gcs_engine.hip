@@ -1085,11 +1085,19 @@ int gcs_restore(gcs_engine* e, const void* buf, int64_t bytes, int32_t* id)
     int rc = gcs_instance_add(e, reinterpret_cast<const gcs_star*>(in + 32), h.n, &nid);
     if (rc) return rc;
     const int slot = find_slot(e, nid);
-    HIPCHK(hipMemcpyAsync(e->plo[e->cur] + e->h_segs[slot].start, in + 32 + 32 * (size_t)h.n,
-                          16 * (size_t)h.n, hipMemcpyHostToDevice, e->stream));
-    HIPCHK(hipMemcpyAsync(e->tsim + slot, &h.sim_time, sizeof(double), hipMemcpyHostToDevice,
-                          e->stream));
-    HIPCHK(hipStreamSynchronize(e->stream));
+    hipError_t he = hipMemcpyAsync(e->plo[e->cur] + e->h_segs[slot].start,
+                                   in + 32 + 32 * (size_t)h.n, 16 * (size_t)h.n,
+                                   hipMemcpyHostToDevice, e->stream);
+    if (he == hipSuccess)
+        he = hipMemcpyAsync(e->tsim + slot, &h.sim_time, sizeof(double), hipMemcpyHostToDevice,
+                            e->stream);
+    if (he == hipSuccess) he = hipStreamSynchronize(e->stream);
+    if (he != hipSuccess) {
+        // Leave the engine as it was: no half-restored instance.
+        const std::string msg = std::string("restore: ") + hipGetErrorString(he);
+        gcs_instance_remove(e, nid);
+        return fail(he == hipErrorOutOfMemory ? GCS_ERR_OOM : GCS_ERR_HIP, msg);
+    }
     if (id) *id = nid;
     return GCS_OK;
 }
