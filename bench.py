@@ -29,6 +29,12 @@ sys.path.insert(0, os.path.join(REPO, "galaxy-collision-simulator_amd"))
 
 PEAK_FP32_TFLOPS = 157.3
 FLOP_PER_INTERACTION = 20
+# FP32 operations the kernel executes per interaction (a multiply-add counted as
+# two, the reciprocal square root as one): softened quadrupole node 52,
+# particle 19 (interact_n / interact_p in csrc/gcs_traverse.hpp).
+EXEC_FLOP_NODE = 52
+EXEC_FLOP_PARTICLE = 19
+PEAK_FP32_VECTOR_TFLOPS = 78.6  # vector FP32 without packed operations
 PEAK_HBM_GBS = 8000.0
 TREE_BYTES_PER_PARTICLE = 376  # Morton 24 + sort 24 + permute 68 + init 88 + build 24 + agg 148
 
@@ -430,6 +436,8 @@ def main():
     trav_ms = float(np.mean(trav))
     n_inter = inter[0] + inter[1]
     achieved = FLOP_PER_INTERACTION * n_inter / (trav_ms * 1e-3) / 1e12
+    executed = ((EXEC_FLOP_NODE * inter[0] + EXEC_FLOP_PARTICLE * inter[1])
+                / (trav_ms * 1e-3) / 1e12)
     stage_ms = [float(x) for x in np.mean(np.array(stages), 0)]
     tree_ms = sum(stage_ms[:5])
     tree_gbs = TREE_BYTES_PER_PARTICLE * n / (tree_ms * 1e-3) / 1e9
@@ -456,6 +464,7 @@ def main():
                    "multipole": "quadrupole",
                    "parallelism": "single GPU"},
         "interactions_per_step": n_inter,
+        "interactions_split": {"node": inter[0], "particle": inter[1]},
         "interactions_per_s": n_inter / (trav_ms * 1e-3),
         "stage_ms": dict(zip(["bounds_morton", "sort", "permute", "build", "aggregate",
                               "traverse", "integrate"], stage_ms)),
@@ -468,7 +477,13 @@ def main():
                                                       a.team)),
                      "kernel": "k_traverse_team" if a.team > 1 else "k_traverse_group",
                      "kernel_ms": trav_ms,
-                     "flop_per_interaction": FLOP_PER_INTERACTION},
+                     "flop_per_interaction": FLOP_PER_INTERACTION,
+                     "executed": {"tflops": executed,
+                                  "flop_per_node": EXEC_FLOP_NODE,
+                                  "flop_per_particle": EXEC_FLOP_PARTICLE,
+                                  "frac_of_peak": executed / PEAK_FP32_TFLOPS,
+                                  "frac_of_unpacked_vector_peak":
+                                      executed / PEAK_FP32_VECTOR_TFLOPS}},
         "roofline_tree": {"bound": "hbm", "unit": "GB/s", "peak": PEAK_HBM_GBS,
                           "achieved": tree_gbs, "frac": tree_gbs / PEAK_HBM_GBS,
                           "bytes_per_particle": TREE_BYTES_PER_PARTICLE,
