@@ -140,6 +140,10 @@ struct gcs_engine {
     uint32_t* chist = nullptr;  // per-chunk digit counts, one block per pass
     uint32_t* seg_bounds = nullptr;
     uint32_t* bpart = nullptr;  // per quarter tile (min, max) encodings
+    // bpart holds the current positions' partials (written by the last
+    // k_integrate or k_bounds; any change of the composition clears it).
+    bool bpart_valid = false;
+    bool capturing = false;  // enqueue_step is recording a step graph
     unsigned* qctr = nullptr;
     unsigned long long* counters = nullptr;
     unsigned* err = nullptr;
@@ -256,6 +260,7 @@ int rebuild_tables(gcs_engine* e)
     }
     e->graph_valid = false;
     e->have_tree = false;
+    e->bpart_valid = false;
     return GCS_OK;
 }
 
@@ -463,8 +468,13 @@ int enqueue_step(gcs_engine* e, const gcs_params& p, bool integrate)
     const int64_t cwords = (int64_t)kSortPasses * e->nchunks * kRadix;
     const int ngchunk = (4 * ntiles + kGroupChunk - 1) / kGroupChunk;
     if (!p.use_bounds) {
-        k_bounds<<<qblocks, kTileThreads, 0, e->stream>>>(e->tiles, e->segs, e->posm[c],
-                                                           e->bpart);
+        // A recorded step relies on the previous step's k_integrate (gcs_step
+        // runs k_bounds once before replays when the partials are stale).
+        if (!e->bpart_valid && !e->capturing) {
+            k_bounds<<<qblocks, kTileThreads, 0, e->stream>>>(e->tiles, e->segs, e->posm[c],
+                                                               e->bpart);
+            e->bpart_valid = true;
+        }
         k_bounds_reduce<<<nseg, 256, 0, e->stream>>>(e->segs, e->bpart, e->seg_bounds);
     }
     StepReset rs;
@@ -584,7 +594,8 @@ int enqueue_step(gcs_engine* e, const gcs_params& p, bool integrate)
         k_integrate<<<qblocks, kTileThreads, 0, e->stream>>>(
             e->tiles, e->segs, e->acc, posm, e->vel[e->cur], e->plo[e->cur], p.dt,
             adaptive ? e->dt_inst : nullptr, comp ? 1 : 0, p.colours, p.diagnostics,
-            e->diag_part, e->amax_part, e->tsim, e->err);
+            e->diag_part, e->amax_part, e->tsim, e->err, e->bpart);
+        if (!e->capturing) e->bpart_valid = true;
         if (p.diagnostics)
             k_diag_final<<<nseg, 64, 0, e->stream>>>(e->segs, nseg, e->diag_part, e->diag_out);
         LAUNCHCHK();
@@ -606,7 +617,9 @@ int capture_graphs(gcs_engine* e, const gcs_params& p)
         e->cur = k;
         hipGraph_t g = nullptr;
         HIPCHK(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+        e->capturing = true;
         int rc = enqueue_step(e, p, true);
+        e->capturing = false;
         hipError_t ec = hipStreamEndCapture(e->stream, &g);
         if (rc != GCS_OK) {
             if (g) (void)hipGraphDestroy(g);
@@ -1149,10 +1162,17 @@ int gcs_step(gcs_engine* e, const gcs_params* p, int32_t nsteps)
             rc = capture_graphs(e, *p);
             if (rc) return rc;
         }
+        if (!p->use_bounds && !e->bpart_valid) {
+            const int qblocks = 4 * (int)e->h_tiles.size();
+            k_bounds<<<qblocks, kTileThreads, 0, e->stream>>>(e->tiles, e->segs,
+                                                               e->posm[e->cur], e->bpart);
+            LAUNCHCHK();
+        }
         for (int32_t s = 0; s < nsteps; ++s) {
             HIPCHK(hipGraphLaunch(e->gexec[e->cur], e->stream));
             e->cur ^= 1;
         }
+        e->bpart_valid = true;  // every replayed step ends with k_integrate
         e->have_tree = true;
         e->last_mac = p->mac;
         return GCS_OK;

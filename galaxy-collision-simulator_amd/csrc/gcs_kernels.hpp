@@ -34,15 +34,18 @@ __device__ __forceinline__ QuarterTile quarter_tile(const TileInfo* __restrict__
 // order-preserving integer encodings, k_bounds_reduce folds an instance's
 // quarter tiles (no atomics, so no contention on one address).  NaN
 // coordinates are ignored, as the reference's `<` comparisons ignore them.
-__global__ __launch_bounds__(256) void k_bounds(const TileInfo* __restrict__ tiles,
-                                                const SegInfo* __restrict__ segs,
-                                                const float4* __restrict__ posm,
-                                                uint32_t* __restrict__ bpart)
+// A step's k_integrate writes the same partials from the positions it stores,
+// so the next step needs only the fold; k_bounds runs after the positions
+// were set in any other way.
+//
+// The quarter tile's (min, max) over every coordinate of its positions, as
+// order-preserving encodings, into bpart[2 * blockIdx.x ..]; NaN coordinates
+// are left out.  Every thread of the 256-thread block calls it.
+__device__ __forceinline__ void bounds_partial(const float4 p, bool valid,
+                                               uint32_t* __restrict__ bpart)
 {
-    const QuarterTile q = quarter_tile(tiles, segs);
     float lo = __builtin_inff(), hi = -__builtin_inff();
-    if (q.valid) {
-        const float4 p = posm[q.i];
+    if (valid) {
         lo = fminf(p.x, fminf(p.y, p.z));
         hi = fmaxf(p.x, fmaxf(p.y, p.z));
         if (lo != lo) lo = __builtin_inff();
@@ -60,6 +63,15 @@ __global__ __launch_bounds__(256) void k_bounds(const TileInfo* __restrict__ til
         bpart[2 * blockIdx.x] = min(min(slo[0], slo[1]), min(slo[2], slo[3]));
         bpart[2 * blockIdx.x + 1] = max(max(shi[0], shi[1]), max(shi[2], shi[3]));
     }
+}
+
+__global__ __launch_bounds__(256) void k_bounds(const TileInfo* __restrict__ tiles,
+                                                const SegInfo* __restrict__ segs,
+                                                const float4* __restrict__ posm,
+                                                uint32_t* __restrict__ bpart)
+{
+    const QuarterTile q = quarter_tile(tiles, segs);
+    bounds_partial(q.valid ? posm[q.i] : make_float4(0.f, 0.f, 0.f, 0.f), q.valid, bpart);
 }
 
 __global__ __launch_bounds__(256) void k_bounds_reduce(const SegInfo* __restrict__ segs,
@@ -1211,10 +1223,11 @@ __global__ __launch_bounds__(256) void k_integrate(
     const float4* __restrict__ acc, float4* __restrict__ posm, float4* __restrict__ vel,
     float4* __restrict__ plo, float dt_fixed, const float* __restrict__ dt_inst, int compensated,
     int colours, int diagnostics, double* __restrict__ diag_part, float* __restrict__ amax_part,
-    double* __restrict__ tsim, unsigned* err)
+    double* __restrict__ tsim, unsigned* err, uint32_t* __restrict__ bpart)
 {
     const QuarterTile q = quarter_tile(tiles, segs);
     const float dt = dt_inst ? dt_inst[q.seg] : dt_fixed;
+    float4 pn = make_float4(0.f, 0.f, 0.f, 0.f);  // the new position
     double d[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
     {
         const float4 a0 = q.valid ? acc[q.i] : make_float4(0.f, 0.f, 0.f, 0.f);
@@ -1263,7 +1276,11 @@ __global__ __launch_bounds__(256) void k_integrate(
             v.w = __uint_as_float(speed_colour(__float_as_uint(v.w), v.x, v.y, v.z));
         vel[q.i] = v;
         posm[q.i] = p;
+        pn = p;
     }
+    // The next step's bounds partials, from the positions just written (what
+    // k_bounds would read at the start of that step).
+    if (bpart) bounds_partial(pn, q.valid, bpart);
     if (diagnostics) {
         __shared__ double sd[4][5];
 #pragma unroll
