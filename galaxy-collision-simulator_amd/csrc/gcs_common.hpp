@@ -40,6 +40,9 @@ constexpr int kATile = kTile;
 constexpr int kStackCap = 8192;      // per-wave traversal stack entries
 constexpr int kOrderClasses = 8;     // traversal dispatch classes by group size
 constexpr int kNumXcd = 8;           // MI355X: 8 XCDs, blocks dealt round-robin
+// Most particles an engine holds: node records are addressed as 2 * v with a
+// 32-bit node index v < 2 n, so 4 n must stay below 2^31.
+constexpr int64_t kMaxParticles = (int64_t)1 << 29;
 
 // Sort tile: a run of at most kTile particles of one instance.
 struct TileInfo {

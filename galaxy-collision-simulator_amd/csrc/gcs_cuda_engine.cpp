@@ -6,7 +6,6 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
-#include <cstring>
 
 #include "gcs.h"
 
@@ -80,16 +79,10 @@ void cuda_unregister_vbo() {}
 
 void cuda_get_center_of_mass(float* cx, float* cy, float* cz)
 {
+    // cuda_engine.cu:365-376: the root record of the last built tree (one
+    // small copy), zeros when nothing has been built.
     float c[4] = {0.f, 0.f, 0.f, 0.f};
-    int32_t k = 0;
-    int64_t n = 0;
-    if (g_engine && gcs_instance_count(g_engine, &k, &n) == GCS_OK && k == 1 && n >= 1) {
-        float* com4 = new float[4 * (2 * n - 1)];
-        if (gcs_read_tree(g_engine, 0, nullptr, nullptr, nullptr, com4, nullptr, nullptr,
-                          nullptr) == GCS_OK)
-            std::memcpy(c, com4, sizeof(c));
-        delete[] com4;
-    }
+    if (g_engine && gcs_read_root(g_engine, 0, c) != GCS_OK) c[0] = c[1] = c[2] = 0.f;
     if (cx) *cx = c[0];
     if (cy) *cy = c[1];
     if (cz) *cz = c[2];

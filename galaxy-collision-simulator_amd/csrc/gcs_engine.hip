@@ -723,8 +723,9 @@ int gcs_create(int64_t cap_n, int32_t cap_k, int32_t device, gcs_engine** out)
 {
     if (!out) return fail(GCS_ERR_INVALID, "out is null");
     *out = nullptr;
-    if (cap_n < 1 || cap_k < 1 || cap_n > (int64_t)1 << 30)
-        return fail(GCS_ERR_INVALID, "capacity out of range");
+    // Node records are indexed as node[2 * v] with a 32-bit v < 2 cap_n.
+    if (cap_n < 1 || cap_k < 1 || cap_n > kMaxParticles)
+        return fail(GCS_ERR_INVALID, "capacity out of range (1 .. 2^29 particles)");
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
         return fail(GCS_ERR_NO_DEVICE, "no HIP device");
@@ -1371,13 +1372,35 @@ int gcs_read_tree(gcs_engine* e, int32_t id, int32_t* left, int32_t* right, int3
     return GCS_OK;
 }
 
+int gcs_read_root(gcs_engine* e, int32_t id, float* com4)
+{
+    if (!e || !com4) return fail(GCS_ERR_INVALID, "null argument");
+    const int slot = find_slot(e, id);
+    if (slot < 0) return fail(GCS_ERR_NOT_FOUND, "unknown instance id");
+    if (!e->have_tree) return fail(GCS_ERR_INVALID, "no tree has been built");
+    HIPCHK(hipSetDevice(e->device));
+    // One 16-byte copy of the root's (COM, mass): an internal node's merge,
+    // or for a single particle its leaf record as the build wrote it.
+    float4 a;
+    HIPCHK(hipMemcpyAsync(&a, e->node + 2 * (size_t)e->h_segs[slot].root, sizeof(float4),
+                          hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    com4[0] = a.x;
+    com4[1] = a.y;
+    com4[2] = a.z;
+    com4[3] = a.w;
+    return GCS_OK;
+}
+
 int gcs_read_bounds(gcs_engine* e, int32_t id, float* mn, float* mx)
 {
     if (!e) return fail(GCS_ERR_INVALID, "engine is null");
     const int slot = find_slot(e, id);
     if (slot < 0) return fail(GCS_ERR_NOT_FOUND, "unknown instance id");
     uint32_t b[2];
-    HIPCHK(hipMemcpy(b, e->seg_bounds + 2 * slot, 8, hipMemcpyDeviceToHost));
+    HIPCHK(hipSetDevice(e->device));
+    HIPCHK(hipMemcpyAsync(b, e->seg_bounds + 2 * slot, 8, hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(hipStreamSynchronize(e->stream));
     if (mn) *mn = fdec(b[0]);
     if (mx) *mx = fdec(b[1]);
     return GCS_OK;
@@ -1470,7 +1493,7 @@ int gcs_morton_keys(const float* x, const float* y, const float* z, int64_t n, f
 {
     if (n < 0 || (n > 0 && (!x || !y || !z || !keys))) return fail(GCS_ERR_INVALID, "bad argument");
     if (n == 0) return GCS_OK;
-    if (n > (int64_t)1 << 30) return fail(GCS_ERR_CAPACITY, "too many keys");
+    if (n > kMaxParticles) return fail(GCS_ERR_CAPACITY, "too many keys (at most 2^29)");
     gcs_engine* e = nullptr;
     int rc = gcs_create(n, 1, 0, &e);
     if (rc) return rc;
@@ -1503,7 +1526,7 @@ int gcs_sort_pairs(const uint64_t* keys, int64_t n, uint64_t* keys_out, int32_t*
 {
     if (n < 0 || (n > 0 && (!keys || !keys_out || !perm_out))) return fail(GCS_ERR_INVALID, "bad argument");
     if (n == 0) return GCS_OK;
-    if (n > (int64_t)1 << 30) return fail(GCS_ERR_CAPACITY, "too many keys");
+    if (n > kMaxParticles) return fail(GCS_ERR_CAPACITY, "too many keys (at most 2^29)");
     gcs_engine* e = nullptr;
     int rc = gcs_create(n, 1, 0, &e);
     if (rc) return rc;

@@ -202,7 +202,9 @@ __device__ __forceinline__ void permute_one(const QuarterTile& q, const SegInfo*
     posm_out[q.i] = p;
     vel_out[q.i] = vel_in[src];
     if (compensated) plo_out[q.i] = plo_in[src];
-    if (write_leaves) {
+    // The group traversal reads no leaf records; an instance of one particle
+    // gets its leaf (= root) record anyway, the root the COM readback copies.
+    if (write_leaves || segs[q.seg].n == 1) {
         const int v = q.i + segs[q.seg].leaf_base;
         node[2 * v] = p;
         node[2 * v + 1] = make_float4(0.0f, __int_as_float(q.i), __int_as_float(q.i),
@@ -578,7 +580,8 @@ __device__ __forceinline__ void quad_shift(NodeData& o, const NodeData& c)
 
 // Merge of one internal node from its children (karras_bvh.cuh:190-221):
 //   AABB = union; width = max extent; m = ml + mr;
-//   com = (cl*ml + cr*mr)/m, or the AABB centre when m == 0.
+//   com = fma(cl, ml, cr*mr)/m (the contraction nvcc applies to the
+//   reference's cl*ml + cr*mr), or the AABB centre when m == 0.
 // Extension: opening value = width for the reference traversal (mode 0); for
 // the group traversal (width*inv_theta)^2 (mode 1) or
 // (width*inv_theta + |com - AABB centre|)^2 (mode 2); quadrupole moments by the
@@ -612,9 +615,10 @@ __device__ __forceinline__ NodeData merge(const NodeData& l, const NodeData& r)
     o.hx = fmaxf(l.hx, r.hx); o.hy = fmaxf(l.hy, r.hy); o.hz = fmaxf(l.hz, r.hz);
     o.m = l.m + r.m;
     if (o.m > 0.0f) {
-        o.cx = (l.cx * l.m + r.cx * r.m) / o.m;
-        o.cy = (l.cy * l.m + r.cy * r.m) / o.m;
-        o.cz = (l.cz * l.m + r.cz * r.m) / o.m;
+        // The left product fused, as nvcc contracts cl*ml + cr*mr.
+        o.cx = __builtin_fmaf(l.cx, l.m, r.cx * r.m) / o.m;
+        o.cy = __builtin_fmaf(l.cy, l.m, r.cy * r.m) / o.m;
+        o.cz = __builtin_fmaf(l.cz, l.m, r.cz * r.m) / o.m;
     } else {
         o.cx = (o.lx + o.hx) * 0.5f;
         o.cy = (o.ly + o.hy) * 0.5f;
@@ -1141,7 +1145,7 @@ __device__ __forceinline__ uint32_t speed_colour(uint32_t rgbt, float vx, float 
 {
     const uint32_t type = rgbt & 0xFFu;
     if (type != 0u && type != 2u) return rgbt;
-    const float sp = sqrtf(__builtin_fmaf(vz, vz, __builtin_fmaf(vy, vy, vx * vx)));
+    const float sp = sqrtf(__builtin_fmaf(vz, vz, __builtin_fmaf(vx, vx, vy * vy)));
     const float t = fminf(sp / 150.0f, 1.0f);
     uint32_t r, g, b;
     if (type == 0u) {

@@ -24,6 +24,7 @@
 #include <vector>
 
 #include "gcs.h"
+#include "gcs_common.hpp"
 #include "gcs_error.hpp"
 
 namespace {
@@ -172,7 +173,7 @@ int gcs_server_create(const gcs_server_config* c, gcs_server** out)
     const int64_t fixed = std::max<int64_t>(0, b1 - per * (1 << 20)) + ((int64_t)1 << 30);
     int64_t cap = (c->hbm_budget_bytes - fixed) / per;
     if (c->max_resident > 0) cap = std::min(cap, c->max_resident);
-    cap = std::min<int64_t>(cap, (int64_t)1 << 30);
+    cap = std::min<int64_t>(cap, gcs::kMaxParticles);
     if (cap < 1) {
         delete s;
         return set_error(GCS_ERR_CAPACITY, "HBM budget holds no particles");

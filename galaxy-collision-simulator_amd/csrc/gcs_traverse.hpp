@@ -854,8 +854,9 @@ __global__ __launch_bounds__(TEAM * 64) void k_traverse_team(
 // pushed before right; leaf taken when d^2+eps^2 > 1.01*eps^2; internal node
 // taken when width^2 < d^2+eps^2; force G*m*rsqrt(d2)^3 * d.  `limit` (at most
 // STACK) is the stack depth; a traversal that would exceed it stops and sets
-// the stack error bit.  Decisions are computed exactly as written (no
-// contraction); the accumulation uses the multiply-adds nvcc forms.
+// the stack error bit.  Every expression rounds as nvcc compiles the
+// reference (--fmad=true): d^2 = fma(dz,dz, fma(dx,dx, dy*dy)) + eps^2, and
+// the accumulation's multiply-adds; nothing else is contracted.
 template <int STACK>
 __global__ __launch_bounds__(256) void k_traverse_reference(
     const TileInfo* __restrict__ tiles, const SegInfo* __restrict__ segs,
@@ -883,7 +884,7 @@ __global__ __launch_bounds__(256) void k_traverse_reference(
             const int first = __float_as_int(bb.y), last = __float_as_int(bb.z),
                       split = __float_as_int(bb.w);
             const float dx = A.x - p.x, dy = A.y - p.y, dz = A.z - p.z;
-            const float d2 = dx * dx + dy * dy + dz * dz + eps2;
+            const float d2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dx, dx, dy * dy)) + eps2;
             bool take;
             if (split < 0) {
                 take = d2 > eps2 * 1.01f;

@@ -108,6 +108,7 @@ SIGNATURES = {
     "gcs_read_keys": ([_P, _P, _P, _P], _I32),
     "gcs_read_tree": ([_P, _I32, _P, _P, _P, _P, _P, _P, _P], _I32),
     "gcs_read_bounds": ([_P, _I32, _P, _P], _I32),
+    "gcs_read_root": ([_P, _I32, _P], _I32),
     "gcs_read_interactions": ([_P, _P], _I32),
     "gcs_read_diagnostics": ([_P, _I32, _P], _I32),
     "gcs_read_stage_ms": ([_P, _P], _I32),

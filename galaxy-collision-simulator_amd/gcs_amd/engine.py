@@ -198,6 +198,12 @@ class Engine:
         return dict(left=left, right=right, parent=parent, com4=com4, width=width, bmin=bmin,
                     bmax=bmax)
 
+    def root(self, iid=0):
+        """Root record (com x, y, z, mass) of the last built tree (gcs_read_root)."""
+        out = np.zeros(4, np.float32)
+        L.check(L.lib().gcs_read_root(self._h, int(iid), L.ptr(out)))
+        return out
+
     def bounds(self, iid=0):
         a = ctypes.c_float()
         b = ctypes.c_float()

@@ -114,9 +114,10 @@ void gcs_params_default(gcs_params* p);
 typedef struct gcs_engine gcs_engine;
 
 /* ---- Lifetime (cuda_engine.cu:219-285 cuda_init, :303-340 cuda_cleanup) --
- * capacity_particles: total particles over all instances; capacity_instances:
- * the most instances held at once.  device: HIP ordinal.  The engine creates
- * its own stream unless gcs_set_stream is called. */
+ * capacity_particles: total particles over all instances, 1 .. 2^29 (larger
+ * capacities return GCS_ERR_INVALID); capacity_instances: the most instances
+ * held at once.  device: HIP ordinal.  The engine creates its own stream
+ * unless gcs_set_stream is called. */
 int gcs_create(int64_t capacity_particles, int32_t capacity_instances,
                int32_t device, gcs_engine** out);
 int gcs_destroy(gcs_engine* e);
@@ -191,10 +192,18 @@ int gcs_read_keys(gcs_engine* e, uint64_t* keys_unsorted, uint64_t* keys_sorted,
 /* Tree of one instance in the reference's numbering (karras_bvh.cuh:114-124:
  * internal nodes 0..n-2, leaves n-1..2n-2), 2n-1 entries each:
  * left/right children (-1 for leaves), parent (-1 for the root), com4
- * (com x,y,z, mass), width (max AABB extent), bmin/bmax (3 floats). */
+ * (com x,y,z, mass), width (max AABB extent), bmin/bmax (3 floats).  Leaf
+ * entries are taken from the current positions: after gcs_compute_accels
+ * they are the positions the tree was built from, after gcs_step the
+ * drifted ones (the internal nodes keep the built tree's values). */
 int gcs_read_tree(gcs_engine* e, int32_t id, int32_t* left, int32_t* right,
                   int32_t* parent, float* com4, float* width, float* bmin,
                   float* bmax);
+/* Root record of instance `id` in the last built tree: com x, y, z and mass
+ * (4 floats), one 16-byte device copy.  Replaces cuda_get_center_of_mass's
+ * copy of node 0 (cuda_engine.cu:365-376); a one-particle instance gives its
+ * leaf, the position the tree was built from. */
+int gcs_read_root(gcs_engine* e, int32_t id, float* com4);
 /* Cube bounds used for instance `id` in the last step. */
 int gcs_read_bounds(gcs_engine* e, int32_t id, float* mn, float* mx);
 /* Interaction counts of the last traversal: [0] monopole (node)

@@ -7,7 +7,14 @@
  * Build: oracle/Makefile (gcc -O2 -fopenmp -ffp-contract=off).  Contraction
  * is off so every float expression rounds exactly as written; where the
  * reference's nvcc build fuses a multiply-add (nvcc defaults to --fmad=true)
- * the fused form is written out with fmaf().
+ * the fused form is written out with fmaf().  A sum of two products a*b + c*d
+ * is written fmaf(a, b, c*d) -- the left product fused, the right one rounded
+ * -- and a*a + b*b + c*c as fmaf(c, c, fmaf(a, a, b*b)): the operand choice
+ * of LLVM's DAG combiner (visitFADDForFMACombine, left operand first when
+ * both multiplies have one use), the code generator nvcc's NVVM derives
+ * from.  nvcc itself cannot run here, so which product nvcc rounds is parity
+ * unpinned (a last-bit difference in a COM coordinate, a traversal distance
+ * or a colour speed).
  *
  * Parity pins (tests/test_oracle.py):
  *   orc_expand_bits / orc_morton   naive per-bit interleave over all 2^21
@@ -18,7 +25,11 @@
  *   orc_karras_build               orc_radix_tree_definition (split at the
  *                                  highest differing bit), tree invariants
  *   orc_aggregate                  FP64 subtree sums, exact AABB min/max
- *   orc_traverse_reference         theta->0 limit equals the direct sum
+ *   orc_traverse_reference         theta->0 limit equals the direct sum;
+ *                                  every mass taken or skipped once; the
+ *                                  opening rule w^2 < d^2 + eps^2 and the
+ *                                  leaf skip d^2 + eps^2 <= 1.01 eps^2 on
+ *                                  cases just either side of each threshold
  *   orc_traverse_group             theta=0 and leaf>=n equal the direct sum;
  *                                  Plummer two-body closed form; every mass
  *                                  counted once; error bound vs FP64
@@ -27,6 +38,8 @@
  *                                  third law (zero total force)
  *   orc_integrate_reference        constant-acceleration closed form, colour
  *                                  table spot values
+ *   (compensated positions have no oracle function: the GPU test pins them
+ *    to the free-drift closed form x0 + k v dt in FP64)
  *   orc_octree_step                leaf_cap >= n reduces to the direct sum;
  *                                  error bound vs FP64 at theta=0.5;
  *                                  bit-exact against the compiled reference
@@ -322,7 +335,8 @@ static void quad_shift(float* q, const float* qc, const float* cc, const float* 
  * is order-independent given the children, so a post-order walk reproduces
  * the atomic climb exactly.  Children read: AABB, mass, COM.  Written:
  *   AABB = union; width = max extent (:206); m = ml + mr;
- *   com = (cl*ml + cr*mr)/m, or the AABB centre when m == 0 (:208-221).
+ *   com = fmaf(cl, ml, cr*mr)/m (nvcc's contraction of cl*ml + cr*mr), or the
+ *   AABB centre when m == 0 (:208-221).
  * Extension (not in the reference): crit = (width*inv_theta)^2, or with
  * offset != 0 (width*inv_theta + |com - c|)^2 with c the AABB centre, the
  * opening radius squared used by the group traversal. */
@@ -363,7 +377,7 @@ void orc_aggregate(int32_t n, const int32_t* left, const int32_t* right,
         com4[4 * v + 3] = mt;
         for (int a = 0; a < 3; ++a) {
             if (mt > 0.0f)
-                com4[4 * v + a] = (com4[4 * l + a] * ml + com4[4 * r + a] * mr) / mt;
+                com4[4 * v + a] = fmaf(com4[4 * l + a], ml, com4[4 * r + a] * mr) / mt;
             else
                 com4[4 * v + a] = (lo[a] + hi[a]) * 0.5f;
         }
@@ -406,7 +420,7 @@ int64_t orc_traverse_reference(int32_t n, const float* x, const float* y,
             float dx = com4[4 * v + 0] - x[i];
             float dy = com4[4 * v + 1] - y[i];
             float dz = com4[4 * v + 2] - z[i];
-            float d2 = dx * dx + dy * dy + dz * dz + eps2;
+            float d2 = fmaf(dz, dz, fmaf(dx, dx, dy * dy)) + eps2;
             int take = 0;
             if (v >= n - 1) {
                 take = d2 > eps2 * 1.01f;
@@ -624,8 +638,8 @@ static uint8_t to_u8(float f) { return (uint8_t)orc_f2u_sat(f); }
  * that nvcc's default --fmad=true forms for `v += a*dt` and `x += v*dt`.
  * Colour from speed t = min(|v|/150, 1) (:185-215): fire palette for type 0,
  * ice palette for type 2, other types untouched; `50 + 150*u` and the like
- * are the fused multiply-adds nvcc forms, |v|^2 = fma(vz,vz, fma(vy,vy,
- * vx*vx)). */
+ * are the fused multiply-adds nvcc forms, |v|^2 = fma(vz,vz, fma(vx,vx,
+ * vy*vy)). */
 void orc_integrate_reference(int64_t n, float* x, float* y, float* z,
                              float* vx, float* vy, float* vz, uint8_t* r,
                              uint8_t* g, uint8_t* b, const uint8_t* type,
@@ -640,7 +654,7 @@ void orc_integrate_reference(int64_t n, float* x, float* y, float* z,
         y[i] = fmaf(vy[i], dt, y[i]);
         z[i] = fmaf(vz[i], dt, z[i]);
         if (type[i] != 0 && type[i] != 2) continue;
-        float sp = sqrtf(fmaf(vz[i], vz[i], fmaf(vy[i], vy[i], vx[i] * vx[i])));
+        float sp = sqrtf(fmaf(vz[i], vz[i], fmaf(vx[i], vx[i], vy[i] * vy[i])));
         float t = fminf(sp / 150.0f, 1.0f);
         if (type[i] == 0) {
             if (t < 0.5f) {

@@ -1,0 +1,203 @@
+"""The benchmark configurations run through the HIP path, against the oracle.
+
+C2  2 x 131,072 particles, 500 steps: Morton keys, the sort's permutation and
+    the Karras topology bit-exact at every step; aggregates at five
+    checkpoints; accelerations within 1e-3 of an FP64 direct sum at steps 0,
+    250 and 500.
+C3  2 x 1,048,576 particles, theta 0.5 and 0.7: keys, permutation, topology
+    and aggregates bit-exact at step 0 and after 20 steps; accelerations on
+    4,096 FP64 direct-sum targets (<= 1e-3 relative RMS at theta 0.5; the
+    theta 0.7 error is reported and bounded by 3e-3).
+C4  the 256-collision forest (N log-uniform in [4,096, 65,536]): for a sample
+    of instances the pipeline is bit-exact against the oracle, the forest
+    equals each instance run alone (accelerations and 3 steps, bit for bit),
+    and accelerations are within 1e-3 of FP64 direct sums.
+C5  a Poisson stream drawn from the served distribution (N log-uniform in
+    [8,192, 524,288], steps uniform in [200, 2,000]; 11 jobs, two above
+    300,000 particles): every job finishes, and sampled jobs give the same
+    final state alone as in the mix, bit for bit.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+G = 1.0
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def rel_rms(a, b):
+    return float(np.sqrt(((a - b) ** 2).sum() / (b ** 2).sum()))
+
+
+def frame_topology(orc, stars):
+    """Bounds, keys, stable sort and Karras topology of one instance's frame."""
+    x, y, z = (np.ascontiguousarray(stars[k], np.float32) for k in ("x", "y", "z"))
+    mn, mx = orc.bounds_cube(x, y, z)
+    keys = orc.morton_n(x, y, z, mn, mx)
+    skeys, perm = orc.stable_sort_pairs(keys)
+    left, right, parent = orc.karras_build(skeys)
+    return dict(mn=mn, mx=mx, keys=keys, skeys=skeys, perm=perm, left=left, right=right,
+                parent=parent)
+
+
+def check_instance(orc, e, iid, before, keys, theta, offset=False, aggregates=True):
+    """Instance `iid` of the engine's last frame against the oracle's frame of
+    `before` (its state before that frame): bounds, keys, permutation,
+    topology (and node aggregates) bit-exact.  `keys` = e.keys()."""
+    n, off = e.info(iid)
+    raw, srt, perm = (k[off:off + n] for k in keys)
+    if aggregates:
+        cpu = orc.build_tree_pipeline(before, theta, offset=offset)
+    else:
+        cpu = frame_topology(orc, before)
+    mn, mx = e.bounds(iid)
+    assert (mn, mx) == (np.float32(cpu["mn"]), np.float32(cpu["mx"])), iid
+    assert np.array_equal(raw, cpu["keys"]), iid
+    assert np.array_equal(srt, cpu["skeys"]), iid
+    assert np.array_equal(perm, cpu["perm"]), iid
+    t = e.tree(iid)
+    for k in ("left", "right", "parent"):
+        assert np.array_equal(t[k], cpu[k]), (iid, k)
+    if aggregates:
+        agg = cpu["agg"]
+        for k in ("com4", "width", "bmin", "bmax"):
+            assert np.array_equal(t[k].view(np.uint32), agg[k].view(np.uint32)), (iid, k)
+    return cpu
+
+
+def direct_error(orc, e, cpu, eps2, targets, acc):
+    s = cpu["sorted"]
+    n = len(s)
+    tgt = np.linspace(0, n - 1, min(targets, n)).astype(np.int64)
+    ref = orc.direct_sum_f64(s["x"], s["y"], s["z"], s["mass"], G, eps2, tgt)
+    return rel_rms(acc[tgt, :3], ref[:, :3])
+
+
+def test_c2_every_step(gcs, orc):
+    """C2: 2 x 131,072, eps 0.02, dt 0.005, 500 steps at the bench's settings."""
+    stars = gcs.collision(131072, 1002)
+    n = len(stars)
+    eps2 = float(np.float32(0.02) ** 2)
+    p = gcs.params(G=G, eps2=eps2, dt=0.005, theta=0.5, leaf_size=4, group_size=96, team=4,
+                   block_size=256, waves_per_simd=4)
+    with gcs.Engine(n) as e:
+        e.load(stars)
+        for step in range(501):
+            before = e.stars()
+            e.compute_accels(p)  # the frame of this step (re-sorts; same order as a step)
+            full = step % 125 == 0
+            cpu = check_instance(orc, e, 0, before, e.keys(), 0.5, aggregates=full)
+            if step in (0, 250, 500):
+                err = direct_error(orc, e, cpu, eps2, 4096, e.accels())
+                say("C2 step %d: rel rms vs FP64 direct sum %.3g" % (step, err))
+                assert err < 1e-3
+            if step % 50 == 0:
+                say("C2 step %d: keys, permutation, topology bit-exact%s"
+                    % (step, " (and aggregates)" if full else ""))
+            if step < 500:
+                e.step(p, 1)
+
+
+@pytest.mark.parametrize("theta,group", [(0.5, 64), (0.7, 96)])
+def test_c3_precision_form(gcs, orc, theta, group):
+    """C3: 2 x 1,048,576, eps 0.02, dt 0.005, compensated positions, the
+    autotuner's settings for this workload."""
+    stars = gcs.collision(1048576, 1003)
+    n = len(stars)
+    eps2 = float(np.float32(0.02) ** 2)
+    p = gcs.params(G=G, eps2=eps2, dt=0.005, theta=theta, leaf_size=4, group_size=group, team=1,
+                   block_size=64, waves_per_simd=4, precision=gcs.PREC_COMPENSATED)
+    with gcs.Engine(n) as e:
+        e.load(stars)
+        say("C3 theta %.1f: step 0" % theta)
+        e.compute_accels(p)
+        cpu = check_instance(orc, e, 0, stars, e.keys(), theta)
+        err = direct_error(orc, e, cpu, eps2, 4096, e.accels())
+        say("C3 theta %.1f step 0: pipeline bit-exact, rel rms vs FP64 direct sum %.3g"
+            % (theta, err))
+        assert err < (1e-3 if theta == 0.5 else 3e-3)
+        e.step(p, 20)
+        before = e.stars()
+        e.compute_accels(p)
+        check_instance(orc, e, 0, before, e.keys(), theta)
+        say("C3 theta %.1f step 20: pipeline bit-exact" % theta)
+
+
+def test_c4_forest(gcs, orc):
+    """C4: 256 collisions generated on the device, compensated positions, the
+    forest's autotuned settings."""
+    k = 256
+    specs = [gcs.random_spec(1004, i, 4096, 65536) for i in range(k)]
+    counts = np.array([s.n_a + s.n_b for s in specs])
+    eps2 = float(np.float32(0.02) ** 2)
+    p = gcs.params(G=G, eps2=eps2, dt=0.005, theta=0.5, leaf_size=4, group_size=128, team=1,
+                   block_size=256, waves_per_simd=4, precision=gcs.PREC_COMPENSATED)
+    rng = np.random.default_rng(4)
+    sample = sorted({int(np.argmin(counts)), int(np.argmax(counts))} |
+                    set(int(i) for i in rng.choice(k, 6, replace=False)))
+    with gcs.Engine(int(counts.sum()), k) as e:
+        say("C4: %d collisions, %d particles" % (k, counts.sum()))
+        e.load_generated(specs)
+        before = e.stars()
+        e.compute_accels(p)
+        keys = e.keys()
+        acc = e.accels()
+        offs = [e.info(i)[1] for i in range(k)]
+        cpus = {}
+        for i in sample:
+            sl = slice(offs[i], offs[i] + counts[i])
+            cpus[i] = check_instance(orc, e, i, before[sl], keys, 0.5)
+            err = direct_error(orc, e, cpus[i], eps2, 1024, acc[sl])
+            say("C4 instance %d (n=%d): pipeline bit-exact, rel rms vs FP64 %.3g"
+                % (i, counts[i], err))
+            assert err < 1e-3
+        e.step(p, 3)
+        after = {i: e.instance(i) for i in sample}
+    for i in sample:
+        sl = slice(offs[i], offs[i] + counts[i])
+        with gcs.Engine(int(counts[i])) as e1:
+            e1.load(before[sl])
+            e1.compute_accels(p)
+            assert np.array_equal(e1.accels().view(np.uint32), acc[sl].view(np.uint32)), i
+            e1.step(p, 3)
+            assert np.array_equal(e1.instance(0).view(np.uint8), after[i].view(np.uint8)), i
+        say("C4 instance %d: forest == alone (accelerations and 3 steps)" % i)
+
+
+def test_c5_served_stream(gcs):
+    """C5: 11 jobs from the served distribution arriving over 0.2 s (Poisson,
+    40 jobs/s), admitted under the HBM budget and continuously batched; the
+    largest and the smallest job alone give the same final state."""
+    jobs = gcs.poisson_jobs(5008, 40.0, 0.2, 8192, 524288, 200, 2000)
+    assert len(jobs) == 11 and sum(1 for j in jobs if j[1] > 300000) == 2
+    kw = dict(G=G, eps2=float(np.float32(0.02) ** 2), dt=0.005, theta=0.5, leaf_size=4,
+              group_size=128, team=1, block_size=64, waves_per_simd=4,
+              precision=gcs.PREC_COMPENSATED, use_graph=1)
+
+    def run(js):
+        with gcs.Server(hbm_budget_bytes=16 << 30, keep_final=True, steps_per_round=25,
+                        gen_threads=0, **kw) as s:
+            s.submit(js, 5008)
+            s.run(0.0)
+            res = s.results()
+            return res, {r["id"]: s.final(r["id"], r["n"]) for r in res}, s.stats()
+
+    say("C5: %d jobs, %.3g particle-steps" % (len(jobs), sum(n * k for _, n, k, _ in jobs)))
+    res, finals, st = run(jobs)
+    assert sorted(r["id"] for r in res) == [j[0] for j in jobs]
+    assert st["particle_steps"] == sum(n * k for _, n, k, _ in jobs)
+    say("C5: all jobs finished, %.3g particle-steps/s" % (st["particle_steps"] / st["run_s"]))
+    big = max(jobs, key=lambda j: j[1])
+    small = min(jobs, key=lambda j: j[1])
+    for jid, n, steps, _ in (big, small):
+        say("C5 job %d (n=%d, %d steps) alone" % (jid, n, steps))
+        _, alone, _ = run([(jid, n, steps, 0.0)])
+        a, b = alone[jid], finals[jid]
+        assert np.array_equal(a.view(np.uint8), b.view(np.uint8)), jid

@@ -416,6 +416,78 @@ def test_reference_traversal_partition(orc):
     assert rel_rms(acc[:, :3], ref[:, :3]) < 0.1  # theta = 1 monopole
 
 
+def reference_opening_case(d2_over):
+    """Particles of mass 3 and 1 at x = 0.1 and 2.1 form a node of width 2
+    whose COM is x = 0.6; a light target sits at distance D on the other side
+    (D^2 = 3 * d2_over).  In the cube [-4, 4] the target's Morton key differs
+    from the pair's in the top x bit, so the tree is {target, {pair}} and the
+    root (width ~3.2) is opened for the target.  With eps^2 = 1 the reference
+    takes the pair node whole iff width^2 < D^2 + eps^2, i.e. D^2 > 3
+    (cuda_engine.cu:131,143)."""
+    D = np.sqrt(3.0 * d2_over)
+    st = np.zeros(3, STAR_DT)
+    st["x"] = np.array([0.1, 2.1, 0.6 - D], np.float32)
+    st["mass"] = np.array([3.0, 1.0, 1e-3], np.float32)
+    return st, D, (-4.0, 4.0), 1.0
+
+
+def reference_skip_case(s_over):
+    """Two unit masses a distance s = 1e-3 * s_over apart straddle x = 0.5,
+    the top x bit of the cube [0, 1], and a third sits at x = 0.52: the tree
+    is {P1, {P2, P3}}.  With eps^2 = 1e-4 both internal nodes are opened for
+    P1 (widths^2 4.2e-4 and 3.8e-4 exceed d^2 + eps^2), so P1 reaches the leaf
+    P2, which the reference takes iff d^2 + eps^2 > 1.01 eps^2, i.e. s > 0.1 eps
+    = 1e-3 (cuda_engine.cu:133-134)."""
+    s = 1e-3 * s_over
+    st = np.zeros(3, STAR_DT)
+    st["x"] = np.array([0.5 - s / 2, 0.5 + s / 2, 0.52], np.float32)
+    st["y"] = st["z"] = np.float32(0.3)
+    st["mass"] = 1.0
+    return st, (0.0, 1.0), 1e-4
+
+
+STAR_DT = np.dtype([("x", "<f4"), ("y", "<f4"), ("z", "<f4"), ("vx", "<f4"), ("vy", "<f4"),
+                    ("vz", "<f4"), ("mass", "<f4"), ("type", "u1"), ("r", "u1"), ("g", "u1"),
+                    ("b", "u1")])
+
+
+def monopole_vs_pair(D, eps2):
+    """x-acceleration of the target of reference_opening_case from the pair
+    taken whole (mass 4 at distance D) and from its two particles."""
+    whole = 4.0 * D / (D * D + eps2) ** 1.5
+    a, b = D - 0.5, D + 1.5
+    parts = 3.0 * a / (a * a + eps2) ** 1.5 + 1.0 * b / (b * b + eps2) ** 1.5
+    return whole, parts
+
+
+@pytest.mark.parametrize("d2_over,taken", [(0.99, False), (1.01, True)])
+def test_reference_node_opening_threshold(orc, d2_over, taken):
+    st, D, (mn, mx), eps2 = reference_opening_case(d2_over)
+    t = orc.build_tree_pipeline(st, 1.0, mn, mx)
+    s = t["sorted"]
+    assert list(t["perm"]) == [2, 0, 1] and t["left"][0] == 2  # {target, {pair}}
+    acc, _, cnt = orc.traverse_reference(t["left"], t["right"], s["x"], s["y"], s["z"],
+                                         t["agg"]["com4"], t["agg"]["width"], G, eps2)
+    whole, parts = monopole_vs_pair(D, eps2)
+    want, other = (whole, parts) if taken else (parts, whole)
+    assert abs(whole - parts) > 0.1 * abs(whole)
+    assert np.isclose(acc[0, 0], want, rtol=1e-5), (acc[0, 0], whole, parts)
+
+
+@pytest.mark.parametrize("s_over,skipped", [(0.98, True), (1.02, False)])
+def test_reference_leaf_skip_threshold(orc, s_over, skipped):
+    st, (mn, mx), eps2 = reference_skip_case(s_over)
+    t = orc.build_tree_pipeline(st, 1.0, mn, mx)
+    s = t["sorted"]
+    assert list(t["perm"]) == [0, 1, 2] and t["left"][0] == 2 and t["right"][0] == 1
+    acc, (msum, mskip), cnt = orc.traverse_reference(t["left"], t["right"], s["x"], s["y"],
+                                                     s["z"], t["agg"]["com4"],
+                                                     t["agg"]["width"], G, eps2)
+    # P1 skips itself, and P2 iff it lies within 0.1 eps.
+    assert mskip[0] == (2.0 if skipped else 1.0)
+    assert msum[0] == (1.0 if skipped else 2.0)
+
+
 # ---- Integrate (cuda_engine.cu:164-216) -----------------------------------------
 
 def test_integrate_constant_acceleration_closed_form(orc):
