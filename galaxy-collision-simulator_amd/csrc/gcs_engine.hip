@@ -11,6 +11,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -1602,3 +1603,146 @@ int gcs_karras_tree(const uint64_t* sorted_keys, int32_t n, int32_t* left, int32
 }
 
 }  // extern "C"
+
+// ============================================================================
+// Allocation-free sort (gcs_sort_pairs_noalloc)
+// ============================================================================
+// The sort's working memory -- the second key/payload buffer of the
+// ping-pong, the per-tile and per-chunk digit histograms and the tile table --
+// sits in one file-scope device array, so a call allocates nothing.  Its
+// name is a host-side stand-in: host code uses only the address
+// hipGetSymbolAddress returns for the current device (kept per device).
+// Every call on a GPU shares that GPU's array: calls that overlap on one GPU
+// (two streams, two threads) race on it.
+namespace {
+
+constexpr int64_t kNaMaxKeys = GCS_NOALLOC_MAX_KEYS;
+constexpr int kNaMaxTiles = (int)(kNaMaxKeys / kTile);
+constexpr int kNaMaxChunks = (kNaMaxTiles + kChunkTiles - 1) / kChunkTiles;
+constexpr size_t na_align(size_t b) { return (b + 255) / 256 * 256; }
+constexpr size_t kNaKeys = 0;
+constexpr size_t kNaVals = kNaKeys + na_align(sizeof(uint64_t) * kNaMaxKeys);
+constexpr size_t kNaHist = kNaVals + na_align(sizeof(uint32_t) * kNaMaxKeys);
+constexpr size_t kNaChist = kNaHist + na_align(sizeof(uint32_t) * kRadix * kNaMaxTiles);
+constexpr size_t kNaTiles =
+    kNaChist + na_align(sizeof(uint32_t) * kSortPasses * kRadix * kNaMaxChunks);
+constexpr size_t kNaSegs = kNaTiles + na_align(sizeof(TileInfo) * kNaMaxTiles);
+constexpr size_t kNaBytes = kNaSegs + na_align(sizeof(SegInfo));
+static_assert(kNaMaxKeys % kTile == 0, "whole tiles");
+
+}  // namespace
+
+// 53 MiB per device, held until the process ends.
+static __device__ __align__(256) uint8_t scratch[kNaBytes];
+
+namespace {
+
+constexpr int kNaMaxDevices = 64;
+void* g_scratch_dev[kNaMaxDevices] = {};
+std::mutex g_scratch_mu;
+
+// The scratch array's device address on the current device (looked up once
+// per device, with that device current).
+int scratch_address(uint8_t** out)
+{
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kNaMaxDevices) return fail(GCS_ERR_NO_DEVICE, "device ordinal out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(GCS_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") +
+                                         hipGetErrorString(e));
+        g_scratch_dev[dev] = p;
+    }
+    *out = static_cast<uint8_t*>(g_scratch_dev[dev]);
+    return GCS_OK;
+}
+
+// Set-up of one sort of n keys: the tile table and the instance record of a
+// single instance, the chunk histograms of all passes cleared, and the
+// payload 0 .. n-1 written into perm (the first pass's payload input).  A
+// kernel rather than hipMemsetAsync, so the call can be captured into a graph.
+__global__ __launch_bounds__(256) void k_noalloc_setup(int n, int ntiles, int chist_words,
+                                                       TileInfo* __restrict__ tiles,
+                                                       SegInfo* __restrict__ segs,
+                                                       uint32_t* __restrict__ chist,
+                                                       uint32_t* __restrict__ perm)
+{
+    const int tid = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    const int stride = (int)(gridDim.x * blockDim.x);
+    for (int i = tid; i < n; i += stride) perm[i] = (uint32_t)i;
+    for (int k = tid; k < chist_words; k += stride) chist[k] = 0u;
+    for (int t = tid; t < ntiles; t += stride) {
+        TileInfo ti;
+        ti.start = t * kTile;
+        ti.count = min(kTile, n - t * kTile);
+        ti.seg = 0;
+        ti.tile_in_seg = t;
+        ti.pad0 = ti.pad1 = ti.pad2 = ti.pad3 = 0;
+        tiles[t] = ti;
+    }
+    if (tid == 0) {
+        SegInfo sg;
+        sg.start = 0;
+        sg.n = n;
+        sg.int_base = 0;
+        sg.leaf_base = n - 1;
+        sg.root = 0;
+        sg.tile0 = 0;
+        sg.ntiles = ntiles;
+        sg.chunk0 = 0;
+        segs[0] = sg;
+    }
+}
+
+}  // namespace
+
+extern "C" int gcs_sort_pairs_noalloc(const uint64_t* keys, int64_t n, uint64_t* keys_out,
+                                      int32_t* perm_out, void* hip_stream)
+{
+    if (n < 0) return fail(GCS_ERR_INVALID, "n < 0");
+    if (n > kNaMaxKeys)
+        return fail(GCS_ERR_CAPACITY, "gcs_sort_pairs_noalloc sorts at most " +
+                                          std::to_string(kNaMaxKeys) + " keys");
+    if (n == 0) return GCS_OK;
+    if (!keys || !keys_out || !perm_out) return fail(GCS_ERR_INVALID, "null argument");
+    uint8_t* base = nullptr;
+    int rc = scratch_address(&base);
+    if (rc) return rc;
+    hipStream_t st = static_cast<hipStream_t>(hip_stream);
+    const int nk = (int)n;
+    const int ntiles = (nk + kTile - 1) / kTile;
+    const int nchunks = (ntiles + kChunkTiles - 1) / kChunkTiles;
+    const size_t cstride = (size_t)nchunks * kRadix;
+    uint64_t* sk = reinterpret_cast<uint64_t*>(base + kNaKeys);
+    uint32_t* sv = reinterpret_cast<uint32_t*>(base + kNaVals);
+    uint32_t* hist = reinterpret_cast<uint32_t*>(base + kNaHist);
+    uint32_t* chist = reinterpret_cast<uint32_t*>(base + kNaChist);
+    TileInfo* tiles = reinterpret_cast<TileInfo*>(base + kNaTiles);
+    SegInfo* segs = reinterpret_cast<SegInfo*>(base + kNaSegs);
+    uint32_t* pv = reinterpret_cast<uint32_t*>(perm_out);
+    k_noalloc_setup<<<std::min(1024, (nk + 255) / 256), 256, 0, st>>>(
+        nk, ntiles, (int)(kSortPasses * cstride), tiles, segs, chist, pv);
+    LAUNCHCHK();
+    // Even passes read the caller's buffers (the input keys first, then
+    // keys_out) and write the scratch pair; odd passes write keys_out and
+    // perm_out, so the eighth pass leaves the result there.
+    static_assert(kSortPasses % 2 == 0, "the last pass writes the caller's buffers");
+    for (int pass = 0; pass < kSortPasses; ++pass) {
+        const int shift = pass * kRadixBits;
+        const bool even = (pass & 1) == 0;
+        const uint64_t* kin = even ? (pass == 0 ? keys : keys_out) : sk;
+        const uint32_t* vin = even ? pv : sv;
+        uint64_t* kdst = even ? sk : keys_out;
+        uint32_t* vdst = even ? sv : pv;
+        uint32_t* ch = chist + (size_t)pass * cstride;
+        k_sort_count<<<ntiles, kTileThreads, 0, st>>>(tiles, segs, kin, hist, ch, shift);
+        k_sort_scatter<<<ntiles, kTileThreads, 0, st>>>(tiles, segs, kin, vin, kdst, vdst, hist,
+                                                         ch, shift);
+    }
+    LAUNCHCHK();
+    return GCS_OK;
+}

@@ -29,6 +29,8 @@ ERR_STACK = -6
 ERR_NONFINITE = -7
 ERR_NOT_FOUND = -8
 
+NOALLOC_MAX_KEYS = 1 << 22  # GCS_NOALLOC_MAX_KEYS
+
 MAC_REFERENCE = 0
 MAC_GROUP = 1
 PREC_PLAIN = 0
@@ -116,6 +118,7 @@ SIGNATURES = {
     "gcs_morton_keys": ([_P, _P, _P, _I64, _F, _F, _P], _I32),
     "gcs_sort_pairs": ([_P, _I64, _P, _P], _I32),
     "gcs_karras_tree": ([_P, _I32, _P, _P, _P], _I32),
+    "gcs_sort_pairs_noalloc": ([_P, _I64, _P, _P, _P], _I32),
     "gcs_spec_default": ([_I32, ctypes.c_uint64, _P], None),
     "gcs_spec_random": ([ctypes.c_uint64, _I64, _I32, _I32, _P], None),
     "gcs_generate": ([_P, _P], _I32),

@@ -240,6 +240,22 @@ int gcs_sort_pairs(const uint64_t* keys, int64_t n, uint64_t* keys_out,
 int gcs_karras_tree(const uint64_t* sorted_keys, int32_t n, int32_t* left,
                     int32_t* right, int32_t* parent);
 
+/* Allocation-free variant of gcs_sort_pairs.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * The same stable ascending sort of n 64-bit keys with the payload 0..n-1,
+ * on device buffers of the current device: keys (n), keys_out (n; may equal
+ * keys), perm_out (n int32, must not overlap the others).  Queued on
+ * hip_stream (a hipStream_t, NULL = the null stream) and returns at once;
+ * results are ready when that stream is.  No device or host memory is
+ * allocated: the working memory is a static device array of 53 MiB per GPU,
+ * held until the process ends, shared by every call on that GPU -- calls
+ * that overlap on one GPU race.  n above GCS_NOALLOC_MAX_KEYS returns
+ * GCS_ERR_CAPACITY and queues nothing. */
+enum { GCS_NOALLOC_MAX_KEYS = 1 << 22 };
+int gcs_sort_pairs_noalloc(const uint64_t* keys, int64_t n, uint64_t* keys_out,
+                           int32_t* perm_out, void* hip_stream);
+
 /* ---- Initial conditions --------------------------------------------------
  * Two-galaxy collision: per galaxy a Miyamoto-Nagai disc (a=3, b=0.3,
  * M=1), a Hernquist bulge (a=0.5, M=0.25) and a Hernquist dark halo (a=12,
