@@ -11,9 +11,11 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <mutex>
 #include <new>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "gcs.h"
@@ -644,6 +646,58 @@ int capture_graphs(gcs_engine* e, const gcs_params& p)
     return GCS_OK;
 }
 
+// Autotuner results per (traversal kind, theta, opening form, precision,
+// log2 particle bucket, log2 instance bucket), shared by every engine in the
+// process and readable/writable as a text table.
+struct TuneKey {
+    int mac, open_offset, precision, nbucket, kbucket;
+    uint32_t theta_bits;
+    bool operator<(const TuneKey& o) const
+    {
+        return std::tie(mac, open_offset, precision, nbucket, kbucket, theta_bits) <
+               std::tie(o.mac, o.open_offset, o.precision, o.nbucket, o.kbucket, o.theta_bits);
+    }
+};
+struct TuneVal {
+    int group, leaf, block, team, wps, stack;
+    float ms;
+};
+std::map<TuneKey, TuneVal> g_tune;
+std::mutex g_tune_mu;
+
+int log2_bucket(int64_t v)
+{
+    int b = 0;
+    while (v > 1) {
+        v >>= 1;
+        ++b;
+    }
+    return b;
+}
+
+TuneKey tune_key(const gcs_engine* e, const gcs_params& p)
+{
+    TuneKey k;
+    k.mac = p.mac;
+    k.open_offset = p.mac == GCS_MAC_GROUP ? p.open_offset : 0;
+    k.precision = p.precision;
+    k.nbucket = log2_bucket(e->n_total);
+    k.kbucket = log2_bucket((int64_t)e->inst.size());
+    float th = p.mac == GCS_MAC_GROUP ? p.theta : 0.0f;
+    std::memcpy(&k.theta_bits, &th, 4);
+    return k;
+}
+
+void apply_tune(const TuneVal& v, gcs_params* q)
+{
+    q->group_size = v.group;
+    q->leaf_size = v.leaf;
+    q->block_size = v.block;
+    q->team = v.team;
+    q->waves_per_simd = v.wps;
+    q->stack_depth = v.stack;
+}
+
 int find_slot(gcs_engine* e, int32_t id)
 {
     for (size_t s = 0; s < e->inst.size(); ++s)
@@ -1229,6 +1283,16 @@ int gcs_autotune(gcs_engine* e, const gcs_params* base, int32_t reps, gcs_params
     *best = *base;
     if (best_ms) *best_ms = 0.0f;
     if (e->n_total == 0) return GCS_OK;
+    const TuneKey key = tune_key(e, *base);
+    {
+        std::lock_guard<std::mutex> lock(g_tune_mu);
+        auto it = g_tune.find(key);
+        if (it != g_tune.end()) {
+            apply_tune(it->second, best);
+            if (best_ms) *best_ms = it->second.ms;
+            return check_params(best);
+        }
+    }
     HIPCHK(hipSetDevice(e->device));
     rc = gcs_sync(e);
     if (rc) return rc;
@@ -1293,6 +1357,72 @@ int gcs_autotune(gcs_engine* e, const gcs_params* base, int32_t reps, gcs_params
     }
     if (!found) return fail(GCS_ERR_STACK, "every candidate overflowed its traversal stack");
     if (best_ms) *best_ms = best_t;
+    {
+        std::lock_guard<std::mutex> lock(g_tune_mu);
+        g_tune[key] = TuneVal{best->group_size, best->leaf_size, best->block_size, best->team,
+                              best->waves_per_simd, best->stack_depth, best_t};
+    }
+    return GCS_OK;
+}
+
+int gcs_autotune_cache_clear(void)
+{
+    std::lock_guard<std::mutex> lock(g_tune_mu);
+    g_tune.clear();
+    return GCS_OK;
+}
+
+int gcs_autotune_cache_save(const char* path)
+{
+    if (!path) return fail(GCS_ERR_INVALID, "null path");
+    std::lock_guard<std::mutex> lock(g_tune_mu);
+    FILE* f = std::fopen(path, "w");
+    if (!f) return fail(GCS_ERR_INVALID, std::string("cannot write ") + path);
+    std::fprintf(f, "gcs-autotune 1\n");
+    std::fprintf(f, "# mac open_offset precision log2_n log2_k theta_bits group leaf block team "
+                    "waves_per_simd stack_depth ms\n");
+    for (const auto& kv : g_tune) {
+        const TuneKey& k = kv.first;
+        const TuneVal& v = kv.second;
+        std::fprintf(f, "%d %d %d %d %d %08x %d %d %d %d %d %d %.6g\n", k.mac, k.open_offset,
+                     k.precision, k.nbucket, k.kbucket, k.theta_bits, v.group, v.leaf, v.block,
+                     v.team, v.wps, v.stack, (double)v.ms);
+    }
+    const bool ok = std::fclose(f) == 0;
+    return ok ? GCS_OK : fail(GCS_ERR_INVALID, std::string("cannot write ") + path);
+}
+
+int gcs_autotune_cache_load(const char* path)
+{
+    if (!path) return fail(GCS_ERR_INVALID, "null path");
+    FILE* f = std::fopen(path, "r");
+    if (!f) return fail(GCS_ERR_NOT_FOUND, std::string("cannot read ") + path);
+    char line[512];
+    std::map<TuneKey, TuneVal> got;
+    bool header = false, bad = false;
+    while (std::fgets(line, sizeof(line), f)) {
+        if (!header) {
+            header = std::strncmp(line, "gcs-autotune 1", 14) == 0;
+            if (!header) { bad = true; break; }
+            continue;
+        }
+        if (line[0] == '#' || line[0] == '\n') continue;
+        TuneKey k;
+        TuneVal v;
+        double ms = 0.0;
+        if (std::sscanf(line, "%d %d %d %d %d %x %d %d %d %d %d %d %lf", &k.mac, &k.open_offset,
+                        &k.precision, &k.nbucket, &k.kbucket, &k.theta_bits, &v.group, &v.leaf,
+                        &v.block, &v.team, &v.wps, &v.stack, &ms) != 13) {
+            bad = true;
+            break;
+        }
+        v.ms = (float)ms;
+        got[k] = v;
+    }
+    std::fclose(f);
+    if (bad || !header) return fail(GCS_ERR_INVALID, std::string("not an autotune table: ") + path);
+    std::lock_guard<std::mutex> lock(g_tune_mu);
+    for (const auto& kv : got) g_tune[kv.first] = kv.second;
     return GCS_OK;
 }
 

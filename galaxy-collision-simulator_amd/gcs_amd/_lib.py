@@ -125,6 +125,9 @@ SIGNATURES = {
     "gcs_load_generated": ([_P, _P, _I32], _I32),
     "gcs_instance_add_generated": ([_P, _P, _P], _I32),
     "gcs_autotune": ([_P, _P, _I32, _P, _P], _I32),
+    "gcs_autotune_cache_clear": ([], _I32),
+    "gcs_autotune_cache_save": ([ctypes.c_char_p], _I32),
+    "gcs_autotune_cache_load": ([ctypes.c_char_p], _I32),
     "gcs_read_max_accel": ([_P, _I32, _P], _I32),
     "gcs_read_sim_time": ([_P, _I32, _P], _I32),
     "gcs_checkpoint_bytes": ([_I32], ctypes.c_int64),

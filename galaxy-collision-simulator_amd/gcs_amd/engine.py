@@ -243,6 +243,18 @@ class Engine:
         return out
 
 
+def autotune_cache_clear():
+    L.check(L.lib().gcs_autotune_cache_clear())
+
+
+def autotune_cache_save(path):
+    L.check(L.lib().gcs_autotune_cache_save(path.encode()))
+
+
+def autotune_cache_load(path):
+    L.check(L.lib().gcs_autotune_cache_load(path.encode()))
+
+
 def morton_keys(x, y, z, mn, mx):
     x, y, z = (np.ascontiguousarray(a, np.float32) for a in (x, y, z))
     out = np.zeros(len(x), np.uint64)

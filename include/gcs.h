@@ -178,6 +178,14 @@ int gcs_compute_accels(gcs_engine* e, const gcs_params* p);
  * timed passes per candidate after one warm-up pass, 1 <= reps <= 100. */
 int gcs_autotune(gcs_engine* e, const gcs_params* base, int32_t reps,
                  gcs_params* best, float* best_ms);
+/* The autotuner keeps each result per (traversal kind, theta, opening form,
+ * precision, floor(log2 particles), floor(log2 instances)), process-wide:
+ * gcs_autotune returns a kept result without timing anything (its best_ms is
+ * the time measured when it was found).  _clear forgets them; _save writes
+ * them as a text table, _load adds a saved table (an earlier run's tuning). */
+int gcs_autotune_cache_clear(void);
+int gcs_autotune_cache_save(const char* path);
+int gcs_autotune_cache_load(const char* path);
 /* Host n_total*4 floats: ax, ay, az, potential (potential only for
  * GCS_MAC_GROUP), in current particle order. */
 int gcs_read_accels(gcs_engine* e, float* out);

@@ -117,3 +117,24 @@ def test_checkpoint_size(gcs):
 def test_bake_driver_built(gcs):
     path = os.path.join(os.path.dirname(gcs.LIB_PATH), "gcs_bake")
     assert os.access(path, os.X_OK)
+
+
+def test_autotune_table_save_and_load(gcs, tmp_path):
+    """The autotuner's table round-trips through its text form (host only)."""
+    path = str(tmp_path / "tune.txt")
+    gcs.autotune_cache_clear()
+    gcs.autotune_cache_save(path)
+    assert open(path).read().startswith("gcs-autotune 1\n")
+    row = "1 0 1 22 8 3f000000 128 4 256 1 4 256 7.5\n"
+    with open(path, "a") as f:
+        f.write(row)
+    gcs.autotune_cache_load(path)
+    path2 = str(tmp_path / "tune2.txt")
+    gcs.autotune_cache_save(path2)
+    assert row in open(path2).read()
+    bad = tmp_path / "bad.txt"
+    bad.write_text("not a table\n")
+    with pytest.raises(gcs.GcsError) as ex:
+        gcs.autotune_cache_load(str(bad))
+    assert ex.value.code == gcs.ERR_INVALID
+    gcs.autotune_cache_clear()

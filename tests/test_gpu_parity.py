@@ -430,6 +430,7 @@ def test_stack_overflow_is_reported(gcs):
 
 
 def test_autotune_picks_a_candidate_and_keeps_the_state(gcs):
+    gcs.autotune_cache_clear()
     stars = gcs.collision(8192, 44)
     p = gcs.params(G=G, eps2=0.0025, dt=0.01, use_graph=1)
     with gcs.Engine(len(stars)) as e:
@@ -441,6 +442,13 @@ def test_autotune_picks_a_candidate_and_keeps_the_state(gcs):
         assert best.waves_per_simd in (2, 3, 4) and best.use_graph == 1
         assert best.group_size in (48, 64, 96, 128)
         assert best.theta == p.theta and best.eps2 == p.eps2
+        # Kept per (kind, theta, form, precision, N bucket, k bucket): the
+        # same answer again without timing.
+        again, ms2 = e.autotune(p, reps=2)
+        assert (again.group_size, again.leaf_size, again.block_size, again.team,
+                again.waves_per_simd, ms2) == (best.group_size, best.leaf_size,
+                                               best.block_size, best.team,
+                                               best.waves_per_simd, ms)
         # positions unchanged (re-sorted only)
         after = np.sort(e.stars(), order=["x", "y", "z"])
         before = np.sort(stars, order=["x", "y", "z"])
