@@ -183,7 +183,7 @@ def base_params(**kw):
 # ---------------------------------------------------------------------------
 
 def batched_leg(n_inst, steps, warmup, seed=1004, check_instances=8, check_targets=512,
-                vs_plain=True):
+                vs_plain=True, eager=False):
     import numpy as np
     import gcs_amd
 
@@ -193,6 +193,8 @@ def batched_leg(n_inst, steps, warmup, seed=1004, check_instances=8, check_targe
     e = gcs_amd.Engine(total, n_inst)
     e.load_generated(specs)  # initial conditions generated on the device
     p, tune_ms = e.autotune(base_params(precision=gcs_amd.PREC_COMPENSATED), reps=2)
+    if eager:
+        p.use_graph = 0
     say("batched: %d collisions, %d particles, settings %s" % (n_inst, total, settings(p)))
     # Accuracy of the first frame on a sample of instances (CPU checker).
     e.compute_accels(p)
@@ -572,6 +574,9 @@ def main():
                     help="collisions of the batched headline")
     ap.add_argument("--vs-plain", type=int, default=1,
                     help="1: run the headline's collisions one at a time too (0 = skip)")
+    ap.add_argument("--eager", action="store_true",
+                    help="launch the headline's steps one by one instead of replaying a "
+                         "hipGraph (the same kernels; for profilers)")
     ap.add_argument("--served", type=float, default=30.0,
                     help="seconds of each served leg (0 = skip)")
     ap.add_argument("--served-rate", type=float, default=40.0)
@@ -607,7 +612,7 @@ def main():
     if not a.retune and os.path.exists(TUNE_TABLE):
         gcs_amd.autotune_cache_load(TUNE_TABLE)
 
-    b = batched_leg(a.batched, a.steps, a.warmup, vs_plain=bool(a.vs_plain))
+    b = batched_leg(a.batched, a.steps, a.warmup, vs_plain=bool(a.vs_plain), eager=a.eager)
     line = {
         "metric": METRIC,
         "value": b["value"],

@@ -295,6 +295,8 @@ int check_params(const gcs_params* p)
             return fail(GCS_ERR_INVALID, "team must be 0, 1, 2, 4 or 8");
         if (p->team > 1 && p->block_size != p->team * kWave)
             return fail(GCS_ERR_INVALID, "a team of w waves takes block_size 64 w");
+        if (!std::isfinite(p->mono_far) || (p->mono_far != 0.0f && p->mono_far < 1.0f))
+            return fail(GCS_ERR_INVALID, "mono_far must be 0 (off) or >= 1");
     } else {
         if (p->stack_depth < 2 || p->stack_depth > 1024)
             return fail(GCS_ERR_INVALID, "stack_depth must be in [2, 1024]");
@@ -319,12 +321,20 @@ int group_waves(const gcs_engine* e, const gcs_params& p)
     return std::max(1, std::min(waves / wpb, (int)((e->n_total + wpb - 1) / wpb))) * wpb;
 }
 
+// Squared monopole radius factor of the group traversal: an accepted node
+// farther than mono_far times its opening radius is taken as a monopole
+// (+inf: every accepted node is a quadrupole).
+float far2_of(const gcs_params& p)
+{
+    return p.mono_far > 0.0f ? p.mono_far * p.mono_far : __builtin_inff();
+}
+
 template <bool POT, bool QUAD, int WPB>
 void launch_group(gcs_engine* e, int blocks, const gcs_params& p, float4* posm, bool order)
 {
     k_traverse_group<POT, QUAD, WPB><<<blocks, WPB * kWave, 0, e->stream>>>(
         e->groups, order ? e->gorder : nullptr, e->ngroups, e->segs, posm, e->node, e->nodeQ, e->acc,
-        e->slot_mem, e->qctr, e->counters, e->err, p.G, p.eps2, p.leaf_size);
+        e->slot_mem, e->qctr, e->counters, e->err, p.G, p.eps2, p.leaf_size, far2_of(p));
 }
 
 template <bool POT, int TEAM>
@@ -332,7 +342,7 @@ void launch_team(gcs_engine* e, int blocks, const gcs_params& p, float4* posm, b
 {
     k_traverse_team<POT, true, TEAM><<<blocks, TEAM * kWave, 0, e->stream>>>(
         e->groups, order ? e->gorder : nullptr, e->ngroups, e->segs, posm, e->node, e->nodeQ,
-        e->acc, e->slot_mem, e->qctr, e->counters, e->err, p.G, p.eps2, p.leaf_size);
+        e->acc, e->slot_mem, e->qctr, e->counters, e->err, p.G, p.eps2, p.leaf_size, far2_of(p));
 }
 
 template <int STACK>
@@ -845,7 +855,7 @@ int gcs_create(int64_t cap_n, int32_t cap_k, int32_t device, gcs_engine** out)
     if (!rc) rc = dalloc(&e->seg_bounds, 2 * (size_t)cap_k);
     if (!rc) rc = dalloc(&e->bpart, 8 * nt);
     if (!rc) rc = dalloc(&e->qctr, kNumXcd + 1);  // traversal queues; aggregation items
-    if (!rc) rc = dalloc(&e->counters, 2);
+    if (!rc) rc = dalloc(&e->counters, 3);
     if (!rc) rc = dalloc(&e->err, 1);
     if (!rc) rc = dalloc(&e->diag_part, 5 * 4 * nt);
     if (!rc) rc = dalloc(&e->diag_out, 5 * (size_t)cap_k);
@@ -877,7 +887,7 @@ int gcs_create(int64_t cap_n, int32_t cap_k, int32_t device, gcs_engine** out)
             return fail(GCS_ERR_HIP, "hipEventCreate failed");
         }
     if (hipMemset(e->err, 0, sizeof(unsigned)) != hipSuccess ||
-        hipMemset(e->counters, 0, 2 * sizeof(unsigned long long)) != hipSuccess) {
+        hipMemset(e->counters, 0, 3 * sizeof(unsigned long long)) != hipSuccess) {
         (void)hipStreamDestroy(e->stream);
         free_all(e);
         delete e;
@@ -1540,11 +1550,23 @@ int gcs_read_bounds(gcs_engine* e, int32_t id, float* mn, float* mx)
 int gcs_read_interactions(gcs_engine* e, int64_t* out2)
 {
     if (!e || !out2) return fail(GCS_ERR_INVALID, "null argument");
-    unsigned long long c[2];
+    unsigned long long c[3];
+    HIPCHK(hipSetDevice(e->device));
     HIPCHK(hipStreamSynchronize(e->stream));
     HIPCHK(hipMemcpy(c, e->counters, sizeof(c), hipMemcpyDeviceToHost));
     out2[0] = (int64_t)c[0];
     out2[1] = (int64_t)c[1];
+    return GCS_OK;
+}
+
+int gcs_read_monopole_interactions(gcs_engine* e, int64_t* out)
+{
+    if (!e || !out) return fail(GCS_ERR_INVALID, "null argument");
+    unsigned long long c = 0;
+    HIPCHK(hipSetDevice(e->device));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    HIPCHK(hipMemcpy(&c, e->counters + 2, sizeof(c), hipMemcpyDeviceToHost));
+    *out = (int64_t)c;
     return GCS_OK;
 }
 

@@ -150,7 +150,7 @@ __global__ __launch_bounds__(256) void k_morton(const TileInfo* __restrict__ til
             }
         }
         if (tid <= kNumXcd) rs.qctr[tid] = 0u;
-        if (tid < 2) rs.counters[tid] = 0ull;
+        if (tid < 3) rs.counters[tid] = 0ull;
         for (int64_t k = tid; k < rs.chist_words; k += stride) rs.chist[k] = 0u;
         if (threadIdx.x == 0) {
             rs.gm.perq[blockIdx.x] = 0u;

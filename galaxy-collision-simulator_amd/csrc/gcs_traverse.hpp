@@ -229,25 +229,37 @@ __device__ __forceinline__ void eval_node_list(WaveLds& L, const int* nl, int k,
     }
 }
 
+// A particle-list entry: a particle index p >= 0 (its position and mass), or
+// ~v < 0 for a node v taken as a monopole (its COM and mass, the same float4
+// layout); kNoEntry past the list end (a zero-mass source).
+constexpr int kNoEntry = (int)0x80000000;
+
+__device__ __forceinline__ float4 point_source(int e, const float4* __restrict__ posm,
+                                               const float4* __restrict__ node)
+{
+    if (e == kNoEntry) return make_float4(0.f, 0.f, 0.f, 0.f);
+    return e >= 0 ? posm[e] : node[2 * (~e)];
+}
+
 // Phase 2 over the particle list pl[0..k), same chunks, pipeline and split.
+// Entries are particles or far nodes taken as monopoles (point_source).
 template <bool POT, int P>
 __device__ __forceinline__ void eval_part_list(WaveLds& L, const int* pl, int k,
-                                               const float4* __restrict__ posm, Acc* a,
+                                               const float4* __restrict__ posm,
+                                               const float4* __restrict__ node, Acc* a,
                                                const Targets& tg, float eps2, int S, int h,
                                                int first, int step)
 {
     const int l = lane_id();
-    float4 X = make_float4(0.f, 0.f, 0.f, 0.f);
-    const int idx = first + l < k ? pl[first + l] : -1;
-    int idx_next = first + step + l < k ? pl[first + step + l] : -1;
-    if (idx >= 0) X = posm[idx];
+    const int idx = first + l < k ? pl[first + l] : kNoEntry;
+    int idx_next = first + step + l < k ? pl[first + step + l] : kNoEntry;
+    float4 X = point_source(idx, posm, node);
     for (int base = first; base < k; base += step) {
         L.ca[l] = X;
         __builtin_amdgcn_wave_barrier();
-        X = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (idx_next >= 0) X = posm[idx_next];
+        X = point_source(idx_next, posm, node);
         const int j2 = base + 2 * step + l;
-        idx_next = j2 < k ? pl[j2] : -1;
+        idx_next = j2 < k ? pl[j2] : kNoEntry;
         const int trips = uniform((min(kWave, k - base) + S - 1) / S);
         int t = 0;
         // About four interactions in flight per iteration.
@@ -290,12 +302,16 @@ __device__ __forceinline__ void eval_nodes(WaveLds& L, const int* nl, int k, con
 
 template <bool POT>
 __device__ __forceinline__ void eval_parts(WaveLds& L, const int* pl, int k, const float4* posm,
-                                           Acc* a, const Targets& tg, float eps2, const Share& sh,
-                                           int first = 0, int step = kWave)
+                                           const float4* node, Acc* a, const Targets& tg,
+                                           float eps2, const Share& sh, int first = 0,
+                                           int step = kWave)
 {
-    if (sh.P == 1) eval_part_list<POT, 1>(L, pl, k, posm, a, tg, eps2, sh.S, sh.h, first, step);
-    else if (sh.P == 2) eval_part_list<POT, 2>(L, pl, k, posm, a, tg, eps2, sh.S, sh.h, first, step);
-    else eval_part_list<POT, 3>(L, pl, k, posm, a, tg, eps2, sh.S, sh.h, first, step);
+    if (sh.P == 1)
+        eval_part_list<POT, 1>(L, pl, k, posm, node, a, tg, eps2, sh.S, sh.h, first, step);
+    else if (sh.P == 2)
+        eval_part_list<POT, 2>(L, pl, k, posm, node, a, tg, eps2, sh.S, sh.h, first, step);
+    else
+        eval_part_list<POT, 3>(L, pl, k, posm, node, a, tg, eps2, sh.S, sh.h, first, step);
 }
 
 __device__ __forceinline__ float box_dist2(float lx, float ly, float lz, float hx, float hy,
@@ -314,7 +330,8 @@ __global__ __launch_bounds__(WPB * 64) void k_traverse_group(
     const int* __restrict__ ngroups_dev, const SegInfo* __restrict__ segs,
     const float4* __restrict__ posm, const float4* __restrict__ node,
     const float4* __restrict__ nodeQ, float4* __restrict__ acc, int* __restrict__ slot_mem,
-    unsigned* qctr, unsigned long long* counters, unsigned* err, float G, float eps2, int leaf)
+    unsigned* qctr, unsigned long long* counters, unsigned* err, float G, float eps2, int leaf,
+    float far2)
 {
     __shared__ WaveLds lds[WPB];
     const int l = lane_id();
@@ -326,7 +343,7 @@ __global__ __launch_bounds__(WPB * 64) void k_traverse_group(
     const int xcd = blockIdx.x % kNumXcd;
     const int ngroups = uniform(*ngroups_dev);
     const int per = (ngroups + kNumXcd - 1) / kNumXcd;
-    unsigned long long n_node = 0, n_part = 0;
+    unsigned long long n_node = 0, n_part = 0, n_far = 0;
     int q = 0;
     const float self_inv = __builtin_amdgcn_rsqf(eps2);
     const int pbits = 32 - __builtin_clz((unsigned)(leaf + 2));  // bits of a round's count
@@ -380,7 +397,7 @@ __global__ __launch_bounds__(WPB * 64) void k_traverse_group(
             a[u] = Acc{0.f, 0.f, 0.f, 0.f};
         }
         int nn = 0, np = 0, top = 0, spilled = 0;
-        long long tot_nodes = 0, tot_parts = 0;
+        long long tot_nodes = 0, tot_parts = 0, tot_far = 0;
         // List stores of one round are issued in the next round after its node
         // loads, so that the wait for those loads does not also wait for them
         // (vector memory operations complete in issue order).
@@ -410,7 +427,7 @@ __global__ __launch_bounds__(WPB * 64) void k_traverse_group(
             if (have) v = L.stk[top + l];
             float4 A = make_float4(0.f, 0.f, 0.f, 0.f);
             int4 B = make_int4(0, 0, 0, 0);
-            bool accept = false, self_direct = false, open = false;
+            bool accept = false, far = false, self_direct = false, open = false;
             float4 bb = make_float4(0.f, 0.f, 0.f, 0.f);
             if (have) {
                 A = node[2 * v];
@@ -427,37 +444,43 @@ __global__ __launch_bounds__(WPB * 64) void k_traverse_group(
             if (have) {
                 B = make_int4(__float_as_int(bb.x), __float_as_int(bb.y), __float_as_int(bb.z),
                               __float_as_int(bb.w));
-                accept = box_dist2(lx, ly, lz, hx, hy, hz, A.x, A.y, A.z) > bb.x;
+                const float d2b = box_dist2(lx, ly, lz, hx, hy, hz, A.x, A.y, A.z);
+                accept = d2b > bb.x;
+                far = accept && d2b > far2 * bb.x;
                 self_direct = !accept && (B.z - B.y + 1) <= leaf;
                 open = !accept && !self_direct;
             }
-            // accepted nodes -> node list
-            const uint64_t mA = ballot(accept);
+            // accepted nodes within far * opening radius -> node list
+            // (quadrupole); farther ones -> particle list (monopole)
+            const bool near = accept && !far;
+            const uint64_t mA = ballot(near);
             const int na = __popcll(mA);
             if (nn + na > kNodeList) {
                 eval_nodes<POT, QUAD>(L, nl, nn, node, nodeQ, a, tg, eps2, sh);
                 tot_nodes += nn;
                 nn = 0;
             }
-            if (accept) {
+            if (near) {
                 dn_pos = nn + mbcnt(mA);
                 dn_val = v;
             }
             nn = uniform(nn + na);
-            // particles: the node's own (summed) or its single-particle children
+            tot_far += __popcll(ballot(far));
+            // particles: the node's own (summed), its single-particle children,
+            // or the node itself as a monopole (~v)
             const bool lleaf = open && B.y == B.w, rleaf = open && B.z == B.w + 1;
-            const int c_self = self_direct ? B.z - B.y + 1 : 0;
+            const int c_self = self_direct ? B.z - B.y + 1 : (far ? 1 : 0);
             const int cnt_p = c_self + (lleaf ? 1 : 0) + (rleaf ? 1 : 0);
             int tp;
             const int ex = wave_scan_excl_bits(cnt_p, pbits, &tp);
             if (np + tp > kPartList) {
-                eval_parts<POT>(L, pl, np, posm, a, tg, eps2, sh);
+                eval_parts<POT>(L, pl, np, posm, node, a, tg, eps2, sh);
                 tot_parts += np;
                 np = 0;
             }
             dp_pos = np + ex;
             dp_self = c_self;
-            dp_first = B.y;
+            dp_first = far ? ~v : B.y;
             dp_lr = (lleaf ? 1 : 0) | (rleaf ? 2 : 0);
             dp_split = B.w;
             np = uniform(np + tp);
@@ -497,7 +520,7 @@ __global__ __launch_bounds__(WPB * 64) void k_traverse_group(
         if (dp_lr & 1) pl[dp_pos + dp_self] = dp_split;
         if (dp_lr & 2) pl[dp_pos + dp_self + (dp_lr & 1)] = dp_split + 1;
         if (nn > 0) eval_nodes<POT, QUAD>(L, nl, nn, node, nodeQ, a, tg, eps2, sh);
-        if (np > 0) eval_parts<POT>(L, pl, np, posm, a, tg, eps2, sh);
+        if (np > 0) eval_parts<POT>(L, pl, np, posm, node, a, tg, eps2, sh);
         tot_nodes += nn;
         tot_parts += np;
         // Combine the sub-wave partial sums through LDS (the chunk arrays, 3 x 64
@@ -528,13 +551,15 @@ __global__ __launch_bounds__(WPB * 64) void k_traverse_group(
             const float ph = POT ? -G * (r.w - posm[gi.first + j].w * self_inv) : 0.0f;
             acc[gi.first + j] = make_float4(G * r.x, G * r.y, G * r.z, ph);
         }
-        n_node += (unsigned long long)tot_nodes * (unsigned long long)gi.count;
-        n_part += (unsigned long long)(tot_parts - 1) * (unsigned long long)gi.count;
+        n_node += (unsigned long long)(tot_nodes + tot_far) * (unsigned long long)gi.count;
+        n_far += (unsigned long long)tot_far * (unsigned long long)gi.count;
+        n_part += (unsigned long long)(tot_parts - tot_far - 1) * (unsigned long long)gi.count;
         __builtin_amdgcn_wave_barrier();
     }
     if (l == 0) {
         atomicAdd(&counters[0], n_node);
         atomicAdd(&counters[1], n_part);
+        atomicAdd(&counters[2], n_far);
     }
 }
 
@@ -594,10 +619,13 @@ __global__ __launch_bounds__(TEAM * 64) void k_traverse_team(
     const int* __restrict__ ngroups_dev, const SegInfo* __restrict__ segs,
     const float4* __restrict__ posm, const float4* __restrict__ node,
     const float4* __restrict__ nodeQ, float4* __restrict__ acc, int* __restrict__ slot_mem,
-    unsigned* qctr, unsigned long long* counters, unsigned* err, float G, float eps2, int leaf)
+    unsigned* qctr, unsigned long long* counters, unsigned* err, float G, float eps2, int leaf,
+    float far2)
 {
     __shared__ WaveLds lds[TEAM];  // the team's stack is lds[0].stk
-    __shared__ int xch[TEAM][4];   // per wave and round: accepted, particles, left, right pushes
+    // per wave and round: accepted (quadrupole), particle-list entries, left
+    // pushes, right pushes, far nodes (monopole entries)
+    __shared__ int xch[TEAM][5];
     __shared__ int ctl[4];          // claimed group; stack top; spilled entries; overflow
     const int l = lane_id();
     const int w = threadIdx.x >> 6;
@@ -610,7 +638,7 @@ __global__ __launch_bounds__(TEAM * 64) void k_traverse_team(
     const int xcd = blockIdx.x % kNumXcd;
     const int ngroups = uniform(*ngroups_dev);
     const int per = (ngroups + kNumXcd - 1) / kNumXcd;
-    unsigned long long n_node = 0, n_part = 0;
+    unsigned long long n_node = 0, n_part = 0, n_far = 0;
     int q = 0;
     const float self_inv = __builtin_amdgcn_rsqf(eps2);
     const int pbits = 32 - __builtin_clz((unsigned)(leaf + 2));
@@ -657,7 +685,7 @@ __global__ __launch_bounds__(TEAM * 64) void k_traverse_team(
         lx = wave_min(lx); ly = wave_min(ly); lz = wave_min(lz);
         hx = wave_max(hx); hy = wave_max(hy); hz = wave_max(hz);
         const int n = gi.count;
-        int nn = 0, np = 0;
+        int nn = 0, np = 0, nfar = 0;
         int top = 0, spilled = 0;
         bool flushed = false;
         if (sg.n == 1) {
@@ -697,14 +725,20 @@ __global__ __launch_bounds__(TEAM * 64) void k_traverse_team(
                 }
                 const int4 B = make_int4(__float_as_int(bb.x), __float_as_int(bb.y),
                                          __float_as_int(bb.z), __float_as_int(bb.w));
-                const bool accept = have && box_dist2(lx, ly, lz, hx, hy, hz, A.x, A.y, A.z) > bb.x;
+                const float d2b = box_dist2(lx, ly, lz, hx, hy, hz, A.x, A.y, A.z);
+                const bool accept = have && d2b > bb.x;
+                // beyond far * the opening radius: a monopole in the particle list
+                const bool far = accept && d2b > far2 * bb.x;
+                const bool near = accept && !far;
                 const bool self_direct = have && !accept && (B.z - B.y + 1) <= leaf;
                 const bool open = have && !accept && !self_direct;
                 const bool lleaf = open && B.y == B.w, rleaf = open && B.z == B.w + 1;
-                const int c_self = self_direct ? B.z - B.y + 1 : 0;
+                const int c_self = self_direct ? B.z - B.y + 1 : (far ? 1 : 0);
+                const int p_first = far ? ~v : B.y;
                 const int cnt_p = c_self + (lleaf ? 1 : 0) + (rleaf ? 1 : 0);
                 const bool lp = open && !lleaf, rp = open && !rleaf;
-                const uint64_t mA = ballot(accept), m3 = ballot(lp), m4 = ballot(rp);
+                const uint64_t mA = ballot(near), m3 = ballot(lp), m4 = ballot(rp);
+                const uint64_t mF = ballot(far);
                 int tp;
                 const int ex = wave_scan_excl_bits(cnt_p, pbits, &tp);
                 if (l == 0) {
@@ -712,10 +746,11 @@ __global__ __launch_bounds__(TEAM * 64) void k_traverse_team(
                     xch[w][1] = tp;
                     xch[w][2] = __popcll(m3);
                     xch[w][3] = __popcll(m4);
+                    xch[w][4] = __popcll(mF);
                 }
                 __syncthreads();  // every entry of the round has been read; counts exchanged
                 int na_before = 0, tp_before = 0, nl_before = 0, nr_before = 0;
-                int na_all = 0, tp_all = 0, nl_all = 0, nr_all = 0;
+                int na_all = 0, tp_all = 0, nl_all = 0, nr_all = 0, nf_all = 0;
 #pragma unroll
                 for (int k = 0; k < TEAM; ++k) {
                     const int a0 = xch[k][0], a1 = xch[k][1], a2 = xch[k][2], a3 = xch[k][3];
@@ -729,16 +764,19 @@ __global__ __launch_bounds__(TEAM * 64) void k_traverse_team(
                     tp_all += a1;
                     nl_all += a2;
                     nr_all += a3;
+                    nf_all += xch[k][4];
                 }
-                // accepted nodes -> node list; particles -> particle list
-                if (accept) nl[nn + na_before + mbcnt(mA)] = v;
+                // accepted nodes -> node list; particles and far nodes ->
+                // particle list
+                if (near) nl[nn + na_before + mbcnt(mA)] = v;
                 int o = np + tp_before + ex;
-                for (int k = 0; k < c_self; ++k) pl[o + k] = B.y + k;
+                for (int k = 0; k < c_self; ++k) pl[o + k] = p_first + k;
                 o += c_self;
                 if (lleaf) pl[o++] = B.w;
                 if (rleaf) pl[o] = B.w + 1;
                 nn += na_all;
                 np += tp_all;
+                nfar += nf_all;
                 // larger children -> stack, in place of the popped entries
                 int base = top - cnt;
                 const int npush = nl_all + nr_all;
@@ -791,7 +829,7 @@ __global__ __launch_bounds__(TEAM * 64) void k_traverse_team(
                     eval_nodes<POT, QUAD>(L, nl, nn, node, nodeQ, a, tg, eps2, sh, w * kWave,
                                           TEAM * kWave);
                 if (np > w * kWave)
-                    eval_parts<POT>(L, pl, np, posm, a, tg, eps2, sh, w * kWave, TEAM * kWave);
+                    eval_parts<POT>(L, pl, np, posm, node, a, tg, eps2, sh, w * kWave, TEAM * kWave);
                 float4 r[kMaxPer];
                 combine_share(L, a, sh, n, r);
                 float4* const scr = L.ca;
@@ -829,11 +867,13 @@ __global__ __launch_bounds__(TEAM * 64) void k_traverse_team(
                         acc[i] = v4;
                     }
                 }
-                n_node += (unsigned long long)nn * (unsigned long long)n;
-                n_part += (unsigned long long)np * (unsigned long long)n;
+                n_node += (unsigned long long)(nn + nfar) * (unsigned long long)n;
+                n_far += (unsigned long long)nfar * (unsigned long long)n;
+                n_part += (unsigned long long)(np - nfar) * (unsigned long long)n;
             }
             nn = 0;
             np = 0;
+            nfar = 0;
             flushed = true;
             if (!need) break;
             __syncthreads();  // the waves are done with the lists and LDS sums before reuse
@@ -847,6 +887,7 @@ __global__ __launch_bounds__(TEAM * 64) void k_traverse_team(
     if (l == 0) {
         atomicAdd(&counters[0], n_node);
         atomicAdd(&counters[1], n_part);
+        atomicAdd(&counters[2], n_far);
     }
 }
 

@@ -215,6 +215,11 @@ class Engine:
         L.check(L.lib().gcs_read_interactions(self._h, L.ptr(out)))
         return int(out[0]), int(out[1])
 
+    def monopole_interactions(self):
+        out = np.zeros(1, np.int64)
+        L.check(L.lib().gcs_read_monopole_interactions(self._h, L.ptr(out)))
+        return int(out[0])
+
     def diagnostics(self, iid=0):
         out = np.zeros(5, np.float64)
         L.check(L.lib().gcs_read_diagnostics(self._h, int(iid), L.ptr(out)))

@@ -103,6 +103,11 @@ typedef struct gcs_params {
                            and evaluates a group; 2, 4 or 8 = a block of that many
                            waves walks it together and splits its targets
                            (block_size is then 64 x team) */
+    float mono_far;     /* GCS_MAC_GROUP: 0 = every accepted node is taken as a
+                           quadrupole; >= 1 = a node whose distance from the group's
+                           box exceeds mono_far times its opening radius is taken
+                           as a softened monopole (its COM and mass), the rest as
+                           quadrupoles */
 } gcs_params;
 
 /* Defaults: G=1, eps2=0.0025, dt=0.01, theta=0.5, GCS_MAC_GROUP,
@@ -214,9 +219,12 @@ int gcs_read_tree(gcs_engine* e, int32_t id, int32_t* left, int32_t* right,
 int gcs_read_root(gcs_engine* e, int32_t id, float* com4);
 /* Cube bounds used for instance `id` in the last step. */
 int gcs_read_bounds(gcs_engine* e, int32_t id, float* mn, float* mx);
-/* Interaction counts of the last traversal: [0] monopole (node)
- * interactions, [1] particle-particle interactions, summed over targets. */
+/* Interaction counts of the last traversal: [0] particle-node interactions
+ * (quadrupole or monopole), [1] particle-particle interactions, summed over
+ * targets. */
 int gcs_read_interactions(gcs_engine* e, int64_t* out2);
+/* Of [0] above, the nodes taken as monopoles (mono_far). */
+int gcs_read_monopole_interactions(gcs_engine* e, int64_t* out);
 /* FP64 diagnostics of instance `id` from the last step run with
  * diagnostics=1: kinetic energy, potential energy, momentum x, y, z, all at
  * the force time (velocities re-synchronised by half a kick). */

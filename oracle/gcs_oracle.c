@@ -470,13 +470,14 @@ void orc_traverse_group(int32_t n, const float* x, const float* y,
                         const float* crit, const float* quad, int32_t group,
                         int32_t leaf, float G, float eps2, double* acc,
                         int64_t* counts, double* msum, double* aabs,
-                        int64_t* gcost, int32_t ngl, const int32_t* glist)
+                        int64_t* gcost, int32_t ngl, const int32_t* glist,
+                        float far2)
 {
     if (n <= 0) return;
     int32_t ngroups = glist ? ngl : (n + group - 1) / group;
-    int64_t cnode = 0, cpart = 0;
+    int64_t cnode = 0, cpart = 0, cfar = 0;
     const double Gd = G, e2 = eps2;
-#pragma omp parallel for schedule(dynamic, 1) reduction(+ : cnode, cpart)
+#pragma omp parallel for schedule(dynamic, 1) reduction(+ : cnode, cpart, cfar)
     for (int32_t g = 0; g < ngroups; ++g) {
         int32_t g0 = glist ? glist[2 * g] : g * group;
         int32_t g1 = glist ? g0 + glist[2 * g + 1] : (g0 + group < n ? g0 + group : n);
@@ -500,13 +501,18 @@ void orc_traverse_group(int32_t n, const float* x, const float* y,
             int32_t v = stack[--top];
             int32_t cnt = last[v] - first[v] + 1;
             int internal = v < n - 1;
-            if (internal &&
-                box_dist2(lo, hi, com4[4 * v], com4[4 * v + 1], com4[4 * v + 2]) > crit[v]) {
+            float bd2 = internal ? box_dist2(lo, hi, com4[4 * v], com4[4 * v + 1],
+                                             com4[4 * v + 2])
+                                 : 0.0f;
+            if (internal && bd2 > crit[v]) {
                 double cx = com4[4 * v], cy = com4[4 * v + 1], cz = com4[4 * v + 2];
                 double mm = com4[4 * v + 3];
                 double q[7] = {0, 0, 0, 0, 0, 0, 0};
-                if (quad)
+                const float far_r2 = far2 * crit[v];
+                const int mono = bd2 > far_r2;
+                if (quad && !mono)
                     for (int c = 0; c < 7; ++c) q[c] = quad[7 * (int64_t)v + c];
+                if (mono) cfar += g1 - g0;
                 for (int32_t i = g0; i < g1; ++i) {
                     double dx = cx - (double)x[i], dy = cy - (double)y[i], dz = cz - (double)z[i];
                     double r2 = dx * dx + dy * dy + dz * dz + e2;
@@ -571,6 +577,7 @@ void orc_traverse_group(int32_t n, const float* x, const float* y,
     }
     counts[0] += cnode;
     counts[1] += cpart;
+    counts[2] += cfar;
 }
 
 /* Groups taken from the tree: the maximal subtrees holding at most gmax

@@ -81,7 +81,11 @@ int64_t orc_traverse_reference(int32_t n, const float* x, const float* y,
  * (the total mass less its own when every particle is counted once); aabs
  * (optional, n doubles) the sum of the magnitudes of the acceleration terms,
  * the scale of the rounding error of any summation order; gcost
- * (optional, 2 per group) the node and particle list lengths of each group. */
+ * (optional, 2 per group) the node and particle list lengths of each group.
+ * far2: an accepted node whose box distance squared also exceeds
+ * far2 * crit[node] (float product) is taken as a plain softened monopole,
+ * its quadrupole and trace left out; counts[2] += those interactions (they
+ * are in counts[0] too); +inf: none. */
 void orc_traverse_group(int32_t n, const float* x, const float* y,
                         const float* z, const float* m, const int32_t* left,
                         const int32_t* right, const int32_t* first,
@@ -89,7 +93,8 @@ void orc_traverse_group(int32_t n, const float* x, const float* y,
                         const float* crit, const float* quad, int32_t group,
                         int32_t leaf, float G, float eps2, double* acc,
                         int64_t* counts, double* msum, double* aabs,
-                        int64_t* gcost, int32_t ngl, const int32_t* glist);
+                        int64_t* gcost, int32_t ngl, const int32_t* glist,
+                        float far2);
 /* With glist non-null the groups are the ngl (first, count) pairs of glist
  * instead of runs of `group` particles.
  * Tree groups: the maximal subtrees of at most gmax particles (every node of

@@ -131,20 +131,21 @@ def check_pipeline(gcs, orc, e, stars, p, iid=0, theta=0.5, bounds=None):
     return cpu
 
 
-@pytest.mark.parametrize("gsize,offset,team", [(64, 0, 1), (24, 0, 1), (128, 0, 1), (192, 0, 1),
-                                               (64, 1, 1), (64, 0, 2), (24, 0, 2), (192, 0, 2),
-                                               (96, 0, 4), (5, 0, 4), (192, 0, 8)])
-def test_engine_c1_group_parity(gcs, orc, gsize, offset, team):
+@pytest.mark.parametrize("gsize,offset,team,mono", [
+    (64, 0, 1, 0.0), (24, 0, 1, 0.0), (128, 0, 1, 0.0), (192, 0, 1, 0.0), (64, 1, 1, 0.0),
+    (64, 0, 2, 0.0), (24, 0, 2, 0.0), (192, 0, 2, 0.0), (96, 0, 4, 0.0), (5, 0, 4, 0.0),
+    (192, 0, 8, 0.0), (128, 0, 1, 1.7), (96, 0, 4, 1.7), (24, 0, 2, 1.3), (64, 1, 1, 2.0)])
+def test_engine_c1_group_parity(gcs, orc, gsize, offset, team, mono):
     stars = gcs.collision(8192, 1001)
     n = len(stars)
     eps2 = np.float32(0.05) ** 2
     p = gcs.params(G=G, eps2=float(eps2), dt=0.01, theta=0.5, leaf_size=16, diagnostics=1,
-                   group_size=gsize, open_offset=offset, team=team,
+                   group_size=gsize, open_offset=offset, team=team, mono_far=mono,
                    block_size=64 * team if team > 1 else 256)
     with gcs.Engine(n) as e:
         e.load(stars)
-        say("engine C1 compute, groups of at most %d, opening offset %d, team %d"
-            % (gsize, offset, team))
+        say("engine C1 compute, groups of at most %d, opening offset %d, team %d, monopoles "
+            "beyond %.1f x the opening radius" % (gsize, offset, team, mono))
         e.compute_accels(p)
         cpu = check_pipeline(gcs, orc, e, stars, p)
         acc = e.accels()
@@ -153,8 +154,10 @@ def test_engine_c1_group_parity(gcs, orc, gsize, offset, team):
         want, msum, cnt, aabs = orc.traverse_group(cpu["left"], cpu["right"], cpu["first"],
                                                    cpu["last"], s["x"], s["y"], s["z"],
                                                    s["mass"], cpu["agg"], G, float(eps2), 64, 16,
-                                                   want_abs=True, groups=grp)
+                                                   want_abs=True, groups=grp, mono_far=mono)
         assert e.engine_info()["groups"] == len(grp)
+        assert e.monopole_interactions() == int(cnt[2])
+        assert (cnt[2] > 0) == (mono > 0)
         # Same interaction lists: identical counts.  Same sums: the FP32
         # accumulation error is bounded by a multiple of unit roundoff times
         # the sum of term magnitudes (3e-5 ~ 500 u here; a dropped or doubled
@@ -170,7 +173,8 @@ def test_engine_c1_group_parity(gcs, orc, gsize, offset, team):
         err = rel_rms(acc[:, :3], ref[:, :3])
         say("C1 rel rms vs FP64 direct sum: %.3g" % err)
         assert err < 1e-3
-        assert rel_rms(acc[:, 3], ref[:, 3]) < 1e-4
+        # potential: second order for quadrupoles, first for far monopoles
+        assert rel_rms(acc[:, 3], ref[:, 3]) < (1e-4 if mono == 0 else 5e-4)
 
 
 def test_engine_reference_mode_parity(gcs, orc):

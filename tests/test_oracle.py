@@ -247,6 +247,55 @@ def test_group_opening_threshold(orc, dist, classic, offset):
         assert np.allclose(msum + s["mass"], 5.0)
 
 
+@pytest.mark.parametrize("dist,mono", [(2.9, False), (3.1, True)])
+def test_group_far_monopole_threshold(orc, dist, mono):
+    # The node of test_group_opening_threshold (width 1, mass 4, COM x = 0.25)
+    # is accepted from distance > 2 at theta = 0.5; with mono_far = 1.5 it is
+    # taken as a plain softened monopole beyond 1.5 x 2 = 3.  Closed forms in
+    # FP64: the monopole 4 D / (D^2 + eps^2)^1.5, and the two particles'
+    # direct sum that the quadrupole approximates to second order.
+    x = np.array([0.0, 1.0, 0.25 + dist], np.float32)
+    st = np.zeros(3, orc.STAR)
+    st["x"], st["mass"] = x, np.array([3.0, 1.0, 1.0], np.float32)
+    t = orc.build_tree_pipeline(st, 0.5)
+    s = t["sorted"]
+    acc, msum, cnt = orc.traverse_group(t["left"], t["right"], t["first"], t["last"],
+                                        s["x"], s["y"], s["z"], s["mass"], t["agg"], G, EPS2,
+                                        1, 1, mono_far=1.5)
+    it = int(np.argmax(s["x"]))  # the target
+    D = float(x[2]) - 0.25
+    whole = -4.0 * D / (D * D + EPS2) ** 1.5
+    assert cnt[0] == 1 and cnt[2] == (1 if mono else 0)
+    if mono:
+        assert np.isclose(acc[it, 0], whole, rtol=1e-6)
+    else:
+        a, b = D + 0.25, D - 0.75  # from x = 0 (mass 3) and x = 1 (mass 1)
+        parts = -(3.0 * a / (a * a + EPS2) ** 1.5 + b / (b * b + EPS2) ** 1.5)
+        # the quadrupole term is there: its third-order remainder (~w/D of the
+        # monopole's error here) instead of the monopole's second-order one
+        assert abs(acc[it, 0] - parts) < 0.5 * abs(whole - parts)
+
+
+def test_far_monopoles_keep_the_lists_and_the_error_bound(orc):
+    # mono_far changes how accepted nodes are evaluated, not which: the same
+    # node and particle counts; the nodes beyond 1.7 x their opening radius
+    # become monopoles and the error against FP64 stays within 1e-3.
+    t = pipeline(orc, 6000, 8)
+    s = t["sorted"]
+    ref = orc.direct_sum_f64(s["x"], s["y"], s["z"], s["mass"], G, EPS2)
+    grp = orc.tree_groups(t["left"], t["right"], t["first"], t["last"], 96)
+    a0, _, c0 = orc.traverse_group(t["left"], t["right"], t["first"], t["last"], s["x"], s["y"],
+                                   s["z"], s["mass"], t["agg"], G, EPS2, 64, 4, groups=grp)
+    a1, m1, c1 = orc.traverse_group(t["left"], t["right"], t["first"], t["last"], s["x"],
+                                    s["y"], s["z"], s["mass"], t["agg"], G, EPS2, 64, 4,
+                                    groups=grp, mono_far=1.7)
+    assert c1[0] == c0[0] and c1[1] == c0[1] and c0[2] == 0
+    assert 0.1 * c1[0] < c1[2] < c1[0]
+    assert np.allclose(m1 + s["mass"], s["mass"].astype(np.float64).sum(), rtol=1e-5)
+    e0, e1 = rel_rms(a0[:, :3], ref[:, :3]), rel_rms(a1[:, :3], ref[:, :3])
+    assert e0 < e1 < 1e-3
+
+
 def test_opening_offset_only_adds(orc):
     t0 = pipeline(orc, 3000, 12)
     s = t0["sorted"]
