@@ -540,6 +540,38 @@ def served_leg(window, rate, max_resident, seed=1005, budget=200e9, steps_per_ro
     return out
 
 
+def bake_leg(frames, per=150000):
+    """The Bake frame at the reference README's conditions (300,000 particles,
+    eps^2 = 10, dt = 0.01, the reference's traversal rule; README.md:92-104
+    quotes ~31 ms per frame on an RTX 3050 Laptop): the gcs_bake driver's
+    MI355X pipeline and the reference's own frame through the cuda_engine.h
+    functions, each writing `frames` frames of simulation.bin to a scratch
+    file.  Context only: the initial conditions are this repository's
+    generator, not the reference's."""
+    import subprocess
+    import tempfile
+    import gcs_amd
+    exe = os.path.join(os.path.dirname(gcs_amd.LIB_PATH), "gcs_bake")
+    out = {"particles": 2 * per, "frames": frames, "eps2": 10.0, "dt": 0.01,
+           "reference_published_ms_per_frame": 31.4,
+           "reference_hardware": "RTX 3050 Laptop (README.md:102)"}
+    with tempfile.TemporaryDirectory() as d:
+        for name, extra in (("pipeline", []), ("via_shim", ["--via-shim"])):
+            path = os.path.join(d, "simulation.bin")
+            r = subprocess.run([exe, str(frames), "--n", str(per), "--eps2", "10", "--dt",
+                                "0.01", "--seed", "1001", "--out", path] + extra,
+                               stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True,
+                               timeout=300)
+            if r.returncode != 0:
+                raise RuntimeError("gcs_bake failed: " + r.stderr[-500:])
+            out[name] = json.loads(r.stdout.strip().splitlines()[-1])
+            out[name]["file_bytes"] = os.path.getsize(path)
+            os.remove(path)
+    say("bake: pipeline %.3f ms/frame, via shim %.3f ms/frame"
+        % (out["pipeline"]["ms_per_frame"], out["via_shim"]["ms_per_frame"]))
+    return out
+
+
 def copy_calibration(mib):
     """Device-to-device copies of `mib` MiB through the HIP runtime (the
     runtime's copy kernel), for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE."""
@@ -591,6 +623,8 @@ def main():
                     help="steps of the full 2 x 1,048,576 precision study at theta 0.5 and "
                          "0.7 (0 = skip)")
     ap.add_argument("--precision-targets", type=int, default=65536)
+    ap.add_argument("--bake-frames", type=int, default=30,
+                    help="frames of each Bake mode at the README's conditions (0 = skip)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--retune", action="store_true",
@@ -650,6 +684,8 @@ def main():
                             4096)
     if a.precision > 0:
         line["precision"] = precision_study(a.precision, a.precision_targets)
+    if a.bake_frames > 0:
+        line["bake"] = bake_leg(a.bake_frames)
     if a.tune_out:
         gcs_amd.autotune_cache_save(a.tune_out)
     print(json.dumps(line), flush=True)

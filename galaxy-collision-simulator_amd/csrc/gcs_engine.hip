@@ -14,7 +14,9 @@
 #include <map>
 #include <mutex>
 #include <new>
+#include <chrono>
 #include <string>
+#include <thread>
 #include <tuple>
 #include <vector>
 
@@ -1211,6 +1213,91 @@ int gcs_read_playback(gcs_engine* e, gcs_playback_star* out)
     HIPCHK(hipMemcpyAsync(out, e->staging, 16 * (size_t)n, hipMemcpyDeviceToHost, e->stream));
     HIPCHK(hipStreamSynchronize(e->stream));
     return GCS_OK;
+}
+
+// ---- Bake output pipeline (main.cpp:133-297) ---------------------------------
+// Per frame: one step on the engine stream; the PlaybackStar records of every
+// particle (16 bytes, current device order) packed on the device into one half
+// of the staging buffer; a 16-byte-per-particle copy into one of two pinned
+// host buffers on a second stream, which overlaps the next frame's step; a
+// writer thread appending the frame to the file once its copy has landed.
+// Frame f uses staging half and pinned buffer f % 2.  As in the reference's
+// loop one writer is in flight: frame f's copy is queued after frame f - 1's
+// writer has been joined, so the pinned buffer of frame f - 2 is free, and
+// frame f's pack waits (on the device) for the copy of frame f - 2 out of the
+// same staging half.
+int gcs_bake(gcs_engine* e, const gcs_params* p, int32_t frames, const char* path,
+             double* seconds)
+{
+    if (!e || !path) return fail(GCS_ERR_INVALID, "null argument");
+    int rc = check_params(p);
+    if (rc) return rc;
+    if (frames < 0) return fail(GCS_ERR_INVALID, "frames < 0");
+    if (seconds) *seconds = 0.0;
+    const int64_t n = e->n_total;
+    HIPCHK(hipSetDevice(e->device));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    FILE* f = std::fopen(path, "wb");
+    if (!f) return fail(GCS_ERR_INVALID, std::string("cannot open ") + path);
+    if (n == 0 || frames == 0) {
+        std::fclose(f);
+        return GCS_OK;
+    }
+    const size_t bytes = 16 * (size_t)n;  // two halves fit the 32 B/particle staging
+    uint8_t* host[2] = {nullptr, nullptr};
+    hipStream_t cs = nullptr;
+    hipEvent_t packed[2] = {nullptr, nullptr}, copied[2] = {nullptr, nullptr};
+    std::thread writer;
+    bool write_failed = false;
+    hipError_t he = hipSuccess;
+    for (int b = 0; b < 2 && he == hipSuccess; ++b) {
+        he = hipHostMalloc((void**)&host[b], bytes, hipHostMallocDefault);
+        if (he == hipSuccess) he = hipEventCreateWithFlags(&packed[b], hipEventDisableTiming);
+        if (he == hipSuccess) he = hipEventCreateWithFlags(&copied[b], hipEventDisableTiming);
+    }
+    if (he == hipSuccess) he = hipStreamCreateWithFlags(&cs, hipStreamNonBlocking);
+    const auto t0 = std::chrono::steady_clock::now();
+    for (int32_t fr = 0; fr < frames && he == hipSuccess; ++fr) {
+        const int b = fr & 1;
+        rc = gcs_step(e, p, 1);
+        if (rc) break;
+        uint8_t* stage = e->staging + (size_t)b * bytes;
+        if (fr >= 2) he = hipStreamWaitEvent(e->stream, copied[b], 0);
+        if (he != hipSuccess) break;
+        const int c = e->cur;
+        k_pack_playback<<<(unsigned)((n + 255) / 256), 256, 0, e->stream>>>(n, 0, e->posm[c],
+                                                                             e->vel[c], stage);
+        he = hipGetLastError();
+        if (he == hipSuccess) he = hipEventRecord(packed[b], e->stream);
+        if (writer.joinable()) writer.join();  // frame fr - 1 is in the file
+        if (write_failed || he != hipSuccess) break;
+        he = hipStreamWaitEvent(cs, packed[b], 0);
+        if (he == hipSuccess) he = hipMemcpyAsync(host[b], stage, bytes, hipMemcpyDeviceToHost, cs);
+        if (he == hipSuccess) he = hipEventRecord(copied[b], cs);
+        if (he != hipSuccess) break;
+        hipEvent_t ev = copied[b];
+        uint8_t* buf = host[b];
+        writer = std::thread([f, ev, buf, bytes, &write_failed]() {
+            write_failed = hipEventSynchronize(ev) != hipSuccess ||
+                           std::fwrite(buf, 1, bytes, f) != bytes;
+        });
+    }
+    if (writer.joinable()) writer.join();
+    if (he == hipSuccess && rc == GCS_OK) he = hipStreamSynchronize(e->stream);
+    if (he == hipSuccess && rc == GCS_OK) he = hipStreamSynchronize(cs);
+    if (seconds)
+        *seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    const bool closed = std::fclose(f) == 0;
+    if (cs) (void)hipStreamDestroy(cs);
+    for (int b = 0; b < 2; ++b) {
+        if (packed[b]) (void)hipEventDestroy(packed[b]);
+        if (copied[b]) (void)hipEventDestroy(copied[b]);
+        if (host[b]) (void)hipHostFree(host[b]);
+    }
+    if (rc) return rc;
+    if (he != hipSuccess) return fail(GCS_ERR_HIP, std::string("bake: ") + hipGetErrorString(he));
+    if (write_failed || !closed) return fail(GCS_ERR_INVALID, std::string("writing ") + path + " failed");
+    return gcs_sync(e);
 }
 
 int gcs_step(gcs_engine* e, const gcs_params* p, int32_t nsteps)

@@ -55,10 +55,10 @@ def build(verbose=False):
     os.replace(tmp, LIB)
     # Bake driver linked against the library (cuda_engine.h entry points).
     bake_obj = os.path.join(obj_dir, "gcs_bake_main.o")
-    _run([hipcc(), "-O2", "-std=c++17", "-I", INCLUDE, "-c",
+    _run([hipcc(), "-O2", "-std=c++17", "-fopenmp", "-I", INCLUDE, "-c",
           os.path.join(CSRC, "gcs_bake_main.cpp"), "-o", bake_obj], verbose)
-    _run([hipcc(), "-o", BAKE + ".tmp", bake_obj, LIB, "-Wl,-rpath,$ORIGIN", "-lpthread"],
-         verbose)
+    _run([hipcc(), "-fopenmp", "-o", BAKE + ".tmp", bake_obj, LIB, "-Wl,-rpath,$ORIGIN",
+          "-lpthread"], verbose)
     os.replace(BAKE + ".tmp", BAKE)
     return LIB
 

@@ -103,6 +103,7 @@ SIGNATURES = {
     "gcs_read_instance": ([_P, _I32, _P], _I32),
     "gcs_read_position_lo": ([_P, _P], _I32),
     "gcs_read_playback": ([_P, _P], _I32),
+    "gcs_bake": ([_P, _P, _I32, ctypes.c_char_p, _P], _I32),
     "gcs_step": ([_P, _P, _I32], _I32),
     "gcs_sync": ([_P], _I32),
     "gcs_compute_accels": ([_P, _P], _I32),

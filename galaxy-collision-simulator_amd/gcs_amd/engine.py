@@ -148,6 +148,14 @@ class Engine:
         L.check(L.lib().gcs_read_playback(self._h, L.ptr(out)))
         return out
 
+    def bake(self, p, frames, path):
+        """gcs_bake: `frames` steps written to `path` as PlaybackStar frames;
+        returns the wall seconds."""
+        sec = ctypes.c_double()
+        L.check(L.lib().gcs_bake(self._h, ctypes.byref(p), int(frames), str(path).encode(),
+                                 ctypes.byref(sec)))
+        return sec.value
+
     # ---- stepping --------------------------------------------------------
     def step(self, p, nsteps=1, sync=True):
         L.check(L.lib().gcs_step(self._h, ctypes.byref(p), int(nsteps)))

@@ -162,6 +162,15 @@ int gcs_checkpoint(gcs_engine* e, int32_t id, void* buf, int64_t cap);
 int gcs_restore(gcs_engine* e, const void* buf, int64_t bytes, int32_t* id);
 /* PlaybackStar frame of all particles (simulation.bin record, main.cpp:266). */
 int gcs_read_playback(gcs_engine* e, gcs_playback_star* out);
+/* Bake (main.cpp:133-297): `frames` steps, each frame's PlaybackStar records
+ * of every particle (device order, as gcs_read_playback gives them after that
+ * step) appended to the file at `path` -- the reference's headerless
+ * simulation.bin.  Records are packed on the device and copied 16 bytes per
+ * particle into pinned double buffers on a second stream while the next
+ * frame steps; writer threads append them in order.  *seconds (optional):
+ * wall time of the frames, the last write included. */
+int gcs_bake(gcs_engine* e, const gcs_params* p, int32_t frames, const char* path,
+             double* seconds);
 
 /* ---- Stepping (cuda_engine.cu:379-451) -----------------------------------
  * Advances every instance by nsteps full steps.  Returns when the work is

@@ -522,23 +522,31 @@ def test_max_accel_and_adaptive_step(gcs):
         assert e.sim_time() > t0
 
 
-def test_bake_driver_writes_reference_frames(gcs, tmp_path):
+@pytest.mark.parametrize("mode", ["pipeline", "via-shim"])
+def test_bake_driver_writes_reference_frames(gcs, tmp_path, mode):
+    """gcs_bake (the driver) writes simulation.bin: N x 16-byte PlaybackStar
+    frames in device order.  Its default mode runs the MI355X pipeline
+    (gcs_bake: device bounds, device pack, async 16-byte copies); --via-shim
+    runs the reference's frame through the cuda_engine.h functions (host
+    bounds, host sort, cuda_physics_step, host pack).  Both must equal the
+    frames of the engine stepped with the reference's semantics and host
+    bounds, read back after each step."""
     exe = os.path.join(os.path.dirname(gcs.LIB_PATH), "gcs_bake")
     out = tmp_path / "simulation.bin"
     n_per = 3000
-    say("bake driver")
-    r = subprocess.run([exe, "3", "--n", str(n_per), "--seed", "5", "--eps2", "0.0025",
-                        "--out", str(out)], stdout=subprocess.PIPE, stderr=subprocess.STDOUT,
-                       text=True, timeout=120)
+    say("bake driver, %s" % mode)
+    r = subprocess.run([exe, "5", "--n", str(n_per), "--seed", "5", "--eps2", "0.0025",
+                        "--out", str(out)] + (["--via-shim"] if mode == "via-shim" else []),
+                       stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, timeout=120)
     assert r.returncode == 0, r.stdout
+    assert '"frames": 5' in r.stdout
     data = np.fromfile(out, gcs.PLAYBACK_STAR)
-    assert len(data) == 3 * 2 * n_per
-    # Same frames from the engine with the same semantics.
+    assert len(data) == 5 * 2 * n_per
     stars = gcs.collision(n_per, 5)
     with gcs.Engine(len(stars)) as e:
         e.load(stars)
         host = stars
-        for f in range(3):
+        for f in range(5):
             mn = min(host["x"].min(), host["y"].min(), host["z"].min())
             mx = max(host["x"].max(), host["y"].max(), host["z"].max())
             e.step(gcs.reference_params(G, 0.0025, 0.01, float(mn), float(mx)), 1)
@@ -546,6 +554,33 @@ def test_bake_driver_writes_reference_frames(gcs, tmp_path):
             pb = e.playback()
             assert np.array_equal(pb.view(np.uint8), data[f * 2 * n_per:(f + 1) * 2 * n_per]
                                   .view(np.uint8)), f
+
+
+def test_bake_entry_point_frames(gcs, tmp_path):
+    """gcs_bake on a forest of two collisions with the group traversal: each
+    frame equals gcs_read_playback after the same step of an engine stepped
+    one frame at a time (frames of 7 and 12 steps cover both buffers and
+    their reuse)."""
+    parts = [gcs.collision(2500, 71), gcs.collision(1200, 72)]
+    counts = [len(a) for a in parts]
+    p = gcs.params(G=G, eps2=0.0025, dt=0.01, theta=0.5, leaf_size=8, use_graph=1,
+                   precision=gcs.PREC_COMPENSATED)
+    for frames in (7, 12):
+        out = tmp_path / ("bake%d.bin" % frames)
+        with gcs.Engine(sum(counts), 2) as e:
+            e.load(np.concatenate(parts), counts)
+            say("gcs_bake %d frames" % frames)
+            sec = e.bake(p, frames, out)
+            assert sec > 0
+        data = np.fromfile(out, gcs.PLAYBACK_STAR)
+        n = sum(counts)
+        assert len(data) == frames * n
+        with gcs.Engine(n, 2) as e:
+            e.load(np.concatenate(parts), counts)
+            for f in range(frames):
+                e.step(p, 1)
+                assert np.array_equal(e.playback().view(np.uint8),
+                                      data[f * n:(f + 1) * n].view(np.uint8)), (frames, f)
 
 
 # ---- reference traversal thresholds, root record, compensated drift -----------------
