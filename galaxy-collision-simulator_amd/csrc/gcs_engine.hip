@@ -150,7 +150,12 @@ struct gcs_engine {
     bool bpart_valid = false;
     bool capturing = false;  // enqueue_step is recording a step graph
     unsigned* qctr = nullptr;
-    unsigned long long* counters = nullptr;
+    unsigned long long* counters = nullptr;  // reference traversal: [1] interactions
+    // Group traversal: node, particle and monopole interactions per wave slot
+    // (3 per slot) of the last launch, which wrote the first wslots slots.
+    unsigned long long* wcount = nullptr;
+    int wslots = 0;
+    int graph_wslots = 0;  // wslots of the captured step graphs
     unsigned* err = nullptr;
     double* diag_part = nullptr;
     double* diag_out = nullptr;
@@ -219,7 +224,7 @@ void free_all(gcs_engine* e)
     dfree(e->gmark); dfree(e->gq); dfree(e->gorder); dfree(e->gchunk); dfree(e->gcnt); dfree(e->ngroups);
     dfree(e->front); dfree(e->nfront);
     dfree(e->tlist); dfree(e->ntop); dfree(e->tslot);
-    dfree(e->seg_bounds); dfree(e->bpart); dfree(e->qctr); dfree(e->counters); dfree(e->err);
+    dfree(e->seg_bounds); dfree(e->bpart); dfree(e->qctr); dfree(e->counters); dfree(e->wcount); dfree(e->err);
     dfree(e->diag_part); dfree(e->diag_out); dfree(e->amax_part); dfree(e->dt_inst); dfree(e->tsim); dfree(e->slot_mem); dfree(e->staging); dfree(e->mn_dev);
     for (auto& v : e->ev)
         if (v) (void)hipEventDestroy(v), v = nullptr;
@@ -334,17 +339,31 @@ float far2_of(const gcs_params& p)
 template <bool POT, bool QUAD, int WPB>
 void launch_group(gcs_engine* e, int blocks, const gcs_params& p, float4* posm, bool order)
 {
-    k_traverse_group<POT, QUAD, WPB><<<blocks, WPB * kWave, 0, e->stream>>>(
+    if (p.mono_far > 0.0f) {
+        k_traverse_group<POT, QUAD, WPB, true><<<blocks, WPB * kWave, 0, e->stream>>>(
+            e->groups, order ? e->gorder : nullptr, e->ngroups, e->segs, posm, e->node, e->nodeQ,
+            e->acc, e->slot_mem, e->qctr, e->wcount, e->err, p.G, p.eps2, p.leaf_size,
+            far2_of(p));
+        return;
+    }
+    k_traverse_group<POT, QUAD, WPB, false><<<blocks, WPB * kWave, 0, e->stream>>>(
         e->groups, order ? e->gorder : nullptr, e->ngroups, e->segs, posm, e->node, e->nodeQ, e->acc,
-        e->slot_mem, e->qctr, e->counters, e->err, p.G, p.eps2, p.leaf_size, far2_of(p));
+        e->slot_mem, e->qctr, e->wcount, e->err, p.G, p.eps2, p.leaf_size, far2_of(p));
 }
 
 template <bool POT, int TEAM>
 void launch_team(gcs_engine* e, int blocks, const gcs_params& p, float4* posm, bool order)
 {
-    k_traverse_team<POT, true, TEAM><<<blocks, TEAM * kWave, 0, e->stream>>>(
+    if (p.mono_far > 0.0f) {
+        k_traverse_team<POT, true, TEAM, true><<<blocks, TEAM * kWave, 0, e->stream>>>(
+            e->groups, order ? e->gorder : nullptr, e->ngroups, e->segs, posm, e->node, e->nodeQ,
+            e->acc, e->slot_mem, e->qctr, e->wcount, e->err, p.G, p.eps2, p.leaf_size,
+            far2_of(p));
+        return;
+    }
+    k_traverse_team<POT, true, TEAM, false><<<blocks, TEAM * kWave, 0, e->stream>>>(
         e->groups, order ? e->gorder : nullptr, e->ngroups, e->segs, posm, e->node, e->nodeQ,
-        e->acc, e->slot_mem, e->qctr, e->counters, e->err, p.G, p.eps2, p.leaf_size, far2_of(p));
+        e->acc, e->slot_mem, e->qctr, e->wcount, e->err, p.G, p.eps2, p.leaf_size, far2_of(p));
 }
 
 template <int STACK>
@@ -571,6 +590,7 @@ int enqueue_step(gcs_engine* e, const gcs_params& p, bool integrate)
         const int wpb = p.team > 1 ? p.team : p.block_size / kWave;
         const int blocks = group_waves(e, p) / wpb;
         const bool pot = p.diagnostics != 0;
+        e->wslots = p.team > 1 ? blocks : blocks * wpb;  // count slots the launch writes
         if (p.team == 2) {
             if (pot) launch_team<true, 2>(e, blocks, p, posm, order);
             else launch_team<false, 2>(e, blocks, p, posm, order);
@@ -655,6 +675,7 @@ int capture_graphs(gcs_engine* e, const gcs_params& p)
     e->cur = saved;
     e->gparams = p;
     e->graph_valid = true;
+    e->graph_wslots = e->wslots;
     return GCS_OK;
 }
 
@@ -807,7 +828,7 @@ int gcs_create(int64_t cap_n, int32_t cap_k, int32_t device, gcs_engine** out)
     if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
         e->num_cu = prop.multiProcessorCount;
     int occ = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_traverse_group<true, true, 4>, 256, 0) !=
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_traverse_group<true, true, 4, false>, 256, 0) !=
             hipSuccess ||
         occ < 1)
         occ = 2;
@@ -869,6 +890,7 @@ int gcs_create(int64_t cap_n, int32_t cap_k, int32_t device, gcs_engine** out)
     e->trav_slots = (int)std::max<int64_t>(4, std::min<int64_t>((int64_t)e->trav_blocks_cap * 4,
                                                               (cap_n + 3) / 4 * 4));
     if (!rc) rc = dalloc(&e->slot_mem, (size_t)e->trav_slots * kSlotInts);
+    if (!rc) rc = dalloc(&e->wcount, 3 * (size_t)e->trav_slots);
     if (!rc) rc = dalloc(&e->staging, 32 * n);
     if (rc) {
         free_all(e);
@@ -1328,6 +1350,7 @@ int gcs_step(gcs_engine* e, const gcs_params* p, int32_t nsteps)
         e->bpart_valid = true;  // every replayed step ends with k_integrate
         e->have_tree = true;
         e->last_mac = p->mac;
+        e->wslots = e->graph_wslots;
         return GCS_OK;
     }
     for (int32_t s = 0; s < nsteps; ++s) {
@@ -1634,13 +1657,34 @@ int gcs_read_bounds(gcs_engine* e, int32_t id, float* mn, float* mx)
     return GCS_OK;
 }
 
+// Interaction counts of the last traversal: node, particle, monopole.
+static int read_counts(gcs_engine* e, unsigned long long* c3)
+{
+    HIPCHK(hipSetDevice(e->device));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    c3[0] = c3[1] = c3[2] = 0;
+    if (e->last_mac == GCS_MAC_GROUP) {
+        std::vector<unsigned long long> w(3 * (size_t)e->wslots);
+        if (!w.empty())
+            HIPCHK(hipMemcpy(w.data(), e->wcount, sizeof(unsigned long long) * w.size(),
+                             hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < w.size(); i += 3) {
+            c3[0] += w[i];
+            c3[1] += w[i + 1];
+            c3[2] += w[i + 2];
+        }
+    } else {
+        HIPCHK(hipMemcpy(c3, e->counters, 3 * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+    }
+    return GCS_OK;
+}
+
 int gcs_read_interactions(gcs_engine* e, int64_t* out2)
 {
     if (!e || !out2) return fail(GCS_ERR_INVALID, "null argument");
     unsigned long long c[3];
-    HIPCHK(hipSetDevice(e->device));
-    HIPCHK(hipStreamSynchronize(e->stream));
-    HIPCHK(hipMemcpy(c, e->counters, sizeof(c), hipMemcpyDeviceToHost));
+    int rc = read_counts(e, c);
+    if (rc) return rc;
     out2[0] = (int64_t)c[0];
     out2[1] = (int64_t)c[1];
     return GCS_OK;
@@ -1649,11 +1693,10 @@ int gcs_read_interactions(gcs_engine* e, int64_t* out2)
 int gcs_read_monopole_interactions(gcs_engine* e, int64_t* out)
 {
     if (!e || !out) return fail(GCS_ERR_INVALID, "null argument");
-    unsigned long long c = 0;
-    HIPCHK(hipSetDevice(e->device));
-    HIPCHK(hipStreamSynchronize(e->stream));
-    HIPCHK(hipMemcpy(&c, e->counters + 2, sizeof(c), hipMemcpyDeviceToHost));
-    *out = (int64_t)c;
+    unsigned long long c[3];
+    int rc = read_counts(e, c);
+    if (rc) return rc;
+    *out = (int64_t)c[2];
     return GCS_OK;
 }
 

@@ -234,16 +234,18 @@ __device__ __forceinline__ void eval_node_list(WaveLds& L, const int* nl, int k,
 // layout); kNoEntry past the list end (a zero-mass source).
 constexpr int kNoEntry = (int)0x80000000;
 
+template <bool FAR>
 __device__ __forceinline__ float4 point_source(int e, const float4* __restrict__ posm,
                                                const float4* __restrict__ node)
 {
-    if (e == kNoEntry) return make_float4(0.f, 0.f, 0.f, 0.f);
-    return e >= 0 ? posm[e] : node[2 * (~e)];
+    if (e >= 0) return posm[e];
+    if (FAR && e != kNoEntry) return node[2 * (~e)];
+    return make_float4(0.f, 0.f, 0.f, 0.f);
 }
 
 // Phase 2 over the particle list pl[0..k), same chunks, pipeline and split.
 // Entries are particles or far nodes taken as monopoles (point_source).
-template <bool POT, int P>
+template <bool POT, int P, bool FAR>
 __device__ __forceinline__ void eval_part_list(WaveLds& L, const int* pl, int k,
                                                const float4* __restrict__ posm,
                                                const float4* __restrict__ node, Acc* a,
@@ -253,11 +255,11 @@ __device__ __forceinline__ void eval_part_list(WaveLds& L, const int* pl, int k,
     const int l = lane_id();
     const int idx = first + l < k ? pl[first + l] : kNoEntry;
     int idx_next = first + step + l < k ? pl[first + step + l] : kNoEntry;
-    float4 X = point_source(idx, posm, node);
+    float4 X = point_source<FAR>(idx, posm, node);
     for (int base = first; base < k; base += step) {
         L.ca[l] = X;
         __builtin_amdgcn_wave_barrier();
-        X = point_source(idx_next, posm, node);
+        X = point_source<FAR>(idx_next, posm, node);
         const int j2 = base + 2 * step + l;
         idx_next = j2 < k ? pl[j2] : kNoEntry;
         const int trips = uniform((min(kWave, k - base) + S - 1) / S);
@@ -300,18 +302,18 @@ __device__ __forceinline__ void eval_nodes(WaveLds& L, const int* nl, int k, con
         eval_node_list<POT, QUAD, 3>(L, nl, k, node, nodeQ, a, tg, eps2, sh.S, sh.h, first, step);
 }
 
-template <bool POT>
+template <bool POT, bool FAR>
 __device__ __forceinline__ void eval_parts(WaveLds& L, const int* pl, int k, const float4* posm,
                                            const float4* node, Acc* a, const Targets& tg,
                                            float eps2, const Share& sh, int first = 0,
                                            int step = kWave)
 {
     if (sh.P == 1)
-        eval_part_list<POT, 1>(L, pl, k, posm, node, a, tg, eps2, sh.S, sh.h, first, step);
+        eval_part_list<POT, 1, FAR>(L, pl, k, posm, node, a, tg, eps2, sh.S, sh.h, first, step);
     else if (sh.P == 2)
-        eval_part_list<POT, 2>(L, pl, k, posm, node, a, tg, eps2, sh.S, sh.h, first, step);
+        eval_part_list<POT, 2, FAR>(L, pl, k, posm, node, a, tg, eps2, sh.S, sh.h, first, step);
     else
-        eval_part_list<POT, 3>(L, pl, k, posm, node, a, tg, eps2, sh.S, sh.h, first, step);
+        eval_part_list<POT, 3, FAR>(L, pl, k, posm, node, a, tg, eps2, sh.S, sh.h, first, step);
 }
 
 __device__ __forceinline__ float box_dist2(float lx, float ly, float lz, float hx, float hy,
@@ -324,7 +326,7 @@ __device__ __forceinline__ float box_dist2(float lx, float ly, float lz, float h
     return ex * ex + ey * ey + ez * ez;
 }
 
-template <bool POT, bool QUAD, int WPB>
+template <bool POT, bool QUAD, int WPB, bool FAR>
 __global__ __launch_bounds__(WPB * 64) void k_traverse_group(
     const GroupInfo* __restrict__ groups, const int* __restrict__ gorder,
     const int* __restrict__ ngroups_dev, const SegInfo* __restrict__ segs,
@@ -446,7 +448,7 @@ __global__ __launch_bounds__(WPB * 64) void k_traverse_group(
                               __float_as_int(bb.w));
                 const float d2b = box_dist2(lx, ly, lz, hx, hy, hz, A.x, A.y, A.z);
                 accept = d2b > bb.x;
-                far = accept && d2b > far2 * bb.x;
+                far = FAR && accept && d2b > far2 * bb.x;
                 self_direct = !accept && (B.z - B.y + 1) <= leaf;
                 open = !accept && !self_direct;
             }
@@ -465,7 +467,7 @@ __global__ __launch_bounds__(WPB * 64) void k_traverse_group(
                 dn_val = v;
             }
             nn = uniform(nn + na);
-            tot_far += __popcll(ballot(far));
+            if (FAR) tot_far += __popcll(ballot(far));
             // particles: the node's own (summed), its single-particle children,
             // or the node itself as a monopole (~v)
             const bool lleaf = open && B.y == B.w, rleaf = open && B.z == B.w + 1;
@@ -474,7 +476,7 @@ __global__ __launch_bounds__(WPB * 64) void k_traverse_group(
             int tp;
             const int ex = wave_scan_excl_bits(cnt_p, pbits, &tp);
             if (np + tp > kPartList) {
-                eval_parts<POT>(L, pl, np, posm, node, a, tg, eps2, sh);
+                eval_parts<POT, FAR>(L, pl, np, posm, node, a, tg, eps2, sh);
                 tot_parts += np;
                 np = 0;
             }
@@ -520,7 +522,7 @@ __global__ __launch_bounds__(WPB * 64) void k_traverse_group(
         if (dp_lr & 1) pl[dp_pos + dp_self] = dp_split;
         if (dp_lr & 2) pl[dp_pos + dp_self + (dp_lr & 1)] = dp_split + 1;
         if (nn > 0) eval_nodes<POT, QUAD>(L, nl, nn, node, nodeQ, a, tg, eps2, sh);
-        if (np > 0) eval_parts<POT>(L, pl, np, posm, node, a, tg, eps2, sh);
+        if (np > 0) eval_parts<POT, FAR>(L, pl, np, posm, node, a, tg, eps2, sh);
         tot_nodes += nn;
         tot_parts += np;
         // Combine the sub-wave partial sums through LDS (the chunk arrays, 3 x 64
@@ -556,10 +558,13 @@ __global__ __launch_bounds__(WPB * 64) void k_traverse_group(
         n_part += (unsigned long long)(tot_parts - tot_far - 1) * (unsigned long long)gi.count;
         __builtin_amdgcn_wave_barrier();
     }
+    // This wave's counts in its own slot (plain stores: no contention on one
+    // address at the end of the launch); the host adds the launch's slots.
     if (l == 0) {
-        atomicAdd(&counters[0], n_node);
-        atomicAdd(&counters[1], n_part);
-        atomicAdd(&counters[2], n_far);
+        unsigned long long* c = counters + 3 * ((size_t)blockIdx.x * WPB + w);
+        c[0] = n_node;
+        c[1] = n_part;
+        c[2] = n_far;
     }
 }
 
@@ -613,7 +618,7 @@ __device__ __forceinline__ void combine_share(WaveLds& L, const Acc* a, const Sh
 // few groups per wave.  Lists that would overflow are evaluated early into
 // partial sums (kept unscaled in acc until the final evaluation).
 
-template <bool POT, bool QUAD, int TEAM>
+template <bool POT, bool QUAD, int TEAM, bool FAR>
 __global__ __launch_bounds__(TEAM * 64) void k_traverse_team(
     const GroupInfo* __restrict__ groups, const int* __restrict__ gorder,
     const int* __restrict__ ngroups_dev, const SegInfo* __restrict__ segs,
@@ -624,8 +629,10 @@ __global__ __launch_bounds__(TEAM * 64) void k_traverse_team(
 {
     __shared__ WaveLds lds[TEAM];  // the team's stack is lds[0].stk
     // per wave and round: accepted (quadrupole), particle-list entries, left
-    // pushes, right pushes, far nodes (monopole entries)
-    __shared__ int xch[TEAM][5];
+    // pushes, right pushes (one 16-byte row per wave); far nodes (monopole
+    // entries) apart, so the rows stay 16-byte reads
+    __shared__ __align__(16) int xch[TEAM][4];
+    __shared__ int xfar[TEAM];
     __shared__ int ctl[4];          // claimed group; stack top; spilled entries; overflow
     const int l = lane_id();
     const int w = threadIdx.x >> 6;
@@ -728,7 +735,7 @@ __global__ __launch_bounds__(TEAM * 64) void k_traverse_team(
                 const float d2b = box_dist2(lx, ly, lz, hx, hy, hz, A.x, A.y, A.z);
                 const bool accept = have && d2b > bb.x;
                 // beyond far * the opening radius: a monopole in the particle list
-                const bool far = accept && d2b > far2 * bb.x;
+                const bool far = FAR && accept && d2b > far2 * bb.x;
                 const bool near = accept && !far;
                 const bool self_direct = have && !accept && (B.z - B.y + 1) <= leaf;
                 const bool open = have && !accept && !self_direct;
@@ -738,15 +745,16 @@ __global__ __launch_bounds__(TEAM * 64) void k_traverse_team(
                 const int cnt_p = c_self + (lleaf ? 1 : 0) + (rleaf ? 1 : 0);
                 const bool lp = open && !lleaf, rp = open && !rleaf;
                 const uint64_t mA = ballot(near), m3 = ballot(lp), m4 = ballot(rp);
-                const uint64_t mF = ballot(far);
                 int tp;
                 const int ex = wave_scan_excl_bits(cnt_p, pbits, &tp);
+                int nfw = 0;
+                if (FAR) nfw = __popcll(ballot(far));
                 if (l == 0) {
                     xch[w][0] = __popcll(mA);
                     xch[w][1] = tp;
                     xch[w][2] = __popcll(m3);
                     xch[w][3] = __popcll(m4);
-                    xch[w][4] = __popcll(mF);
+                    if (FAR) xfar[w] = nfw;
                 }
                 __syncthreads();  // every entry of the round has been read; counts exchanged
                 int na_before = 0, tp_before = 0, nl_before = 0, nr_before = 0;
@@ -764,7 +772,7 @@ __global__ __launch_bounds__(TEAM * 64) void k_traverse_team(
                     tp_all += a1;
                     nl_all += a2;
                     nr_all += a3;
-                    nf_all += xch[k][4];
+                    if (FAR) nf_all += xfar[k];
                 }
                 // accepted nodes -> node list; particles and far nodes ->
                 // particle list
@@ -829,7 +837,7 @@ __global__ __launch_bounds__(TEAM * 64) void k_traverse_team(
                     eval_nodes<POT, QUAD>(L, nl, nn, node, nodeQ, a, tg, eps2, sh, w * kWave,
                                           TEAM * kWave);
                 if (np > w * kWave)
-                    eval_parts<POT>(L, pl, np, posm, node, a, tg, eps2, sh, w * kWave, TEAM * kWave);
+                    eval_parts<POT, FAR>(L, pl, np, posm, node, a, tg, eps2, sh, w * kWave, TEAM * kWave);
                 float4 r[kMaxPer];
                 combine_share(L, a, sh, n, r);
                 float4* const scr = L.ca;
@@ -884,10 +892,12 @@ __global__ __launch_bounds__(TEAM * 64) void k_traverse_team(
         }
         if (w == 0) n_part -= (unsigned long long)n;  // the particle itself, in its own list
     }
-    if (l == 0) {
-        atomicAdd(&counters[0], n_node);
-        atomicAdd(&counters[1], n_part);
-        atomicAdd(&counters[2], n_far);
+    // The team's counts (accumulated by wave 0) in the block's slot.
+    if (tid == 0) {
+        unsigned long long* c = counters + 3 * (size_t)blockIdx.x;
+        c[0] = n_node;
+        c[1] = n_part;
+        c[2] = n_far;
     }
 }
 
