@@ -23,9 +23,16 @@ Beside it, in the same line:
             4,096 FP64 targets, energy drift with plain and compensated
             positions, and the reference CPU octree on the same input.
 
+Far field: softened quadrupoles per accepted node, and for nodes farther than
+`--local-kappa` (4) half-diagonals of the target group's box a second-order
+local expansion at the box centre (20 coefficients per group, applied per
+target), which keeps the error within 1e-3 at theta 0.5 (checked every run).
+
 `roofline` is the dominant kernel of the headline workload, the group
-traversal: 20 flop per interaction (counted exactly by the kernel) over its
-average duration from HIP events on the engine stream, against MI355X's FP32
+traversal: 20 flop per particle or node interaction, 119 per local-expansion
+term and 72 per target's local evaluation (all counted exactly by the kernel)
+over its average duration from HIP events on the engine stream, against
+MI355X's FP32
 vector peak (157.3 TFLOP/s with packed FP32; the build issues unpacked FP32,
 whose ceiling is 78.6 -- both fractions are given).  `cpu_baseline` is the
 reference's own CPU octree (compiled from its sources into oracle/_ref) timed
@@ -47,6 +54,11 @@ METRIC = "particle-steps/s @θ=0.5, err≤1e-3: served/batched vs plain; p50/p99
 PEAK_FP32_TFLOPS = 157.3
 PEAK_FP32_VECTOR_TFLOPS = 78.6  # vector FP32 without packed operations
 FLOP_PER_INTERACTION = 20
+# A node's contribution to a group's local expansion (local_terms) and one
+# target's evaluation of the expansion (local_apply), executed FP32 operations
+# of csrc/gcs_traverse.hpp, a multiply-add counted as two.
+FLOP_PER_LOCAL_TERM = 119
+FLOP_PER_LOCAL_APPLY = 72
 # FP32 operations the kernel executes per interaction (a multiply-add counted as
 # two, the reciprocal square root as one): softened quadrupole node 52,
 # particle 19 (interact_n / interact_p in csrc/gcs_traverse.hpp).
@@ -56,6 +68,7 @@ PEAK_HBM_GBS = 8000.0
 TREE_BYTES_PER_PARTICLE = 376  # Morton 24 + sort 24 + permute 68 + init 88 + build 24 + agg 148
 EPS = 0.02
 DT = 0.005
+LOCAL_KAPPA = 4.0  # --local-kappa
 TUNE_TABLE = os.path.join(REPO, "galaxy-collision-simulator_amd", "gcs_amd",
                           "autotune_gfx950.txt")
 
@@ -122,30 +135,36 @@ def cpu_octree_rate(stars, p, sample_seconds, max_steps=50, fast=False):
 def kernel_roofline(e, p, n_steps=5):
     """Event-timed eager steps of the current state: the traversal's mean
     duration (HIP events on the engine stream around its launch), its
-    interaction counts, and the tree stage's time."""
+    interaction counts (node, particle, far monopole, local-expansion terms,
+    node-target pairs those terms stand for), and the tree stage's time."""
     import numpy as np
     import gcs_amd
     pe = gcs_amd.params(**{f: getattr(p, f) for f, _ in p._fields_})
     pe.use_graph = 0
     trav, stages = [], []
-    inter = (0, 0)
+    cnt = [0] * 5
     for _ in range(n_steps):
         e.step(pe, 1)
         st = e.stage_ms()
         stages.append(st)
         trav.append(float(st[5]))
-        inter = e.interactions()
+        cnt = e.interaction_counts()
     trav_ms = float(np.mean(trav))
-    n_inter = inter[0] + inter[1]
-    achieved = FLOP_PER_INTERACTION * n_inter / (trav_ms * 1e-3) / 1e12
-    executed = ((EXEC_FLOP_NODE * inter[0] + EXEC_FLOP_PARTICLE * inter[1])
+    applies = e.count()[1] if p.local_kappa > 0 else 0
+    flop = (FLOP_PER_INTERACTION * (cnt[0] + cnt[1] + cnt[2]) + FLOP_PER_LOCAL_TERM * cnt[3]
+            + FLOP_PER_LOCAL_APPLY * applies)
+    achieved = flop / (trav_ms * 1e-3) / 1e12
+    executed = ((EXEC_FLOP_NODE * cnt[0] + EXEC_FLOP_PARTICLE * (cnt[1] + cnt[2])
+                 + FLOP_PER_LOCAL_TERM * cnt[3] + FLOP_PER_LOCAL_APPLY * applies)
                 / (trav_ms * 1e-3) / 1e12)
     stage_ms = [float(x) for x in np.mean(np.array(stages), 0)]
-    return trav_ms, inter, achieved, executed, stage_ms
+    return trav_ms, (cnt, applies), achieved, executed, stage_ms
 
 
 def roofline_obj(workload, p, trav_ms, inter, achieved, executed):
-    key = (workload, p.leaf_size, p.block_size, p.waves_per_simd, p.group_size, p.team)
+    cnt, applies = inter
+    key = (workload, p.leaf_size, p.block_size, p.waves_per_simd, p.group_size, p.team,
+           p.local_kappa)
     return {"bound": "fp32-valu",
             "issue": "vector FP32 (VALU) with packed FP32 off; no matrix-core work",
             "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
@@ -154,8 +173,13 @@ def roofline_obj(workload, p, trav_ms, inter, achieved, executed):
             "traffic": TRAVERSE_TRAFFIC.get(key),
             "kernel": "k_traverse_team" if p.team > 1 else "k_traverse_group",
             "kernel_ms": trav_ms, "flop_per_interaction": FLOP_PER_INTERACTION,
-            "interactions_per_launch": inter[0] + inter[1],
-            "interactions_split": {"node": inter[0], "particle": inter[1]},
+            "flop_per_local_term": FLOP_PER_LOCAL_TERM,
+            "flop_per_local_apply": FLOP_PER_LOCAL_APPLY,
+            "interactions_per_launch": cnt[0] + cnt[1] + cnt[2],
+            "interactions_split": {"node": cnt[0], "particle": cnt[1], "far_monopole": cnt[2],
+                                   "local_terms": cnt[3], "local_applies": applies,
+                                   "node_target_pairs_via_local": cnt[4]},
+            "local_kappa": p.local_kappa,
             "executed": {"tflops": executed, "flop_per_node": EXEC_FLOP_NODE,
                          "flop_per_particle": EXEC_FLOP_PARTICLE,
                          "frac_of_unpacked_vector_peak": executed / PEAK_FP32_VECTOR_TFLOPS}}
@@ -174,6 +198,7 @@ def tree_roofline(stage_ms, n):
 def base_params(**kw):
     import numpy as np
     import gcs_amd
+    kw.setdefault("local_kappa", LOCAL_KAPPA)
     return gcs_amd.params(G=1.0, eps2=float(np.float32(EPS) ** 2), dt=DT, theta=0.5,
                           use_graph=1, **kw)
 
@@ -596,6 +621,7 @@ def copy_calibration(mib):
 
 
 def main():
+    global LOCAL_KAPPA
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20, help="timed steps of the headline")
@@ -625,6 +651,9 @@ def main():
     ap.add_argument("--precision-targets", type=int, default=65536)
     ap.add_argument("--bake-frames", type=int, default=30,
                     help="frames of each Bake mode at the README's conditions (0 = skip)")
+    ap.add_argument("--local-kappa", type=float, default=LOCAL_KAPPA,
+                    help="local expansions for nodes beyond this many group half-diagonals "
+                         "(0 = off: quadrupoles only)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--retune", action="store_true",
@@ -642,6 +671,7 @@ def main():
         copy_calibration(a.copy_calibration)
         return
 
+    LOCAL_KAPPA = a.local_kappa
     import gcs_amd
     if not a.retune and os.path.exists(TUNE_TABLE):
         gcs_amd.autotune_cache_load(TUNE_TABLE)
@@ -664,7 +694,8 @@ def main():
                                % b["instances"],
                    "instances": b["instances"], "particles": b["particles"], "theta": 0.5,
                    "eps": EPS, "dt": DT, "precision": "compensated FP32 hi/lo positions",
-                   "multipole": "quadrupole", "parallelism": "single GPU", **b["settings"]},
+                   "multipole": "quadrupole", "local_kappa": LOCAL_KAPPA,
+                   "parallelism": "single GPU", **b["settings"]},
         "vs_plain": b["vs_plain"],
         "accuracy": b["accuracy"],
         "roofline": b["roofline"],

@@ -304,6 +304,8 @@ int check_params(const gcs_params* p)
             return fail(GCS_ERR_INVALID, "a team of w waves takes block_size 64 w");
         if (!std::isfinite(p->mono_far) || (p->mono_far != 0.0f && p->mono_far < 1.0f))
             return fail(GCS_ERR_INVALID, "mono_far must be 0 (off) or >= 1");
+        if (!std::isfinite(p->local_kappa) || (p->local_kappa != 0.0f && p->local_kappa < 1.0f))
+            return fail(GCS_ERR_INVALID, "local_kappa must be 0 (off) or >= 1");
     } else {
         if (p->stack_depth < 2 || p->stack_depth > 1024)
             return fail(GCS_ERR_INVALID, "stack_depth must be in [2, 1024]");
@@ -336,34 +338,41 @@ float far2_of(const gcs_params& p)
     return p.mono_far > 0.0f ? p.mono_far * p.mono_far : __builtin_inff();
 }
 
+// Squared local-expansion radius factor (+inf: off; the kernels are then the
+// LOCAL = false instantiations and never read it).
+float kappa2_of(const gcs_params& p)
+{
+    return p.local_kappa > 0.0f ? p.local_kappa * p.local_kappa : __builtin_inff();
+}
+
 template <bool POT, bool QUAD, int WPB>
 void launch_group(gcs_engine* e, int blocks, const gcs_params& p, float4* posm, bool order)
 {
-    if (p.mono_far > 0.0f) {
+    if (p.local_kappa > 0.0f)
         k_traverse_group<POT, QUAD, WPB, true><<<blocks, WPB * kWave, 0, e->stream>>>(
             e->groups, order ? e->gorder : nullptr, e->ngroups, e->segs, posm, e->node, e->nodeQ,
             e->acc, e->slot_mem, e->qctr, e->wcount, e->err, p.G, p.eps2, p.leaf_size,
-            far2_of(p));
-        return;
-    }
-    k_traverse_group<POT, QUAD, WPB, false><<<blocks, WPB * kWave, 0, e->stream>>>(
-        e->groups, order ? e->gorder : nullptr, e->ngroups, e->segs, posm, e->node, e->nodeQ, e->acc,
-        e->slot_mem, e->qctr, e->wcount, e->err, p.G, p.eps2, p.leaf_size, far2_of(p));
+            far2_of(p), kappa2_of(p));
+    else
+        k_traverse_group<POT, QUAD, WPB, false><<<blocks, WPB * kWave, 0, e->stream>>>(
+            e->groups, order ? e->gorder : nullptr, e->ngroups, e->segs, posm, e->node, e->nodeQ,
+            e->acc, e->slot_mem, e->qctr, e->wcount, e->err, p.G, p.eps2, p.leaf_size,
+            far2_of(p), kappa2_of(p));
 }
 
 template <bool POT, int TEAM>
 void launch_team(gcs_engine* e, int blocks, const gcs_params& p, float4* posm, bool order)
 {
-    if (p.mono_far > 0.0f) {
+    if (p.local_kappa > 0.0f)
         k_traverse_team<POT, true, TEAM, true><<<blocks, TEAM * kWave, 0, e->stream>>>(
             e->groups, order ? e->gorder : nullptr, e->ngroups, e->segs, posm, e->node, e->nodeQ,
             e->acc, e->slot_mem, e->qctr, e->wcount, e->err, p.G, p.eps2, p.leaf_size,
-            far2_of(p));
-        return;
-    }
-    k_traverse_team<POT, true, TEAM, false><<<blocks, TEAM * kWave, 0, e->stream>>>(
-        e->groups, order ? e->gorder : nullptr, e->ngroups, e->segs, posm, e->node, e->nodeQ,
-        e->acc, e->slot_mem, e->qctr, e->wcount, e->err, p.G, p.eps2, p.leaf_size, far2_of(p));
+            far2_of(p), kappa2_of(p));
+    else
+        k_traverse_team<POT, true, TEAM, false><<<blocks, TEAM * kWave, 0, e->stream>>>(
+            e->groups, order ? e->gorder : nullptr, e->ngroups, e->segs, posm, e->node, e->nodeQ,
+            e->acc, e->slot_mem, e->qctr, e->wcount, e->err, p.G, p.eps2, p.leaf_size,
+            far2_of(p), kappa2_of(p));
 }
 
 template <int STACK>
@@ -680,15 +689,17 @@ int capture_graphs(gcs_engine* e, const gcs_params& p)
 }
 
 // Autotuner results per (traversal kind, theta, opening form, precision,
-// log2 particle bucket, log2 instance bucket), shared by every engine in the
-// process and readable/writable as a text table.
+// log2 particle bucket, log2 instance bucket, far-monopole factor, local
+// expansion factor), shared by every engine in the process and
+// readable/writable as a text table.
 struct TuneKey {
     int mac, open_offset, precision, nbucket, kbucket;
-    uint32_t theta_bits;
+    uint32_t theta_bits, far_bits, kappa_bits;
     bool operator<(const TuneKey& o) const
     {
-        return std::tie(mac, open_offset, precision, nbucket, kbucket, theta_bits) <
-               std::tie(o.mac, o.open_offset, o.precision, o.nbucket, o.kbucket, o.theta_bits);
+        return std::tie(mac, open_offset, precision, nbucket, kbucket, theta_bits, far_bits,
+                        kappa_bits) < std::tie(o.mac, o.open_offset, o.precision, o.nbucket,
+                                               o.kbucket, o.theta_bits, o.far_bits, o.kappa_bits);
     }
 };
 struct TuneVal {
@@ -716,8 +727,12 @@ TuneKey tune_key(const gcs_engine* e, const gcs_params& p)
     k.precision = p.precision;
     k.nbucket = log2_bucket(e->n_total);
     k.kbucket = log2_bucket((int64_t)e->inst.size());
-    float th = p.mac == GCS_MAC_GROUP ? p.theta : 0.0f;
+    const bool grp = p.mac == GCS_MAC_GROUP;
+    const float th = grp ? p.theta : 0.0f, far = grp ? p.mono_far : 0.0f,
+                kap = grp ? p.local_kappa : 0.0f;
     std::memcpy(&k.theta_bits, &th, 4);
+    std::memcpy(&k.far_bits, &far, 4);
+    std::memcpy(&k.kappa_bits, &kap, 4);
     return k;
 }
 
@@ -890,7 +905,7 @@ int gcs_create(int64_t cap_n, int32_t cap_k, int32_t device, gcs_engine** out)
     e->trav_slots = (int)std::max<int64_t>(4, std::min<int64_t>((int64_t)e->trav_blocks_cap * 4,
                                                               (cap_n + 3) / 4 * 4));
     if (!rc) rc = dalloc(&e->slot_mem, (size_t)e->trav_slots * kSlotInts);
-    if (!rc) rc = dalloc(&e->wcount, 3 * (size_t)e->trav_slots);
+    if (!rc) rc = dalloc(&e->wcount, (size_t)kCounts * e->trav_slots);
     if (!rc) rc = dalloc(&e->staging, 32 * n);
     if (rc) {
         free_all(e);
@@ -1498,15 +1513,16 @@ int gcs_autotune_cache_save(const char* path)
     std::lock_guard<std::mutex> lock(g_tune_mu);
     FILE* f = std::fopen(path, "w");
     if (!f) return fail(GCS_ERR_INVALID, std::string("cannot write ") + path);
-    std::fprintf(f, "gcs-autotune 1\n");
-    std::fprintf(f, "# mac open_offset precision log2_n log2_k theta_bits group leaf block team "
-                    "waves_per_simd stack_depth ms\n");
+    std::fprintf(f, "gcs-autotune 2\n");
+    std::fprintf(f, "# mac open_offset precision log2_n log2_k theta_bits mono_far_bits "
+                    "local_kappa_bits group leaf block team waves_per_simd stack_depth ms\n");
     for (const auto& kv : g_tune) {
         const TuneKey& k = kv.first;
         const TuneVal& v = kv.second;
-        std::fprintf(f, "%d %d %d %d %d %08x %d %d %d %d %d %d %.6g\n", k.mac, k.open_offset,
-                     k.precision, k.nbucket, k.kbucket, k.theta_bits, v.group, v.leaf, v.block,
-                     v.team, v.wps, v.stack, (double)v.ms);
+        std::fprintf(f, "%d %d %d %d %d %08x %08x %08x %d %d %d %d %d %d %.6g\n", k.mac,
+                     k.open_offset, k.precision, k.nbucket, k.kbucket, k.theta_bits, k.far_bits,
+                     k.kappa_bits, v.group, v.leaf, v.block, v.team, v.wps, v.stack,
+                     (double)v.ms);
     }
     const bool ok = std::fclose(f) == 0;
     return ok ? GCS_OK : fail(GCS_ERR_INVALID, std::string("cannot write ") + path);
@@ -1522,7 +1538,7 @@ int gcs_autotune_cache_load(const char* path)
     bool header = false, bad = false;
     while (std::fgets(line, sizeof(line), f)) {
         if (!header) {
-            header = std::strncmp(line, "gcs-autotune 1", 14) == 0;
+            header = std::strncmp(line, "gcs-autotune 2", 14) == 0;
             if (!header) { bad = true; break; }
             continue;
         }
@@ -1530,9 +1546,10 @@ int gcs_autotune_cache_load(const char* path)
         TuneKey k;
         TuneVal v;
         double ms = 0.0;
-        if (std::sscanf(line, "%d %d %d %d %d %x %d %d %d %d %d %d %lf", &k.mac, &k.open_offset,
-                        &k.precision, &k.nbucket, &k.kbucket, &k.theta_bits, &v.group, &v.leaf,
-                        &v.block, &v.team, &v.wps, &v.stack, &ms) != 13) {
+        if (std::sscanf(line, "%d %d %d %d %d %x %x %x %d %d %d %d %d %d %lf", &k.mac,
+                        &k.open_offset, &k.precision, &k.nbucket, &k.kbucket, &k.theta_bits,
+                        &k.far_bits, &k.kappa_bits, &v.group, &v.leaf, &v.block, &v.team, &v.wps,
+                        &v.stack, &ms) != 15) {
             bad = true;
             break;
         }
@@ -1657,32 +1674,41 @@ int gcs_read_bounds(gcs_engine* e, int32_t id, float* mn, float* mx)
     return GCS_OK;
 }
 
-// Interaction counts of the last traversal: node, particle, monopole.
-static int read_counts(gcs_engine* e, unsigned long long* c3)
+// Interaction counts of the last traversal: per-target node, particle, of
+// the nodes the monopoles, local-expansion (group, node) pairs, the target
+// interactions those stand for.
+static int read_counts(gcs_engine* e, unsigned long long* c5)
 {
     HIPCHK(hipSetDevice(e->device));
     HIPCHK(hipStreamSynchronize(e->stream));
-    c3[0] = c3[1] = c3[2] = 0;
+    for (int k = 0; k < kCounts; ++k) c5[k] = 0;
     if (e->last_mac == GCS_MAC_GROUP) {
-        std::vector<unsigned long long> w(3 * (size_t)e->wslots);
+        std::vector<unsigned long long> w((size_t)kCounts * e->wslots);
         if (!w.empty())
             HIPCHK(hipMemcpy(w.data(), e->wcount, sizeof(unsigned long long) * w.size(),
                              hipMemcpyDeviceToHost));
-        for (size_t i = 0; i < w.size(); i += 3) {
-            c3[0] += w[i];
-            c3[1] += w[i + 1];
-            c3[2] += w[i + 2];
-        }
+        for (size_t i = 0; i < w.size(); i += kCounts)
+            for (int k = 0; k < kCounts; ++k) c5[k] += w[i + k];
     } else {
-        HIPCHK(hipMemcpy(c3, e->counters, 3 * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(c5, e->counters, 3 * sizeof(unsigned long long), hipMemcpyDeviceToHost));
     }
+    return GCS_OK;
+}
+
+int gcs_read_interaction_counts(gcs_engine* e, int64_t* out5)
+{
+    if (!e || !out5) return fail(GCS_ERR_INVALID, "null argument");
+    unsigned long long c[kCounts];
+    int rc = read_counts(e, c);
+    if (rc) return rc;
+    for (int k = 0; k < kCounts; ++k) out5[k] = (int64_t)c[k];
     return GCS_OK;
 }
 
 int gcs_read_interactions(gcs_engine* e, int64_t* out2)
 {
     if (!e || !out2) return fail(GCS_ERR_INVALID, "null argument");
-    unsigned long long c[3];
+    unsigned long long c[kCounts];
     int rc = read_counts(e, c);
     if (rc) return rc;
     out2[0] = (int64_t)c[0];
@@ -1693,7 +1719,7 @@ int gcs_read_interactions(gcs_engine* e, int64_t* out2)
 int gcs_read_monopole_interactions(gcs_engine* e, int64_t* out)
 {
     if (!e || !out) return fail(GCS_ERR_INVALID, "null argument");
-    unsigned long long c[3];
+    unsigned long long c[kCounts];
     int rc = read_counts(e, c);
     if (rc) return rc;
     *out = (int64_t)c[2];

@@ -48,8 +48,11 @@ namespace gcs {
 constexpr int kLdsStack = 1024;    // LDS stack entries per wave
 constexpr int kNodeList = 4096;    // node-list entries per wave slot
 constexpr int kPartList = 16384;   // particle-list entries per wave slot
+constexpr int kLocList = 4096;     // local-expansion list entries per wave slot
 constexpr int kMaxLeaf = 64;       // largest direct-sum bucket
-constexpr int kSlotInts = kStackCap + kNodeList + kPartList;
+constexpr int kSlotInts = kStackCap + kNodeList + kPartList + kLocList;
+constexpr int kLocTerms = 20;      // a0 (3), A (6), B (10), phi0
+constexpr int kCounts = 5;         // per-wave interaction counts
 
 constexpr int kMaxSplit = 16;               // most sub-waves a list is split into
 constexpr int kMaxPer = 3;                  // most targets per lane
@@ -60,6 +63,7 @@ struct WaveLds {
     float4 ca[kChunk];   // COM and mass, or position and mass
     float4 cq0[kChunk];  // Qxx, Qyy, Qzz, Qxy
     float4 cq1[kChunk];  // Qxz, Qyz, (width), T
+    float loc[kLocTerms + 4];  // the group's local-expansion terms (this wave's share)
 };
 
 struct Acc {
@@ -234,18 +238,17 @@ __device__ __forceinline__ void eval_node_list(WaveLds& L, const int* nl, int k,
 // layout); kNoEntry past the list end (a zero-mass source).
 constexpr int kNoEntry = (int)0x80000000;
 
-template <bool FAR>
 __device__ __forceinline__ float4 point_source(int e, const float4* __restrict__ posm,
                                                const float4* __restrict__ node)
 {
     if (e >= 0) return posm[e];
-    if (FAR && e != kNoEntry) return node[2 * (~e)];
+    if (e != kNoEntry) return node[2 * (~e)];
     return make_float4(0.f, 0.f, 0.f, 0.f);
 }
 
 // Phase 2 over the particle list pl[0..k), same chunks, pipeline and split.
 // Entries are particles or far nodes taken as monopoles (point_source).
-template <bool POT, int P, bool FAR>
+template <bool POT, int P>
 __device__ __forceinline__ void eval_part_list(WaveLds& L, const int* pl, int k,
                                                const float4* __restrict__ posm,
                                                const float4* __restrict__ node, Acc* a,
@@ -255,11 +258,11 @@ __device__ __forceinline__ void eval_part_list(WaveLds& L, const int* pl, int k,
     const int l = lane_id();
     const int idx = first + l < k ? pl[first + l] : kNoEntry;
     int idx_next = first + step + l < k ? pl[first + step + l] : kNoEntry;
-    float4 X = point_source<FAR>(idx, posm, node);
+    float4 X = point_source(idx, posm, node);
     for (int base = first; base < k; base += step) {
         L.ca[l] = X;
         __builtin_amdgcn_wave_barrier();
-        X = point_source<FAR>(idx_next, posm, node);
+        X = point_source(idx_next, posm, node);
         const int j2 = base + 2 * step + l;
         idx_next = j2 < k ? pl[j2] : kNoEntry;
         const int trips = uniform((min(kWave, k - base) + S - 1) / S);
@@ -302,18 +305,146 @@ __device__ __forceinline__ void eval_nodes(WaveLds& L, const int* nl, int k, con
         eval_node_list<POT, QUAD, 3>(L, nl, k, node, nodeQ, a, tg, eps2, sh.S, sh.h, first, step);
 }
 
-template <bool POT, bool FAR>
+template <bool POT>
 __device__ __forceinline__ void eval_parts(WaveLds& L, const int* pl, int k, const float4* posm,
                                            const float4* node, Acc* a, const Targets& tg,
                                            float eps2, const Share& sh, int first = 0,
                                            int step = kWave)
 {
     if (sh.P == 1)
-        eval_part_list<POT, 1, FAR>(L, pl, k, posm, node, a, tg, eps2, sh.S, sh.h, first, step);
+        eval_part_list<POT, 1>(L, pl, k, posm, node, a, tg, eps2, sh.S, sh.h, first, step);
     else if (sh.P == 2)
-        eval_part_list<POT, 2, FAR>(L, pl, k, posm, node, a, tg, eps2, sh.S, sh.h, first, step);
+        eval_part_list<POT, 2>(L, pl, k, posm, node, a, tg, eps2, sh.S, sh.h, first, step);
     else
-        eval_part_list<POT, 3, FAR>(L, pl, k, posm, node, a, tg, eps2, sh.S, sh.h, first, step);
+        eval_part_list<POT, 3>(L, pl, k, posm, node, a, tg, eps2, sh.S, sh.h, first, step);
+}
+
+// ---- Far field through local expansions -------------------------------------
+// An accepted node whose COM lies farther than kappa group radii from the
+// centre c of the group's box is not evaluated per target: its field enters a
+// second-order expansion at c -- the softened quadrupole field a0 at c and the
+// softened monopole's first and second derivatives A and B -- and each target
+// adds a0 + A y + B:yy/2 (y its offset from c) once, with the potential
+// phi0 + a0.y + y.A.y/2 (the accumulated sign convention: phi = -G x this).
+// Terms: a0 x y z, A xx yy zz xy xz yz, B xxx yyy zzz xxy xxz xyy yyz xzz yzz
+// xyz, phi0.
+
+// The group's box centre and local-expansion radius squared, rounded as the
+// oracle forms them (no contraction).
+__device__ __forceinline__ void local_frame(float lx, float ly, float lz, float hx, float hy,
+                                            float hz, float kappa2, float& cx, float& cy,
+                                            float& cz, float& lim)
+{
+#pragma clang fp contract(off)
+    cx = (lx + hx) * 0.5f;
+    cy = (ly + hy) * 0.5f;
+    cz = (lz + hz) * 0.5f;
+    const float ex = hx - lx, ey = hy - ly, ez = hz - lz;
+    const float rg2 = 0.25f * (ex * ex + ey * ey + ez * ez);
+    lim = kappa2 * rg2;
+}
+
+__device__ __forceinline__ float dist2_nc(float dx, float dy, float dz)
+{
+#pragma clang fp contract(off)
+    return dx * dx + dy * dy + dz * dz;
+}
+
+// Adds a node's terms (r = COM - c, mass m, quadrupole q0/q1) to L.
+template <bool POT>
+__device__ __forceinline__ void local_terms(float rx, float ry, float rz, float m, float4 q0,
+                                            float4 q1, float eps2, float* L)
+{
+    const float r2 = __builtin_fmaf(rx, rx, __builtin_fmaf(ry, ry, __builtin_fmaf(rz, rz, eps2)));
+    const float iR = __builtin_amdgcn_rsqf(r2);
+    const float iR2 = iR * iR, iR3 = iR * iR2, iR5 = iR3 * iR2, iR7 = iR5 * iR2;
+    const float qx = __builtin_fmaf(q0.x, rx, __builtin_fmaf(q0.w, ry, q1.x * rz));
+    const float qy = __builtin_fmaf(q0.w, rx, __builtin_fmaf(q0.y, ry, q1.y * rz));
+    const float qz = __builtin_fmaf(q1.x, rx, __builtin_fmaf(q1.y, ry, q0.z * rz));
+    const float dqd =
+        __builtin_fmaf(-q1.w, eps2, __builtin_fmaf(rx, qx, __builtin_fmaf(ry, qy, rz * qz)));
+    const float t = dqd * iR5;
+    const float f = __builtin_fmaf(2.5f * t, iR2, m * iR3);
+    L[0] += __builtin_fmaf(rx, f, -qx * iR5);
+    L[1] += __builtin_fmaf(ry, f, -qy * iR5);
+    L[2] += __builtin_fmaf(rz, f, -qz * iR5);
+    const float m3 = m * iR3, m5 = 3.0f * m * iR5, m7 = 15.0f * m * iR7;
+    const float xx = rx * rx, yy = ry * ry, zz = rz * rz;
+    L[3] += __builtin_fmaf(m5, xx, -m3);
+    L[4] += __builtin_fmaf(m5, yy, -m3);
+    L[5] += __builtin_fmaf(m5, zz, -m3);
+    L[6] += m5 * rx * ry;
+    L[7] += m5 * rx * rz;
+    L[8] += m5 * ry * rz;
+    const float m7x = m7 * rx, m7y = m7 * ry, m7z = m7 * rz;
+    L[9] += rx * __builtin_fmaf(m7, xx, -3.0f * m5);
+    L[10] += ry * __builtin_fmaf(m7, yy, -3.0f * m5);
+    L[11] += rz * __builtin_fmaf(m7, zz, -3.0f * m5);
+    L[12] += ry * __builtin_fmaf(m7x, rx, -m5);
+    L[13] += rz * __builtin_fmaf(m7x, rx, -m5);
+    L[14] += rx * __builtin_fmaf(m7y, ry, -m5);
+    L[15] += rz * __builtin_fmaf(m7y, ry, -m5);
+    L[16] += rx * __builtin_fmaf(m7z, rz, -m5);
+    L[17] += ry * __builtin_fmaf(m7z, rz, -m5);
+    L[18] += m7x * ry * rz;
+    if (POT) L[19] += __builtin_fmaf(0.5f, t, m * iR);
+}
+
+// The local list ll[0..k): the chunks of 64 entries that start at first,
+// first + step, ... (one node per lane), summed over the wave (a butterfly
+// in a fixed pairing, identical on every lane) and added to tot (LDS).
+template <bool POT>
+__device__ __forceinline__ void eval_local_list(float* tot, const int* ll, int k,
+                                                const float4* __restrict__ node,
+                                                const float4* __restrict__ nodeQ, float cx,
+                                                float cy, float cz, float eps2, int first,
+                                                int step)
+{
+    const int l = lane_id();
+    float c[kLocTerms];
+#pragma unroll
+    for (int t = 0; t < kLocTerms; ++t) c[t] = 0.0f;
+    for (int base = first; base < k; base += step) {
+        const int j = base + l;
+        if (j < k) {
+            const int v = ll[j];
+            const float4 A = node[2 * v], q0 = nodeQ[2 * v], q1 = nodeQ[2 * v + 1];
+            local_terms<POT>(A.x - cx, A.y - cy, A.z - cz, A.w, q0, q1, eps2, c);
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < kLocTerms; ++t) c[t] = wave_sum(c[t]);
+    if (l == 0) {
+#pragma unroll
+        for (int t = 0; t < kLocTerms; ++t) tot[t] += c[t];
+    }
+    __builtin_amdgcn_wave_barrier();
+}
+
+// The expansion L at offset y from the centre, added to r (x, y, z, phi).
+template <bool POT>
+__device__ __forceinline__ void local_apply(const float* L, float yx, float yy, float yz,
+                                            float4& r)
+{
+    const float xx = yx * yx, y2 = yy * yy, zz = yz * yz, xy = yx * yy, xz = yx * yz,
+                yz2 = yy * yz;
+    const float ax = __builtin_fmaf(L[3], yx, __builtin_fmaf(L[6], yy, L[7] * yz));
+    const float ay = __builtin_fmaf(L[6], yx, __builtin_fmaf(L[4], yy, L[8] * yz));
+    const float az = __builtin_fmaf(L[7], yx, __builtin_fmaf(L[8], yy, L[5] * yz));
+    const float bx = __builtin_fmaf(L[9], xx, __builtin_fmaf(L[14], y2, L[16] * zz)) +
+                     2.0f * __builtin_fmaf(L[12], xy, __builtin_fmaf(L[13], xz, L[18] * yz2));
+    const float by = __builtin_fmaf(L[12], xx, __builtin_fmaf(L[10], y2, L[17] * zz)) +
+                     2.0f * __builtin_fmaf(L[14], xy, __builtin_fmaf(L[18], xz, L[15] * yz2));
+    const float bz = __builtin_fmaf(L[13], xx, __builtin_fmaf(L[15], y2, L[11] * zz)) +
+                     2.0f * __builtin_fmaf(L[18], xy, __builtin_fmaf(L[16], xz, L[17] * yz2));
+    r.x += __builtin_fmaf(0.5f, bx, L[0] + ax);
+    r.y += __builtin_fmaf(0.5f, by, L[1] + ay);
+    r.z += __builtin_fmaf(0.5f, bz, L[2] + az);
+    if (POT) {
+        const float a0y = __builtin_fmaf(L[0], yx, __builtin_fmaf(L[1], yy, L[2] * yz));
+        const float yAy = __builtin_fmaf(ax, yx, __builtin_fmaf(ay, yy, az * yz));
+        r.w += L[19] + __builtin_fmaf(0.5f, yAy, a0y);
+    }
 }
 
 __device__ __forceinline__ float box_dist2(float lx, float ly, float lz, float hx, float hy,
@@ -326,14 +457,14 @@ __device__ __forceinline__ float box_dist2(float lx, float ly, float lz, float h
     return ex * ex + ey * ey + ez * ez;
 }
 
-template <bool POT, bool QUAD, int WPB, bool FAR>
+template <bool POT, bool QUAD, int WPB, bool LOCAL>
 __global__ __launch_bounds__(WPB * 64) void k_traverse_group(
     const GroupInfo* __restrict__ groups, const int* __restrict__ gorder,
     const int* __restrict__ ngroups_dev, const SegInfo* __restrict__ segs,
     const float4* __restrict__ posm, const float4* __restrict__ node,
     const float4* __restrict__ nodeQ, float4* __restrict__ acc, int* __restrict__ slot_mem,
     unsigned* qctr, unsigned long long* counters, unsigned* err, float G, float eps2, int leaf,
-    float far2)
+    float far2, float kappa2)
 {
     __shared__ WaveLds lds[WPB];
     const int l = lane_id();
@@ -342,10 +473,11 @@ __global__ __launch_bounds__(WPB * 64) void k_traverse_group(
     int* const spill = slot_mem + (size_t)(blockIdx.x * WPB + w) * kSlotInts;
     int* const nl = spill + kStackCap;
     int* const pl = nl + kNodeList;
+    int* const ll = pl + kPartList;
     const int xcd = blockIdx.x % kNumXcd;
     const int ngroups = uniform(*ngroups_dev);
     const int per = (ngroups + kNumXcd - 1) / kNumXcd;
-    unsigned long long n_node = 0, n_part = 0, n_far = 0;
+    unsigned long long n_node = 0, n_part = 0, n_far = 0, n_loc = 0, n_loct = 0;
     int q = 0;
     const float self_inv = __builtin_amdgcn_rsqf(eps2);
     const int pbits = 32 - __builtin_clz((unsigned)(leaf + 2));  // bits of a round's count
@@ -398,8 +530,14 @@ __global__ __launch_bounds__(WPB * 64) void k_traverse_group(
             tg.z[u] = pu.z;
             a[u] = Acc{0.f, 0.f, 0.f, 0.f};
         }
-        int nn = 0, np = 0, top = 0, spilled = 0;
-        long long tot_nodes = 0, tot_parts = 0, tot_far = 0;
+        float gcx = 0.f, gcy = 0.f, gcz = 0.f, glim = 0.f;
+        if (LOCAL) {
+            local_frame(lx, ly, lz, hx, hy, hz, kappa2, gcx, gcy, gcz, glim);
+            if (l < kLocTerms) L.loc[l] = 0.0f;
+        }
+        int nn = 0, np = 0, nloc = 0, top = 0, spilled = 0;
+        long long tot_nodes = 0, tot_parts = 0, tot_far = 0, tot_loc = 0;
+        int dl_pos = -1, dl_val = 0;  // local-list entry of the previous round
         // List stores of one round are issued in the next round after its node
         // loads, so that the wait for those loads does not also wait for them
         // (vector memory operations complete in issue order).
@@ -429,7 +567,7 @@ __global__ __launch_bounds__(WPB * 64) void k_traverse_group(
             if (have) v = L.stk[top + l];
             float4 A = make_float4(0.f, 0.f, 0.f, 0.f);
             int4 B = make_int4(0, 0, 0, 0);
-            bool accept = false, far = false, self_direct = false, open = false;
+            bool accept = false, far = false, loc = false, self_direct = false, open = false;
             float4 bb = make_float4(0.f, 0.f, 0.f, 0.f);
             if (have) {
                 A = node[2 * v];
@@ -437,6 +575,8 @@ __global__ __launch_bounds__(WPB * 64) void k_traverse_group(
             }
             // The previous round's list entries.
             if (dn_pos >= 0) nl[dn_pos] = dn_val;
+            if (LOCAL && dl_pos >= 0) ll[dl_pos] = dl_val;
+            dl_pos = -1;
             for (int k = 0; k < dp_self; ++k) pl[dp_pos + k] = dp_first + k;
             if (dp_lr & 1) pl[dp_pos + dp_self] = dp_split;
             if (dp_lr & 2) pl[dp_pos + dp_self + (dp_lr & 1)] = dp_split + 1;
@@ -448,13 +588,30 @@ __global__ __launch_bounds__(WPB * 64) void k_traverse_group(
                               __float_as_int(bb.w));
                 const float d2b = box_dist2(lx, ly, lz, hx, hy, hz, A.x, A.y, A.z);
                 accept = d2b > bb.x;
-                far = FAR && accept && d2b > far2 * bb.x;
+                loc = LOCAL && accept && dist2_nc(A.x - gcx, A.y - gcy, A.z - gcz) > glim;
+                far = accept && !loc && d2b > far2 * bb.x;
                 self_direct = !accept && (B.z - B.y + 1) <= leaf;
                 open = !accept && !self_direct;
             }
-            // accepted nodes within far * opening radius -> node list
-            // (quadrupole); farther ones -> particle list (monopole)
-            const bool near = accept && !far;
+            // accepted nodes far from the group -> local list; beyond far x the
+            // opening radius -> particle list (monopole); the others -> node
+            // list (quadrupole)
+            if (LOCAL) {
+                const uint64_t mL = ballot(loc);
+                const int nlo = __popcll(mL);
+                if (nloc + nlo > kLocList) {
+                    eval_local_list<POT>(L.loc, ll, nloc, node, nodeQ, gcx, gcy, gcz, eps2, 0,
+                                         kWave);
+                    nloc = 0;
+                }
+                if (loc) {
+                    dl_pos = nloc + mbcnt(mL);
+                    dl_val = v;
+                }
+                nloc = uniform(nloc + nlo);
+                tot_loc += nlo;
+            }
+            const bool near = accept && !far && !loc;
             const uint64_t mA = ballot(near);
             const int na = __popcll(mA);
             if (nn + na > kNodeList) {
@@ -467,7 +624,7 @@ __global__ __launch_bounds__(WPB * 64) void k_traverse_group(
                 dn_val = v;
             }
             nn = uniform(nn + na);
-            if (FAR) tot_far += __popcll(ballot(far));
+            tot_far += __popcll(ballot(far));
             // particles: the node's own (summed), its single-particle children,
             // or the node itself as a monopole (~v)
             const bool lleaf = open && B.y == B.w, rleaf = open && B.z == B.w + 1;
@@ -476,7 +633,7 @@ __global__ __launch_bounds__(WPB * 64) void k_traverse_group(
             int tp;
             const int ex = wave_scan_excl_bits(cnt_p, pbits, &tp);
             if (np + tp > kPartList) {
-                eval_parts<POT, FAR>(L, pl, np, posm, node, a, tg, eps2, sh);
+                eval_parts<POT>(L, pl, np, posm, node, a, tg, eps2, sh);
                 tot_parts += np;
                 np = 0;
             }
@@ -518,11 +675,14 @@ __global__ __launch_bounds__(WPB * 64) void k_traverse_group(
             break;
         }
         if (dn_pos >= 0) nl[dn_pos] = dn_val;
+        if (LOCAL && dl_pos >= 0) ll[dl_pos] = dl_val;
         for (int k = 0; k < dp_self; ++k) pl[dp_pos + k] = dp_first + k;
         if (dp_lr & 1) pl[dp_pos + dp_self] = dp_split;
         if (dp_lr & 2) pl[dp_pos + dp_self + (dp_lr & 1)] = dp_split + 1;
+        if (LOCAL && nloc > 0)
+            eval_local_list<POT>(L.loc, ll, nloc, node, nodeQ, gcx, gcy, gcz, eps2, 0, kWave);
         if (nn > 0) eval_nodes<POT, QUAD>(L, nl, nn, node, nodeQ, a, tg, eps2, sh);
-        if (np > 0) eval_parts<POT, FAR>(L, pl, np, posm, node, a, tg, eps2, sh);
+        if (np > 0) eval_parts<POT>(L, pl, np, posm, node, a, tg, eps2, sh);
         tot_nodes += nn;
         tot_parts += np;
         // Combine the sub-wave partial sums through LDS (the chunk arrays, 3 x 64
@@ -550,21 +710,27 @@ __global__ __launch_bounds__(WPB * 64) void k_traverse_group(
                 r.z += v.z;
                 r.w += v.w;
             }
-            const float ph = POT ? -G * (r.w - posm[gi.first + j].w * self_inv) : 0.0f;
+            const float4 pj = posm[gi.first + j];
+            if (LOCAL && tot_loc > 0) local_apply<POT>(L.loc, pj.x - gcx, pj.y - gcy, pj.z - gcz, r);
+            const float ph = POT ? -G * (r.w - pj.w * self_inv) : 0.0f;
             acc[gi.first + j] = make_float4(G * r.x, G * r.y, G * r.z, ph);
         }
         n_node += (unsigned long long)(tot_nodes + tot_far) * (unsigned long long)gi.count;
         n_far += (unsigned long long)tot_far * (unsigned long long)gi.count;
         n_part += (unsigned long long)(tot_parts - tot_far - 1) * (unsigned long long)gi.count;
+        n_loc += (unsigned long long)tot_loc;
+        n_loct += (unsigned long long)tot_loc * (unsigned long long)gi.count;
         __builtin_amdgcn_wave_barrier();
     }
     // This wave's counts in its own slot (plain stores: no contention on one
     // address at the end of the launch); the host adds the launch's slots.
     if (l == 0) {
-        unsigned long long* c = counters + 3 * ((size_t)blockIdx.x * WPB + w);
+        unsigned long long* c = counters + kCounts * ((size_t)blockIdx.x * WPB + w);
         c[0] = n_node;
         c[1] = n_part;
         c[2] = n_far;
+        c[3] = n_loc;
+        c[4] = n_loct;
     }
 }
 
@@ -618,21 +784,22 @@ __device__ __forceinline__ void combine_share(WaveLds& L, const Acc* a, const Sh
 // few groups per wave.  Lists that would overflow are evaluated early into
 // partial sums (kept unscaled in acc until the final evaluation).
 
-template <bool POT, bool QUAD, int TEAM, bool FAR>
+template <bool POT, bool QUAD, int TEAM, bool LOCAL>
 __global__ __launch_bounds__(TEAM * 64) void k_traverse_team(
     const GroupInfo* __restrict__ groups, const int* __restrict__ gorder,
     const int* __restrict__ ngroups_dev, const SegInfo* __restrict__ segs,
     const float4* __restrict__ posm, const float4* __restrict__ node,
     const float4* __restrict__ nodeQ, float4* __restrict__ acc, int* __restrict__ slot_mem,
     unsigned* qctr, unsigned long long* counters, unsigned* err, float G, float eps2, int leaf,
-    float far2)
+    float far2, float kappa2)
 {
     __shared__ WaveLds lds[TEAM];  // the team's stack is lds[0].stk
     // per wave and round: accepted (quadrupole), particle-list entries, left
     // pushes, right pushes (one 16-byte row per wave); far nodes (monopole
-    // entries) apart, so the rows stay 16-byte reads
+    // entries) and local-list entries apart, so the rows stay 16-byte reads
     __shared__ __align__(16) int xch[TEAM][4];
     __shared__ int xfar[TEAM];
+    __shared__ int xloc[TEAM];
     __shared__ int ctl[4];          // claimed group; stack top; spilled entries; overflow
     const int l = lane_id();
     const int w = threadIdx.x >> 6;
@@ -642,10 +809,12 @@ __global__ __launch_bounds__(TEAM * 64) void k_traverse_team(
     int* const spill = slot_mem + (size_t)blockIdx.x * TEAM * kSlotInts;
     int* const nl = spill + kStackCap;
     int* const pl = nl + kNodeList;
+    int* const ll = pl + kPartList;
     const int xcd = blockIdx.x % kNumXcd;
     const int ngroups = uniform(*ngroups_dev);
     const int per = (ngroups + kNumXcd - 1) / kNumXcd;
-    unsigned long long n_node = 0, n_part = 0, n_far = 0;
+    const bool use_far = far2 < __builtin_inff();
+    unsigned long long n_node = 0, n_part = 0, n_far = 0, n_loc = 0, n_loct = 0;
     int q = 0;
     const float self_inv = __builtin_amdgcn_rsqf(eps2);
     const int pbits = 32 - __builtin_clz((unsigned)(leaf + 2));
@@ -654,6 +823,7 @@ __global__ __launch_bounds__(TEAM * 64) void k_traverse_team(
     // particle entries; lists are evaluated before a round could overflow them.
     const int node_room = kNodeList - TEAM * kWave;
     const int part_room = kPartList - TEAM * kWave * (leaf + 2);
+    const int loc_room = kLocList - TEAM * kWave;
     if (l < kMaxSplit) {
         const float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
         L.ca[kWave + l] = z;
@@ -692,9 +862,15 @@ __global__ __launch_bounds__(TEAM * 64) void k_traverse_team(
         lx = wave_min(lx); ly = wave_min(ly); lz = wave_min(lz);
         hx = wave_max(hx); hy = wave_max(hy); hz = wave_max(hz);
         const int n = gi.count;
-        int nn = 0, np = 0, nfar = 0;
+        int nn = 0, np = 0, nfar = 0, nloc = 0;
         int top = 0, spilled = 0;
         bool flushed = false;
+        float gcx = 0.f, gcy = 0.f, gcz = 0.f, glim = 0.f;
+        long long tot_loc = 0;
+        if (LOCAL) {
+            local_frame(lx, ly, lz, hx, hy, hz, kappa2, gcx, gcy, gcz, glim);
+            if (l < kLocTerms) L.loc[l] = 0.0f;
+        }
         if (sg.n == 1) {
             if (tid == 0) pl[0] = sg.start;
             np = 1;
@@ -706,7 +882,7 @@ __global__ __launch_bounds__(TEAM * 64) void k_traverse_team(
             // Walk until the stack is empty or a list could overflow.
             bool need = false;
             while (top > 0 || spilled > 0) {
-                if (nn > node_room || np > part_room) {
+                if (nn > node_room || np > part_room || (LOCAL && nloc > loc_room)) {
                     need = true;
                     break;
                 }
@@ -734,9 +910,12 @@ __global__ __launch_bounds__(TEAM * 64) void k_traverse_team(
                                          __float_as_int(bb.z), __float_as_int(bb.w));
                 const float d2b = box_dist2(lx, ly, lz, hx, hy, hz, A.x, A.y, A.z);
                 const bool accept = have && d2b > bb.x;
-                // beyond far * the opening radius: a monopole in the particle list
-                const bool far = FAR && accept && d2b > far2 * bb.x;
-                const bool near = accept && !far;
+                // far from the group: the local list; beyond far * the opening
+                // radius: a monopole in the particle list
+                const bool loc =
+                    LOCAL && accept && dist2_nc(A.x - gcx, A.y - gcy, A.z - gcz) > glim;
+                const bool far = accept && !loc && d2b > far2 * bb.x;
+                const bool near = accept && !far && !loc;
                 const bool self_direct = have && !accept && (B.z - B.y + 1) <= leaf;
                 const bool open = have && !accept && !self_direct;
                 const bool lleaf = open && B.y == B.w, rleaf = open && B.z == B.w + 1;
@@ -748,17 +927,20 @@ __global__ __launch_bounds__(TEAM * 64) void k_traverse_team(
                 int tp;
                 const int ex = wave_scan_excl_bits(cnt_p, pbits, &tp);
                 int nfw = 0;
-                if (FAR) nfw = __popcll(ballot(far));
+                if (use_far) nfw = __popcll(ballot(far));
+                const uint64_t mL = LOCAL ? ballot(loc) : 0ull;
                 if (l == 0) {
                     xch[w][0] = __popcll(mA);
                     xch[w][1] = tp;
                     xch[w][2] = __popcll(m3);
                     xch[w][3] = __popcll(m4);
-                    if (FAR) xfar[w] = nfw;
+                    if (use_far) xfar[w] = nfw;
+                    if (LOCAL) xloc[w] = __popcll(mL);
                 }
                 __syncthreads();  // every entry of the round has been read; counts exchanged
                 int na_before = 0, tp_before = 0, nl_before = 0, nr_before = 0;
                 int na_all = 0, tp_all = 0, nl_all = 0, nr_all = 0, nf_all = 0;
+                int lo_before = 0, lo_all = 0;
 #pragma unroll
                 for (int k = 0; k < TEAM; ++k) {
                     const int a0 = xch[k][0], a1 = xch[k][1], a2 = xch[k][2], a3 = xch[k][3];
@@ -772,11 +954,17 @@ __global__ __launch_bounds__(TEAM * 64) void k_traverse_team(
                     tp_all += a1;
                     nl_all += a2;
                     nr_all += a3;
-                    if (FAR) nf_all += xfar[k];
+                    if (use_far) nf_all += xfar[k];
+                    if (LOCAL) {
+                        const int a4 = xloc[k];
+                        if (k < w) lo_before += a4;
+                        lo_all += a4;
+                    }
                 }
                 // accepted nodes -> node list; particles and far nodes ->
                 // particle list
                 if (near) nl[nn + na_before + mbcnt(mA)] = v;
+                if (LOCAL && loc) ll[nloc + lo_before + mbcnt(mL)] = v;
                 int o = np + tp_before + ex;
                 for (int k = 0; k < c_self; ++k) pl[o + k] = p_first + k;
                 o += c_self;
@@ -785,6 +973,7 @@ __global__ __launch_bounds__(TEAM * 64) void k_traverse_team(
                 nn += na_all;
                 np += tp_all;
                 nfar += nf_all;
+                nloc += lo_all;
                 // larger children -> stack, in place of the popped entries
                 int base = top - cnt;
                 const int npush = nl_all + nr_all;
@@ -822,6 +1011,11 @@ __global__ __launch_bounds__(TEAM * 64) void k_traverse_team(
             // This wave's chunks of the lists, for all targets; then the
             // waves' sums of each target in wave order.
             {
+                // This wave's chunks of the local list first (its running
+                // share of the group's expansion, in its own LDS row).
+                if (LOCAL && nloc > w * kWave)
+                    eval_local_list<POT>(L.loc, ll, nloc, node, nodeQ, gcx, gcy, gcz, eps2,
+                                         w * kWave, TEAM * kWave);
                 const Share sh = make_share(n, l);
                 Targets tg;
                 Acc a[kMaxPer];
@@ -837,7 +1031,8 @@ __global__ __launch_bounds__(TEAM * 64) void k_traverse_team(
                     eval_nodes<POT, QUAD>(L, nl, nn, node, nodeQ, a, tg, eps2, sh, w * kWave,
                                           TEAM * kWave);
                 if (np > w * kWave)
-                    eval_parts<POT, FAR>(L, pl, np, posm, node, a, tg, eps2, sh, w * kWave, TEAM * kWave);
+                    eval_parts<POT>(L, pl, np, posm, node, a, tg, eps2, sh, w * kWave,
+                                    TEAM * kWave);
                 float4 r[kMaxPer];
                 combine_share(L, a, sh, n, r);
                 float4* const scr = L.ca;
@@ -848,7 +1043,20 @@ __global__ __launch_bounds__(TEAM * 64) void k_traverse_team(
                 }
             }
             __syncthreads();  // every wave's sums are in LDS
+            tot_loc += nloc;
             if (w == 0) {
+                // The group's local expansion: the waves' shares added in wave
+                // order into wave 0's row (applied with the final sums).
+                const bool apply = LOCAL && !need && tot_loc > 0;
+                if (apply) {
+                    if (l < kLocTerms) {
+                        float v = lds[0].loc[l];
+#pragma unroll
+                        for (int ww = 1; ww < TEAM; ++ww) v += lds[ww].loc[l];
+                        lds[0].loc[l] = v;
+                    }
+                    __builtin_amdgcn_wave_barrier();
+                }
 #pragma unroll
                 for (int u = 0; u < kMaxPer; ++u) {
                     const int t = l + u * kWave;
@@ -869,7 +1077,10 @@ __global__ __launch_bounds__(TEAM * 64) void k_traverse_team(
                             v4 = make_float4(p4.x + v4.x, p4.y + v4.y, p4.z + v4.z, p4.w + v4.w);
                         }
                         if (!need) {
-                            const float ph = POT ? -G * (v4.w - posm[i].w * self_inv) : 0.0f;
+                            const float4 pi = posm[i];
+                            if (apply)
+                                local_apply<POT>(lds[0].loc, pi.x - gcx, pi.y - gcy, pi.z - gcz, v4);
+                            const float ph = POT ? -G * (v4.w - pi.w * self_inv) : 0.0f;
                             v4 = make_float4(G * v4.x, G * v4.y, G * v4.z, ph);
                         }
                         acc[i] = v4;
@@ -878,10 +1089,13 @@ __global__ __launch_bounds__(TEAM * 64) void k_traverse_team(
                 n_node += (unsigned long long)(nn + nfar) * (unsigned long long)n;
                 n_far += (unsigned long long)nfar * (unsigned long long)n;
                 n_part += (unsigned long long)(np - nfar) * (unsigned long long)n;
+                n_loc += (unsigned long long)nloc;
+                n_loct += (unsigned long long)nloc * (unsigned long long)n;
             }
             nn = 0;
             np = 0;
             nfar = 0;
+            nloc = 0;
             flushed = true;
             if (!need) break;
             __syncthreads();  // the waves are done with the lists and LDS sums before reuse
@@ -894,10 +1108,12 @@ __global__ __launch_bounds__(TEAM * 64) void k_traverse_team(
     }
     // The team's counts (accumulated by wave 0) in the block's slot.
     if (tid == 0) {
-        unsigned long long* c = counters + 3 * (size_t)blockIdx.x;
+        unsigned long long* c = counters + kCounts * (size_t)blockIdx.x;
         c[0] = n_node;
         c[1] = n_part;
         c[2] = n_far;
+        c[3] = n_loc;
+        c[4] = n_loct;
     }
 }
 

@@ -47,7 +47,8 @@ class Params(ctypes.Structure):
                 ("diagnostics", ctypes.c_int32), ("use_graph", ctypes.c_int32),
                 ("waves_per_simd", ctypes.c_int32), ("dt_eta", ctypes.c_float),
                 ("group_size", ctypes.c_int32), ("open_offset", ctypes.c_int32),
-                ("team", ctypes.c_int32), ("mono_far", ctypes.c_float)]
+                ("team", ctypes.c_int32), ("mono_far", ctypes.c_float),
+                ("local_kappa", ctypes.c_float)]
 
 
 class CollisionSpec(ctypes.Structure):
@@ -114,6 +115,7 @@ SIGNATURES = {
     "gcs_read_root": ([_P, _I32, _P], _I32),
     "gcs_read_interactions": ([_P, _P], _I32),
     "gcs_read_monopole_interactions": ([_P, _P], _I32),
+    "gcs_read_interaction_counts": ([_P, _P], _I32),
     "gcs_read_diagnostics": ([_P, _I32, _P], _I32),
     "gcs_read_stage_ms": ([_P, _P], _I32),
     "gcs_engine_info": ([_P, _P], _I32),

@@ -223,6 +223,13 @@ class Engine:
         L.check(L.lib().gcs_read_interactions(self._h, L.ptr(out)))
         return int(out[0]), int(out[1])
 
+    def interaction_counts(self):
+        """[per-target node, particle, monopole, local (group, node) pairs,
+        target interactions of those] of the last group traversal."""
+        out = np.zeros(5, np.int64)
+        L.check(L.lib().gcs_read_interaction_counts(self._h, L.ptr(out)))
+        return [int(v) for v in out]
+
     def monopole_interactions(self):
         out = np.zeros(1, np.int64)
         L.check(L.lib().gcs_read_monopole_interactions(self._h, L.ptr(out)))

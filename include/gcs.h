@@ -108,6 +108,13 @@ typedef struct gcs_params {
                            box exceeds mono_far times its opening radius is taken
                            as a softened monopole (its COM and mass), the rest as
                            quadrupoles */
+    float local_kappa;  /* GCS_MAC_GROUP: 0 = off; >= 1 = an accepted node whose COM
+                           lies farther than local_kappa times the half-diagonal of
+                           the group's box from the box centre is not evaluated per
+                           target: its softened quadrupole field at the centre and
+                           its monopole's first and second derivatives enter a
+                           second-order local expansion that every target of the
+                           group evaluates once (a cell-cell far field) */
 } gcs_params;
 
 /* Defaults: G=1, eps2=0.0025, dt=0.01, theta=0.5, GCS_MAC_GROUP,
@@ -193,7 +200,8 @@ int gcs_compute_accels(gcs_engine* e, const gcs_params* p);
 int gcs_autotune(gcs_engine* e, const gcs_params* base, int32_t reps,
                  gcs_params* best, float* best_ms);
 /* The autotuner keeps each result per (traversal kind, theta, opening form,
- * precision, floor(log2 particles), floor(log2 instances)), process-wide:
+ * precision, floor(log2 particles), floor(log2 instances), mono_far,
+ * local_kappa), process-wide:
  * gcs_autotune returns a kept result without timing anything (its best_ms is
  * the time measured when it was found).  _clear forgets them; _save writes
  * them as a text table, _load adds a saved table (an earlier run's tuning). */
@@ -234,6 +242,11 @@ int gcs_read_bounds(gcs_engine* e, int32_t id, float* mn, float* mx);
 int gcs_read_interactions(gcs_engine* e, int64_t* out2);
 /* Of [0] above, the nodes taken as monopoles (mono_far). */
 int gcs_read_monopole_interactions(gcs_engine* e, int64_t* out);
+/* All counts of the last group traversal: [0] node interactions evaluated
+ * per target, [1] particle interactions, [2] of [0] the monopoles, [3]
+ * (group, node) pairs taken through local expansions (local_kappa), [4] the
+ * target interactions those stand for (not in [0]). */
+int gcs_read_interaction_counts(gcs_engine* e, int64_t* out5);
 /* FP64 diagnostics of instance `id` from the last step run with
  * diagnostics=1: kinetic energy, potential energy, momentum x, y, z, all at
  * the force time (velocities re-synchronised by half a kick). */

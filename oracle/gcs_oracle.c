@@ -463,6 +463,77 @@ static float box_dist2(const float* lo, const float* hi, float cx, float cy,
     return ex * ex + ey * ey + ez * ez;
 }
 
+/* Second-order local expansion at a group's centre c of a node at r = COM - c
+ * (mass m, quadrupole and trace q[0..6]): its field at c + y is
+ *   a(y) = a0 + A y + (1/2) B:yy,   phi(y) = phi0 - a0.y - (1/2) y.A.y,
+ * with a0 and phi0 the softened quadrupole field at c and, from the softened
+ * monopole (R^2 = |r|^2 + eps^2),
+ *   A_ij  = m (3 r_i r_j / R^5 - delta_ij / R^3),
+ *   B_ijk = m (15 r_i r_j r_k / R^7 - 3 (delta_ij r_k + delta_ik r_j + delta_jk r_i) / R^5).
+ * L[20] accumulates a0 (3), A (xx yy zz xy xz yz), B (xxx yyy zzz xxy xxz xyy
+ * yyz xzz yzz xyz) and phi0, each a sum over the group's far nodes. */
+static void local_terms(double rx, double ry, double rz, double m, const double* q, double e2,
+                        double* L)
+{
+    const double R2 = rx * rx + ry * ry + rz * rz + e2;
+    const double iR = 1.0 / sqrt(R2), iR2 = iR * iR, iR3 = iR * iR2, iR5 = iR3 * iR2,
+                 iR7 = iR5 * iR2;
+    const double qx = q[0] * rx + q[3] * ry + q[4] * rz;
+    const double qy = q[3] * rx + q[1] * ry + q[5] * rz;
+    const double qz = q[4] * rx + q[5] * ry + q[2] * rz;
+    const double dqd = rx * qx + ry * qy + rz * qz - q[6] * e2;
+    const double f = m * iR3 + 2.5 * dqd * iR7;
+    L[0] += rx * f - qx * iR5;
+    L[1] += ry * f - qy * iR5;
+    L[2] += rz * f - qz * iR5;
+    const double m3 = m * iR3, m5 = 3.0 * m * iR5, m7 = 15.0 * m * iR7;
+    L[3] += m5 * rx * rx - m3;
+    L[4] += m5 * ry * ry - m3;
+    L[5] += m5 * rz * rz - m3;
+    L[6] += m5 * rx * ry;
+    L[7] += m5 * rx * rz;
+    L[8] += m5 * ry * rz;
+    L[9] += m7 * rx * rx * rx - 3.0 * m5 * rx;
+    L[10] += m7 * ry * ry * ry - 3.0 * m5 * ry;
+    L[11] += m7 * rz * rz * rz - 3.0 * m5 * rz;
+    L[12] += m7 * rx * rx * ry - m5 * ry;
+    L[13] += m7 * rx * rx * rz - m5 * rz;
+    L[14] += m7 * rx * ry * ry - m5 * rx;
+    L[15] += m7 * ry * ry * rz - m5 * rz;
+    L[16] += m7 * rx * rz * rz - m5 * rx;
+    L[17] += m7 * ry * rz * rz - m5 * ry;
+    L[18] += m7 * rx * ry * rz;
+    L[19] += -(m * iR + 0.5 * dqd * iR5);
+}
+
+static int cloc_any(const double* L)
+{
+    for (int c = 0; c < 20; ++c)
+        if (L[c] != 0.0) return 1;
+    return 0;
+}
+
+/* Adds G x (the accumulated expansion at offset y from the centre) to a[0..3]. */
+static void local_apply(const double* L, double yx, double yy, double yz, double G, double* a)
+{
+    const double xx = yx * yx, yy2 = yy * yy, zz = yz * yz, xy = yx * yy, xz = yx * yz,
+                 yz2 = yy * yz;
+    const double Ax = L[3] * yx + L[6] * yy + L[7] * yz;
+    const double Ay = L[6] * yx + L[4] * yy + L[8] * yz;
+    const double Az = L[7] * yx + L[8] * yy + L[5] * yz;
+    const double bx = L[9] * xx + L[14] * yy2 + L[16] * zz +
+                      2.0 * (L[12] * xy + L[13] * xz + L[18] * yz2);
+    const double by = L[12] * xx + L[10] * yy2 + L[17] * zz +
+                      2.0 * (L[14] * xy + L[18] * xz + L[15] * yz2);
+    const double bz = L[13] * xx + L[15] * yy2 + L[11] * zz +
+                      2.0 * (L[18] * xy + L[16] * xz + L[17] * yz2);
+    a[0] += G * (L[0] + Ax + 0.5 * bx);
+    a[1] += G * (L[1] + Ay + 0.5 * by);
+    a[2] += G * (L[2] + Az + 0.5 * bz);
+    a[3] += G * (L[19] - (L[0] * yx + L[1] * yy + L[2] * yz) -
+                 0.5 * (Ax * yx + Ay * yy + Az * yz));
+}
+
 void orc_traverse_group(int32_t n, const float* x, const float* y,
                         const float* z, const float* m, const int32_t* left,
                         const int32_t* right, const int32_t* first,
@@ -471,13 +542,13 @@ void orc_traverse_group(int32_t n, const float* x, const float* y,
                         int32_t leaf, float G, float eps2, double* acc,
                         int64_t* counts, double* msum, double* aabs,
                         int64_t* gcost, int32_t ngl, const int32_t* glist,
-                        float far2)
+                        float far2, float kappa2)
 {
     if (n <= 0) return;
     int32_t ngroups = glist ? ngl : (n + group - 1) / group;
-    int64_t cnode = 0, cpart = 0, cfar = 0;
+    int64_t cnode = 0, cpart = 0, cfar = 0, cloc = 0, cloct = 0;
     const double Gd = G, e2 = eps2;
-#pragma omp parallel for schedule(dynamic, 1) reduction(+ : cnode, cpart, cfar)
+#pragma omp parallel for schedule(dynamic, 1) reduction(+ : cnode, cpart, cfar, cloc, cloct)
     for (int32_t g = 0; g < ngroups; ++g) {
         int32_t g0 = glist ? glist[2 * g] : g * group;
         int32_t g1 = glist ? g0 + glist[2 * g + 1] : (g0 + group < n ? g0 + group : n);
@@ -487,6 +558,15 @@ void orc_traverse_group(int32_t n, const float* x, const float* y,
             lo[1] = fminf(lo[1], y[i]); hi[1] = fmaxf(hi[1], y[i]);
             lo[2] = fminf(lo[2], z[i]); hi[2] = fmaxf(hi[2], z[i]);
         }
+        /* The group's box centre and squared half-diagonal, in float as the
+         * kernel forms them; the local-expansion radius squared. */
+        const float gc[3] = {(lo[0] + hi[0]) * 0.5f, (lo[1] + hi[1]) * 0.5f,
+                             (lo[2] + hi[2]) * 0.5f};
+        const float ex = hi[0] - lo[0], ey = hi[1] - lo[1], ez = hi[2] - lo[2];
+        const float rg2 = 0.25f * (ex * ex + ey * ey + ez * ez);
+        const float lim = kappa2 * rg2;
+        double L[20];
+        for (int c = 0; c < 20; ++c) L[c] = 0.0;
         int32_t cap = 4096;
         int64_t gc_nodes = 0, gc_parts = 0;
         int32_t* stack = (int32_t*)malloc(sizeof(int32_t) * (size_t)cap);
@@ -507,6 +587,30 @@ void orc_traverse_group(int32_t n, const float* x, const float* y,
             if (internal && bd2 > crit[v]) {
                 double cx = com4[4 * v], cy = com4[4 * v + 1], cz = com4[4 * v + 2];
                 double mm = com4[4 * v + 3];
+                const float dlx = com4[4 * v] - gc[0], dly = com4[4 * v + 1] - gc[1],
+                            dlz = com4[4 * v + 2] - gc[2];
+                if (dlx * dlx + dly * dly + dlz * dlz > lim) {
+                    /* Far from the group: its field in a second-order local
+                     * expansion at the group's centre. */
+                    double qn[7] = {0, 0, 0, 0, 0, 0, 0};
+                    if (quad)
+                        for (int c = 0; c < 7; ++c) qn[c] = quad[7 * (int64_t)v + c];
+                    local_terms(cx - gc[0], cy - gc[1], cz - gc[2], mm, qn, e2, L);
+                    cloc += 1;
+                    cloct += g1 - g0;
+                    cnode += 0;
+                    gc_nodes += 1;
+                    for (int32_t i = g0; i < g1; ++i) {
+                        if (msum) msum[i] += mm;
+                        if (aabs) {
+                            double dx = cx - (double)x[i], dy = cy - (double)y[i],
+                                   dz = cz - (double)z[i];
+                            double r2 = dx * dx + dy * dy + dz * dz + e2;
+                            aabs[i] += fabs(Gd) * mm / r2;
+                        }
+                    }
+                    continue;
+                }
                 double q[7] = {0, 0, 0, 0, 0, 0, 0};
                 const float far_r2 = far2 * crit[v];
                 const int mono = bd2 > far_r2;
@@ -570,6 +674,10 @@ void orc_traverse_group(int32_t n, const float* x, const float* y,
             }
         }
         free(stack);
+        if (cloc_any(L))
+            for (int32_t i = g0; i < g1; ++i)
+                local_apply(L, (double)x[i] - gc[0], (double)y[i] - gc[1], (double)z[i] - gc[2], Gd,
+                            &acc[4 * i]);
         if (gcost) {
             gcost[2 * g] = gc_nodes;
             gcost[2 * g + 1] = gc_parts;
@@ -578,6 +686,8 @@ void orc_traverse_group(int32_t n, const float* x, const float* y,
     counts[0] += cnode;
     counts[1] += cpart;
     counts[2] += cfar;
+    counts[3] += cloc;
+    counts[4] += cloct;
 }
 
 /* Groups taken from the tree: the maximal subtrees holding at most gmax

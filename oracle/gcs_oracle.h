@@ -85,7 +85,14 @@ int64_t orc_traverse_reference(int32_t n, const float* x, const float* y,
  * far2: an accepted node whose box distance squared also exceeds
  * far2 * crit[node] (float product) is taken as a plain softened monopole,
  * its quadrupole and trace left out; counts[2] += those interactions (they
- * are in counts[0] too); +inf: none. */
+ * are in counts[0] too); +inf: none.
+ * kappa2: an accepted node whose COM lies farther from the group's box
+ * centre c than sqrt(kappa2) times the box's half-diagonal (squares and
+ * products in float) enters a second-order local expansion at c instead
+ * (the quadrupole field at c plus the monopole's first and second
+ * derivatives), which every target of the group evaluates at its offset from
+ * c; counts[3] += 1 per such (group, node) pair, counts[4] += the group's
+ * targets (not in counts[0]); +inf: none.  counts has 5 entries. */
 void orc_traverse_group(int32_t n, const float* x, const float* y,
                         const float* z, const float* m, const int32_t* left,
                         const int32_t* right, const int32_t* first,
@@ -94,7 +101,7 @@ void orc_traverse_group(int32_t n, const float* x, const float* y,
                         int32_t leaf, float G, float eps2, double* acc,
                         int64_t* counts, double* msum, double* aabs,
                         int64_t* gcost, int32_t ngl, const int32_t* glist,
-                        float far2);
+                        float far2, float kappa2);
 /* With glist non-null the groups are the ngl (first, count) pairs of glist
  * instead of runs of `group` particles.
  * Tree groups: the maximal subtrees of at most gmax particles (every node of

@@ -37,7 +37,7 @@ _SIG = {
     "orc_aggregate": ([_I32, _P, _P, _P, _P, _P, _P, _F, _I32, _P, _P, _P, _P, _P, _P], None),
     "orc_traverse_reference": ([_I32, _P, _P, _P, _P, _P, _P, _P, _F, _F, _P, _P, _P], _I64),
     "orc_traverse_group": ([_I32, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _I32, _I32, _F,
-                            _F, _P, _P, _P, _P, _P, _I32, _P, _F], None),
+                            _F, _P, _P, _P, _P, _P, _I32, _P, _F, _F], None),
     "orc_tree_groups": ([_I32, _P, _P, _P, _P, _I32, _P], _I32),
     "orc_direct_sum_f64": ([_I64, _P, _P, _P, _P, _I64, _P, _D, _D, _P], None),
     "orc_integrate_reference": ([_I64, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _F],
@@ -158,22 +158,28 @@ def traverse_reference(left, right, x, y, z, com4, width, G, eps2):
 
 
 def traverse_group(left, right, first, last, x, y, z, m, agg, G, eps2, group=64, leaf=16,
-                   quad=True, want_abs=False, gcost=None, groups=None, mono_far=0.0):
-    """counts: [node interactions, particle interactions, of the nodes those
-    taken as monopoles (beyond mono_far times the opening radius; 0: none)]."""
+                   quad=True, want_abs=False, gcost=None, groups=None, mono_far=0.0,
+                   local_kappa=0.0):
+    """counts: [node interactions evaluated per target, particle interactions,
+    of the nodes those taken as monopoles (beyond mono_far times the opening
+    radius; 0: none), (group, node) pairs taken through local expansions
+    (nodes beyond local_kappa group radii from the group's centre; 0: none),
+    the target interactions those stand for]."""
     n = len(x)
     x, y, z, m = c32(x), c32(y), c32(z), c32(m)
     acc = np.zeros((n, 4), np.float64)
     msum = np.zeros(n, np.float64)
-    counts = np.zeros(3, np.int64)
+    counts = np.zeros(5, np.int64)
     far2 = float(np.float32(mono_far) * np.float32(mono_far)) if mono_far > 0 else float("inf")
+    kappa2 = (float(np.float32(local_kappa) * np.float32(local_kappa)) if local_kappa > 0
+              else float("inf"))
     aabs = np.zeros(n, np.float64) if want_abs else None
     lib().orc_traverse_group(n, P(x), P(y), P(z), P(m), P(left), P(right), P(first), P(last),
                              P(agg["com4"]), P(agg["crit"]), P(agg["quad"]) if quad else None,
                              group, leaf, G, eps2, P(acc), P(counts), P(msum), P(aabs),
                              P(gcost), 0 if groups is None else len(groups),
                              None if groups is None else
-                             P(np.ascontiguousarray(groups, np.int32)), far2)
+                             P(np.ascontiguousarray(groups, np.int32)), far2, kappa2)
     if want_abs:
         return acc, msum, counts, aabs
     return acc, msum, counts

@@ -124,8 +124,8 @@ def test_autotune_table_save_and_load(gcs, tmp_path):
     path = str(tmp_path / "tune.txt")
     gcs.autotune_cache_clear()
     gcs.autotune_cache_save(path)
-    assert open(path).read().startswith("gcs-autotune 1\n")
-    row = "1 0 1 22 8 3f000000 128 4 256 1 4 256 7.5\n"
+    assert open(path).read().startswith("gcs-autotune 2\n")
+    row = "1 0 1 22 8 3f000000 00000000 40800000 128 4 256 1 4 256 7.5\n"
     with open(path, "a") as f:
         f.write(row)
     gcs.autotune_cache_load(path)

@@ -296,6 +296,71 @@ def test_far_monopoles_keep_the_lists_and_the_error_bound(orc):
     assert e0 < e1 < 1e-3
 
 
+def local_case(orc, s, src_x, kappa, eps2=EPS2):
+    """Two unit-mass targets at x = +-s/2 (one group, centre 0, half-diagonal
+    s/2) and a source pair of mass 2 + 2, 1e-3 apart, at x = src_x (another
+    group); groups of at most 2.  Returns (targets' FP64 accelerations from
+    the oracle with local expansions at kappa, FP64 direct sum, counts)."""
+    st = np.zeros(4, orc.STAR)
+    st["x"] = np.array([-s / 2, s / 2, src_x - 5e-4, src_x + 5e-4], np.float32)
+    st["mass"] = np.array([1.0, 1.0, 2.0, 2.0], np.float32)
+    t = orc.build_tree_pipeline(st, 0.5)
+    srt = t["sorted"]
+    grp = orc.tree_groups(t["left"], t["right"], t["first"], t["last"], 2)
+    assert [tuple(g) for g in grp] == [(0, 2), (2, 2)]
+    acc, _, cnt = orc.traverse_group(t["left"], t["right"], t["first"], t["last"], srt["x"],
+                                     srt["y"], srt["z"], srt["mass"], t["agg"], G, eps2, 2, 1,
+                                     groups=grp, local_kappa=kappa)
+    ref = orc.direct_sum_f64(srt["x"], srt["y"], srt["z"], srt["mass"], G, eps2)
+    return acc[:2], ref[:2], cnt
+
+
+def test_local_expansion_is_second_order(orc):
+    # The far pair enters the targets' group through a local expansion at the
+    # group's centre.  Its error against the direct sum is third order in the
+    # targets' offset: halving the group cuts it about 8x (a dropped B term
+    # would leave 4x, a dropped A term 2x), for the force and the potential.
+    errs = []
+    for s in (1.0, 0.5, 0.25):
+        acc, ref, cnt = local_case(orc, s, 20.0, 2.0)
+        # both groups take the other pair locally: two (group, node) pairs
+        assert cnt[3] == 2 and cnt[4] == 4 and cnt[0] == 0
+        errs.append((np.abs(acc[:, 0] - ref[:, 0]).max(), np.abs(acc[:, 3] - ref[:, 3]).max()))
+    for k in range(2):
+        for e1, e2 in ((errs[0][k], errs[1][k]), (errs[1][k], errs[2][k])):
+            assert 6.5 < e1 / e2 < 9.5, errs
+    assert errs[0][0] < 1e-4 * abs(ref[0, 0])
+
+
+@pytest.mark.parametrize("over,local", [(0.99, False), (1.01, True)])
+def test_local_expansion_threshold(orc, over, local):
+    # The targets' group has half-diagonal 0.5; with kappa = 6 a node is taken
+    # through the local expansion iff its COM lies farther than 3 from the
+    # group's centre.
+    acc, ref, cnt = local_case(orc, 1.0, 3.0 * over, 6.0)
+    # (the source group, 1e-3 across, takes the target pair locally either way)
+    assert cnt[3] == (2 if local else 1)
+    assert cnt[0] == (0 if local else 2)
+    assert np.allclose(acc[:, :3], ref[:, :3], rtol=0.05)
+
+
+def test_local_expansions_keep_every_mass_and_the_error_bound(orc):
+    t = pipeline(orc, 6000, 8)
+    s = t["sorted"]
+    ref = orc.direct_sum_f64(s["x"], s["y"], s["z"], s["mass"], G, EPS2)
+    grp = orc.tree_groups(t["left"], t["right"], t["first"], t["last"], 96)
+    a0, _, c0 = orc.traverse_group(t["left"], t["right"], t["first"], t["last"], s["x"], s["y"],
+                                   s["z"], s["mass"], t["agg"], G, EPS2, 64, 4, groups=grp)
+    a1, m1, c1 = orc.traverse_group(t["left"], t["right"], t["first"], t["last"], s["x"],
+                                    s["y"], s["z"], s["mass"], t["agg"], G, EPS2, 64, 4,
+                                    groups=grp, local_kappa=6.0)
+    # the same accepted nodes: those taken locally leave the per-target count
+    assert c1[0] + c1[4] == c0[0] and c1[1] == c0[1] and c1[3] > 0
+    assert np.allclose(m1 + s["mass"], s["mass"].astype(np.float64).sum(), rtol=1e-5)
+    e0, e1 = rel_rms(a0[:, :3], ref[:, :3]), rel_rms(a1[:, :3], ref[:, :3])
+    assert e1 < 1.5 * e0 and e1 < 1e-3
+
+
 def test_opening_offset_only_adds(orc):
     t0 = pipeline(orc, 3000, 12)
     s = t0["sorted"]
