@@ -1,5 +1,8 @@
 """The benchmark configurations run through the HIP path, against the oracle.
 
+All at the benchmark's traversal settings (local expansions beyond 4 group
+half-diagonals; C3 also without them).
+
 C2  2 x 131,072 particles, 500 steps: Morton keys, the sort's permutation and
     the Karras topology bit-exact at every step; aggregates at five
     checkpoints; accelerations within 1e-3 of an FP64 direct sum at steps 0,
@@ -85,8 +88,8 @@ def test_c2_every_step(gcs, orc):
     stars = gcs.collision(131072, 1002)
     n = len(stars)
     eps2 = float(np.float32(0.02) ** 2)
-    p = gcs.params(G=G, eps2=eps2, dt=0.005, theta=0.5, leaf_size=4, group_size=96, team=4,
-                   block_size=256, waves_per_simd=4)
+    p = gcs.params(G=G, eps2=eps2, dt=0.005, theta=0.5, leaf_size=4, group_size=96, team=2,
+                   block_size=128, waves_per_simd=3, local_kappa=4.0)
     with gcs.Engine(n) as e:
         e.load(stars)
         for step in range(501):
@@ -105,18 +108,20 @@ def test_c2_every_step(gcs, orc):
                 e.step(p, 1)
 
 
-@pytest.mark.parametrize("theta,group", [(0.5, 64), (0.7, 96)])
-def test_c3_precision_form(gcs, orc, theta, group):
+@pytest.mark.parametrize("theta,group,kappa", [(0.5, 128, 4.0), (0.5, 64, 0.0), (0.7, 96, 0.0)])
+def test_c3_precision_form(gcs, orc, theta, group, kappa):
     """C3: 2 x 1,048,576, eps 0.02, dt 0.005, compensated positions, the
-    autotuner's settings for this workload."""
+    autotuner's settings for this workload (local expansions beyond 4 group
+    half-diagonals, as the benchmark runs it, and quadrupoles only)."""
     stars = gcs.collision(1048576, 1003)
     n = len(stars)
     eps2 = float(np.float32(0.02) ** 2)
     p = gcs.params(G=G, eps2=eps2, dt=0.005, theta=theta, leaf_size=4, group_size=group, team=1,
-                   block_size=64, waves_per_simd=4, precision=gcs.PREC_COMPENSATED)
+                   block_size=64, waves_per_simd=4, precision=gcs.PREC_COMPENSATED,
+                   local_kappa=kappa)
     with gcs.Engine(n) as e:
         e.load(stars)
-        say("C3 theta %.1f: step 0" % theta)
+        say("C3 theta %.1f local_kappa %.1f: step 0" % (theta, kappa))
         e.compute_accels(p)
         cpu = check_instance(orc, e, 0, stars, e.keys(), theta)
         err = direct_error(orc, e, cpu, eps2, 4096, e.accels())
@@ -138,7 +143,8 @@ def test_c4_forest(gcs, orc):
     counts = np.array([s.n_a + s.n_b for s in specs])
     eps2 = float(np.float32(0.02) ** 2)
     p = gcs.params(G=G, eps2=eps2, dt=0.005, theta=0.5, leaf_size=4, group_size=128, team=1,
-                   block_size=256, waves_per_simd=4, precision=gcs.PREC_COMPENSATED)
+                   block_size=64, waves_per_simd=4, precision=gcs.PREC_COMPENSATED,
+                   local_kappa=4.0)
     rng = np.random.default_rng(4)
     sample = sorted({int(np.argmin(counts)), int(np.argmax(counts))} |
                     set(int(i) for i in rng.choice(k, 6, replace=False)))
@@ -179,7 +185,7 @@ def test_c5_served_stream(gcs):
     assert len(jobs) == 11 and sum(1 for j in jobs if j[1] > 300000) == 2
     kw = dict(G=G, eps2=float(np.float32(0.02) ** 2), dt=0.005, theta=0.5, leaf_size=4,
               group_size=128, team=1, block_size=64, waves_per_simd=4,
-              precision=gcs.PREC_COMPENSATED, use_graph=1)
+              precision=gcs.PREC_COMPENSATED, use_graph=1, local_kappa=4.0)
 
     def run(js):
         with gcs.Server(hbm_budget_bytes=16 << 30, keep_final=True, steps_per_round=25,
