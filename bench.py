@@ -629,7 +629,8 @@ def main():
     ap.add_argument("--config", default="batched-256", choices=["batched-256"],
                     help="headline workload (the batched form of 256 collisions)")
     ap.add_argument("--batched", type=int, default=256,
-                    help="collisions of the batched headline")
+                    help="collisions of the batched headline (0 = skip it: profiling of the "
+                         "plain and c3 legs alone)")
     ap.add_argument("--vs-plain", type=int, default=1,
                     help="1: run the headline's collisions one at a time too (0 = skip)")
     ap.add_argument("--eager", action="store_true",
@@ -676,6 +677,16 @@ def main():
     if not a.retune and os.path.exists(TUNE_TABLE):
         gcs_amd.autotune_cache_load(TUNE_TABLE)
 
+    if a.batched == 0:
+        # Profiling form: only the legs asked for, no headline value.
+        line = {"metric": METRIC, "value": None, "unit": "particle-steps/s", "n_gpus": 1,
+                "note": "--batched 0: headline skipped (profiling of the other legs)"}
+        if a.plain > 0:
+            line["plain"] = plain_leg(a.plain, a.steps, a.warmup, 1002, 0)
+        if a.c3_steps > 0:
+            line["c3"] = c3_leg(a.c3_steps, 2, a.c3_drift_steps, 0, 4096)
+        print(json.dumps(line), flush=True)
+        return
     b = batched_leg(a.batched, a.steps, a.warmup, vs_plain=bool(a.vs_plain), eager=a.eager)
     line = {
         "metric": METRIC,

@@ -33,6 +33,14 @@ constexpr int kGroupMax = 192;       // largest group a wave takes (3 particles 
 constexpr int kGroupChunk = 128;    // quarter tiles per group-numbering chunk
 constexpr int kChunkTiles = 32;      // tiles per sort-histogram and aggregation chunk
 constexpr int kTileShift = 10;       // kTile == 1 << kTileShift
+// Radix-sort blocks: runs of at most kSortItems x (threads) keys of one
+// instance (aligned to that size within it): kSortTileBig keys per block of
+// 1,024 threads for sorts of at least kSortBigMin keys, else kTile keys per
+// block of 256 threads.
+constexpr int kSortItems = 4;
+constexpr int kSortTileBig = 4096;
+constexpr int64_t kSortBigMin = (int64_t)3 << 18;
+static_assert(kSortTileBig == kSortItems * 1024 && kTile == kSortItems * kTileThreads, "tiles");
 static_assert(kTile == 1 << kTileShift, "shift");
 // Aggregation subtrees: the maximal subtrees of at most kATile particles are
 // finished one per block, the nodes above them by the instance pass.

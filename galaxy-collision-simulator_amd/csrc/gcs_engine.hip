@@ -118,6 +118,8 @@ struct gcs_engine {
     int* flag = nullptr;
     // tables
     TileInfo* tiles = nullptr;
+    TileInfo* stiles = nullptr;  // radix-sort tiles (at most stile_keys keys)
+    int stile_keys = kTile;      // kSortTileBig for engines of at least kSortBigMin
     SegInfo* segs = nullptr;
     GroupInfo* groups = nullptr;  // traversal groups, rebuilt on the device each step
     int* gorder = nullptr;        // their dispatch order (size classes within each XCD queue)
@@ -170,6 +172,8 @@ struct gcs_engine {
     std::vector<Instance> inst;
     std::vector<SegInfo> h_segs;
     std::vector<TileInfo> h_tiles;
+    std::vector<TileInfo> h_stiles;
+    std::vector<int> h_stile0;  // first sort tile of each instance
     int64_t n_total = 0;
     int32_t next_id = 0;
     int nchunks = 0;  // sort-histogram chunks in use
@@ -220,7 +224,7 @@ void free_all(gcs_engine* e)
     dfree(e->acc); dfree(e->keys_raw);
     dfree(e->node); dfree(e->nodeQ); dfree(e->boxLo); dfree(e->boxHi);
     dfree(e->parent); dfree(e->flag);
-    dfree(e->tiles); dfree(e->segs); dfree(e->amark); dfree(e->aq); dfree(e->achunk); dfree(e->acnt); dfree(e->aroot); dfree(e->atiles); dfree(e->natiles); dfree(e->groups); dfree(e->hist); dfree(e->chist);
+    dfree(e->tiles); dfree(e->stiles); dfree(e->segs); dfree(e->amark); dfree(e->aq); dfree(e->achunk); dfree(e->acnt); dfree(e->aroot); dfree(e->atiles); dfree(e->natiles); dfree(e->groups); dfree(e->hist); dfree(e->chist);
     dfree(e->gmark); dfree(e->gq); dfree(e->gorder); dfree(e->gchunk); dfree(e->gcnt); dfree(e->ngroups);
     dfree(e->front); dfree(e->nfront);
     dfree(e->tlist); dfree(e->ntop); dfree(e->tslot);
@@ -235,6 +239,12 @@ int rebuild_tables(gcs_engine* e)
 {
     e->h_segs.clear();
     e->h_tiles.clear();
+    e->h_stiles.clear();
+    e->h_stile0.clear();
+    int64_t n_all = 0;
+    for (const Instance& in : e->inst) n_all += in.n;
+    const int st_keys = n_all >= kSortBigMin ? kSortTileBig : kTile;
+    e->stile_keys = st_keys;
     int start = 0, chunks = 0;
     for (size_t s = 0; s < e->inst.size(); ++s) {
         const int n = e->inst[s].n;
@@ -257,6 +267,16 @@ int rebuild_tables(gcs_engine* e)
             t.pad0 = t.pad1 = t.pad2 = t.pad3 = 0;
             e->h_tiles.push_back(t);
         }
+        e->h_stile0.push_back((int)e->h_stiles.size());
+        for (int j = 0; j * st_keys < n; ++j) {
+            TileInfo t;
+            t.start = start + j * st_keys;
+            t.count = std::min(st_keys, n - j * st_keys);
+            t.seg = (int)s;
+            t.tile_in_seg = j;
+            t.pad0 = t.pad1 = t.pad2 = t.pad3 = 0;
+            e->h_stiles.push_back(t);
+        }
         e->h_segs.push_back(sg);
         start += n;
     }
@@ -266,6 +286,8 @@ int rebuild_tables(gcs_engine* e)
         HIPCHK(hipMemcpy(e->segs, e->h_segs.data(), sizeof(SegInfo) * e->h_segs.size(),
                          hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(e->tiles, e->h_tiles.data(), sizeof(TileInfo) * e->h_tiles.size(),
+                         hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(e->stiles, e->h_stiles.data(), sizeof(TileInfo) * e->h_stiles.size(),
                          hipMemcpyHostToDevice));
     }
     e->graph_valid = false;
@@ -395,26 +417,41 @@ Marks atile_marks(gcs_engine* e) { return Marks{e->amark, e->aq, e->acnt, e->aro
 // state[cur ^ 1] and flips cur.
 // The eight sort passes over keys[0]/vals[0] (sorted result back in
 // buffer 0).  The chunk histograms of all passes must be zero on entry.
-// Sorts the instances whose tiles are tile0 .. tile0 + ntiles - 1 (whole
-// instances); the other instances' keys are left alone.
-void enqueue_sort_range(gcs_engine* e, int tile0, int ntiles)
+// Sorts instances seg0 .. seg0 + nseg - 1; the other instances' keys are
+// left alone.
+void enqueue_sort_range(gcs_engine* e, int seg0, int nseg)
 {
+    if (nseg <= 0) return;
     const size_t cstride = (size_t)e->nchunks * kRadix;
-    const TileInfo* tiles = e->tiles + tile0;
+    const int st0 = e->h_stile0[(size_t)seg0];
+    const int st1 = seg0 + nseg < (int)e->h_stile0.size() ? e->h_stile0[(size_t)(seg0 + nseg)]
+                                                          : (int)e->h_stiles.size();
+    const TileInfo* tiles = e->stiles + st0;
+    const int ntiles = st1 - st0;
+    // Tile rows are indexed by the launch's block index: the rows of the
+    // range start at row 0, the chunk rows stay global.
     for (int pass = 0; pass < kSortPasses; ++pass) {
         const int shift = pass * kRadixBits;
         const int src = pass & 1, dst = src ^ 1;
         uint32_t* ch = e->chist + (size_t)pass * cstride;
-        k_sort_count<<<ntiles, kTileThreads, 0, e->stream>>>(tiles, e->segs, e->keys[src],
-                                                              e->hist, ch, shift);
-        k_sort_scatter<<<ntiles, kTileThreads, 0, e->stream>>>(
-            tiles, e->segs, e->keys[src], e->vals[src], e->keys[dst], e->vals[dst], e->hist,
-            ch, shift);
+        if (e->stile_keys == kSortTileBig) {
+            k_sort_count<1024><<<ntiles, 1024, 0, e->stream>>>(tiles, e->segs, e->keys[src],
+                                                                e->hist, ch, shift);
+            k_sort_scatter<1024><<<ntiles, 1024, 0, e->stream>>>(
+                tiles, e->segs, e->keys[src], e->vals[src], e->keys[dst], e->vals[dst],
+                e->hist, ch, shift);
+        } else {
+            k_sort_count<kTileThreads><<<ntiles, kTileThreads, 0, e->stream>>>(
+                tiles, e->segs, e->keys[src], e->hist, ch, shift);
+            k_sort_scatter<kTileThreads><<<ntiles, kTileThreads, 0, e->stream>>>(
+                tiles, e->segs, e->keys[src], e->vals[src], e->keys[dst], e->vals[dst],
+                e->hist, ch, shift);
+        }
     }
     static_assert(kSortPasses % 2 == 0, "sorted keys end in buffer 0");
 }
 
-void enqueue_sort(gcs_engine* e) { enqueue_sort_range(e, 0, (int)e->h_tiles.size()); }
+void enqueue_sort(gcs_engine* e) { enqueue_sort_range(e, 0, (int)e->h_segs.size()); }
 
 // The disc sampler table on the device (uploaded on first use).
 int mn_table_dev(gcs_engine* e, gcs_gen::MnTableView* mt)
@@ -480,7 +517,7 @@ int generate_into(gcs_engine* e, const GenInst* insts, int seg0, int k)
     if (le == hipSuccess) {
         k_gen_pos<<<4 * ntiles, kTileThreads, 0, e->stream>>>(tiles, e->segs, di, seg0, mt, xyz,
                                                                e->keys[0], e->vals[0]);
-        enqueue_sort_range(e, tile0, ntiles);
+        enqueue_sort_range(e, seg0, k);
         k_gen_count<<<ntiles, kTileThreads, 0, e->stream>>>(tiles, e->segs, di, seg0, e->vals[0],
                                                              e->hist);
         k_gen_scan<<<(k + 255) / 256, 256, 0, e->stream>>>(e->segs, seg0, k, tile0, e->hist);
@@ -807,7 +844,7 @@ int64_t gcs_device_bytes(int64_t cap_n, int32_t cap_k)
     b += 16 * n;               // acc
     b += 3 * 8 * n + 2 * 4 * n; // keys x2 + raw, vals x2
     b += 2 * n * (32 + 32 + 16 + 16 + 4 + 4);  // nodes
-    b += (int64_t)(sizeof(TileInfo) * max_tiles(cap_n, cap_k) + sizeof(SegInfo) * cap_k +
+    b += (int64_t)(2 * sizeof(TileInfo) * max_tiles(cap_n, cap_k) + sizeof(SegInfo) * cap_k +
                    sizeof(GroupInfo) * (cap_n + 1) + 12 * cap_n);
     b += (int64_t)(4 * hist_entries(cap_n, cap_k) +
                    4 * (size_t)kSortPasses * kRadix * max_chunks(cap_n, cap_k));
@@ -868,6 +905,7 @@ int gcs_create(int64_t cap_n, int32_t cap_k, int32_t device, gcs_engine** out)
     if (!rc) rc = dalloc(&e->parent, 2 * n);
     if (!rc) rc = dalloc(&e->flag, 2 * n);
     if (!rc) rc = dalloc(&e->tiles, nt);
+    if (!rc) rc = dalloc(&e->stiles, nt);
     if (!rc) rc = dalloc(&e->segs, (size_t)cap_k);
     if (!rc) rc = dalloc(&e->groups, n + 1);
     if (!rc) rc = dalloc(&e->gmark, n);
@@ -1936,7 +1974,7 @@ constexpr size_t kNaTiles =
     kNaChist + na_align(sizeof(uint32_t) * kSortPasses * kRadix * kNaMaxChunks);
 constexpr size_t kNaSegs = kNaTiles + na_align(sizeof(TileInfo) * kNaMaxTiles);
 constexpr size_t kNaBytes = kNaSegs + na_align(sizeof(SegInfo));
-static_assert(kNaMaxKeys % kTile == 0, "whole tiles");
+static_assert(kNaMaxKeys % kSortTileBig == 0, "whole sort tiles");
 
 }  // namespace
 
@@ -1969,11 +2007,12 @@ int scratch_address(uint8_t** out)
     return GCS_OK;
 }
 
-// Set-up of one sort of n keys: the tile table and the instance record of a
-// single instance, the chunk histograms of all passes cleared, and the
+// Set-up of one sort of n keys: the sort-tile table and the instance record
+// of a single instance, the chunk histograms of all passes cleared, and the
 // payload 0 .. n-1 written into perm (the first pass's payload input).  A
 // kernel rather than hipMemsetAsync, so the call can be captured into a graph.
-__global__ __launch_bounds__(256) void k_noalloc_setup(int n, int ntiles, int chist_words,
+__global__ __launch_bounds__(256) void k_noalloc_setup(int n, int ntiles, int nstiles,
+                                                       int st_keys, int chist_words,
                                                        TileInfo* __restrict__ tiles,
                                                        SegInfo* __restrict__ segs,
                                                        uint32_t* __restrict__ chist,
@@ -1983,10 +2022,10 @@ __global__ __launch_bounds__(256) void k_noalloc_setup(int n, int ntiles, int ch
     const int stride = (int)(gridDim.x * blockDim.x);
     for (int i = tid; i < n; i += stride) perm[i] = (uint32_t)i;
     for (int k = tid; k < chist_words; k += stride) chist[k] = 0u;
-    for (int t = tid; t < ntiles; t += stride) {
+    for (int t = tid; t < nstiles; t += stride) {
         TileInfo ti;
-        ti.start = t * kTile;
-        ti.count = min(kTile, n - t * kTile);
+        ti.start = t * st_keys;
+        ti.count = min(st_keys, n - t * st_keys);
         ti.seg = 0;
         ti.tile_in_seg = t;
         ti.pad0 = ti.pad1 = ti.pad2 = ti.pad3 = 0;
@@ -2023,6 +2062,9 @@ extern "C" int gcs_sort_pairs_noalloc(const uint64_t* keys, int64_t n, uint64_t*
     hipStream_t st = static_cast<hipStream_t>(hip_stream);
     const int nk = (int)n;
     const int ntiles = (nk + kTile - 1) / kTile;
+    const bool big = nk >= kSortBigMin;
+    const int st_keys = big ? kSortTileBig : kTile;
+    const int nstiles = (nk + st_keys - 1) / st_keys;
     const int nchunks = (ntiles + kChunkTiles - 1) / kChunkTiles;
     const size_t cstride = (size_t)nchunks * kRadix;
     uint64_t* sk = reinterpret_cast<uint64_t*>(base + kNaKeys);
@@ -2033,7 +2075,7 @@ extern "C" int gcs_sort_pairs_noalloc(const uint64_t* keys, int64_t n, uint64_t*
     SegInfo* segs = reinterpret_cast<SegInfo*>(base + kNaSegs);
     uint32_t* pv = reinterpret_cast<uint32_t*>(perm_out);
     k_noalloc_setup<<<std::min(1024, (nk + 255) / 256), 256, 0, st>>>(
-        nk, ntiles, (int)(kSortPasses * cstride), tiles, segs, chist, pv);
+        nk, ntiles, nstiles, st_keys, (int)(kSortPasses * cstride), tiles, segs, chist, pv);
     LAUNCHCHK();
     // Even passes read the caller's buffers (the input keys first, then
     // keys_out) and write the scratch pair; odd passes write keys_out and
@@ -2047,9 +2089,16 @@ extern "C" int gcs_sort_pairs_noalloc(const uint64_t* keys, int64_t n, uint64_t*
         uint64_t* kdst = even ? sk : keys_out;
         uint32_t* vdst = even ? sv : pv;
         uint32_t* ch = chist + (size_t)pass * cstride;
-        k_sort_count<<<ntiles, kTileThreads, 0, st>>>(tiles, segs, kin, hist, ch, shift);
-        k_sort_scatter<<<ntiles, kTileThreads, 0, st>>>(tiles, segs, kin, vin, kdst, vdst, hist,
-                                                         ch, shift);
+        if (big) {
+            k_sort_count<1024><<<nstiles, 1024, 0, st>>>(tiles, segs, kin, hist, ch, shift);
+            k_sort_scatter<1024><<<nstiles, 1024, 0, st>>>(tiles, segs, kin, vin, kdst, vdst,
+                                                            hist, ch, shift);
+        } else {
+            k_sort_count<kTileThreads><<<nstiles, kTileThreads, 0, st>>>(tiles, segs, kin, hist,
+                                                                        ch, shift);
+            k_sort_scatter<kTileThreads><<<nstiles, kTileThreads, 0, st>>>(
+                tiles, segs, kin, vin, kdst, vdst, hist, ch, shift);
+        }
     }
     LAUNCHCHK();
     return GCS_OK;

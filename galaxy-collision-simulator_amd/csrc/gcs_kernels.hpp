@@ -970,6 +970,7 @@ __device__ __forceinline__ void climb_rounds(const SegInfo& sg, int* la, int n, 
 // both are such, else parked in the node's own LDS record.  Climbs then run
 // in LDS, and every node record is written once at the end.
 constexpr int kTopCap = 2048;
+constexpr int kTopPhaseNodes = 8192;  // most listed nodes of the phased form
 struct TopLds {
     float4 dat[kTopCap][4];
     float qt[kTopCap];
@@ -1005,12 +1006,28 @@ __device__ __forceinline__ NodeData child_data(int idx, bool leaf, const SegInfo
     return d;
 }
 
-__device__ __forceinline__ void top_lds(TopLds& T, const SegInfo& sg, int nt,
-                                        const int* __restrict__ tlist,
-                                        const int* __restrict__ tslot,
+// Size class of a node of more than kATile particles: eight classes per
+// octave of its particle count.
+constexpr int kATileShift = 10;
+static_assert(kATile == 1 << kATileShift, "shift");
+constexpr int kTopBuckets = 8 * (30 - kATileShift);
+__device__ __forceinline__ int top_bucket(int size)
+{
+    const int lg = 31 - __clz(size);
+    return 8 * (lg - kATileShift) + ((size >> (lg - 3)) & 7);
+}
+
+// list[0, nt): the nodes to finish (slot k = list[k], tslot[node] = k).
+// phase_of == nullptr: every node of more than kATile particles is in the
+// list.  Else only those whose size class c has phase_of[c] == phase: the
+// smaller ones were finished by earlier phases (read like aggregation
+// subtree roots), and a climb stops below a larger one.
+__device__ __forceinline__ void top_lds(TopLds& T, const SegInfo& sg, int nt, const int* list,
+                                        const int* tslot,
                                         const float4* __restrict__ posm, float4* node,
                                         float4* nodeQ, float4* boxLo, float4* boxHi,
-                                        const int* __restrict__ parent, float inv_theta, int mode)
+                                        const int* __restrict__ parent, float inv_theta, int mode,
+                                        const uint8_t* phase_of, int phase)
 {
     constexpr int kPer = kTopCap / 1024;
     const int tid = threadIdx.x;
@@ -1023,7 +1040,7 @@ __device__ __forceinline__ void top_lds(TopLds& T, const SegInfo& sg, int nt,
         vk[r] = -1;
         start[r] = false;
         if (k >= nt) continue;
-        const int v = tlist[sg.start + k];
+        const int v = list[k];
         vk[r] = v;
         const float4 bb = node[2 * v + 1];
         const int first = __float_as_int(bb.y), last = __float_as_int(bb.z),
@@ -1032,8 +1049,18 @@ __device__ __forceinline__ void top_lds(TopLds& T, const SegInfo& sg, int nt,
         const bool lleaf = first == split, rleaf = last == split + 1;
         const int lidx = lleaf ? split + sg.leaf_base : split + sg.int_base;
         const int ridx = rleaf ? split + 1 + sg.leaf_base : split + 1 + sg.int_base;
-        const bool ltop = split - first + 1 > kATile, rtop = last - split > kATile;
-        T.par[k] = (uint16_t)(pv < 0 ? 0 : tslot[pv] + 1);
+        const int nl = split - first + 1, nr = last - split;
+        bool ltop = nl > kATile, rtop = nr > kATile, pin = pv >= 0;
+        if (phase_of) {
+            ltop = ltop && phase_of[top_bucket(nl)] == phase;
+            rtop = rtop && phase_of[top_bucket(nr)] == phase;
+            if (pin) {
+                const float4 pb = node[2 * pv + 1];
+                pin = phase_of[top_bucket(__float_as_int(pb.z) - __float_as_int(pb.y) + 1)] ==
+                      phase;
+            }
+        }
+        T.par[k] = (uint16_t)(pin ? tslot[pv] + 1 : 0);
         T.cl[k] = (uint16_t)(ltop ? tslot[lidx] + 1 : 0);
         T.cr[k] = (uint16_t)(rtop ? tslot[ridx] + 1 : 0);
         if (!ltop || !rtop) {
@@ -1087,7 +1114,11 @@ __device__ __forceinline__ void top_lds(TopLds& T, const SegInfo& sg, int nt,
 
 // k_aggregate_top: one block per instance finishes the nodes of more than
 // kATile particles: in LDS (top_lds) when the build listed at most kTopCap of
-// them, else by climbing from the roots of the aggregation subtrees with
+// them, else (at most kTopPhaseNodes) in LDS in phases of at most kTopCap
+// nodes by size class (a phase's stores are visible to the next through the
+// block's barrier: every access is by this block), else (more nodes, or a
+// size class of more than kTopCap nodes) by climbing from the roots of the
+// aggregation subtrees with
 // arrival marks at workgroup scope (bits in LDS, an LDS hash, or counters in
 // global memory for very large instances) -- every arrival at a node of an
 // instance is in that instance's block, so no write-back or invalidate across
@@ -1109,19 +1140,94 @@ __global__ __launch_bounds__(1024) void k_aggregate_top(
     const SegInfo* __restrict__ segs, const float4* __restrict__ posm, float4* node,
     float4* nodeQ, float4* boxLo, float4* boxHi, const int* __restrict__ parent, int* flag,
     int* front, const int* __restrict__ nfront, const int* __restrict__ tlist,
-    const int* __restrict__ ntop, const int* __restrict__ tslot, float inv_theta, int mode)
+    const int* __restrict__ ntop, int* tslot, float inv_theta, int mode)
 {
     __shared__ __align__(16) uint32_t smem[kTopSmemWords];
     __shared__ int lcount;
+    __shared__ uint32_t bhist[kTopBuckets];
+    __shared__ uint8_t bphase[kTopBuckets];
+    __shared__ int nphase;
     uint32_t* const lmem = smem;
+    TopLds& T = *reinterpret_cast<TopLds*>(smem);
     const int s = blockIdx.x;
+    const int tid = (int)threadIdx.x;
     const SegInfo sg = segs[s];
     const int nt = tlist ? ntop[s] : kTopCap + 1;
     if (nt <= kTopCap) {
         if (nt > 0)
-            top_lds(*reinterpret_cast<TopLds*>(smem), sg, nt, tlist, tslot, posm, node, nodeQ,
-                    boxLo, boxHi, parent, inv_theta, mode);
+            top_lds(T, sg, nt, tlist + sg.start, tslot, posm, node, nodeQ, boxLo, boxHi, parent,
+                    inv_theta, mode, nullptr, 0);
         return;
+    }
+    if (tlist && nt <= kTopPhaseNodes) {
+        // Phases in LDS: the listed nodes by size class, consecutive classes
+        // taken together while they hold at most kTopCap nodes; phase p
+        // finishes its classes after phases 0 .. p-1 finished the smaller
+        // ones (a node's children are smaller than it).  Not possible when
+        // one class alone holds more than kTopCap nodes.  Each thread keeps
+        // its nodes (k = tid + r * 1024) and their classes in registers.
+        constexpr int kPerT = kTopPhaseNodes / 1024;
+        int pv[kPerT];
+        int pc[kPerT];
+        for (int k = tid; k < kTopBuckets; k += blockDim.x) bhist[k] = 0u;
+#pragma unroll
+        for (int r = 0; r < kPerT; ++r) {
+            const int k = tid + r * 1024;
+            pv[r] = k < nt ? tlist[sg.start + k] : -1;
+        }
+        float4 pb[kPerT];
+#pragma unroll
+        for (int r = 0; r < kPerT; ++r)
+            pb[r] = pv[r] >= 0 ? node[2 * pv[r] + 1] : make_float4(0.f, 0.f, 0.f, 0.f);
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < kPerT; ++r) {
+            pc[r] = -1;
+            if (pv[r] >= 0) {
+                pc[r] = top_bucket(__float_as_int(pb[r].z) - __float_as_int(pb[r].y) + 1);
+                atomicAdd(&bhist[pc[r]], 1u);
+            }
+        }
+        __syncthreads();
+        if (tid == 0) {
+            int ph = 0;
+            uint32_t cum = 0;
+            bool ok = true;
+#pragma unroll 16
+            for (int c = 0; c < kTopBuckets; ++c) {
+                const uint32_t h = bhist[c];
+                if (h > (uint32_t)kTopCap) ok = false;
+                if (cum + h > (uint32_t)kTopCap) {
+                    ++ph;
+                    cum = 0;
+                }
+                cum += h;
+                bphase[c] = (uint8_t)ph;
+            }
+            nphase = ok ? ph + 1 : 0;
+        }
+        __syncthreads();
+        const int np = nphase;
+        if (np > 0) {
+            int* plist = front + 2 * (size_t)sg.start;
+            for (int ph = 0; ph < np; ++ph) {
+                if (tid == 0) lcount = 0;
+                __syncthreads();
+#pragma unroll
+                for (int r = 0; r < kPerT; ++r) {
+                    if (pc[r] >= 0 && bphase[pc[r]] == ph) {
+                        const int slot = atomicAdd(&lcount, 1);
+                        plist[slot] = pv[r];
+                        tslot[pv[r]] = slot;
+                    }
+                }
+                __syncthreads();  // the phase's list and slots are visible to the block
+                top_lds(T, sg, lcount, plist, tslot, posm, node, nodeQ, boxLo, boxHi, parent,
+                        inv_theta, mode, bphase, ph);
+                __syncthreads();  // its records are visible to the next phase
+            }
+            return;
+        }
     }
     const int nf = nfront[s];
     if (nf == 0) return;

@@ -185,6 +185,43 @@ def test_engine_c1_group_parity(gcs, orc, gsize, offset, team, mono, kappa):
         assert rel_rms(acc[:, 3], ref[:, 3]) < (1e-4 if mono == 0 and kappa == 0 else 5e-4)
 
 
+def top_classes(cpu, atile=1024):
+    """Nodes of more than `atile` particles (the ones the instance pass
+    finishes) per size class (eight per octave), from the oracle's tree."""
+    size = (cpu["last"] - cpu["first"] + 1).astype(np.int64)
+    size = size[size > atile]
+    lg = np.floor(np.log2(size)).astype(np.int64)
+    cls = 8 * (lg - 10) + ((size >> (lg - 3)) & 7)
+    return len(size), np.bincount(cls)
+
+
+@pytest.mark.parametrize("case", ["phases", "global"])
+def test_instance_pass_large_instances(gcs, orc, case):
+    """The instance pass on instances with more than 2048 nodes above the
+    aggregation subtrees: in LDS phases by size class (two collided galaxies
+    of 1,048,576), and by climbing with arrival marks when one size class
+    alone holds more than 2048 nodes (4.4M particles uniform in a cube).
+    Pipeline bit-exact against the oracle, aggregates included."""
+    if case == "phases":
+        stars = gcs.collision(1048576, 1007)
+    else:
+        stars = cloud(gcs, 4404019, 8)
+        for k in ("x", "y", "z"):
+            stars[k] = np.random.default_rng({"x": 1, "y": 2, "z": 3}[k]).uniform(
+                -1, 1, len(stars)).astype(np.float32)
+    p = gcs.params(G=G, eps2=0.0025, dt=0.01, theta=0.5, leaf_size=8, group_size=128)
+    with gcs.Engine(len(stars)) as e:
+        e.load(stars)
+        say("instance pass, %s: n=%d" % (case, len(stars)))
+        e.compute_accels(p)
+        cpu = check_pipeline(gcs, orc, e, stars, p)
+    nt, hist = top_classes(cpu)
+    say("instance pass, %s: %d nodes above the subtrees, largest size class %d"
+        % (case, nt, hist.max()))
+    assert nt > 2048
+    assert (hist.max() <= 2048) == (case == "phases")
+
+
 def test_engine_reference_mode_parity(gcs, orc):
     stars = cloud(gcs, 20000, 7)
     eps2 = 0.0625
