@@ -74,6 +74,10 @@ struct gcs_server {
     double particle_steps = 0.0, run_s = 0.0, resident_sum = 0.0, gpu_s = 0.0;
     double full_t0 = -1.0, full_steps = 0.0, full_s = 0.0;  // from the first full round
     bool full = false;  // admission was held back by the cap in this run
+    // The server clock: arrival, admission and finish times count from the
+    // start of the first gcs_server_run; later runs continue it.
+    std::chrono::steady_clock::time_point clock0;
+    bool clock_started = false;
     int64_t steps_run = 0, rounds = 0, admitted = 0;
 
     void worker()
@@ -230,7 +234,12 @@ int gcs_server_submit(gcs_server* s, const gcs_job* jobs, int32_t n)
 int gcs_server_run(gcs_server* s, double duration_s)
 {
     if (!s) return GCS_ERR_INVALID;
-    const auto t0 = std::chrono::steady_clock::now();
+    if (!s->clock_started) {
+        s->clock0 = std::chrono::steady_clock::now();
+        s->clock_started = true;
+    }
+    const auto t0 = s->clock0;
+    const double t_begin = now_s(t0);
     int rc = GCS_OK;
     for (;;) {
         const double t = now_s(t0);
@@ -252,7 +261,7 @@ int gcs_server_run(gcs_server* s, double duration_s)
         if (rc != GCS_OK) break;
         const bool all_done = s->next_admit == s->jobs.size() && s->resident.empty();
         if (all_done && duration_s <= 0) break;
-        if (duration_s > 0 && t >= duration_s) break;
+        if (duration_s > 0 && t - t_begin >= duration_s) break;
         if (s->resident.empty()) {
             // Wait for the next arrival or generation.
             std::unique_lock<std::mutex> lk(s->mu);
@@ -308,7 +317,7 @@ int gcs_server_run(gcs_server* s, double duration_s)
         s->resident.swap(keep);
     }
     const double t_end = now_s(t0);
-    s->run_s += t_end;
+    s->run_s += t_end - t_begin;
     if (s->full_t0 >= 0.0) s->full_s += t_end - s->full_t0;
     s->full_t0 = -1.0;
     s->full = false;

@@ -1124,6 +1124,17 @@ __global__ __launch_bounds__(TEAM * 64) void k_traverse_team(
 // the stack error bit.  Every expression rounds as nvcc compiles the
 // reference (--fmad=true): d^2 = fma(dz,dz, fma(dx,dx, dy*dy)) + eps^2, and
 // the accumulation's multiply-adds; nothing else is contracted.
+//
+// One wave walks for its 64 particles (consecutive in Morton order) at once:
+// an LDS stack of (node, lane mask) entries, the mask holding the lanes whose
+// own walk reaches the node.  A popped node is read once (scalar loads) and
+// decided by the lanes of its mask; the lanes that open it push its children
+// with their mask.  Every lane thus meets its own nodes in the order of its own
+// DFS (the tree's preorder, right child first), so its sums are those of a
+// per-particle walk.  The pending entries are left children along the current
+// path, their masks nested, so the stack never holds more entries than the
+// deepest lane's own stack: at most `limit`.  Each lane counts its own depth
+// and stops, with the error bit, where its own walk would overflow.
 template <int STACK>
 __global__ __launch_bounds__(256) void k_traverse_reference(
     const TileInfo* __restrict__ tiles, const SegInfo* __restrict__ segs,
@@ -1132,57 +1143,89 @@ __global__ __launch_bounds__(256) void k_traverse_reference(
     int limit)
 {
 #pragma clang fp contract(off)
+    __shared__ int snode[4][STACK];
+    __shared__ uint64_t smask[4][STACK];
     const TileInfo t = tiles[blockIdx.x >> 2];
+    const int w = (int)threadIdx.x >> 6, l = lane_id();
     const int off = (int)(blockIdx.x & 3) * kTileThreads + (int)threadIdx.x;
+    const bool valid = off < t.count;
+    const int i = t.start + (valid ? off : 0);
+    const SegInfo sg = segs[t.seg];
+    const float4 p = posm[i];
+    float ax = 0.f, ay = 0.f, az = 0.f;
     unsigned long long count = 0;
-    if (off < t.count) {
-        const int i = t.start + off;
-        const SegInfo sg = segs[t.seg];
-        const float4 p = posm[i];
-        float ax = 0.f, ay = 0.f, az = 0.f;
-        int stack[STACK];
-        int top = 0;
-        stack[top++] = sg.root;
-        bool overflow = false;
-        while (top > 0) {
-            const int v = stack[--top];
-            const float4 A = node[2 * v];
-            const float4 bb = node[2 * v + 1];
-            const int first = __float_as_int(bb.y), last = __float_as_int(bb.z),
-                      split = __float_as_int(bb.w);
-            const float dx = A.x - p.x, dy = A.y - p.y, dz = A.z - p.z;
-            const float d2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dx, dx, dy * dy)) + eps2;
-            bool take;
+    int depth = 1;  // this lane's own stack depth
+    uint64_t live = ballot(valid);  // lanes still walking
+    bool overflow = false;
+    int top = 0;
+    if (live) {
+        if (l == 0) {
+            snode[w][0] = sg.root;
+            smask[w][0] = live;
+        }
+        top = 1;
+    }
+    __builtin_amdgcn_wave_barrier();
+    while (top > 0) {
+        --top;
+        const int v = __builtin_amdgcn_readfirstlane(snode[w][top]);
+        const uint64_t m = smask[w][top] & live;
+        __builtin_amdgcn_wave_barrier();
+        if (m == 0ull) continue;
+        const bool act = (m >> l) & 1ull;
+        if (act) --depth;
+        const float4 A = node[2 * v];
+        const float4 bb = node[2 * v + 1];
+        const int first = __float_as_int(bb.y), last = __float_as_int(bb.z),
+                  split = __float_as_int(bb.w);
+        const float dx = A.x - p.x, dy = A.y - p.y, dz = A.z - p.z;
+        const float d2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dx, dx, dy * dy)) + eps2;
+        bool take = false, open = false;
+        if (act) {
             if (split < 0) {
                 take = d2 > eps2 * 1.01f;
             } else {
                 const float width = bb.x;
                 take = width * width < d2 * 1.0f;
                 if (!take) {
-                    if (top + 2 > limit) {
+                    if (depth + 2 > limit) {
                         overflow = true;
-                        break;
+                    } else {
+                        open = true;
+                        depth += 2;
                     }
-                    stack[top++] = (first == split) ? split + sg.leaf_base : split + sg.int_base;
-                    stack[top++] = (last == split + 1) ? split + 1 + sg.leaf_base
-                                                       : split + 1 + sg.int_base;
                 }
             }
-            if (take) {
-                const float inv = __builtin_amdgcn_rsqf(d2);
-                const float inv3 = inv * inv * inv;
-                const float f = G * A.w * inv3;
-                ax = __builtin_fmaf(dx, f, ax);
-                ay = __builtin_fmaf(dy, f, ay);
-                az = __builtin_fmaf(dz, f, az);
-                ++count;
-            }
         }
+        if (take) {
+            const float inv = __builtin_amdgcn_rsqf(d2);
+            const float inv3 = inv * inv * inv;
+            const float f = G * A.w * inv3;
+            ax = __builtin_fmaf(dx, f, ax);
+            ay = __builtin_fmaf(dy, f, ay);
+            az = __builtin_fmaf(dz, f, az);
+            ++count;
+        }
+        live &= ~ballot(overflow);
+        const uint64_t om = ballot(open);
+        if (om != 0ull) {
+            if (l == 0) {
+                snode[w][top] = (first == split) ? split + sg.leaf_base : split + sg.int_base;
+                smask[w][top] = om;
+                snode[w][top + 1] =
+                    (last == split + 1) ? split + 1 + sg.leaf_base : split + 1 + sg.int_base;
+                smask[w][top + 1] = om;
+            }
+            top += 2;
+        }
+        __builtin_amdgcn_wave_barrier();
+    }
+    if (valid) {
         if (overflow) atomicOr(err, kErrStack);
         acc[i] = make_float4(ax, ay, az, 0.0f);
     }
     count = wave_sum(count);
-    if (lane_id() == 0 && count) atomicAdd(&counters[1], count);
+    if (l == 0 && count) atomicAdd(&counters[1], count);
 }
 
 }  // namespace gcs

@@ -348,7 +348,8 @@ typedef struct gcs_job {
     int32_t n;          /* particles (split 50/50 between the two galaxies) */
     int32_t steps;
     uint64_t seed;      /* gcs_spec_random(seed, id, n, n) fixes the scenario */
-    double arrival_s;   /* seconds after the start of gcs_server_run */
+    double arrival_s;   /* server-clock seconds (the clock starts with the first
+                           gcs_server_run call and runs on across calls) */
 } gcs_job;
 
 typedef struct gcs_job_result {
@@ -375,7 +376,8 @@ int gcs_server_destroy(gcs_server* s);
 /* Adds jobs to the arrival schedule (before or between runs). */
 int gcs_server_submit(gcs_server* s, const gcs_job* jobs, int32_t n);
 /* Runs the loop for `duration_s` seconds of wall time, arrivals following the
- * wall clock, or (duration_s <= 0) until every submitted job has finished. */
+ * server clock, or (duration_s <= 0) until every submitted job has finished.
+ * A later call continues where the last stopped (same clock, same jobs). */
 int gcs_server_run(gcs_server* s, double duration_s);
 /* Finished jobs in completion order. */
 int gcs_server_results(gcs_server* s, gcs_job_result* out, int32_t cap, int32_t* n_out);
