@@ -32,7 +32,22 @@
  *                                  cases just either side of each threshold
  *   orc_traverse_group             theta=0 and leaf>=n equal the direct sum;
  *                                  Plummer two-body closed form; every mass
- *                                  counted once; error bound vs FP64
+ *                                  counted once; error bound vs FP64; the
+ *                                  group opening threshold either side of
+ *                                  (w/theta)^2 (and of the offset form); the
+ *                                  softened quadrupole's second-order
+ *                                  convergence (halving a cluster cuts the
+ *                                  error ~8x); far monopoles: the threshold
+ *                                  either side of mono_far x the opening
+ *                                  radius against the monopole's closed form,
+ *                                  unchanged lists; local expansions: third-
+ *                                  order convergence in the target offset
+ *                                  (~8x per halving; a dropped B term gives
+ *                                  4x, a dropped A term 2x) for force and
+ *                                  potential, the threshold either side of
+ *                                  kappa x the group half-diagonal, every
+ *                                  mass kept, the per-target counts split
+ *                                  exactly between node and local terms
  *   orc_tree_groups                maximal-subtree definition by brute force
  *   orc_direct_sum_f64             Plummer two-body closed form, Newton's
  *                                  third law (zero total force)
