@@ -74,8 +74,9 @@ TUNE_TABLE = os.path.join(REPO, "galaxy-collision-simulator_amd", "gcs_amd",
 
 # HBM bytes per traversal launch from rocprofv3 FETCH_SIZE (x2.000) and
 # WRITE_SIZE (x1.000) passes, scaled by a device copy of known size; valid for
-# the (workload, leaf, block, waves per SIMD, group, team) measured.
-TRAVERSE_TRAFFIC = {}
+# the (workload, leaf, block, waves per SIMD, group, team, local_kappa)
+# measured: 3,606,992 KiB fetched, 584,319 KiB written per launch.
+TRAVERSE_TRAFFIC = {("batched-256", 4, 64, 4, 128, 1, 4.0): 7.986e9}
 
 
 def say(*a):
