@@ -80,6 +80,19 @@ def test_sort_pairs_match_oracle(gcs, orc, n):
     assert np.array_equal(ko, eo) and np.array_equal(po, ep)
 
 
+@pytest.mark.parametrize("n", [786431, 786432, 786432 + 4097])
+def test_sort_pairs_at_the_tile_size_threshold(gcs, orc, n):
+    """Sorts of fewer than 786,432 keys run in 1,024-key blocks, larger ones in
+    4,096-key blocks (a last block of one key included): both bit-exact."""
+    r = np.random.default_rng(n)
+    keys = r.integers(0, 1 << 63, n, dtype=np.uint64)
+    keys[::7] = keys[::7] % np.uint64(1000)  # many equal keys
+    say("sort n=%d" % n)
+    ko, po = gcs.sort_pairs(keys)
+    eo, ep = orc.stable_sort_pairs(keys)
+    assert np.array_equal(ko, eo) and np.array_equal(po, ep)
+
+
 def test_sort_pairs_full_64_bits_and_presorted(gcs, orc):
     r = np.random.default_rng(5)
     keys = r.integers(0, 1 << 64, 2_000_000, dtype=np.uint64)  # top bit used too
