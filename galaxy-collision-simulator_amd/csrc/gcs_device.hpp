@@ -51,6 +51,37 @@ __device__ __forceinline__ T wave_sum(T v)
     return v;
 }
 
+// Inclusive prefix sum of an integer over the wave from DPP moves (row
+// shifts by 1, 2, 4, 8 within each row of 16 lanes, then the broadcasts of
+// lanes 15 and 31 into the later rows): six adds, no LDS traffic.  Called
+// with all 64 lanes active.
+__device__ __forceinline__ int wave_scan_incl_dpp(int x)
+{
+    x += __builtin_amdgcn_update_dpp(0, x, 0x111, 0xf, 0xf, false);  // row_shr:1
+    x += __builtin_amdgcn_update_dpp(0, x, 0x112, 0xf, 0xf, false);  // row_shr:2
+    x += __builtin_amdgcn_update_dpp(0, x, 0x114, 0xf, 0xf, false);  // row_shr:4
+    x += __builtin_amdgcn_update_dpp(0, x, 0x118, 0xf, 0xf, false);  // row_shr:8
+    x += __builtin_amdgcn_update_dpp(0, x, 0x142, 0xa, 0xf, false);  // row_bcast:15, rows 1, 3
+    x += __builtin_amdgcn_update_dpp(0, x, 0x143, 0xc, 0xf, false);  // row_bcast:31, rows 2, 3
+    return x;
+}
+
+// 32-bit words at base, as a buffer resource: a store at a byte offset of
+// kDrop (beyond any such range) is dropped by the range check, which makes a
+// per-lane conditional store a plain store.
+constexpr int kDrop = (int)0x80000000;
+constexpr int kBufferDword3 = 0x00020000;  // gfx9 raw buffer: 32-bit data, no swizzle
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t word_rsrc(int* base, int words)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(base, 0, words * 4, kBufferDword3);
+}
+
+__device__ __forceinline__ void store_word(__amdgpu_buffer_rsrc_t r, int byte_off, int v)
+{
+    __builtin_amdgcn_raw_buffer_store_b32(v, r, byte_off, 0, 0);
+}
+
 // Inclusive prefix sum over the wave.
 // Exclusive prefix sum over the wave of a value below 2^bits (bits uniform),
 // from one ballot per bit: no cross-lane data movement.  *total = wave sum.

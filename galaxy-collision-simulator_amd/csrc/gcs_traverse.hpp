@@ -59,7 +59,7 @@ constexpr int kMaxPer = 3;                  // most targets per lane
 constexpr int kChunk = kWave + kMaxSplit;   // a chunk of 64 sources and a zero pad
 
 struct WaveLds {
-    int stk[kLdsStack];
+    int stk[kLdsStack + 1];  // + a spare entry that takes discarded pushes
     float4 ca[kChunk];   // COM and mass, or position and mass
     float4 cq0[kChunk];  // Qxx, Qyy, Qzz, Qxy
     float4 cq1[kChunk];  // Qxz, Qyz, (width), T
@@ -468,7 +468,7 @@ __global__ __launch_bounds__(WPB * 64) void k_traverse_group(
 {
     __shared__ WaveLds lds[WPB];
     const int l = lane_id();
-    const int w = threadIdx.x >> 6;
+    const int w = uniform(threadIdx.x >> 6);  // wave-uniform: the slot addresses stay scalar
     WaveLds& L = lds[w];
     int* const spill = slot_mem + (size_t)(blockIdx.x * WPB + w) * kSlotInts;
     int* const nl = spill + kStackCap;
@@ -480,8 +480,6 @@ __global__ __launch_bounds__(WPB * 64) void k_traverse_group(
     unsigned long long n_node = 0, n_part = 0, n_far = 0, n_loc = 0, n_loct = 0;
     int q = 0;
     const float self_inv = __builtin_amdgcn_rsqf(eps2);
-    const int pbits = 32 - __builtin_clz((unsigned)(leaf + 2));  // bits of a round's count
-    const uint64_t lt = (1ull << l) - 1ull;
     bool overflow = false;
     // The pads after each chunk hold zero-mass sources for the whole launch.
     if (l < kMaxSplit) {
@@ -537,12 +535,18 @@ __global__ __launch_bounds__(WPB * 64) void k_traverse_group(
         }
         int nn = 0, np = 0, nloc = 0, top = 0, spilled = 0;
         long long tot_nodes = 0, tot_parts = 0, tot_far = 0, tot_loc = 0;
-        int dl_pos = -1, dl_val = 0;  // local-list entry of the previous round
-        // List stores of one round are issued in the next round after its node
-        // loads, so that the wait for those loads does not also wait for them
-        // (vector memory operations complete in issue order).
-        int dn_pos = -1, dn_val = 0;          // node-list entry
-        int dp_pos = 0, dp_self = 0, dp_first = 0, dp_lr = 0, dp_split = 0;  // particles
+        // The lists are written through one buffer resource over the wave's
+        // slot: a lane with nothing to write stores at kDrop.  List stores of
+        // one round are issued in the next round after its node loads, so that
+        // the wait for those loads does not also wait for them (vector memory
+        // operations complete in issue order): the node- or local-list byte
+        // offset of the lane's node and its index, and its particle run
+        // [dp_f, dp_f + dp_c) at byte offset dp_off.
+        const __amdgpu_buffer_rsrc_t rs = word_rsrc(spill, kSlotInts);
+        constexpr int kNlOff = 4 * kStackCap, kPlOff = kNlOff + 4 * kNodeList,
+                      kLlOff = kPlOff + 4 * kPartList;
+        int dn_off = kDrop, dl_off = kDrop, dv = 0, dp_off = 0, dp_f = 0, dp_c = 0;
+        int dp_max = 0;  // the longest run of the round (uniform)
         if (sg.n == 1) {
             if (l == 0) pl[0] = sg.start;
             np = 1;
@@ -560,39 +564,32 @@ __global__ __launch_bounds__(WPB * 64) void k_traverse_group(
                 top = uniform(c);
                 __builtin_amdgcn_wave_barrier();
             }
+            // Lanes past the popped entries repeat the last one and decide
+            // nothing.
             const int cnt = min(top, kWave);
             top -= cnt;
             const bool have = l < cnt;
-            int v = 0;
-            if (have) v = L.stk[top + l];
-            float4 A = make_float4(0.f, 0.f, 0.f, 0.f);
-            int4 B = make_int4(0, 0, 0, 0);
-            bool accept = false, far = false, loc = false, self_direct = false, open = false;
-            float4 bb = make_float4(0.f, 0.f, 0.f, 0.f);
-            if (have) {
-                A = node[2 * v];
-                bb = node[2 * v + 1];
-            }
+            const int v = L.stk[top + min(l, cnt - 1)];
+            const float4 A = node[2 * v];
+            const float4 bb = node[2 * v + 1];
             // The previous round's list entries.
-            if (dn_pos >= 0) nl[dn_pos] = dn_val;
-            if (LOCAL && dl_pos >= 0) ll[dl_pos] = dl_val;
-            dl_pos = -1;
-            for (int k = 0; k < dp_self; ++k) pl[dp_pos + k] = dp_first + k;
-            if (dp_lr & 1) pl[dp_pos + dp_self] = dp_split;
-            if (dp_lr & 2) pl[dp_pos + dp_self + (dp_lr & 1)] = dp_split + 1;
-            dn_pos = -1;
-            dp_self = 0;
-            dp_lr = 0;
-            if (have) {
-                B = make_int4(__float_as_int(bb.x), __float_as_int(bb.y), __float_as_int(bb.z),
-                              __float_as_int(bb.w));
-                const float d2b = box_dist2(lx, ly, lz, hx, hy, hz, A.x, A.y, A.z);
-                accept = d2b > bb.x;
-                loc = LOCAL && accept && dist2_nc(A.x - gcx, A.y - gcy, A.z - gcz) > glim;
-                far = accept && !loc && d2b > far2 * bb.x;
-                self_direct = !accept && (B.z - B.y + 1) <= leaf;
-                open = !accept && !self_direct;
-            }
+            store_word(rs, dn_off, dv);
+            if (LOCAL) store_word(rs, dl_off, dv);
+            for (int k = 0; k < dp_max; ++k)
+                store_word(rs, k < dp_c ? dp_off + 4 * k : kDrop, dp_f + k);
+            const int first = __float_as_int(bb.y), last = __float_as_int(bb.z),
+                      split = __float_as_int(bb.w);
+            // (bitwise logic: every lane evaluates every test, no branches)
+            const float d2b = box_dist2(lx, ly, lz, hx, hy, hz, A.x, A.y, A.z);
+            const float d2c = dist2_nc(A.x - gcx, A.y - gcy, A.z - gcz);
+            const bool accept = have & (d2b > bb.x);
+            const bool loc = LOCAL & accept & (d2c > glim);
+            const bool far = accept & !loc & (d2b > far2 * bb.x);
+            const bool near = accept & !far & !loc;
+            const int size = last - first + 1;
+            const bool direct = have & !accept & (size <= leaf);
+            const bool open = have & !accept & !direct;
+            const bool lleaf = open & (first == split), rleaf = open & (last == split + 1);
             // accepted nodes far from the group -> local list; beyond far x the
             // opening radius -> particle list (monopole); the others -> node
             // list (quadrupole)
@@ -604,14 +601,10 @@ __global__ __launch_bounds__(WPB * 64) void k_traverse_group(
                                          kWave);
                     nloc = 0;
                 }
-                if (loc) {
-                    dl_pos = nloc + mbcnt(mL);
-                    dl_val = v;
-                }
+                dl_off = loc ? kLlOff + 4 * (nloc + mbcnt(mL)) : kDrop;
                 nloc = uniform(nloc + nlo);
                 tot_loc += nlo;
             }
-            const bool near = accept && !far && !loc;
             const uint64_t mA = ballot(near);
             const int na = __popcll(mA);
             if (nn + na > kNodeList) {
@@ -619,34 +612,35 @@ __global__ __launch_bounds__(WPB * 64) void k_traverse_group(
                 tot_nodes += nn;
                 nn = 0;
             }
-            if (near) {
-                dn_pos = nn + mbcnt(mA);
-                dn_val = v;
-            }
+            dn_off = near ? kNlOff + 4 * (nn + mbcnt(mA)) : kDrop;
+            dv = v;
             nn = uniform(nn + na);
             tot_far += __popcll(ballot(far));
-            // particles: the node's own (summed), its single-particle children,
-            // or the node itself as a monopole (~v)
-            const bool lleaf = open && B.y == B.w, rleaf = open && B.z == B.w + 1;
-            const int c_self = self_direct ? B.z - B.y + 1 : (far ? 1 : 0);
-            const int cnt_p = c_self + (lleaf ? 1 : 0) + (rleaf ? 1 : 0);
-            int tp;
-            const int ex = wave_scan_excl_bits(cnt_p, pbits, &tp);
+            // particles: the run [pf, pf + pc): the node's own (summed), its
+            // single-particle children, or the node itself as a monopole (~v)
+            int pc = far ? 1 : (int)lleaf + (int)rleaf;
+            pc = direct ? size : pc;
+            int pf = lleaf ? split : split + 1;
+            pf = far ? ~v : pf;
+            pf = direct ? first : pf;
+            const int incl = wave_scan_incl_dpp(pc);
+            const int tp = __builtin_amdgcn_readlane(incl, kWave - 1);
             if (np + tp > kPartList) {
                 eval_parts<POT>(L, pl, np, posm, node, a, tg, eps2, sh);
                 tot_parts += np;
                 np = 0;
             }
-            dp_pos = np + ex;
-            dp_self = c_self;
-            dp_first = far ? ~v : B.y;
-            dp_lr = (lleaf ? 1 : 0) | (rleaf ? 2 : 0);
-            dp_split = B.w;
+            dp_off = kPlOff + 4 * (np + incl - pc);
+            dp_f = pf;
+            dp_c = pc;
+            dp_max = 0;
+            while (ballot(dp_max < pc)) ++dp_max;
             np = uniform(np + tp);
-            // larger children -> stack
-            const bool lp = open && !lleaf, rp = open && !rleaf;
+            // larger children -> stack (lanes with nothing to push write the
+            // spare entry past the stack)
+            const bool lp = open & !lleaf, rp = open & !rleaf;
             const uint64_t m3 = ballot(lp), m4 = ballot(rp);
-            const int npush = __popcll(m3) + __popcll(m4);
+            const int c3 = __popcll(m3), npush = c3 + __popcll(m4);
             if (top + npush > kLdsStack) {
                 // Spill the oldest half of the LDS stack to global memory.
                 const int h = kLdsStack / 2;
@@ -664,21 +658,18 @@ __global__ __launch_bounds__(WPB * 64) void k_traverse_group(
                 spilled = uniform(spilled + h);
                 top -= h;
             }
-            if (lp) L.stk[top + __popcll(m3 & lt)] = B.w + sg.int_base;
-            top += __popcll(m3);
-            if (rp) L.stk[top + __popcll(m4 & lt)] = B.w + 1 + sg.int_base;
-            top = uniform(top + __popcll(m4));
+            L.stk[lp ? top + mbcnt(m3) : kLdsStack] = split + sg.int_base;
+            L.stk[rp ? top + c3 + mbcnt(m4) : kLdsStack] = split + 1 + sg.int_base;
+            top = uniform(top + npush);
             __builtin_amdgcn_wave_barrier();
         }
         if (overflow) {
             if (l == 0) atomicOr(err, kErrStack);
             break;
         }
-        if (dn_pos >= 0) nl[dn_pos] = dn_val;
-        if (LOCAL && dl_pos >= 0) ll[dl_pos] = dl_val;
-        for (int k = 0; k < dp_self; ++k) pl[dp_pos + k] = dp_first + k;
-        if (dp_lr & 1) pl[dp_pos + dp_self] = dp_split;
-        if (dp_lr & 2) pl[dp_pos + dp_self + (dp_lr & 1)] = dp_split + 1;
+        store_word(rs, dn_off, dv);
+        if (LOCAL) store_word(rs, dl_off, dv);
+        for (int k = 0; k < dp_max; ++k) store_word(rs, k < dp_c ? dp_off + 4 * k : kDrop, dp_f + k);
         if (LOCAL && nloc > 0)
             eval_local_list<POT>(L.loc, ll, nloc, node, nodeQ, gcx, gcy, gcz, eps2, 0, kWave);
         if (nn > 0) eval_nodes<POT, QUAD>(L, nl, nn, node, nodeQ, a, tg, eps2, sh);
@@ -806,10 +797,15 @@ __global__ __launch_bounds__(TEAM * 64) void k_traverse_team(
     const int tid = threadIdx.x;
     WaveLds& L = lds[w];
     int* const stk = lds[0].stk;
+    // The block owns TEAM wave slots: one stack spill area and TEAM times each
+    // list, written through one buffer resource (kDrop: nothing to write).
     int* const spill = slot_mem + (size_t)blockIdx.x * TEAM * kSlotInts;
     int* const nl = spill + kStackCap;
-    int* const pl = nl + kNodeList;
-    int* const ll = pl + kPartList;
+    int* const pl = nl + TEAM * kNodeList;
+    int* const ll = pl + TEAM * kPartList;
+    const __amdgpu_buffer_rsrc_t rs = word_rsrc(spill, TEAM * kSlotInts);
+    constexpr int kNlOff = 4 * kStackCap, kPlOff = kNlOff + 4 * TEAM * kNodeList,
+                  kLlOff = kPlOff + 4 * TEAM * kPartList;
     const int xcd = blockIdx.x % kNumXcd;
     const int ngroups = uniform(*ngroups_dev);
     const int per = (ngroups + kNumXcd - 1) / kNumXcd;
@@ -817,13 +813,12 @@ __global__ __launch_bounds__(TEAM * 64) void k_traverse_team(
     unsigned long long n_node = 0, n_part = 0, n_far = 0, n_loc = 0, n_loct = 0;
     int q = 0;
     const float self_inv = __builtin_amdgcn_rsqf(eps2);
-    const int pbits = 32 - __builtin_clz((unsigned)(leaf + 2));
-    const uint64_t lt = (1ull << l) - 1ull;
     // A round adds at most TEAM*64 node entries and TEAM*64*(leaf+2)
-    // particle entries; lists are evaluated before a round could overflow them.
-    const int node_room = kNodeList - TEAM * kWave;
-    const int part_room = kPartList - TEAM * kWave * (leaf + 2);
-    const int loc_room = kLocList - TEAM * kWave;
+    // particle entries; lists are evaluated before a round could overflow them
+    // (each room is positive for every leaf size up to kMaxLeaf).
+    const int node_room = TEAM * (kNodeList - kWave);
+    const int part_room = TEAM * (kPartList - kWave * (leaf + 2));
+    const int loc_room = TEAM * (kLocList - kWave);
     if (l < kMaxSplit) {
         const float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
         L.ca[kWave + l] = z;
@@ -895,37 +890,39 @@ __global__ __launch_bounds__(TEAM * 64) void k_traverse_team(
                     top = c;
                     __syncthreads();
                 }
+                // Lanes past the popped entries repeat the last one and
+                // decide nothing (bitwise logic: no branches).
                 const int cnt = min(top, TEAM * kWave);
                 const int e = w * kWave + l;
                 const bool have = e < cnt;
-                int v = 0;
-                if (have) v = stk[top - cnt + e];
-                float4 A = make_float4(0.f, 0.f, 0.f, 0.f);
-                float4 bb = A;
-                if (have) {
-                    A = node[2 * v];
-                    bb = node[2 * v + 1];
-                }
-                const int4 B = make_int4(__float_as_int(bb.x), __float_as_int(bb.y),
-                                         __float_as_int(bb.z), __float_as_int(bb.w));
+                const int v = stk[top - cnt + min(e, cnt - 1)];
+                const float4 A = node[2 * v];
+                const float4 bb = node[2 * v + 1];
+                const int first = __float_as_int(bb.y), last = __float_as_int(bb.z),
+                          split = __float_as_int(bb.w);
                 const float d2b = box_dist2(lx, ly, lz, hx, hy, hz, A.x, A.y, A.z);
-                const bool accept = have && d2b > bb.x;
+                const float d2c = dist2_nc(A.x - gcx, A.y - gcy, A.z - gcz);
+                const bool accept = have & (d2b > bb.x);
                 // far from the group: the local list; beyond far * the opening
                 // radius: a monopole in the particle list
-                const bool loc =
-                    LOCAL && accept && dist2_nc(A.x - gcx, A.y - gcy, A.z - gcz) > glim;
-                const bool far = accept && !loc && d2b > far2 * bb.x;
-                const bool near = accept && !far && !loc;
-                const bool self_direct = have && !accept && (B.z - B.y + 1) <= leaf;
-                const bool open = have && !accept && !self_direct;
-                const bool lleaf = open && B.y == B.w, rleaf = open && B.z == B.w + 1;
-                const int c_self = self_direct ? B.z - B.y + 1 : (far ? 1 : 0);
-                const int p_first = far ? ~v : B.y;
-                const int cnt_p = c_self + (lleaf ? 1 : 0) + (rleaf ? 1 : 0);
-                const bool lp = open && !lleaf, rp = open && !rleaf;
+                const bool loc = LOCAL & accept & (d2c > glim);
+                const bool far = accept & !loc & (d2b > far2 * bb.x);
+                const bool near = accept & !far & !loc;
+                const int size = last - first + 1;
+                const bool direct = have & !accept & (size <= leaf);
+                const bool open = have & !accept & !direct;
+                const bool lleaf = open & (first == split), rleaf = open & (last == split + 1);
+                // the particle-list run [pf, pf + pc): the node's own particles,
+                // its single-particle children, or the node as a monopole (~v)
+                int pc = far ? 1 : (int)lleaf + (int)rleaf;
+                pc = direct ? size : pc;
+                int pf = lleaf ? split : split + 1;
+                pf = far ? ~v : pf;
+                pf = direct ? first : pf;
+                const bool lp = open & !lleaf, rp = open & !rleaf;
                 const uint64_t mA = ballot(near), m3 = ballot(lp), m4 = ballot(rp);
-                int tp;
-                const int ex = wave_scan_excl_bits(cnt_p, pbits, &tp);
+                const int incl = wave_scan_incl_dpp(pc);
+                const int tp = __builtin_amdgcn_readlane(incl, kWave - 1);
                 int nfw = 0;
                 if (use_far) nfw = __popcll(ballot(far));
                 const uint64_t mL = LOCAL ? ballot(loc) : 0ull;
@@ -963,18 +960,19 @@ __global__ __launch_bounds__(TEAM * 64) void k_traverse_team(
                 }
                 // accepted nodes -> node list; particles and far nodes ->
                 // particle list
-                if (near) nl[nn + na_before + mbcnt(mA)] = v;
-                if (LOCAL && loc) ll[nloc + lo_before + mbcnt(mL)] = v;
-                int o = np + tp_before + ex;
-                for (int k = 0; k < c_self; ++k) pl[o + k] = p_first + k;
-                o += c_self;
-                if (lleaf) pl[o++] = B.w;
-                if (rleaf) pl[o] = B.w + 1;
+                store_word(rs, near ? kNlOff + 4 * (nn + na_before + mbcnt(mA)) : kDrop, v);
+                if (LOCAL)
+                    store_word(rs, loc ? kLlOff + 4 * (nloc + lo_before + mbcnt(mL)) : kDrop, v);
+                const int o = kPlOff + 4 * (np + tp_before + incl - pc);
+                int pmax = 0;
+                while (ballot(pmax < pc)) ++pmax;
+                for (int k = 0; k < pmax; ++k) store_word(rs, k < pc ? o + 4 * k : kDrop, pf + k);
                 nn += na_all;
                 np += tp_all;
                 nfar += nf_all;
                 nloc += lo_all;
                 // larger children -> stack, in place of the popped entries
+                // (lanes with nothing to push write the spare entry past it)
                 int base = top - cnt;
                 const int npush = nl_all + nr_all;
                 if (base + npush > kLdsStack) {
@@ -1000,8 +998,9 @@ __global__ __launch_bounds__(TEAM * 64) void k_traverse_team(
                     spilled += h;
                     base -= h;
                 }
-                if (lp) stk[base + nl_before + mbcnt(m3)] = B.w + sg.int_base;
-                if (rp) stk[base + nl_all + nr_before + mbcnt(m4)] = B.w + 1 + sg.int_base;
+                stk[lp ? base + nl_before + mbcnt(m3) : kLdsStack] = split + sg.int_base;
+                stk[rp ? base + nl_all + nr_before + mbcnt(m4) : kLdsStack] =
+                    split + 1 + sg.int_base;
                 top = base + npush;
             }
             if (overflow) break;
