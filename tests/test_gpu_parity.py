@@ -198,6 +198,36 @@ def test_engine_c1_group_parity(gcs, orc, gsize, offset, team, mono, kappa):
         assert rel_rms(acc[:, 3], ref[:, 3]) < (1e-4 if mono == 0 and kappa == 0 else 5e-4)
 
 
+@pytest.mark.parametrize("team,leaf,kappa", [(8, 64, 0.0), (4, 64, 4.0), (2, 48, 0.0),
+                                            (1, 64, 4.0)])
+def test_group_traversal_large_leaf(gcs, orc, team, leaf, kappa):
+    """Leaf sizes up to the largest allowed: a round of a team of 8 waves can
+    add 8 x 64 x 66 particle-list entries, which its lists hold."""
+    stars = gcs.collision(8192, 1001)
+    eps2 = np.float32(0.05) ** 2
+    p = gcs.params(G=G, eps2=float(eps2), dt=0.01, theta=0.5, leaf_size=leaf, group_size=128,
+                   team=team, local_kappa=kappa, block_size=64 * team if team > 1 else 64)
+    with gcs.Engine(len(stars)) as e:
+        e.load(stars)
+        say("team %d, leaf %d, kappa %.0f" % (team, leaf, kappa))
+        e.compute_accels(p)
+        cpu = check_pipeline(gcs, orc, e, stars, p)
+        acc = e.accels()
+        s = cpu["sorted"]
+        grp = orc.tree_groups(cpu["left"], cpu["right"], cpu["first"], cpu["last"], 128)
+        want, msum, cnt, aabs = orc.traverse_group(cpu["left"], cpu["right"], cpu["first"],
+                                                   cpu["last"], s["x"], s["y"], s["z"],
+                                                   s["mass"], cpu["agg"], G, float(eps2), 128,
+                                                   leaf, want_abs=True, groups=grp,
+                                                   local_kappa=kappa)
+        assert e.interaction_counts() == [int(c) for c in cnt]
+        dev = np.abs(acc[:, :3] - want[:, :3]).max(1) / aabs
+        say("max dev/sum|terms| %.3g" % dev.max())
+        assert dev.max() < 3e-5
+        ref = orc.direct_sum_f64(s["x"], s["y"], s["z"], s["mass"], G, float(eps2))
+        assert rel_rms(acc[:, :3], ref[:, :3]) < 1e-3
+
+
 def top_classes(cpu, atile=1024):
     """Nodes of more than `atile` particles (the ones the instance pass
     finishes) per size class (eight per octave), from the oracle's tree."""
