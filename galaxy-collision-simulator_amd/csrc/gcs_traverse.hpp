@@ -132,8 +132,10 @@ __device__ __forceinline__ void interact_p(Acc& a, float px, float py, float pz,
 // Node source with its traceless quadrupole Q and second-moment trace T,
 // expanded to second order in the softened kernel 1/R, R^2 = |d|^2 + eps^2
 // (the kernel's Laplacian -3 eps^2/R^5 brings in T):
-//   a += (M/R^3 + 5/2 (d.Qd - T eps^2)/R^7) d - Qd/R^5,
+//   a += (M/R^3 + 5/2 (d.Qd - T eps^2)/R^7) d - Qd/R^5
+//      = ((M R^2 + 5/2 (d.Qd - T eps^2)/R^2) d - Qd) / R^5,
 //   phi += M/R + (d.Qd - T eps^2)/(2 R^5).
+// q1.w holds T eps^2 (the list gather forms it once per node).
 template <bool POT, bool QUAD>
 __device__ __forceinline__ void interact_n(Acc& a, float px, float py, float pz, float4 c,
                                            float4 q0, float4 q1, float eps2)
@@ -142,8 +144,8 @@ __device__ __forceinline__ void interact_n(Acc& a, float px, float py, float pz,
     const float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, __builtin_fmaf(dz, dz, eps2)));
     const float inv = __builtin_amdgcn_rsqf(r2);
     const float inv2 = inv * inv;
-    const float minv = c.w * inv;
     if (!QUAD) {
+        const float minv = c.w * inv;
         const float f = minv * inv2;
         a.ax = __builtin_fmaf(dx, f, a.ax);
         a.ay = __builtin_fmaf(dy, f, a.ay);
@@ -154,16 +156,13 @@ __device__ __forceinline__ void interact_n(Acc& a, float px, float py, float pz,
     const float qx = __builtin_fmaf(q0.x, dx, __builtin_fmaf(q0.w, dy, q1.x * dz));
     const float qy = __builtin_fmaf(q0.w, dx, __builtin_fmaf(q0.y, dy, q1.y * dz));
     const float qz = __builtin_fmaf(q1.x, dx, __builtin_fmaf(q1.y, dy, q0.z * dz));
-    const float dqd =
-        __builtin_fmaf(-q1.w, eps2, __builtin_fmaf(dx, qx, __builtin_fmaf(dy, qy, dz * qz)));
-    const float inv3 = inv * inv2;
-    const float inv5 = inv3 * inv2;
-    const float t = dqd * inv5;
-    const float f = __builtin_fmaf(2.5f * t, inv2, c.w * inv3);
-    a.ax = __builtin_fmaf(dx, f, __builtin_fmaf(-qx, inv5, a.ax));
-    a.ay = __builtin_fmaf(dy, f, __builtin_fmaf(-qy, inv5, a.ay));
-    a.az = __builtin_fmaf(dz, f, __builtin_fmaf(-qz, inv5, a.az));
-    if (POT) a.ph += __builtin_fmaf(0.5f, t, minv);
+    const float dqd = __builtin_fmaf(dx, qx, __builtin_fmaf(dy, qy, __builtin_fmaf(dz, qz, -q1.w)));
+    const float inv5 = (inv2 * inv2) * inv;
+    const float g = __builtin_fmaf(2.5f * dqd, inv2, c.w * r2);
+    a.ax = __builtin_fmaf(inv5, __builtin_fmaf(g, dx, -qx), a.ax);
+    a.ay = __builtin_fmaf(inv5, __builtin_fmaf(g, dy, -qy), a.ay);
+    a.az = __builtin_fmaf(inv5, __builtin_fmaf(g, dz, -qz), a.az);
+    if (POT) a.ph += __builtin_fmaf(0.5f, dqd * inv5, c.w * inv);
 }
 
 // Phase 2 over the 64-entry chunks of the node list nl[0..k) that start at
@@ -191,7 +190,7 @@ __device__ __forceinline__ void eval_node_list(WaveLds& L, const int* nl, int k,
         L.ca[l] = A;
         if (QUAD) {
             L.cq0[l] = Q0;
-            L.cq1[l] = Q1;
+            L.cq1[l] = make_float4(Q1.x, Q1.y, Q1.z, Q1.w * eps2);  // T eps^2
         }
         __builtin_amdgcn_wave_barrier();
         A = make_float4(0.f, 0.f, 0.f, 0.f);
