@@ -60,9 +60,9 @@ FLOP_PER_INTERACTION = 20
 FLOP_PER_LOCAL_TERM = 119
 FLOP_PER_LOCAL_APPLY = 72
 # FP32 operations the kernel executes per interaction (a multiply-add counted as
-# two, the reciprocal square root as one): softened quadrupole node 52,
+# two, the reciprocal square root as one): softened quadrupole node 50,
 # particle 19 (interact_n / interact_p in csrc/gcs_traverse.hpp).
-EXEC_FLOP_NODE = 52
+EXEC_FLOP_NODE = 50
 EXEC_FLOP_PARTICLE = 19
 PEAK_HBM_GBS = 8000.0
 TREE_BYTES_PER_PARTICLE = 376  # Morton 24 + sort 24 + permute 68 + init 88 + build 24 + agg 148
@@ -75,8 +75,8 @@ TUNE_TABLE = os.path.join(REPO, "galaxy-collision-simulator_amd", "gcs_amd",
 # HBM bytes per traversal launch from rocprofv3 FETCH_SIZE (x2.000) and
 # WRITE_SIZE (x1.000) passes, scaled by a device copy of known size; valid for
 # the (workload, leaf, block, waves per SIMD, group, team, local_kappa)
-# measured: 3,606,992 KiB fetched, 584,319 KiB written per launch.
-TRAVERSE_TRAFFIC = {("batched-256", 4, 64, 4, 128, 1, 4.0): 7.986e9}
+# measured: 707,508 KiB fetched, 625,450 KiB written per launch.
+TRAVERSE_TRAFFIC = {("batched-256", 4, 256, 4, 96, 1, 4.0): 2.089e9}
 
 
 def say(*a):
