@@ -45,6 +45,12 @@ static_assert(kTile == 1 << kTileShift, "shift");
 // Aggregation subtrees: the maximal subtrees of at most kATile particles are
 // finished one per block, the nodes above them by the instance pass.
 constexpr int kATile = kTile;
+// Instance pass in two levels (for instances of at least kMidSize
+// particles): the nodes above the aggregation subtrees inside each maximal
+// subtree of fewer than kMidSize particles by one block each, then the nodes
+// of at least kMidSize particles by one block per instance.
+constexpr int kMidShift = 15;
+constexpr int kMidSize = 1 << kMidShift;
 constexpr int kStackCap = 8192;      // per-wave traversal stack entries
 constexpr int kOrderClasses = 8;     // traversal dispatch classes by group size
 constexpr int kNumXcd = 8;           // MI355X: 8 XCDs, blocks dealt round-robin

@@ -135,6 +135,11 @@ struct gcs_engine {
     int* tlist = nullptr;
     int* ntop = nullptr;
     int* tslot = nullptr;
+    // Roots of the mid-level subtrees (node, instance) and their count, for
+    // the two-level instance pass.
+    int* mids = nullptr;
+    int* nmid = nullptr;
+    int* mfail = nullptr;  // per instance: a mid-level root left to the one-level pass
     // aggregation subtrees (maximal subtrees of at most kATile particles)
     uint32_t* amark = nullptr;
     uint32_t* aq = nullptr;      // subtrees per quarter tile
@@ -227,7 +232,7 @@ void free_all(gcs_engine* e)
     dfree(e->tiles); dfree(e->stiles); dfree(e->segs); dfree(e->amark); dfree(e->aq); dfree(e->achunk); dfree(e->acnt); dfree(e->aroot); dfree(e->atiles); dfree(e->natiles); dfree(e->groups); dfree(e->hist); dfree(e->chist);
     dfree(e->gmark); dfree(e->gq); dfree(e->gorder); dfree(e->gchunk); dfree(e->gcnt); dfree(e->ngroups);
     dfree(e->front); dfree(e->nfront);
-    dfree(e->tlist); dfree(e->ntop); dfree(e->tslot);
+    dfree(e->tlist); dfree(e->ntop); dfree(e->tslot); dfree(e->mids); dfree(e->nmid); dfree(e->mfail);
     dfree(e->seg_bounds); dfree(e->bpart); dfree(e->qctr); dfree(e->counters); dfree(e->wcount); dfree(e->err);
     dfree(e->diag_part); dfree(e->diag_out); dfree(e->amax_part); dfree(e->dt_inst); dfree(e->tsim); dfree(e->slot_mem); dfree(e->staging); dfree(e->mn_dev);
     for (auto& v : e->ev)
@@ -332,6 +337,8 @@ int check_params(const gcs_params* p)
         if (p->stack_depth < 2 || p->stack_depth > 1024)
             return fail(GCS_ERR_INVALID, "stack_depth must be in [2, 1024]");
     }
+    if (p->instance_pass < 0 || p->instance_pass > 2)
+        return fail(GCS_ERR_INVALID, "instance_pass must be 0 (automatic), 1 or 2 (levels)");
     if (p->precision != GCS_PREC_PLAIN && p->precision != GCS_PREC_COMPENSATED)
         return fail(GCS_ERR_INVALID, "unknown precision mode");
     if (!std::isfinite(p->dt_eta) || p->dt_eta < 0.0f)
@@ -561,6 +568,8 @@ int enqueue_step(gcs_engine* e, const gcs_params& p, bool integrate)
     rs.nseg = nseg;
     rs.nfront = e->nfront;
     rs.ntop = e->ntop;
+    rs.nmid = e->nmid;
+    rs.mfail = e->mfail;
     rs.qctr = e->qctr;
     rs.counters = e->counters;
     rs.chist = e->chist;
@@ -580,6 +589,17 @@ int enqueue_step(gcs_engine* e, const gcs_params& p, bool integrate)
     mark(e, 2);
     // (The permute is timed with the build: one launch does both.)
     mark(e, 3);
+    // The two-level instance pass when an instance has at least kMidSize
+    // particles and the instances number at most half the CUs: one block per
+    // instance would then leave most CUs idle behind the largest instance's
+    // single-block pass (2 x 1,048,576 particles: tree stage 0.656 -> 0.557
+    // ms; 2 x 131,072: 0.218 -> 0.211 ms).  With an instance per CU the pass
+    // is spread already and the extra launch costs more (256 collisions of
+    // 4,096-65,536 particles: 1.201 -> 1.215 ms).
+    bool mid_any = false;
+    for (const SegInfo& sg : e->h_segs) mid_any = mid_any || sg.n >= kMidSize;
+    const bool two_level = p.instance_pass == 2 ||
+                           (p.instance_pass == 0 && mid_any && 2 * nseg <= e->num_cu);
     PermuteArgs pa;
     pa.perm = e->vals[0];
     pa.posm_in = e->posm[c];
@@ -592,7 +612,8 @@ int enqueue_step(gcs_engine* e, const gcs_params& p, bool integrate)
     pa.write_leaves = group ? 0 : 1;
     k_permute_build<<<qblocks, kTileThreads, 0, e->stream>>>(
         e->tiles, e->segs, pa, e->keys[0], e->node, e->nodeQ, e->parent, e->flag, group_marks(e),
-        atile_marks(e), group ? p.group_size : kGroup, TopList{e->tlist, e->ntop, e->tslot});
+        atile_marks(e), group ? p.group_size : kGroup,
+        TopList{e->tlist, e->ntop, e->tslot, two_level ? e->mids : nullptr, e->nmid});
     e->cur = c ^ 1;
     float4* posm = e->posm[e->cur];
     // Number the traversal groups and the aggregation subtrees in particle
@@ -628,10 +649,14 @@ int enqueue_step(gcs_engine* e, const gcs_params& p, bool integrate)
                                                    e->nodeQ, e->boxLo, e->boxHi, e->parent,
                                                    e->front, e->nfront, e->qctr + kNumXcd,
                                                    inv_theta, amode);
+    if (two_level)
+        k_aggregate_mid<<<e->num_cu, 1024, 0, e->stream>>>(
+            e->segs, posm, e->node, e->nodeQ, e->boxLo, e->boxHi, e->parent, e->mids, e->nmid,
+            e->tlist, e->ntop, e->tslot, e->front, e->mfail, inv_theta, amode);
     k_aggregate_top<<<nseg, 1024, 0, e->stream>>>(e->segs, posm, e->node, e->nodeQ, e->boxLo,
                                                   e->boxHi, e->parent, e->flag, e->front,
                                                   e->nfront, e->tlist, e->ntop, e->tslot,
-                                                  inv_theta, amode);
+                                                  e->mfail, inv_theta, amode, two_level ? 1 : 0);
     LAUNCHCHK();
     mark(e, 5);
     if (group) {
@@ -859,6 +884,7 @@ int64_t gcs_device_bytes(int64_t cap_n, int32_t cap_k)
          4 * (int64_t)max_gchunks(cap_n, cap_k) + 4;  // aggregation subtrees
     b += 4 * 4 * (int64_t)max_tiles(cap_n, cap_k) + 12 * (int64_t)cap_k;  // amax, dt, time
     b += 4 * n + 8 * n + 4 * (int64_t)cap_k;  // nodes above the aggregation subtrees
+    b += 8 * (n / kATile + (int64_t)cap_k + 1) + 4 + 4 * (int64_t)cap_k;  // mid-level roots
     b += 32 * n;               // staging (also the device generator's scratch)
     b += 8 * ((gcs_gen::kNR + 1) + (gcs_gen::kNZ + 1) + 2 * gcs_gen::kNR * gcs_gen::kNZ);
     return b;
@@ -924,6 +950,9 @@ int gcs_create(int64_t cap_n, int32_t cap_k, int32_t device, gcs_engine** out)
     if (!rc) rc = dalloc(&e->tlist, n);
     if (!rc) rc = dalloc(&e->ntop, (size_t)cap_k);
     if (!rc) rc = dalloc(&e->tslot, 2 * n);
+    if (!rc) rc = dalloc(&e->mids, 2 * (n / kATile + (size_t)cap_k + 1));
+    if (!rc) rc = dalloc(&e->nmid, 1);
+    if (!rc) rc = dalloc(&e->mfail, (size_t)cap_k);
     if (!rc) rc = dalloc(&e->amark, n);
     if (!rc) rc = dalloc(&e->aq, 4 * nt);
     if (!rc) rc = dalloc(&e->achunk, max_gchunks(cap_n, cap_k));
@@ -1925,7 +1954,8 @@ int gcs_karras_tree(const uint64_t* sorted_keys, int32_t n, int32_t* left, int32
         k_build<<<qblocks, kTileThreads, 0, e->stream>>>(e->tiles, e->segs, e->keys[0], e->node,
                                                          e->parent, e->flag, group_marks(e),
                                                          atile_marks(e), kGroup,
-                                                         TopList{nullptr, nullptr, nullptr});
+                                                         TopList{nullptr, nullptr, nullptr,
+                                                                 nullptr, nullptr});
         hipError_t le = hipGetLastError();
         if (le != hipSuccess) rc = fail(GCS_ERR_HIP, hipGetErrorString(le));
     }

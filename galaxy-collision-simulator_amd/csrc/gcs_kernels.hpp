@@ -119,6 +119,8 @@ struct StepReset {
     int nseg;
     int* nfront;
     int* ntop;
+    int* nmid;
+    int* mfail;
     unsigned* qctr;
     unsigned long long* counters;
     uint32_t* chist;
@@ -142,6 +144,7 @@ __global__ __launch_bounds__(256) void k_morton(const TileInfo* __restrict__ til
         if (tid < rs.nseg) {
             rs.nfront[tid] = 0;
             rs.ntop[tid] = 0;
+            if (rs.mfail) rs.mfail[tid] = 0;
             // Given bounds are recorded as the ones used (read back per
             // instance); computed ones were written by the bounds kernels.
             if (use_given) {
@@ -150,6 +153,7 @@ __global__ __launch_bounds__(256) void k_morton(const TileInfo* __restrict__ til
             }
         }
         if (tid <= kNumXcd) rs.qctr[tid] = 0u;
+        if (tid == 0 && rs.nmid) *rs.nmid = 0;
         if (tid < 3) rs.counters[tid] = 0ull;
         for (int64_t k = tid; k < rs.chist_words; k += stride) rs.chist[k] = 0u;
         if (threadIdx.x == 0) {
@@ -285,11 +289,15 @@ __device__ __forceinline__ void mark_subtree(const SegInfo& sg, int x, int count
 // The nodes of more than kATile particles of each instance (the ones above
 // the aggregation subtrees), listed from the build in any order: list[start +
 // k] = node, slot[node] = k, count[instance] = number listed.  list == nullptr:
-// not listed.
+// not listed.  mids (unless null): the roots of the maximal subtrees of more
+// than kATile and fewer than kMidSize particles under a node of at least
+// kMidSize, as (node, instance) pairs, *nmid of them, in any order.
 struct TopList {
     int* list;
     int* count;
     int* slot;
+    int* mids;
+    int* nmid;
 };
 
 __device__ __forceinline__ void build_one(const QuarterTile& q, const SegInfo* __restrict__ segs,
@@ -344,6 +352,18 @@ __device__ __forceinline__ void build_one(const QuarterTile& q, const SegInfo* _
             const int k = atomicAdd(&tl.count[q.seg], 1);
             tl.list[sg.start + k] = me;
             tl.slot[me] = k;
+            if (tl.mids && hi - lo + 1 >= kMidSize) {
+                if (nl > kATile && nl < kMidSize) {
+                    const int j = atomicAdd(tl.nmid, 1);
+                    tl.mids[2 * j] = cl;
+                    tl.mids[2 * j + 1] = q.seg;
+                }
+                if (nr > kATile && nr < kMidSize) {
+                    const int j = atomicAdd(tl.nmid, 1);
+                    tl.mids[2 * j] = cr;
+                    tl.mids[2 * j + 1] = q.seg;
+                }
+            }
         }
     }
 }
@@ -1079,8 +1099,9 @@ __device__ __forceinline__ void top_lds(TopLds& T, const SegInfo& sg, int nt, co
 #pragma unroll
     for (int r = 0; r < kPer; ++r) {
         int sl = tid + r * (int)blockDim.x;
-        bool climb = start[r];
-        while (climb) {
+        // A climb passes at most nt nodes (a bound that also ends it should
+        // the listed links ever be inconsistent).
+        for (int steps = start[r] ? nt : 0; steps > 0; --steps) {
             const int ps = (int)T.par[sl] - 1;
             if (ps < 0) break;
             const int lc = (int)T.cl[ps] - 1, rc = (int)T.cr[ps] - 1;
@@ -1131,6 +1152,10 @@ __device__ __forceinline__ void top_lds(TopLds& T, const SegInfo& sg, int nt, co
 // runs to the end.  A thread thus never holds back climbers that have not
 // started behind one long climb.  The lists live in the instance's 2n-entry
 // frontier area: the current one in the first half, the next in the second.
+// Size class of kMidSize particles: the classes below it are the first
+// level of the two-level instance pass, the rest the second.
+constexpr int kMidBucket = 8 * (kMidShift - kATileShift);
+static_assert(kMidShift > kATileShift, "mid-level subtrees hold aggregation subtrees");
 constexpr int kTopSmemWords =
     (int)(sizeof(TopLds) > 4 * (kTopLdsNodes / 32) ? sizeof(TopLds) / 4 : kTopLdsNodes / 32);
 static_assert(sizeof(TopLds) % 16 == 0, "LDS words");
@@ -1140,7 +1165,8 @@ __global__ __launch_bounds__(1024) void k_aggregate_top(
     const SegInfo* __restrict__ segs, const float4* __restrict__ posm, float4* node,
     float4* nodeQ, float4* boxLo, float4* boxHi, const int* __restrict__ parent, int* flag,
     int* front, const int* __restrict__ nfront, const int* __restrict__ tlist,
-    const int* __restrict__ ntop, int* tslot, float inv_theta, int mode)
+    const int* __restrict__ ntop, int* tslot, const int* __restrict__ mfail, float inv_theta,
+    int mode, int two_level)
 {
     __shared__ __align__(16) uint32_t smem[kTopSmemWords];
     __shared__ int lcount;
@@ -1152,6 +1178,45 @@ __global__ __launch_bounds__(1024) void k_aggregate_top(
     const int s = blockIdx.x;
     const int tid = (int)threadIdx.x;
     const SegInfo sg = segs[s];
+    const bool mid_done = two_level && tlist && sg.n >= kMidSize && !mfail[s];
+    if (mid_done) {
+        // Second level of the two-level pass: the nodes of at least kMidSize
+        // particles (their smaller children were finished by k_aggregate_mid
+        // or k_aggregate), listed in the second half of the instance's
+        // frontier area.
+        for (int c = tid; c < kTopBuckets; c += blockDim.x) bphase[c] = c < kMidBucket ? 0 : 1;
+        if (tid == 0) lcount = 0;
+        __syncthreads();
+        int* const plist = front + 2 * (size_t)sg.start + sg.n;
+        const int nt = ntop[s];
+        for (int k = tid; k < nt; k += blockDim.x) {
+            const int v = tlist[sg.start + k];
+            const float4 bb = node[2 * v + 1];
+            if (__float_as_int(bb.z) - __float_as_int(bb.y) + 1 >= kMidSize) {
+                const int slot = atomicAdd(&lcount, 1);
+                if (slot < kTopCap) {
+                    plist[slot] = v;
+                    tslot[v] = slot;
+                }
+            }
+        }
+        __syncthreads();  // the list and its slots are visible to the block
+        const int nb = lcount;
+        if (nb <= kTopCap) {
+            top_lds(T, sg, nb, plist, tslot, posm, node, nodeQ, boxLo, boxHi, parent, inv_theta,
+                    mode, bphase, 1);
+            return;
+        }
+        // More such nodes than one LDS pass holds: the one-level paths below
+        // finish every listed node (those of the first level again, alike).
+        __syncthreads();
+    }
+    // With the two-level pass, k_aggregate_mid rewrote the slots of the
+    // nodes below kMidSize particles of the instances of at least kMidSize
+    // (smaller instances have no mid-level roots).  Such an instance gets
+    // here only when it listed more than kTopCap nodes (a given-up root alone
+    // holds more), so the LDS form below, which reads the build's slots, is
+    // not taken; the phases relist their own.
     const int nt = tlist ? ntop[s] : kTopCap + 1;
     if (nt <= kTopCap) {
         if (nt > 0)
@@ -1241,6 +1306,59 @@ __global__ __launch_bounds__(1024) void k_aggregate_top(
         for (int k = threadIdx.x; k < (1 << kTopHashBits); k += blockDim.x) lmem[k] = 0xFFFFFFFFu;
     climb_rounds(sg, front + 2 * (size_t)sg.start, sg.n, nf, posm, node, nodeQ, boxLo, boxHi,
                  parent, flag, lmem, marks, inv_theta, mode, &lcount);
+}
+
+// k_aggregate_mid: the first level of the two-level instance pass.  Block b
+// takes the mid-level roots b, b + gridDim.x, ... that the build recorded.
+// For root r over particles [rf, rl], the instance's listed nodes whose
+// particle range lies inside r's (the nodes of more than kATile particles of
+// r's subtree, fewer than rl - rf + 1) are listed with their slots at
+// rf - start of the second half of the instance's frontier area (the lists of
+// disjoint roots do not overlap; the first half keeps the frontier), and
+// top_lds finishes them as the phase of the size classes below kMidSize: r's
+// parent, of at least kMidSize particles, is of the other phase, so the
+// climbs stop at r.  A root with more than kTopCap such nodes is left to
+// k_aggregate_top's one-level paths: mfail[instance] = 1.
+__global__ __launch_bounds__(1024) void k_aggregate_mid(
+    const SegInfo* __restrict__ segs, const float4* __restrict__ posm, float4* node,
+    float4* nodeQ, float4* boxLo, float4* boxHi, const int* __restrict__ parent,
+    const int* __restrict__ mids, const int* __restrict__ nmid, const int* __restrict__ tlist,
+    const int* __restrict__ ntop, int* tslot, int* front, int* mfail, float inv_theta, int mode)
+{
+    __shared__ __align__(16) uint32_t smem[sizeof(TopLds) / 4];
+    __shared__ uint8_t phase_of[kTopBuckets];
+    __shared__ int lcount;
+    TopLds& T = *reinterpret_cast<TopLds*>(smem);
+    const int tid = (int)threadIdx.x;
+    for (int c = tid; c < kTopBuckets; c += blockDim.x) phase_of[c] = c < kMidBucket ? 0 : 1;
+    const int nm = *nmid;
+    for (int j = blockIdx.x; j < nm; j += gridDim.x) {
+        const int r = mids[2 * j], s = mids[2 * j + 1];
+        const SegInfo sg = segs[s];
+        const float4 rb = node[2 * r + 1];
+        const int rf = __float_as_int(rb.y), rl = __float_as_int(rb.z);
+        int* const mlist = front + 2 * (size_t)sg.start + sg.n + (rf - sg.start);
+        if (tid == 0) lcount = 0;
+        __syncthreads();  // the previous root's use of the LDS is over
+        const int nt = ntop[s];
+        for (int k = tid; k < nt; k += blockDim.x) {
+            const int v = tlist[sg.start + k];
+            const float4 bb = node[2 * v + 1];
+            if (__float_as_int(bb.y) >= rf && __float_as_int(bb.z) <= rl) {
+                const int slot = atomicAdd(&lcount, 1);
+                mlist[slot] = v;
+                tslot[v] = slot;
+            }
+        }
+        __syncthreads();  // the list and its slots are visible to the block
+        const int nl = lcount;
+        if (nl <= kTopCap)
+            top_lds(T, sg, nl, mlist, tslot, posm, node, nodeQ, boxLo, boxHi, parent, inv_theta,
+                    mode, phase_of, 0);
+        else if (tid == 0)
+            mfail[s] = 1;
+        __syncthreads();
+    }
 }
 
 __device__ __forceinline__ uint8_t to_u8(float f) { return (uint8_t)f2u_sat(f); }

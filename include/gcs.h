@@ -115,6 +115,14 @@ typedef struct gcs_params {
                            its monopole's first and second derivatives enter a
                            second-order local expansion that every target of the
                            group evaluates once (a cell-cell far field) */
+    int32_t instance_pass; /* aggregation of the nodes above the subtrees of at most
+                           1,024 particles: 1 = one block per instance; 2 = for
+                           instances of at least 32,768 particles in two levels,
+                           the maximal subtrees of fewer than 32,768 particles one
+                           block each, then the rest one block per instance; 0 =
+                           automatic: two levels when an instance has at least
+                           32,768 particles and the instances number at most half
+                           the CUs.  The node records are the same either way */
 } gcs_params;
 
 /* Defaults: G=1, eps2=0.0025, dt=0.01, theta=0.5, GCS_MAC_GROUP,

@@ -238,13 +238,16 @@ def top_classes(cpu, atile=1024):
     return len(size), np.bincount(cls)
 
 
-@pytest.mark.parametrize("case", ["phases", "global"])
-def test_instance_pass_large_instances(gcs, orc, case):
+@pytest.mark.parametrize("case,levels", [("phases", 1), ("global", 1), ("phases", 2),
+                                         ("global", 2)])
+def test_instance_pass_large_instances(gcs, orc, case, levels):
     """The instance pass on instances with more than 2048 nodes above the
-    aggregation subtrees: in LDS phases by size class (two collided galaxies
-    of 1,048,576), and by climbing with arrival marks when one size class
-    alone holds more than 2048 nodes (4.4M particles uniform in a cube).
-    Pipeline bit-exact against the oracle, aggregates included."""
+    aggregation subtrees.  In one level: in LDS phases by size class (two
+    collided galaxies of 1,048,576), and by climbing with arrival marks when
+    one size class alone holds more than 2048 nodes (4.4M particles uniform
+    in a cube).  In two levels: the subtrees of fewer than 32,768 particles one
+    block each, then the rest.  Pipeline bit-exact against the oracle,
+    aggregates included."""
     if case == "phases":
         stars = gcs.collision(1048576, 1007)
     else:
@@ -252,10 +255,11 @@ def test_instance_pass_large_instances(gcs, orc, case):
         for k in ("x", "y", "z"):
             stars[k] = np.random.default_rng({"x": 1, "y": 2, "z": 3}[k]).uniform(
                 -1, 1, len(stars)).astype(np.float32)
-    p = gcs.params(G=G, eps2=0.0025, dt=0.01, theta=0.5, leaf_size=8, group_size=128)
+    p = gcs.params(G=G, eps2=0.0025, dt=0.01, theta=0.5, leaf_size=8, group_size=128,
+                   instance_pass=levels)
     with gcs.Engine(len(stars)) as e:
         e.load(stars)
-        say("instance pass, %s: n=%d" % (case, len(stars)))
+        say("instance pass, %s, %d level(s): n=%d" % (case, levels, len(stars)))
         e.compute_accels(p)
         cpu = check_pipeline(gcs, orc, e, stars, p)
     nt, hist = top_classes(cpu)
@@ -358,23 +362,70 @@ def test_pipeline_over_many_steps(gcs, orc, per, eps, dt, seed, checks):
                 assert err < 1e-3
 
 
-@pytest.mark.parametrize("half", [450000, 600000])
-def test_large_instance_aggregates(gcs, orc, half):
+@pytest.mark.parametrize("half,levels", [(450000, 1), (600000, 1), (600000, 2)])
+def test_large_instance_aggregates(gcs, orc, half, levels):
     """Large instances: 2 x 450,000 particles has between 1024 and 2048 nodes
     above the aggregation subtrees (the LDS top pass with two list slots per
     thread); 2 x 600,000 has more than 2048 (the global top pass, arrival
-    marks in an LDS hash since the instance exceeds 2^20 particles).  Topology
-    and aggregates bit-exact against the oracle."""
+    marks in an LDS hash since the instance exceeds 2^20 particles) -- in one
+    level, or in two.  Topology and aggregates bit-exact against the oracle."""
     stars = gcs.collision(half, 1006)
     n = len(stars)
     eps2 = float(np.float32(0.02) ** 2)
-    p = gcs.params(G=G, eps2=eps2, dt=0.005, theta=0.5, leaf_size=8)
+    p = gcs.params(G=G, eps2=eps2, dt=0.005, theta=0.5, leaf_size=8, instance_pass=levels)
     with gcs.Engine(n) as e:
         e.load(stars)
         say("large instance compute n=%d" % n)
         e.compute_accels(p)
         check_pipeline(gcs, orc, e, stars, p)
         e.step(p, 2)
+
+
+def mid_level_nodes(cpu, atile=1024, mid=32768):
+    """Per mid-level subtree (a subtree of more than `atile` and fewer than
+    `mid` particles whose parent has at least `mid`), the number of its nodes
+    of more than `atile` particles, from the oracle's tree."""
+    first, last = cpu["first"], cpu["last"]
+    size = last - first + 1
+    parent = np.full(2 * len(first) + 1, -1)
+    parent[cpu["left"]] = np.arange(len(first))
+    parent[cpu["right"]] = np.arange(len(first))
+    top = np.where(size > atile)[0]
+    roots = [v for v in top if size[v] < mid and parent[v] >= 0 and size[parent[v]] >= mid]
+    return [int(np.sum((first[top] >= first[r]) & (last[top] <= last[r]))) for r in roots]
+
+
+@pytest.mark.parametrize("levels", [1, 2])
+def test_forest_mid_level_subtrees(gcs, orc, levels):
+    """Three collisions of 38,233-58,347 particles in one forest, whose
+    mid-level subtrees hold up to 78 nodes of more than 1024 particles: each
+    instance's keys, topology and aggregates bit-exact against the oracle, in
+    one level and in two."""
+    parts = [gcs.generate(gcs.random_spec(1004, k, 4096, 65536)) for k in (8, 24, 132)]
+    counts = [len(a) for a in parts]
+    p = gcs.params(G=G, eps2=0.0025, dt=0.01, theta=0.5, leaf_size=4, group_size=96,
+                   instance_pass=levels)
+    with gcs.Engine(sum(counts), len(parts)) as e:
+        e.load(np.concatenate(parts), counts)
+        say("forest of %s particles, instance pass in %d level(s)" % (counts, levels))
+        e.compute_accels(p)
+        raw, srt, _ = e.keys()
+        trees = [e.tree(i) for i in range(len(parts))]
+    most = 0
+    off = 0
+    for i, a in enumerate(parts):
+        sl = slice(off, off + len(a))
+        off += len(a)
+        cpu = orc.build_tree_pipeline(a, 0.5)
+        assert np.array_equal(raw[sl], cpu["keys"]) and np.array_equal(srt[sl], cpu["skeys"])
+        for k in ("left", "right", "parent"):
+            assert np.array_equal(trees[i][k], cpu[k]), (i, k)
+        for k in ("com4", "width", "bmin", "bmax"):
+            assert np.array_equal(trees[i][k].view(np.uint32), cpu["agg"][k].view(np.uint32)), \
+                (i, k)
+        most = max([most] + mid_level_nodes(cpu))
+    say("most nodes of more than 1024 particles in a mid-level subtree: %d" % most)
+    assert most > 64
 
 
 # ---- forests, graphs, composition ----------------------------------------------------
@@ -411,6 +462,53 @@ def test_forest_equals_standalone(gcs, orc, team, kappa):
         for k in ("left", "right", "parent", "com4"):
             assert np.array_equal(trees[i][k], t1[k]), (i, k)
         off += len(a)
+
+
+@pytest.mark.parametrize("team,gsize", [(1, 96), (2, 64)])
+def test_ragged_forest_matches_oracle(gcs, orc, team, gsize):
+    """Instances at the size edges -- one particle, two or three (the root's
+    children leaves), a group's size and one more, an aggregation subtree's
+    size and one either side -- walked and evaluated in one forest: each
+    instance's keys and tree equal the oracle's, the forest's interaction
+    counts equal the sum of the oracle's per instance, and every acceleration
+    equals the oracle's FP64 sum of the same lists within 3e-5 of its terms."""
+    sizes = [1, 2, 3, 5, gsize, gsize + 1, 1023, 1024, 1025, 3000]
+    parts = [cloud(gcs, n, 50 + n) for n in sizes]
+    eps2 = 0.0025
+    p = gcs.params(G=G, eps2=eps2, dt=0.01, theta=0.5, leaf_size=4, group_size=gsize,
+                   team=team, local_kappa=4.0, block_size=64 * team if team > 1 else 256)
+    counts = [len(a) for a in parts]
+    with gcs.Engine(sum(counts), len(parts)) as e:
+        e.load(np.concatenate(parts), counts)
+        say("ragged forest, team %d, groups of %d" % (team, gsize))
+        e.compute_accels(p)
+        acc = e.accels()
+        raw, srt, perm = e.keys()
+        got_counts = e.interaction_counts()
+        trees = [e.tree(i) for i in range(len(parts))]
+    want = np.zeros(5, np.int64)
+    off = 0
+    for i, a in enumerate(parts):
+        n = len(a)
+        sl = slice(off, off + n)
+        off += n
+        if n == 1:
+            assert np.all(acc[sl, :3] == 0.0)
+            continue
+        cpu = orc.build_tree_pipeline(a, 0.5)
+        assert np.array_equal(raw[sl], cpu["keys"]) and np.array_equal(srt[sl], cpu["skeys"])
+        for k in ("left", "right", "parent"):
+            assert np.array_equal(trees[i][k], cpu[k]), (n, k)
+        s = cpu["sorted"]
+        grp = orc.tree_groups(cpu["left"], cpu["right"], cpu["first"], cpu["last"], gsize)
+        ref, _, cnt, aabs = orc.traverse_group(cpu["left"], cpu["right"], cpu["first"],
+                                               cpu["last"], s["x"], s["y"], s["z"], s["mass"],
+                                               cpu["agg"], G, eps2, gsize, 4, want_abs=True,
+                                               groups=grp, local_kappa=4.0)
+        want += cnt
+        dev = np.abs(acc[sl, :3] - ref[:, :3]).max(1) / np.maximum(aabs, 1e-30)
+        assert dev.max() < 3e-5, (n, dev.max())
+    assert got_counts == [int(c) for c in want]
 
 
 def test_graph_replays_match_eager(gcs):
