@@ -74,10 +74,14 @@ struct DevBuf {
 struct Instance {
     int32_t id;
     int32_t n;
+    // Its particles are stored in the order of the last step's sort, so its
+    // next keys are nearly sorted (the bucket sort applies).
+    bool ordered = false;
 };
 
 size_t max_tiles(int64_t cap_n, int32_t cap_k) { return (size_t)(cap_n / kTile) + (size_t)cap_k + 1; }
 size_t hist_entries(int64_t cap_n, int32_t cap_k) { return (size_t)kRadix * max_tiles(cap_n, cap_k); }
+size_t max_buckets(int64_t cap_n, int32_t cap_k) { return (size_t)(cap_n / kBucketKeys) + (size_t)cap_k + 1; }
 // Sort-histogram chunks: at most one partial chunk per instance.
 size_t max_chunks(int64_t cap_n, int32_t cap_k)
 {
@@ -140,6 +144,17 @@ struct gcs_engine {
     int* mids = nullptr;
     int* nmid = nullptr;
     int* mfail = nullptr;  // per instance: a mid-level root left to the one-level pass
+    // Bucket sort: buckets [bkt0[s], bkt0[s+1]) of instance s, their
+    // splitters, counts, offsets and fill counters, and each key's bucket.
+    int* bkt0 = nullptr;
+    uint64_t* bsplit = nullptr;
+    uint32_t* bcnt = nullptr;
+    uint32_t* boff = nullptr;
+    uint32_t* bfill = nullptr;
+    uint32_t* bid = nullptr;
+    std::vector<int> h_bkt0;
+    // The instances the captured step graphs sort with buckets.
+    int graph_bucket_segs = -1;
     // aggregation subtrees (maximal subtrees of at most kATile particles)
     uint32_t* amark = nullptr;
     uint32_t* aq = nullptr;      // subtrees per quarter tile
@@ -233,6 +248,7 @@ void free_all(gcs_engine* e)
     dfree(e->gmark); dfree(e->gq); dfree(e->gorder); dfree(e->gchunk); dfree(e->gcnt); dfree(e->ngroups);
     dfree(e->front); dfree(e->nfront);
     dfree(e->tlist); dfree(e->ntop); dfree(e->tslot); dfree(e->mids); dfree(e->nmid); dfree(e->mfail);
+    dfree(e->bkt0); dfree(e->bsplit); dfree(e->bcnt); dfree(e->boff); dfree(e->bfill); dfree(e->bid);
     dfree(e->seg_bounds); dfree(e->bpart); dfree(e->qctr); dfree(e->counters); dfree(e->wcount); dfree(e->err);
     dfree(e->diag_part); dfree(e->diag_out); dfree(e->amax_part); dfree(e->dt_inst); dfree(e->tsim); dfree(e->slot_mem); dfree(e->staging); dfree(e->mn_dev);
     for (auto& v : e->ev)
@@ -287,7 +303,12 @@ int rebuild_tables(gcs_engine* e)
     }
     e->n_total = start;
     e->nchunks = chunks;
+    e->h_bkt0.assign(1, 0);
+    for (const SegInfo& sg : e->h_segs)
+        e->h_bkt0.push_back(e->h_bkt0.back() + std::max(1, (sg.n + kBucketKeys - 1) / kBucketKeys));
     if (!e->h_segs.empty()) {
+        HIPCHK(hipMemcpy(e->bkt0, e->h_bkt0.data(), sizeof(int) * e->h_bkt0.size(),
+                         hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(e->segs, e->h_segs.data(), sizeof(SegInfo) * e->h_segs.size(),
                          hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(e->tiles, e->h_tiles.data(), sizeof(TileInfo) * e->h_tiles.size(),
@@ -339,6 +360,8 @@ int check_params(const gcs_params* p)
     }
     if (p->instance_pass < 0 || p->instance_pass > 2)
         return fail(GCS_ERR_INVALID, "instance_pass must be 0 (automatic), 1 or 2 (levels)");
+    if (p->sort < 0 || p->sort > 2)
+        return fail(GCS_ERR_INVALID, "sort must be 0 (automatic), 1 (radix) or 2 (buckets)");
     if (p->precision != GCS_PREC_PLAIN && p->precision != GCS_PREC_COMPENSATED)
         return fail(GCS_ERR_INVALID, "unknown precision mode");
     if (!std::isfinite(p->dt_eta) || p->dt_eta < 0.0f)
@@ -459,6 +482,40 @@ void enqueue_sort_range(gcs_engine* e, int seg0, int nseg)
 }
 
 void enqueue_sort(gcs_engine* e) { enqueue_sort_range(e, 0, (int)e->h_segs.size()); }
+
+// The bucket sort of instances 0 .. nseg - 1 over keys[0]/vals[0] (sorted
+// result back in buffer 0, buffer 1 the buckets); the other instances' keys
+// are left alone.
+void enqueue_bucket_sort(gcs_engine* e, int nseg)
+{
+    if (nseg <= 0) return;
+    const int ntiles = nseg < (int)e->h_segs.size() ? e->h_segs[(size_t)nseg].tile0
+                                                    : (int)e->h_tiles.size();
+    const int nbkt = e->h_bkt0[(size_t)nseg];
+    k_bucket_split<<<nseg, 1024, 0, e->stream>>>(e->segs, e->bkt0, e->keys[0], e->bsplit, e->bcnt,
+                                                 e->bfill);
+    k_bucket_count<<<ntiles, kTileThreads, 0, e->stream>>>(e->tiles, e->segs, e->bkt0, e->keys[0],
+                                                           e->bsplit, e->bid, e->bcnt);
+    k_bucket_scan<<<nseg, 1024, 0, e->stream>>>(e->segs, e->bkt0, e->bcnt, e->boff);
+    k_bucket_scatter<<<ntiles, kTileThreads, 0, e->stream>>>(e->tiles, e->segs, e->bkt0,
+                                                             e->keys[0], e->bid, e->boff, e->bfill,
+                                                             e->keys[1], e->vals[1]);
+    k_bucket_sort<<<nbkt, kBucketThreads, 0, e->stream>>>(e->bcnt, e->boff, e->keys[1],
+                                                          e->vals[1], e->keys[0], e->vals[0]);
+}
+
+// Instances 0 .. k - 1 get the bucket sort, the rest the radix sort: with
+// sort 0 the leading instances whose particles are in the last step's sorted
+// order (instances joined since then come after them).
+int bucket_segs(const gcs_engine* e, const gcs_params& p)
+{
+    const int nseg = (int)e->h_segs.size();
+    if (p.sort == 1) return 0;
+    if (p.sort == 2) return nseg;
+    int k = 0;
+    while (k < nseg && e->inst[(size_t)k].ordered) ++k;
+    return k;
+}
 
 // The disc sampler table on the device (uploaded on first use).
 int mn_table_dev(gcs_engine* e, gcs_gen::MnTableView* mt)
@@ -584,7 +641,9 @@ int enqueue_step(gcs_engine* e, const gcs_params& p, bool integrate)
         HIPCHK(hipMemcpyAsync(e->keys_raw, e->keys[0], sizeof(uint64_t) * e->n_total,
                               hipMemcpyDeviceToDevice, e->stream));
     mark(e, 1);
-    enqueue_sort(e);
+    const int nbs = bucket_segs(e, p);
+    enqueue_bucket_sort(e, nbs);
+    enqueue_sort_range(e, nbs, nseg - nbs);
     LAUNCHCHK();
     mark(e, 2);
     // (The permute is timed with the build: one launch does both.)
@@ -714,6 +773,8 @@ int enqueue_step(gcs_engine* e, const gcs_params& p, bool integrate)
     mark(e, 7);
     e->have_tree = true;
     e->last_mac = p.mac;
+    if (!e->capturing)
+        for (Instance& in : e->inst) in.ordered = true;  // the permute stored the sorted order
     return GCS_OK;
 }
 
@@ -752,6 +813,7 @@ int capture_graphs(gcs_engine* e, const gcs_params& p)
     e->gparams = p;
     e->graph_valid = true;
     e->graph_wslots = e->wslots;
+    e->graph_bucket_segs = bucket_segs(e, p);
     return GCS_OK;
 }
 
@@ -885,6 +947,7 @@ int64_t gcs_device_bytes(int64_t cap_n, int32_t cap_k)
     b += 4 * 4 * (int64_t)max_tiles(cap_n, cap_k) + 12 * (int64_t)cap_k;  // amax, dt, time
     b += 4 * n + 8 * n + 4 * (int64_t)cap_k;  // nodes above the aggregation subtrees
     b += 8 * (n / kATile + (int64_t)cap_k + 1) + 4 + 4 * (int64_t)cap_k;  // mid-level roots
+    b += 4 * ((int64_t)cap_k + 1) + 20 * (int64_t)max_buckets(cap_n, cap_k) + 4 * n;  // buckets
     b += 32 * n;               // staging (also the device generator's scratch)
     b += 8 * ((gcs_gen::kNR + 1) + (gcs_gen::kNZ + 1) + 2 * gcs_gen::kNR * gcs_gen::kNZ);
     return b;
@@ -953,6 +1016,12 @@ int gcs_create(int64_t cap_n, int32_t cap_k, int32_t device, gcs_engine** out)
     if (!rc) rc = dalloc(&e->mids, 2 * (n / kATile + (size_t)cap_k + 1));
     if (!rc) rc = dalloc(&e->nmid, 1);
     if (!rc) rc = dalloc(&e->mfail, (size_t)cap_k);
+    if (!rc) rc = dalloc(&e->bkt0, (size_t)cap_k + 1);
+    if (!rc) rc = dalloc(&e->bsplit, max_buckets(cap_n, cap_k));
+    if (!rc) rc = dalloc(&e->bcnt, max_buckets(cap_n, cap_k));
+    if (!rc) rc = dalloc(&e->boff, max_buckets(cap_n, cap_k));
+    if (!rc) rc = dalloc(&e->bfill, max_buckets(cap_n, cap_k));
+    if (!rc) rc = dalloc(&e->bid, n);
     if (!rc) rc = dalloc(&e->amark, n);
     if (!rc) rc = dalloc(&e->aq, 4 * nt);
     if (!rc) rc = dalloc(&e->achunk, max_gchunks(cap_n, cap_k));
@@ -1419,7 +1488,17 @@ int gcs_step(gcs_engine* e, const gcs_params* p, int32_t nsteps)
     HIPCHK(hipSetDevice(e->device));
     e->capture_keys = false;
     if (p->use_graph) {
-        if (!e->graph_valid || std::memcmp(&e->gparams, p, sizeof(gcs_params)) != 0) {
+        if (p->sort == 0 && bucket_segs(e, *p) < (int)e->h_segs.size()) {
+            // Instances not in a sorted order (loaded or joined since the
+            // last step): one step runs eagerly, its radix sort orders them,
+            // and the graphs then sort every instance with buckets.
+            e->timed = false;
+            rc = enqueue_step(e, *p, true);
+            if (rc) return rc;
+            if (--nsteps == 0) return GCS_OK;
+        }
+        if (!e->graph_valid || std::memcmp(&e->gparams, p, sizeof(gcs_params)) != 0 ||
+            e->graph_bucket_segs != bucket_segs(e, *p)) {
             e->timed = false;
             rc = capture_graphs(e, *p);
             if (rc) return rc;

@@ -214,4 +214,528 @@ __global__ __launch_bounds__(THREADS) void k_sort_scatter(
     }
 }
 
+// ---- Bucket sort for keys in nearly sorted order --------------------------
+//
+// After a step the particles are stored in that step's sorted order, and at
+// the next step most keys are within a few hundred places of their sorted
+// rank (2 x 1,048,576 particles, dt 0.005: the median moves 88 places, 0.1 %
+// move across the key space).  Splitters taken from the keys at every
+// kBucketKeys-th place of the current order then cut each instance into
+// buckets of about kBucketKeys keys (at most 3,218 there over 30 steps), and
+// the sort is
+//   k_bucket_split    per instance: splitters from the keys around every
+//                     kBucketKeys-th place (medians, made non-decreasing);
+//                     bucket counters cleared
+//   k_bucket_count    per tile: each key's bucket (a search among the
+//                     tile's window of kBucketWin splitters in LDS, or among
+//                     all of the instance's for a key outside it), counted
+//   k_bucket_scan     per instance: bucket offsets
+//   k_bucket_scatter  per tile: keys and their indices moved into their
+//                     buckets, in any order within a bucket
+//   k_bucket_sort     per bucket: sorted by (key, index) in LDS (a sorting
+//                     network of eight per thread, then merge-path rounds).
+// (key, index) is a total order, so every bucket's order -- and the result
+// -- is the stable sort's, whatever order the scatter left.  A bucket of more
+// than kBucketCap keys (splitters that fit the keys badly) is sorted in
+// kBucketCap-key LDS chunks merged in global memory: slow, still exact.
+// Bucket b of an instance holds the keys k with split[b] <= k < split[b+1]
+// (split[0] = 0, split[nb] = +inf): its index is the number of splitters
+// 1 .. nb-1 at most k.
+constexpr int kBucketKeys = 3072;  // splitter spacing
+constexpr int kBucketCap = 4096;   // keys a block sorts in LDS
+constexpr int kBucketWin = 64;     // splitters per tile in LDS
+constexpr int kBucketThreads = 512;
+constexpr int kBucketPer = kBucketCap / kBucketThreads;
+static_assert(kBucketPer == 8, "the register network sorts eight");
+
+__host__ __device__ __forceinline__ uint64_t min_u64(uint64_t a, uint64_t b) { return a < b ? a : b; }
+__host__ __device__ __forceinline__ uint64_t max_u64(uint64_t a, uint64_t b) { return a < b ? b : a; }
+
+__host__ __device__ __forceinline__ void swap_u64(uint64_t& a, uint64_t& b)
+{
+    const uint64_t t = a;
+    a = b;
+    b = t;
+}
+
+// Median of nine: odd-even transposition sort (nine rounds of fixed
+// compare-exchanges, unrolled into registers).
+__host__ __device__ __forceinline__ uint64_t median9(uint64_t* a)
+{
+#pragma unroll
+    for (int r = 0; r < 9; ++r)
+#pragma unroll
+        for (int j = r & 1; j + 1 < 9; j += 2) {
+            const uint64_t lo = min_u64(a[j], a[j + 1]), hi = max_u64(a[j], a[j + 1]);
+            a[j] = lo;
+            a[j + 1] = hi;
+        }
+    return a[4];
+}
+
+// Splitter candidate b of an instance's keys k[0, n): the median of the
+// nine keys at places b * kBucketKeys + 64 m, m = -4 .. 4 (clamped); 0 for b
+// = 0 and +inf for b = nb.  Keys that crossed a coarse octree boundary
+// together move as runs of neighbours in the current order; nine points
+// 256 places either side keep the median off such a run.
+__host__ __device__ __forceinline__ uint64_t split_candidate(const uint64_t* k, int n, int b,
+                                                             int nb)
+{
+    if (b <= 0) return 0ull;
+    if (b >= nb) return ~0ull;
+    uint64_t a[9];
+#pragma unroll
+    for (int m = 0; m < 9; ++m) {
+        const int q = b * kBucketKeys + 64 * (m - 4);
+        a[m] = k[q < 0 ? 0 : (q >= n ? n - 1 : q)];
+    }
+    return median9(a);
+}
+
+// Splitter b: the median of candidates b - 1, b, b + 1 (one bad candidate
+// never reaches the running maximum), made non-decreasing by a running
+// maximum over the instance.
+__global__ __launch_bounds__(1024) void k_bucket_split(const SegInfo* __restrict__ segs,
+                                                       const int* __restrict__ bkt0,
+                                                       const uint64_t* __restrict__ keys,
+                                                       uint64_t* __restrict__ split,
+                                                       uint32_t* __restrict__ bcnt,
+                                                       uint32_t* __restrict__ bfill)
+{
+    __shared__ uint64_t sc[1024];
+    __shared__ uint64_t cd[1024 + 2];  // candidates c0 - 1 .. c0 + 1024
+    const int s = blockIdx.x;
+    const int tid = (int)threadIdx.x;
+    const SegInfo sg = segs[s];
+    const int b0 = bkt0[s], nb = bkt0[s + 1] - b0;
+    const uint64_t* k = keys + sg.start;
+    uint64_t carry = 0ull;
+    for (int c0 = 0; c0 < nb; c0 += 1024) {
+        const int b = c0 + tid;
+        cd[tid + 1] = split_candidate(k, sg.n, b, nb);
+        if (tid < 2) {
+            const int e = tid == 0 ? c0 - 1 : c0 + 1024;
+            cd[tid == 0 ? 0 : 1025] = split_candidate(k, sg.n, e, nb);
+        }
+        __syncthreads();
+        uint64_t v = 0ull;
+        if (b < nb) {
+            if (b > 0) {
+                uint64_t x = cd[tid], y = cd[tid + 1];
+                const uint64_t z = cd[tid + 2];
+                if (x > y) swap_u64(x, y);
+                v = max_u64(x, min_u64(y, z));  // median of three
+            }
+            bcnt[b0 + b] = 0u;
+            bfill[b0 + b] = 0u;
+        }
+        sc[tid] = v;
+        __syncthreads();
+        for (int off = 1; off < 1024; off <<= 1) {
+            const uint64_t o = tid >= off ? sc[tid - off] : 0ull;
+            __syncthreads();
+            sc[tid] = max_u64(sc[tid], o);
+            __syncthreads();
+        }
+        if (b < nb) split[b0 + b] = max_u64(sc[tid], carry);
+        carry = max_u64(carry, sc[1023]);
+        __syncthreads();
+    }
+}
+
+// The window of splitters of the tile starting at place j0 of an instance
+// of nb buckets: buckets [wbase, wbase + wn).
+__host__ __device__ __forceinline__ void bucket_window(int j0, int nb, int& wbase, int& wn)
+{
+    wn = nb < kBucketWin ? nb : kBucketWin;
+    const int g = j0 / kBucketKeys - kBucketWin / 4;
+    wbase = g < 0 ? 0 : (g > nb - wn ? nb - wn : g);
+}
+
+// Local bucket of key k: the number of splitters 1 .. nb-1 at most k.
+__host__ __device__ __forceinline__ int bucket_of(uint64_t k, const uint64_t* ls, int wbase, int wn,
+                                         const uint64_t* __restrict__ split, int nb, bool& inwin)
+{
+    int lo, hi;
+    inwin = k >= ls[0] && k < ls[wn];
+    if (inwin) {
+        lo = 1;
+        hi = wn;
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (ls[mid] <= k) lo = mid + 1;
+            else hi = mid;
+        }
+        return wbase + lo - 1;
+    }
+    lo = 1;
+    hi = nb;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (split[mid] <= k) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo - 1;
+}
+
+// The window's splitters in LDS: ls[w] = split[wbase + w] for w <= wn
+// (+inf past the last bucket; split[0] = 0).
+__device__ __forceinline__ void load_window(uint64_t* ls, const uint64_t* __restrict__ split,
+                                            int wbase, int wn, int nb)
+{
+    for (int w = (int)threadIdx.x; w <= wn; w += blockDim.x)
+        ls[w] = wbase + w < nb ? split[wbase + w] : ~0ull;
+}
+
+__global__ __launch_bounds__(kTileThreads) void k_bucket_count(
+    const TileInfo* __restrict__ tiles, const SegInfo* __restrict__ segs,
+    const int* __restrict__ bkt0, const uint64_t* __restrict__ keys,
+    const uint64_t* __restrict__ split, uint32_t* __restrict__ bid, uint32_t* __restrict__ bcnt)
+{
+    __shared__ uint64_t ls[kBucketWin + 1];
+    __shared__ uint32_t lc[kBucketWin];
+    const TileInfo t = tiles[blockIdx.x];
+    const SegInfo sg = segs[t.seg];
+    const int b0 = bkt0[t.seg], nb = bkt0[t.seg + 1] - b0;
+    const int tid = (int)threadIdx.x;
+    int wbase, wn;
+    bucket_window(t.start - sg.start, nb, wbase, wn);
+    load_window(ls, split + b0, wbase, wn, nb);
+    for (int w = tid; w < wn; w += kTileThreads) lc[w] = 0u;
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < kSortItems; ++r) {
+        const int idx = r * kTileThreads + tid;
+        if (idx < t.count) {
+            const uint64_t k = keys[t.start + idx];
+            bool inwin;
+            const int b = bucket_of(k, ls, wbase, wn, split + b0, nb, inwin);
+            if (inwin) atomicAdd(&lc[b - wbase], 1u);
+            else atomicAdd(&bcnt[b0 + b], 1u);
+            bid[t.start + idx] = (uint32_t)(b0 + b);
+        }
+    }
+    __syncthreads();
+    for (int w = tid; w < wn; w += kTileThreads)
+        if (lc[w] != 0u) atomicAdd(&bcnt[b0 + wbase + w], lc[w]);
+}
+
+// Per instance: boff[b] = the instance's first particle + the keys of its
+// earlier buckets.
+__global__ __launch_bounds__(1024) void k_bucket_scan(const SegInfo* __restrict__ segs,
+                                                      const int* __restrict__ bkt0,
+                                                      const uint32_t* __restrict__ bcnt,
+                                                      uint32_t* __restrict__ boff)
+{
+    __shared__ uint32_t wsum[16];
+    const int s = blockIdx.x;
+    const int tid = (int)threadIdx.x;
+    const int l = lane_id(), w = tid >> 6;
+    const int b0 = bkt0[s], nb = bkt0[s + 1] - b0;
+    uint32_t carry = (uint32_t)segs[s].start;
+    for (int c0 = 0; c0 < nb; c0 += 1024) {
+        const int b = c0 + tid;
+        const uint32_t v = b < nb ? bcnt[b0 + b] : 0u;
+        const uint32_t incl = (uint32_t)wave_scan_incl((int)v);
+        if (l == 63) wsum[w] = incl;
+        __syncthreads();
+        uint32_t base = carry, tot = 0u;
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            const uint32_t x = wsum[k];
+            if (k < w) base += x;
+            tot += x;
+        }
+        if (b < nb) boff[b0 + b] = base + incl - v;
+        carry += tot;
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(kTileThreads) void k_bucket_scatter(
+    const TileInfo* __restrict__ tiles, const SegInfo* __restrict__ segs,
+    const int* __restrict__ bkt0, const uint64_t* __restrict__ keys,
+    const uint32_t* __restrict__ bid, const uint32_t* __restrict__ boff,
+    uint32_t* __restrict__ bfill, uint64_t* __restrict__ kout, uint32_t* __restrict__ vout)
+{
+    __shared__ uint32_t lc[kBucketWin];
+    __shared__ uint32_t lb[kBucketWin];
+    const TileInfo t = tiles[blockIdx.x];
+    const SegInfo sg = segs[t.seg];
+    const int b0 = bkt0[t.seg], nb = bkt0[t.seg + 1] - b0;
+    const int tid = (int)threadIdx.x;
+    int wbase, wn;
+    bucket_window(t.start - sg.start, nb, wbase, wn);
+    for (int w = tid; w < wn; w += kTileThreads) lc[w] = 0u;
+    __syncthreads();
+    uint64_t key[kSortItems];
+    uint32_t b[kSortItems], rank[kSortItems];
+    bool win[kSortItems];
+#pragma unroll
+    for (int r = 0; r < kSortItems; ++r) {
+        const int idx = r * kTileThreads + tid;
+        win[r] = false;
+        b[r] = 0u;
+        rank[r] = 0u;
+        key[r] = 0ull;
+        if (idx < t.count) {
+            key[r] = keys[t.start + idx];
+            b[r] = bid[t.start + idx];
+            const int w = (int)b[r] - b0 - wbase;
+            win[r] = w >= 0 && w < wn;
+            rank[r] = win[r] ? atomicAdd(&lc[w], 1u) : atomicAdd(&bfill[b[r]], 1u);
+        }
+    }
+    __syncthreads();
+    for (int w = tid; w < wn; w += kTileThreads)
+        lb[w] = lc[w] != 0u ? atomicAdd(&bfill[b0 + wbase + w], lc[w]) : 0u;
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < kSortItems; ++r) {
+        const int idx = r * kTileThreads + tid;
+        if (idx < t.count) {
+            const uint32_t pos =
+                boff[b[r]] + rank[r] + (win[r] ? lb[(int)b[r] - b0 - wbase] : 0u);
+            kout[pos] = key[r];
+            vout[pos] = (uint32_t)(t.start + idx);
+        }
+    }
+}
+
+__host__ __device__ __forceinline__ bool pair_less(uint64_t ka, uint32_t va, uint64_t kb, uint32_t vb)
+{
+    return (ka < kb) | ((ka == kb) & (va < vb));  // bitwise: no branches
+}
+
+// LDS place of element e of a bucket being sorted: bits 6-8 folded into
+// bits 0-2, so that 64 lanes reading elements 1, 2, 4 or 8 apart (the merge
+// rounds' pattern) hit distinct banks.  A bijection on [0, kBucketCap).
+__host__ __device__ __forceinline__ int lds_at(int e) { return e ^ ((e >> 6) & 7); }
+
+__host__ __device__ __forceinline__ void cmp_swap(uint64_t& ka, uint32_t& va, uint64_t& kb,
+                                                  uint32_t& vb)
+{
+    const bool sw = pair_less(kb, vb, ka, va);
+    const uint64_t lk = sw ? kb : ka, hk = sw ? ka : kb;
+    const uint32_t lv = sw ? vb : va, hv = sw ? va : vb;
+    ka = lk;
+    kb = hk;
+    va = lv;
+    vb = hv;
+}
+
+// Eight pairs sorted by a 19-comparator network.
+__host__ __device__ __forceinline__ void sort8(uint64_t* k, uint32_t* v)
+{
+#define GCS_CS(a, b) cmp_swap(k[a], v[a], k[b], v[b])
+    GCS_CS(0, 2); GCS_CS(1, 3); GCS_CS(4, 6); GCS_CS(5, 7);
+    GCS_CS(0, 4); GCS_CS(1, 5); GCS_CS(2, 6); GCS_CS(3, 7);
+    GCS_CS(0, 1); GCS_CS(2, 3); GCS_CS(4, 5); GCS_CS(6, 7);
+    GCS_CS(2, 4); GCS_CS(3, 5);
+    GCS_CS(1, 4); GCS_CS(3, 6);
+    GCS_CS(1, 2); GCS_CS(3, 4); GCS_CS(5, 6);
+#undef GCS_CS
+}
+
+// Outputs [base, base + 8) of a merge round over sk/sv (element e at
+// lds_at(e)), where sorted runs of w pairs are merged pairwise: the split of the merge path before the first
+// output (a binary search), then eight merge steps, each loading the next
+// pair of the run it took from (one LDS load, no branch; a run's last step
+// loads its successor, in the array or clamped to its last place, unused).
+// Ties (only the padding is equal) go to the second run in both the search
+// and the merge, so every output is taken once.
+__host__ __device__ __forceinline__ void merge_round8(const uint64_t* sk, const uint32_t* sv,
+                                                      int w, int base, uint64_t* k, uint32_t* v)
+{
+    const int a0 = base & ~(2 * w - 1), k0 = base & (2 * w - 1);
+    int lo = k0 > w ? k0 - w : 0, hi = k0 < w ? k0 : w;
+    while (lo < hi) {
+        const int i = (lo + hi) >> 1;
+        const int pa = lds_at(a0 + i), pb = lds_at(a0 + w + k0 - i - 1);
+        const uint64_t xa = sk[pa], xb = sk[pb];
+        bool right = xa < xb;
+        if (xa == xb) right = sv[pa] < sv[pb];  // equal keys are rare: indices only then
+        lo = right ? i + 1 : lo;
+        hi = right ? hi : i;
+    }
+    int i = lo, j = k0 - lo;
+    // Heads: places a0 + i and a0 + w + j (at most a0 + 2w - 1 while the run
+    // is not exhausted; an exhausted run's head is never taken).
+    const int last = kBucketCap - 1;
+    const int pa = lds_at(a0 + i < last ? a0 + i : last);
+    const int pb = lds_at(a0 + w + j < last ? a0 + w + j : last);
+    uint64_t ka = sk[pa], kb = sk[pb];
+    uint32_t va = sv[pa], vb = sv[pb];
+#pragma unroll
+    for (int e = 0; e < kBucketPer; ++e) {
+        const bool take_a = (j >= w) | ((i < w) & pair_less(ka, va, kb, vb));
+        k[e] = take_a ? ka : kb;
+        v[e] = take_a ? va : vb;
+        i += take_a ? 1 : 0;
+        j += take_a ? 0 : 1;
+        int q = take_a ? a0 + i : a0 + w + j;
+        q = lds_at(q < last ? q : last);
+        const uint64_t nk = sk[q];
+        const uint32_t nv = sv[q];
+        ka = take_a ? nk : ka;
+        va = take_a ? nv : va;
+        kb = take_a ? kb : nk;
+        vb = take_a ? vb : nv;
+    }
+}
+
+// Sorts kBucketCap (key, index) pairs in LDS, ascending; every thread of the
+// block calls it (sk/sv written and synchronised on entry).  Thread t sorts
+// places [8t, 8t + 8) in registers, then rounds merge sorted runs of w = 8,
+// 16, ..., 2048 pairs: thread t computes outputs [8t, 8t + 8) into registers
+// and writes them back after the block has read the round's input.
+__device__ __forceinline__ void lds_merge_sort(uint64_t* sk, uint32_t* sv)
+{
+    const int base = kBucketPer * (int)threadIdx.x;
+    uint64_t k[kBucketPer];
+    uint32_t v[kBucketPer];
+#pragma unroll
+    for (int e = 0; e < kBucketPer; ++e) {
+        k[e] = sk[lds_at(base + e)];
+        v[e] = sv[lds_at(base + e)];
+    }
+    sort8(k, v);
+#pragma unroll
+    for (int e = 0; e < kBucketPer; ++e) {
+        sk[lds_at(base + e)] = k[e];
+        sv[lds_at(base + e)] = v[e];
+    }
+    __syncthreads();
+    for (int w = kBucketPer; w < kBucketCap; w <<= 1) {
+        merge_round8(sk, sv, w, base, k, v);
+        __syncthreads();  // the round's input has been read
+#pragma unroll
+        for (int e = 0; e < kBucketPer; ++e) {
+            sk[lds_at(base + e)] = k[e];
+            sv[lds_at(base + e)] = v[e];
+        }
+        __syncthreads();
+    }
+}
+
+// One chunk of at most kBucketCap pairs from (kin, vin) sorted into (kout,
+// vout) at the same places.
+__device__ __forceinline__ void sort_chunk(uint64_t* sk, uint32_t* sv,
+                                           const uint64_t* __restrict__ kin,
+                                           const uint32_t* __restrict__ vin, uint64_t* kout,
+                                           uint32_t* vout, int m)
+{
+    const int tid = (int)threadIdx.x;
+    // All of a thread's loads are issued before any is used.
+    uint64_t kr[kBucketPer];
+    uint32_t vr[kBucketPer];
+#pragma unroll
+    for (int r = 0; r < kBucketPer; ++r) {
+        const int i = tid + r * kBucketThreads;
+        kr[r] = i < m ? kin[i] : ~0ull;  // keys are below 2^63: the pads sort last
+        vr[r] = i < m ? vin[i] : 0xFFFFFFFFu;
+    }
+#pragma unroll
+    for (int r = 0; r < kBucketPer; ++r) {
+        const int i = tid + r * kBucketThreads;
+        sk[lds_at(i)] = kr[r];
+        sv[lds_at(i)] = vr[r];
+    }
+    __syncthreads();
+    lds_merge_sort(sk, sv);
+#pragma unroll
+    for (int r = 0; r < kBucketPer; ++r) {
+        const int i = tid + r * kBucketThreads;
+        kr[r] = sk[lds_at(i)];
+        vr[r] = sv[lds_at(i)];
+    }
+#pragma unroll
+    for (int r = 0; r < kBucketPer; ++r) {
+        const int i = tid + r * kBucketThreads;
+        if (i < m) {
+            kout[i] = kr[r];
+            vout[i] = vr[r];
+        }
+    }
+    __syncthreads();
+}
+
+// The block merges sorted runs a[0, la) and b[0, lb) into out[0, la + lb):
+// thread t writes outputs [t E, t E + E), starting from the split of the
+// first t E outputs between the runs (a binary search on the merge path).
+__host__ __device__ __forceinline__ void block_merge(int t, const uint64_t* ka,
+                                                     const uint32_t* va, int la,
+                                                     const uint64_t* kb, const uint32_t* vb,
+                                                     int lb, uint64_t* ko, uint32_t* vo)
+{
+    const int total = la + lb;
+    const int per = (total + kBucketThreads - 1) / kBucketThreads;
+    const int k0 = t * per;
+    if (k0 >= total) return;
+    const int k1 = k0 + per < total ? k0 + per : total;
+    int lo = k0 - lb > 0 ? k0 - lb : 0, hi = k0 < la ? k0 : la;
+    while (lo < hi) {
+        const int i = (lo + hi) >> 1;
+        // Outputs before k0 take i from a and k0 - i from b: too few from a
+        // while a[i] < b[k0 - i - 1].
+        if (pair_less(ka[i], va[i], kb[k0 - i - 1], vb[k0 - i - 1])) lo = i + 1;
+        else hi = i;
+    }
+    int i = lo, j = k0 - lo;
+    for (int k = k0; k < k1; ++k) {
+        const bool take_a = j >= lb || (i < la && pair_less(ka[i], va[i], kb[j], vb[j]));
+        if (take_a) {
+            ko[k] = ka[i];
+            vo[k] = va[i];
+            ++i;
+        } else {
+            ko[k] = kb[j];
+            vo[k] = vb[j];
+            ++j;
+        }
+    }
+}
+
+// Bucket b from (kin, vin) into (kout, vout) at [boff[b], boff[b] + bcnt[b]).
+// More than kBucketCap keys: sorted chunks in (kout, vout), merged pairwise
+// between the two buffers (both free over the bucket's places), the result
+// copied into (kout, vout) if the last merge left it in (kin, vin).
+__global__ __launch_bounds__(kBucketThreads) void k_bucket_sort(
+    const uint32_t* __restrict__ bcnt, const uint32_t* __restrict__ boff, uint64_t* kin,
+    uint32_t* vin, uint64_t* kout, uint32_t* vout)
+{
+    __shared__ uint64_t sk[kBucketCap];
+    __shared__ uint32_t sv[kBucketCap];
+    const int b = blockIdx.x;
+    const int m = (int)bcnt[b];
+    if (m == 0) return;
+    const size_t off = boff[b];
+    if (m <= kBucketCap) {
+        sort_chunk(sk, sv, kin + off, vin + off, kout + off, vout + off, m);
+        return;
+    }
+    for (int c = 0; c < m; c += kBucketCap)
+        sort_chunk(sk, sv, kin + off + c, vin + off + c, kout + off + c, vout + off + c,
+                   min(kBucketCap, m - c));
+    uint64_t* ks = kout + off;
+    uint32_t* vs = vout + off;
+    uint64_t* kd = kin + off;
+    uint32_t* vd = vin + off;
+    for (int w = kBucketCap; w < m; w <<= 1) {
+        for (int s0 = 0; s0 < m; s0 += 2 * w) {
+            const int la = min(w, m - s0), lb = min(w, m - s0 - la);
+            block_merge((int)threadIdx.x, ks + s0, vs + s0, la, ks + s0 + la, vs + s0 + la, lb,
+                        kd + s0, vd + s0);
+        }
+        __syncthreads();  // this round's output is visible to the block
+        uint64_t* tk = ks; ks = kd; kd = tk;
+        uint32_t* tv = vs; vs = vd; vd = tv;
+    }
+    if (ks != kout + off)
+        for (int i = (int)threadIdx.x; i < m; i += kBucketThreads) {
+            kout[off + i] = ks[i];
+            vout[off + i] = vs[i];
+        }
+}
+
 }  // namespace gcs

@@ -48,7 +48,8 @@ class Params(ctypes.Structure):
                 ("waves_per_simd", ctypes.c_int32), ("dt_eta", ctypes.c_float),
                 ("group_size", ctypes.c_int32), ("open_offset", ctypes.c_int32),
                 ("team", ctypes.c_int32), ("mono_far", ctypes.c_float),
-                ("local_kappa", ctypes.c_float), ("instance_pass", ctypes.c_int32)]
+                ("local_kappa", ctypes.c_float), ("instance_pass", ctypes.c_int32),
+                ("sort", ctypes.c_int32)]
 
 
 class CollisionSpec(ctypes.Structure):

@@ -123,6 +123,13 @@ typedef struct gcs_params {
                            automatic: two levels when an instance has at least
                            32,768 particles and the instances number at most half
                            the CUs.  The node records are the same either way */
+    int32_t sort;          /* key sort of a step: 1 = LSD radix sort (8 passes); 2 =
+                           bucket sort (splitters from the keys at every 3,072nd
+                           place of the current order, buckets sorted in LDS), for
+                           keys in nearly sorted order; 0 = automatic: buckets for
+                           the instances stored in the last step's sorted order,
+                           radix for instances loaded or joined since.  The result
+                           is the same stable sort either way */
 } gcs_params;
 
 /* Defaults: G=1, eps2=0.0025, dt=0.01, theta=0.5, GCS_MAC_GROUP,

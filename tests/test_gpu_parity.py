@@ -428,6 +428,86 @@ def test_forest_mid_level_subtrees(gcs, orc, levels):
     assert most > 64
 
 
+def check_instances(gcs, orc, e, parts, theta=0.5):
+    """Each instance of a forest (its stars in the engine's current order):
+    Morton keys, sorted keys, permutation, topology and aggregates equal the
+    oracle's for that instance alone."""
+    raw, srt, perm = e.keys()
+    off = 0
+    for i, a in enumerate(parts):
+        sl = slice(off, off + len(a))
+        off += len(a)
+        cpu = orc.build_tree_pipeline(a, theta)
+        assert np.array_equal(raw[sl], cpu["keys"]), i
+        assert np.array_equal(srt[sl], cpu["skeys"]) and np.array_equal(perm[sl], cpu["perm"]), i
+        if len(a) == 1:
+            continue
+        t = e.tree(i)
+        for k in ("left", "right", "parent"):
+            assert np.array_equal(t[k], cpu[k]), (i, k)
+        for k in ("com4", "width", "bmin", "bmax"):
+            assert np.array_equal(t[k].view(np.uint32), cpu["agg"][k].view(np.uint32)), (i, k)
+
+
+@pytest.mark.parametrize("case", ["cloud", "coincident", "forest"])
+def test_bucket_sort_unordered_keys(gcs, orc, case):
+    """The bucket sort forced on keys in no particular order (sort=2): the
+    splitters fit them badly, and buckets of more than 4,096 keys are sorted
+    in LDS chunks merged in global memory.  A cloud of 60,000; 20,000
+    coincident particles among 10,000 others (one bucket holds every equal
+    key); a forest with instances at the bucket edges (1-3, 3,071-3,073,
+    6,145 particles).  Every instance's keys, permutation, topology and
+    aggregates equal the oracle's."""
+    if case == "cloud":
+        parts = [cloud(gcs, 60000, 9)]
+    elif case == "coincident":
+        a = cloud(gcs, 30000, 10)
+        for k, v in (("x", 0.3), ("y", -0.2), ("z", 0.1)):
+            a[k][5000:25000] = v
+        parts = [a]
+    else:
+        parts = [cloud(gcs, n, 70 + n) for n in (1, 2, 3, 3071, 3072, 3073, 6145, 1025, 20000)]
+    counts = [len(a) for a in parts]
+    p = gcs.params(G=G, eps2=0.0025, dt=0.01, theta=0.5, leaf_size=8, sort=2)
+    with gcs.Engine(sum(counts), len(parts)) as e:
+        e.load(np.concatenate(parts), counts)
+        say("bucket sort on unordered keys: %s, %s particles" % (case, counts))
+        e.compute_accels(p)
+        check_instances(gcs, orc, e, parts)
+
+
+def test_bucket_sort_after_steps_and_joins(gcs, orc):
+    """Automatic sort choice with step graphs: the first step after the load
+    sorts by radix and stores the particles in sorted order, the graphs then
+    sort every instance with buckets; an instance joined later is sorted by
+    radix while the others keep buckets, and then joins them.  Checked against
+    the oracle on the current state after the steps and after the join, and
+    against the radix sort (sort=1) on the same state."""
+    parts = [gcs.collision(n, 80 + n) for n in (1536, 3072, 12000)]
+    parts.append(cloud(gcs, 3073, 5))
+    counts = [len(a) for a in parts]
+    p = gcs.params(G=G, eps2=0.0025, dt=0.01, theta=0.5, leaf_size=8, use_graph=1)
+    with gcs.Engine(sum(counts) + 20000, len(parts) + 1) as e:
+        e.load(np.concatenate(parts), counts)
+        say("bucket sort after steps: %s particles" % counts)
+        e.step(p, 3)
+        cur = e.stars()
+        e.compute_accels(p)
+        check_instances(gcs, orc, e, np.split(cur, np.cumsum(counts)[:-1]))
+        acc = e.accels()
+        e.compute_accels(gcs.params(G=G, eps2=0.0025, dt=0.01, theta=0.5, leaf_size=8, sort=1))
+        assert np.array_equal(acc.view(np.uint32), e.accels().view(np.uint32))
+        joined = gcs.collision(10000, 90)
+        e.add(joined)
+        counts.append(len(joined))
+        for steps in (0, 2):
+            e.step(p, steps)
+            cur = e.stars()
+            e.compute_accels(p)
+            say("%d steps after a join: %s particles" % (steps, counts))
+            check_instances(gcs, orc, e, np.split(cur, np.cumsum(counts)[:-1]))
+
+
 # ---- forests, graphs, composition ----------------------------------------------------
 
 @pytest.mark.parametrize("team,kappa", [(1, 0.0), (2, 0.0), (4, 0.0), (8, 0.0), (1, 6.0),
