@@ -148,6 +148,7 @@ struct gcs_engine {
     // splitters, counts, offsets and fill counters, and each key's bucket.
     int* bkt0 = nullptr;
     uint64_t* bsplit = nullptr;
+    uint64_t* bcand = nullptr;
     uint32_t* bcnt = nullptr;
     uint32_t* boff = nullptr;
     uint32_t* bfill = nullptr;
@@ -248,7 +249,7 @@ void free_all(gcs_engine* e)
     dfree(e->gmark); dfree(e->gq); dfree(e->gorder); dfree(e->gchunk); dfree(e->gcnt); dfree(e->ngroups);
     dfree(e->front); dfree(e->nfront);
     dfree(e->tlist); dfree(e->ntop); dfree(e->tslot); dfree(e->mids); dfree(e->nmid); dfree(e->mfail);
-    dfree(e->bkt0); dfree(e->bsplit); dfree(e->bcnt); dfree(e->boff); dfree(e->bfill); dfree(e->bid);
+    dfree(e->bkt0); dfree(e->bsplit); dfree(e->bcand); dfree(e->bcnt); dfree(e->boff); dfree(e->bfill); dfree(e->bid);
     dfree(e->seg_bounds); dfree(e->bpart); dfree(e->qctr); dfree(e->counters); dfree(e->wcount); dfree(e->err);
     dfree(e->diag_part); dfree(e->diag_out); dfree(e->amax_part); dfree(e->dt_inst); dfree(e->tsim); dfree(e->slot_mem); dfree(e->staging); dfree(e->mn_dev);
     for (auto& v : e->ev)
@@ -492,8 +493,9 @@ void enqueue_bucket_sort(gcs_engine* e, int nseg)
     const int ntiles = nseg < (int)e->h_segs.size() ? e->h_segs[(size_t)nseg].tile0
                                                     : (int)e->h_tiles.size();
     const int nbkt = e->h_bkt0[(size_t)nseg];
-    k_bucket_split<<<nseg, 1024, 0, e->stream>>>(e->segs, e->bkt0, e->keys[0], e->bsplit, e->bcnt,
-                                                 e->bfill);
+    k_bucket_cand<<<(nbkt + 255) / 256, 256, 0, e->stream>>>(e->segs, e->bkt0, nseg, e->keys[0],
+                                                             e->bcand, e->bcnt, e->bfill);
+    k_bucket_split<<<nseg, 1024, 0, e->stream>>>(e->bkt0, e->bcand, e->bsplit);
     k_bucket_count<<<ntiles, kTileThreads, 0, e->stream>>>(e->tiles, e->segs, e->bkt0, e->keys[0],
                                                            e->bsplit, e->bid, e->bcnt);
     k_bucket_scan<<<nseg, 1024, 0, e->stream>>>(e->segs, e->bkt0, e->bcnt, e->boff);
@@ -947,7 +949,7 @@ int64_t gcs_device_bytes(int64_t cap_n, int32_t cap_k)
     b += 4 * 4 * (int64_t)max_tiles(cap_n, cap_k) + 12 * (int64_t)cap_k;  // amax, dt, time
     b += 4 * n + 8 * n + 4 * (int64_t)cap_k;  // nodes above the aggregation subtrees
     b += 8 * (n / kATile + (int64_t)cap_k + 1) + 4 + 4 * (int64_t)cap_k;  // mid-level roots
-    b += 4 * ((int64_t)cap_k + 1) + 20 * (int64_t)max_buckets(cap_n, cap_k) + 4 * n;  // buckets
+    b += 4 * ((int64_t)cap_k + 1) + 28 * (int64_t)max_buckets(cap_n, cap_k) + 4 * n;  // buckets
     b += 32 * n;               // staging (also the device generator's scratch)
     b += 8 * ((gcs_gen::kNR + 1) + (gcs_gen::kNZ + 1) + 2 * gcs_gen::kNR * gcs_gen::kNZ);
     return b;
@@ -1018,6 +1020,7 @@ int gcs_create(int64_t cap_n, int32_t cap_k, int32_t device, gcs_engine** out)
     if (!rc) rc = dalloc(&e->mfail, (size_t)cap_k);
     if (!rc) rc = dalloc(&e->bkt0, (size_t)cap_k + 1);
     if (!rc) rc = dalloc(&e->bsplit, max_buckets(cap_n, cap_k));
+    if (!rc) rc = dalloc(&e->bcand, max_buckets(cap_n, cap_k));
     if (!rc) rc = dalloc(&e->bcnt, max_buckets(cap_n, cap_k));
     if (!rc) rc = dalloc(&e->boff, max_buckets(cap_n, cap_k));
     if (!rc) rc = dalloc(&e->bfill, max_buckets(cap_n, cap_k));

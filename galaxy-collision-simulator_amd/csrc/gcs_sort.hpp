@@ -221,11 +221,12 @@ __global__ __launch_bounds__(THREADS) void k_sort_scatter(
 // rank (2 x 1,048,576 particles, dt 0.005: the median moves 88 places, 0.1 %
 // move across the key space).  Splitters taken from the keys at every
 // kBucketKeys-th place of the current order then cut each instance into
-// buckets of about kBucketKeys keys (at most 3,218 there over 30 steps), and
+// buckets of about kBucketKeys keys (at most 1,762 there over 20 steps), and
 // the sort is
-//   k_bucket_split    per instance: splitters from the keys around every
-//                     kBucketKeys-th place (medians, made non-decreasing);
-//                     bucket counters cleared
+//   k_bucket_cand     per bucket: a splitter candidate from the keys around
+//                     every kBucketKeys-th place; bucket counters cleared
+//   k_bucket_split    per instance: candidates filtered and made
+//                     non-decreasing
 //   k_bucket_count    per tile: each key's bucket (a search among the
 //                     tile's window of kBucketWin splitters in LDS, or among
 //                     all of the instance's for a key outside it), counted
@@ -233,7 +234,7 @@ __global__ __launch_bounds__(THREADS) void k_sort_scatter(
 //   k_bucket_scatter  per tile: keys and their indices moved into their
 //                     buckets, in any order within a bucket
 //   k_bucket_sort     per bucket: sorted by (key, index) in LDS (a sorting
-//                     network of eight per thread, then merge-path rounds).
+//                     network of four per thread, then merge-path rounds).
 // (key, index) is a total order, so every bucket's order -- and the result
 // -- is the stable sort's, whatever order the scatter left.  A bucket of more
 // than kBucketCap keys (splitters that fit the keys badly) is sorted in
@@ -241,12 +242,12 @@ __global__ __launch_bounds__(THREADS) void k_sort_scatter(
 // Bucket b of an instance holds the keys k with split[b] <= k < split[b+1]
 // (split[0] = 0, split[nb] = +inf): its index is the number of splitters
 // 1 .. nb-1 at most k.
-constexpr int kBucketKeys = 3072;  // splitter spacing
-constexpr int kBucketCap = 4096;   // keys a block sorts in LDS
+constexpr int kBucketKeys = 1536;  // splitter spacing
+constexpr int kBucketCap = 2048;   // keys a block sorts in LDS
 constexpr int kBucketWin = 64;     // splitters per tile in LDS
 constexpr int kBucketThreads = 512;
 constexpr int kBucketPer = kBucketCap / kBucketThreads;
-static_assert(kBucketPer == 8, "the register network sorts eight");
+static_assert(kBucketPer == 4, "the register network sorts four");
 
 __host__ __device__ __forceinline__ uint64_t min_u64(uint64_t a, uint64_t b) { return a < b ? a : b; }
 __host__ __device__ __forceinline__ uint64_t max_u64(uint64_t a, uint64_t b) { return a < b ? b : a; }
@@ -292,54 +293,92 @@ __host__ __device__ __forceinline__ uint64_t split_candidate(const uint64_t* k, 
     return median9(a);
 }
 
+// Candidate of every bucket (one thread each, over all instances), and the
+// bucket counters cleared.
+__global__ __launch_bounds__(256) void k_bucket_cand(const SegInfo* __restrict__ segs,
+                                                     const int* __restrict__ bkt0, int nseg,
+                                                     const uint64_t* __restrict__ keys,
+                                                     uint64_t* __restrict__ cand,
+                                                     uint32_t* __restrict__ bcnt,
+                                                     uint32_t* __restrict__ bfill)
+{
+    const int b = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (b >= bkt0[nseg]) return;
+    int lo = 0, hi = nseg - 1;  // the instance: the last s with bkt0[s] <= b
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (bkt0[mid] <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    const SegInfo sg = segs[lo];
+    cand[b] = split_candidate(keys + sg.start, sg.n, b - bkt0[lo], bkt0[lo + 1] - bkt0[lo]);
+    bcnt[b] = 0u;
+    bfill[b] = 0u;
+}
+
+// Running maximum over the block's threads, exclusive: the maximum of v over
+// the threads before this one (0 for the first), and in *total over all of
+// them.  Within each wave by shuffles, then over the waves' totals in LDS;
+// every thread calls it.
+__device__ __forceinline__ uint64_t block_max_before(uint64_t v, uint64_t* wtot, uint64_t* total)
+{
+    const int l = lane_id(), w = (int)threadIdx.x >> 6, nw = (int)blockDim.x >> 6;
+    uint64_t incl = v;
+#pragma unroll
+    for (int o = 1; o < kWave; o <<= 1) {
+        const uint64_t u = __shfl_up(incl, o);
+        incl = l >= o ? max_u64(incl, u) : incl;
+    }
+    const uint64_t prev = __shfl_up(incl, 1);
+    if (l == kWave - 1) wtot[w] = incl;
+    __syncthreads();
+    uint64_t before = l > 0 ? prev : 0ull, all = 0ull;
+    for (int q = 0; q < nw; ++q) {
+        const uint64_t t = wtot[q];
+        before = q < w ? max_u64(before, t) : before;
+        all = max_u64(all, t);
+    }
+    __syncthreads();  // wtot is rewritten by the next call
+    *total = all;
+    return before;
+}
+
 // Splitter b: the median of candidates b - 1, b, b + 1 (one bad candidate
 // never reaches the running maximum), made non-decreasing by a running
-// maximum over the instance.
-__global__ __launch_bounds__(1024) void k_bucket_split(const SegInfo* __restrict__ segs,
-                                                       const int* __restrict__ bkt0,
-                                                       const uint64_t* __restrict__ keys,
-                                                       uint64_t* __restrict__ split,
-                                                       uint32_t* __restrict__ bcnt,
-                                                       uint32_t* __restrict__ bfill)
+// maximum over the instance.  Each thread takes four consecutive buckets.
+__global__ __launch_bounds__(1024) void k_bucket_split(const int* __restrict__ bkt0,
+                                                       const uint64_t* __restrict__ cand,
+                                                       uint64_t* __restrict__ split)
 {
-    __shared__ uint64_t sc[1024];
-    __shared__ uint64_t cd[1024 + 2];  // candidates c0 - 1 .. c0 + 1024
+    __shared__ uint64_t wtot[16];
     const int s = blockIdx.x;
-    const int tid = (int)threadIdx.x;
-    const SegInfo sg = segs[s];
     const int b0 = bkt0[s], nb = bkt0[s + 1] - b0;
-    const uint64_t* k = keys + sg.start;
+    const uint64_t* c = cand + b0;
     uint64_t carry = 0ull;
-    for (int c0 = 0; c0 < nb; c0 += 1024) {
-        const int b = c0 + tid;
-        cd[tid + 1] = split_candidate(k, sg.n, b, nb);
-        if (tid < 2) {
-            const int e = tid == 0 ? c0 - 1 : c0 + 1024;
-            cd[tid == 0 ? 0 : 1025] = split_candidate(k, sg.n, e, nb);
-        }
-        __syncthreads();
-        uint64_t v = 0ull;
-        if (b < nb) {
-            if (b > 0) {
-                uint64_t x = cd[tid], y = cd[tid + 1];
-                const uint64_t z = cd[tid + 2];
+    for (int c0 = 0; c0 < nb; c0 += 4 * 1024) {
+        const int bb = c0 + 4 * (int)threadIdx.x;
+        uint64_t v[4];
+        uint64_t run = 0ull;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const int b = bb + e;
+            uint64_t x = 0ull;
+            if (b > 0 && b < nb) {
+                x = c[b - 1];
+                uint64_t y = c[b];
+                const uint64_t z = b + 1 < nb ? c[b + 1] : ~0ull;
                 if (x > y) swap_u64(x, y);
-                v = max_u64(x, min_u64(y, z));  // median of three
+                x = max_u64(x, min_u64(y, z));  // median of three
             }
-            bcnt[b0 + b] = 0u;
-            bfill[b0 + b] = 0u;
+            run = max_u64(run, x);
+            v[e] = run;
         }
-        sc[tid] = v;
-        __syncthreads();
-        for (int off = 1; off < 1024; off <<= 1) {
-            const uint64_t o = tid >= off ? sc[tid - off] : 0ull;
-            __syncthreads();
-            sc[tid] = max_u64(sc[tid], o);
-            __syncthreads();
-        }
-        if (b < nb) split[b0 + b] = max_u64(sc[tid], carry);
-        carry = max_u64(carry, sc[1023]);
-        __syncthreads();
+        uint64_t total;
+        const uint64_t before = max_u64(carry, block_max_before(run, wtot, &total));
+#pragma unroll
+        for (int e = 0; e < 4; ++e)
+            if (bb + e < nb) split[b0 + bb + e] = max_u64(before, v[e]);
+        carry = max_u64(carry, total);
     }
 }
 
@@ -507,11 +546,6 @@ __host__ __device__ __forceinline__ bool pair_less(uint64_t ka, uint32_t va, uin
     return (ka < kb) | ((ka == kb) & (va < vb));  // bitwise: no branches
 }
 
-// LDS place of element e of a bucket being sorted: bits 6-8 folded into
-// bits 0-2, so that 64 lanes reading elements 1, 2, 4 or 8 apart (the merge
-// rounds' pattern) hit distinct banks.  A bijection on [0, kBucketCap).
-__host__ __device__ __forceinline__ int lds_at(int e) { return e ^ ((e >> 6) & 7); }
-
 __host__ __device__ __forceinline__ void cmp_swap(uint64_t& ka, uint32_t& va, uint64_t& kb,
                                                   uint32_t& vb)
 {
@@ -524,34 +558,32 @@ __host__ __device__ __forceinline__ void cmp_swap(uint64_t& ka, uint32_t& va, ui
     vb = hv;
 }
 
-// Eight pairs sorted by a 19-comparator network.
-__host__ __device__ __forceinline__ void sort8(uint64_t* k, uint32_t* v)
+// Four pairs sorted by a 5-comparator network.
+__host__ __device__ __forceinline__ void sort4(uint64_t* k, uint32_t* v)
 {
-#define GCS_CS(a, b) cmp_swap(k[a], v[a], k[b], v[b])
-    GCS_CS(0, 2); GCS_CS(1, 3); GCS_CS(4, 6); GCS_CS(5, 7);
-    GCS_CS(0, 4); GCS_CS(1, 5); GCS_CS(2, 6); GCS_CS(3, 7);
-    GCS_CS(0, 1); GCS_CS(2, 3); GCS_CS(4, 5); GCS_CS(6, 7);
-    GCS_CS(2, 4); GCS_CS(3, 5);
-    GCS_CS(1, 4); GCS_CS(3, 6);
-    GCS_CS(1, 2); GCS_CS(3, 4); GCS_CS(5, 6);
-#undef GCS_CS
+    cmp_swap(k[0], v[0], k[1], v[1]);
+    cmp_swap(k[2], v[2], k[3], v[3]);
+    cmp_swap(k[0], v[0], k[2], v[2]);
+    cmp_swap(k[1], v[1], k[3], v[3]);
+    cmp_swap(k[1], v[1], k[2], v[2]);
 }
 
-// Outputs [base, base + 8) of a merge round over sk/sv (element e at
-// lds_at(e)), where sorted runs of w pairs are merged pairwise: the split of the merge path before the first
-// output (a binary search), then eight merge steps, each loading the next
-// pair of the run it took from (one LDS load, no branch; a run's last step
-// loads its successor, in the array or clamped to its last place, unused).
+// Outputs [base, base + kBucketPer) of a merge round over sk/sv, where sorted runs of
+// w pairs are merged pairwise: the split of the merge path before the first
+// output (a binary search), then kBucketPer merge steps, each loading the next
+// pair of the run it took from (one LDS load, no branch; an exhausted run's
+// head -- at most place kBucketCap, so the arrays hold kBucketCap + 1 -- is
+// loaded but never taken).
 // Ties (only the padding is equal) go to the second run in both the search
 // and the merge, so every output is taken once.
-__host__ __device__ __forceinline__ void merge_round8(const uint64_t* sk, const uint32_t* sv,
+__host__ __device__ __forceinline__ void merge_round(const uint64_t* sk, const uint32_t* sv,
                                                       int w, int base, uint64_t* k, uint32_t* v)
 {
     const int a0 = base & ~(2 * w - 1), k0 = base & (2 * w - 1);
     int lo = k0 > w ? k0 - w : 0, hi = k0 < w ? k0 : w;
     while (lo < hi) {
         const int i = (lo + hi) >> 1;
-        const int pa = lds_at(a0 + i), pb = lds_at(a0 + w + k0 - i - 1);
+        const int pa = (a0 + i), pb = (a0 + w + k0 - i - 1);
         const uint64_t xa = sk[pa], xb = sk[pb];
         bool right = xa < xb;
         if (xa == xb) right = sv[pa] < sv[pb];  // equal keys are rare: indices only then
@@ -559,13 +591,8 @@ __host__ __device__ __forceinline__ void merge_round8(const uint64_t* sk, const 
         hi = right ? hi : i;
     }
     int i = lo, j = k0 - lo;
-    // Heads: places a0 + i and a0 + w + j (at most a0 + 2w - 1 while the run
-    // is not exhausted; an exhausted run's head is never taken).
-    const int last = kBucketCap - 1;
-    const int pa = lds_at(a0 + i < last ? a0 + i : last);
-    const int pb = lds_at(a0 + w + j < last ? a0 + w + j : last);
-    uint64_t ka = sk[pa], kb = sk[pb];
-    uint32_t va = sv[pa], vb = sv[pb];
+    uint64_t ka = sk[a0 + i], kb = sk[a0 + w + j];
+    uint32_t va = sv[a0 + i], vb = sv[a0 + w + j];
 #pragma unroll
     for (int e = 0; e < kBucketPer; ++e) {
         const bool take_a = (j >= w) | ((i < w) & pair_less(ka, va, kb, vb));
@@ -573,8 +600,7 @@ __host__ __device__ __forceinline__ void merge_round8(const uint64_t* sk, const 
         v[e] = take_a ? va : vb;
         i += take_a ? 1 : 0;
         j += take_a ? 0 : 1;
-        int q = take_a ? a0 + i : a0 + w + j;
-        q = lds_at(q < last ? q : last);
+        const int q = take_a ? a0 + i : a0 + w + j;
         const uint64_t nk = sk[q];
         const uint32_t nv = sv[q];
         ka = take_a ? nk : ka;
@@ -586,9 +612,9 @@ __host__ __device__ __forceinline__ void merge_round8(const uint64_t* sk, const 
 
 // Sorts kBucketCap (key, index) pairs in LDS, ascending; every thread of the
 // block calls it (sk/sv written and synchronised on entry).  Thread t sorts
-// places [8t, 8t + 8) in registers, then rounds merge sorted runs of w = 8,
-// 16, ..., 2048 pairs: thread t computes outputs [8t, 8t + 8) into registers
-// and writes them back after the block has read the round's input.
+// places [4t, 4t + 4) in registers, then rounds merge sorted runs of w = 4,
+// 8, ..., kBucketCap / 2 pairs: thread t computes outputs [4t, 4t + 4) into
+// registers and writes them back after the block has read the round's input.
 __device__ __forceinline__ void lds_merge_sort(uint64_t* sk, uint32_t* sv)
 {
     const int base = kBucketPer * (int)threadIdx.x;
@@ -596,23 +622,23 @@ __device__ __forceinline__ void lds_merge_sort(uint64_t* sk, uint32_t* sv)
     uint32_t v[kBucketPer];
 #pragma unroll
     for (int e = 0; e < kBucketPer; ++e) {
-        k[e] = sk[lds_at(base + e)];
-        v[e] = sv[lds_at(base + e)];
+        k[e] = sk[base + e];
+        v[e] = sv[base + e];
     }
-    sort8(k, v);
+    sort4(k, v);
 #pragma unroll
     for (int e = 0; e < kBucketPer; ++e) {
-        sk[lds_at(base + e)] = k[e];
-        sv[lds_at(base + e)] = v[e];
+        sk[base + e] = k[e];
+        sv[base + e] = v[e];
     }
     __syncthreads();
     for (int w = kBucketPer; w < kBucketCap; w <<= 1) {
-        merge_round8(sk, sv, w, base, k, v);
+        merge_round(sk, sv, w, base, k, v);
         __syncthreads();  // the round's input has been read
 #pragma unroll
         for (int e = 0; e < kBucketPer; ++e) {
-            sk[lds_at(base + e)] = k[e];
-            sv[lds_at(base + e)] = v[e];
+            sk[base + e] = k[e];
+            sv[base + e] = v[e];
         }
         __syncthreads();
     }
@@ -638,16 +664,16 @@ __device__ __forceinline__ void sort_chunk(uint64_t* sk, uint32_t* sv,
 #pragma unroll
     for (int r = 0; r < kBucketPer; ++r) {
         const int i = tid + r * kBucketThreads;
-        sk[lds_at(i)] = kr[r];
-        sv[lds_at(i)] = vr[r];
+        sk[i] = kr[r];
+        sv[i] = vr[r];
     }
     __syncthreads();
     lds_merge_sort(sk, sv);
 #pragma unroll
     for (int r = 0; r < kBucketPer; ++r) {
         const int i = tid + r * kBucketThreads;
-        kr[r] = sk[lds_at(i)];
-        vr[r] = sv[lds_at(i)];
+        kr[r] = sk[i];
+        vr[r] = sv[i];
     }
 #pragma unroll
     for (int r = 0; r < kBucketPer; ++r) {
@@ -700,12 +726,12 @@ __host__ __device__ __forceinline__ void block_merge(int t, const uint64_t* ka,
 // More than kBucketCap keys: sorted chunks in (kout, vout), merged pairwise
 // between the two buffers (both free over the bucket's places), the result
 // copied into (kout, vout) if the last merge left it in (kin, vin).
-__global__ __launch_bounds__(kBucketThreads) void k_bucket_sort(
+__global__ __launch_bounds__(kBucketThreads, 3) void k_bucket_sort(
     const uint32_t* __restrict__ bcnt, const uint32_t* __restrict__ boff, uint64_t* kin,
     uint32_t* vin, uint64_t* kout, uint32_t* vout)
 {
-    __shared__ uint64_t sk[kBucketCap];
-    __shared__ uint32_t sv[kBucketCap];
+    __shared__ uint64_t sk[kBucketCap + 1];
+    __shared__ uint32_t sv[kBucketCap + 1];
     const int b = blockIdx.x;
     const int m = (int)bcnt[b];
     if (m == 0) return;

@@ -455,8 +455,8 @@ def test_bucket_sort_unordered_keys(gcs, orc, case):
     splitters fit them badly, and buckets of more than 4,096 keys are sorted
     in LDS chunks merged in global memory.  A cloud of 60,000; 20,000
     coincident particles among 10,000 others (one bucket holds every equal
-    key); a forest with instances at the bucket edges (1-3, 3,071-3,073,
-    6,145 particles).  Every instance's keys, permutation, topology and
+    key); a forest with instances at the bucket edges (1-3, 1,535-1,537,
+    3,073 particles).  Every instance's keys, permutation, topology and
     aggregates equal the oracle's."""
     if case == "cloud":
         parts = [cloud(gcs, 60000, 9)]
@@ -466,7 +466,7 @@ def test_bucket_sort_unordered_keys(gcs, orc, case):
             a[k][5000:25000] = v
         parts = [a]
     else:
-        parts = [cloud(gcs, n, 70 + n) for n in (1, 2, 3, 3071, 3072, 3073, 6145, 1025, 20000)]
+        parts = [cloud(gcs, n, 70 + n) for n in (1, 2, 3, 1535, 1536, 1537, 3073, 1025, 20000)]
     counts = [len(a) for a in parts]
     p = gcs.params(G=G, eps2=0.0025, dt=0.01, theta=0.5, leaf_size=8, sort=2)
     with gcs.Engine(sum(counts), len(parts)) as e:
@@ -483,8 +483,13 @@ def test_bucket_sort_after_steps_and_joins(gcs, orc):
     radix while the others keep buckets, and then joins them.  Checked against
     the oracle on the current state after the steps and after the join, and
     against the radix sort (sort=1) on the same state."""
-    parts = [gcs.collision(n, 80 + n) for n in (1536, 3072, 12000)]
+    parts = [gcs.collision(n, 80 + n) for n in (768, 1536, 12000)]
     parts.append(cloud(gcs, 3073, 5))
+    # A dense core among 1,000 others: after the steps its keys are still
+    # nearly sorted, with buckets far narrower than the key space.
+    core = cloud(gcs, 25000, 6, spread=1e-3)
+    core[:1000] = cloud(gcs, 1000, 7)
+    parts.append(core)
     counts = [len(a) for a in parts]
     p = gcs.params(G=G, eps2=0.0025, dt=0.01, theta=0.5, leaf_size=8, use_graph=1)
     with gcs.Engine(sum(counts) + 20000, len(parts) + 1) as e:
