@@ -621,7 +621,7 @@ int enqueue_step(gcs_engine* e, const gcs_params& p, bool integrate)
                                                                e->bpart);
             e->bpart_valid = true;
         }
-        k_bounds_reduce<<<nseg, 256, 0, e->stream>>>(e->segs, e->bpart, e->seg_bounds);
+        k_bounds_reduce<<<nseg, 1024, 0, e->stream>>>(e->segs, e->bpart, e->seg_bounds);
     }
     StepReset rs;
     rs.nseg = nseg;
@@ -635,15 +635,17 @@ int enqueue_step(gcs_engine* e, const gcs_params& p, bool integrate)
     rs.chist_words = cwords;
     rs.gm = group_marks(e);
     rs.am = atile_marks(e);
+    // Instances 0 .. nbs - 1 are sorted with buckets, the rest by radix.
+    const int nbs = bucket_segs(e, p);
     k_morton<<<qblocks, kTileThreads, 0, e->stream>>>(e->tiles, e->segs, e->posm[c],
                                                        e->seg_bounds, p.use_bounds, p.global_min,
-                                                       p.global_max, e->keys[0], e->vals[0], rs);
+                                                       p.global_max, e->keys[0], e->vals[0], nbs,
+                                                       rs);
     LAUNCHCHK();
     if (e->capture_keys)
         HIPCHK(hipMemcpyAsync(e->keys_raw, e->keys[0], sizeof(uint64_t) * e->n_total,
                               hipMemcpyDeviceToDevice, e->stream));
     mark(e, 1);
-    const int nbs = bucket_segs(e, p);
     enqueue_bucket_sort(e, nbs);
     enqueue_sort_range(e, nbs, nseg - nbs);
     LAUNCHCHK();
@@ -690,12 +692,10 @@ int enqueue_step(gcs_engine* e, const gcs_params& p, bool integrate)
     // with many groups per wave the tail it shortens is small while the order
     // spreads a queue's concurrent groups over many instances' trees (L2
     // misses): 256 collisions of 4,096-65,536 particles ran 3.5% faster
-    // without it, 2 x 1,048,576 particles 2.3% slower.
+    // without it, 2 x 1,048,576 particles 2.3% slower.  Launched with the
+    // traversal, after the aggregation (the order is the traversal's).
     const int64_t gw = (int64_t)p.group_size * group_waves(e, p);
     const bool order = group && e->n_total >= 4 * gw && e->n_total < 8 * gw;
-    if (order)
-        k_group_order<<<kNumXcd, 1024, 0, e->stream>>>(e->groups, e->ngroups, p.group_size,
-                                                        e->gorder);
     mark(e, 4);
     const float inv_theta = p.theta > 0.0f ? 1.0f / p.theta : __builtin_inff();
     // Opening value of the merge: width (reference traversal), (w/theta)^2 or
@@ -720,6 +720,9 @@ int enqueue_step(gcs_engine* e, const gcs_params& p, bool integrate)
                                                   e->mfail, inv_theta, amode, two_level ? 1 : 0);
     LAUNCHCHK();
     mark(e, 5);
+    if (order)
+        k_group_order<<<kNumXcd, 1024, 0, e->stream>>>(e->groups, e->ngroups, p.group_size,
+                                                        e->gorder);
     if (group) {
         // Persistent: as many blocks as the per-wave memory covers (blocks *
         // waves per block <= trav_slots); waves that find no group left exit
@@ -1974,7 +1977,7 @@ int gcs_morton_keys(const float* x, const float* y, const float* z, int64_t n, f
         std::memset(&none, 0, sizeof(none));
         k_morton<<<qblocks, kTileThreads, 0, e->stream>>>(e->tiles, e->segs, e->posm[e->cur],
                                                            e->seg_bounds, 1, mn, mx, e->keys[0],
-                                                           e->vals[0], none);
+                                                           e->vals[0], 0, none);
         hipError_t le = hipGetLastError();
         if (le != hipSuccess) rc = fail(GCS_ERR_HIP, hipGetErrorString(le));
         else if (hipMemcpyAsync(keys, e->keys[0], 8 * (size_t)n, hipMemcpyDeviceToHost, e->stream) != hipSuccess ||

@@ -74,19 +74,32 @@ __global__ __launch_bounds__(256) void k_bounds(const TileInfo* __restrict__ til
     bounds_partial(q.valid ? posm[q.i] : make_float4(0.f, 0.f, 0.f, 0.f), q.valid, bpart);
 }
 
-__global__ __launch_bounds__(256) void k_bounds_reduce(const SegInfo* __restrict__ segs,
-                                                       const uint32_t* __restrict__ bpart,
-                                                       uint32_t* __restrict__ seg_bounds)
+__global__ __launch_bounds__(1024) void k_bounds_reduce(const SegInfo* __restrict__ segs,
+                                                        const uint32_t* __restrict__ bpart,
+                                                        uint32_t* __restrict__ seg_bounds)
 {
     const SegInfo sg = segs[blockIdx.x];
+    // (min, max) pairs of the instance's quarter tiles, eight loads in flight
+    // per thread.
+    const uint2* __restrict__ bp = reinterpret_cast<const uint2*>(bpart);
+    const int k0 = 4 * sg.tile0, k1 = 4 * (sg.tile0 + sg.ntiles);
     uint32_t lo = 0xFFFFFFFFu, hi = 0u;
-    for (int k = 4 * sg.tile0 + (int)threadIdx.x; k < 4 * (sg.tile0 + sg.ntiles); k += 256) {
-        lo = min(lo, bpart[2 * k]);
-        hi = max(hi, bpart[2 * k + 1]);
+    for (int k = k0 + (int)threadIdx.x; k < k1; k += 8 * 1024) {
+        uint2 v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int kk = k + u * 1024;
+            v[u] = kk < k1 ? bp[kk] : make_uint2(0xFFFFFFFFu, 0u);
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            lo = min(lo, v[u].x);
+            hi = max(hi, v[u].y);
+        }
     }
     lo = wave_min(lo);
     hi = wave_max(hi);
-    __shared__ uint32_t slo[4], shi[4];
+    __shared__ uint32_t slo[16], shi[16];
     const int w = threadIdx.x >> 6;
     if (lane_id() == 0) {
         slo[w] = lo;
@@ -94,8 +107,12 @@ __global__ __launch_bounds__(256) void k_bounds_reduce(const SegInfo* __restrict
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        seg_bounds[2 * blockIdx.x] = min(min(slo[0], slo[1]), min(slo[2], slo[3]));
-        seg_bounds[2 * blockIdx.x + 1] = max(max(shi[0], shi[1]), max(shi[2], shi[3]));
+        for (int q = 1; q < 16; ++q) {
+            slo[0] = min(slo[0], slo[q]);
+            shi[0] = max(shi[0], shi[q]);
+        }
+        seg_bounds[2 * blockIdx.x] = slo[0];
+        seg_bounds[2 * blockIdx.x + 1] = shi[0];
     }
 }
 
@@ -135,7 +152,8 @@ __global__ __launch_bounds__(256) void k_morton(const TileInfo* __restrict__ til
                                                 uint32_t* __restrict__ seg_bounds,
                                                 int use_given, float gmin, float gmax,
                                                 uint64_t* __restrict__ keys,
-                                                uint32_t* __restrict__ vals, StepReset rs)
+                                                uint32_t* __restrict__ vals, int vals_seg0,
+                                                StepReset rs)
 {
     const QuarterTile q = quarter_tile(tiles, segs);
     if (rs.chist) {
@@ -173,7 +191,8 @@ __global__ __launch_bounds__(256) void k_morton(const TileInfo* __restrict__ til
     }
     const float4 p = posm[q.i];
     keys[q.i] = morton_key(p.x, p.y, p.z, mn, mx);
-    vals[q.i] = (uint32_t)q.i;
+    // The radix sort's payload; the bucket sort writes its own.
+    if (q.seg >= vals_seg0) vals[q.i] = (uint32_t)q.i;
 }
 
 // karras_bvh.cuh:44-70 -- gather the state into sorted order (and, for the

@@ -124,7 +124,7 @@ typedef struct gcs_params {
                            32,768 particles and the instances number at most half
                            the CUs.  The node records are the same either way */
     int32_t sort;          /* key sort of a step: 1 = LSD radix sort (8 passes); 2 =
-                           bucket sort (splitters from the keys at every 3,072nd
+                           bucket sort (splitters from the keys at every 1,536th
                            place of the current order, buckets sorted in LDS), for
                            keys in nearly sorted order; 0 = automatic: buckets for
                            the instances stored in the last step's sorted order,
