@@ -276,10 +276,17 @@ __device__ __forceinline__ KeyWin stage_keys(const TileInfo* __restrict__ tiles,
     kw.n = sg.n;
     kw.lo = t.start - sg.start + (int)(blockIdx.x & 3) * kTileThreads -
             (kKeyWin - kTileThreads) / 2;
-    for (int k = threadIdx.x; k < kKeyWin; k += blockDim.x) {
-        const int j = kw.lo + k;
-        if (j >= 0 && j < kw.n) wk[k] = kw.g[j];
+    // All of a thread's loads in flight before the stores (keys outside the
+    // instance are never read: delta() rejects those places first).
+    constexpr int kPer = kKeyWin / kTileThreads;
+    uint64_t v[kPer];
+#pragma unroll
+    for (int r = 0; r < kPer; ++r) {
+        const int j = kw.lo + (int)threadIdx.x + r * kTileThreads;
+        v[r] = j >= 0 && j < kw.n ? kw.g[j] : 0ull;
     }
+#pragma unroll
+    for (int r = 0; r < kPer; ++r) wk[threadIdx.x + r * kTileThreads] = v[r];
     __syncthreads();
     return kw;
 }
