@@ -445,21 +445,6 @@ __global__ __launch_bounds__(kGroupChunk) void k_group_chunksum(const uint32_t* 
     }
 }
 
-// First number of this block's quarter tile among subtrees of one kind: the
-// subtrees of earlier chunks plus those of earlier quarter tiles of its chunk.
-__device__ __forceinline__ uint32_t quarter_base(const uint32_t* __restrict__ perq,
-                                                 const uint32_t* __restrict__ chunk, int b,
-                                                 uint32_t* ws)
-{
-    const int c = b / kGroupChunk;
-    uint32_t part = 0u;
-    for (int k = threadIdx.x; k < c; k += 256) part += chunk[k];
-    for (int k = c * kGroupChunk + (int)threadIdx.x; k < b; k += 256) part += perq[k];
-    uint32_t base;
-    block_exclusive_scan(part, &base, ws);
-    return base;
-}
-
 // Writes the group table and the aggregation-subtree table in particle
 // (Morton) order, and their lengths.  A subtree's number is its quarter
 // tile's first number plus the marks before its first particle within the
@@ -473,31 +458,65 @@ __global__ __launch_bounds__(256) void k_group_emit(const TileInfo* __restrict__
                                                     ATileInfo* __restrict__ atiles,
                                                     int* __restrict__ natiles, int n_total)
 {
-    __shared__ uint32_t ws[4];
+    static_assert(kTileThreads == 4 * kWave, "four waves per quarter tile");
+    __shared__ uint32_t ws[4][4];
     const QuarterTile q = quarter_tile(tiles, segs);
     const int b = blockIdx.x;
-    const uint32_t gbase = quarter_base(gm.perq, gchunk, b, ws);
-    const uint32_t abase = quarter_base(am.perq, achunk, b, ws);
+    const int tid = (int)threadIdx.x, l = lane_id(), w = tid >> 6;
+    // Every load first: the counts before this quarter tile (the chunks
+    // before its chunk, the quarter tiles before it in its chunk), its marks
+    // and the marked subtrees' sizes and roots.
+    const int c = b / kGroupChunk;
+    uint32_t pg = 0u, pa = 0u;
+    for (int k = tid; k < c; k += kTileThreads) {
+        pg += gchunk[k];
+        pa += achunk[k];
+    }
+    for (int k = c * kGroupChunk + tid; k < b; k += kTileThreads) {
+        pg += gm.perq[k];
+        pa += am.perq[k];
+    }
     const uint32_t m = q.valid ? gm.mark[q.i] : 0u;
     const uint32_t ma = q.valid ? am.mark[q.i] : 0u;
-    uint32_t nb, na;
-    const uint32_t ex = block_exclusive_scan(m, &nb, ws);
-    const uint32_t exa = block_exclusive_scan(ma, &na, ws);
+    const int gcount = q.valid ? gm.cnt[q.i] : 0;
+    const int acount = q.valid ? am.cnt[q.i] : 0;
+    const int aroot = q.valid ? am.root[q.i] : 0;
+    // One exchange: per wave the partial counts' sums and the marks' totals.
+    const uint32_t sg = (uint32_t)wave_sum((int)pg), sa = (uint32_t)wave_sum((int)pa);
+    const uint32_t im = (uint32_t)wave_scan_incl_dpp((int)m);
+    const uint32_t ima = (uint32_t)wave_scan_incl_dpp((int)ma);
+    if (l == kWave - 1) {
+        ws[0][w] = sg;
+        ws[1][w] = sa;
+        ws[2][w] = im;
+        ws[3][w] = ima;
+    }
+    __syncthreads();
+    uint32_t gbase = 0u, abase = 0u, bm = 0u, bma = 0u, nb = 0u, na = 0u;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        gbase += ws[0][k];
+        abase += ws[1][k];
+        nb += ws[2][k];
+        na += ws[3][k];
+        bm += k < w ? ws[2][k] : 0u;
+        bma += k < w ? ws[3][k] : 0u;
+    }
     if (m) {
         GroupInfo g;
         g.first = q.i;
-        g.count = gm.cnt[q.i];
+        g.count = gcount;
         g.seg = q.seg;
         g.pad = 0;
-        groups[gbase + ex] = g;
+        groups[gbase + bm + im - m] = g;
     }
     if (ma) {
         ATileInfo a;
         a.first = q.i;
-        a.count = am.cnt[q.i];
+        a.count = acount;
         a.seg = q.seg;
-        a.root = am.root[q.i];
-        atiles[abase + exa] = a;
+        a.root = aroot;
+        atiles[abase + bma + ima - ma] = a;
     }
     if (q.valid && q.i == n_total - 1) {
         *ngroups = (int)(gbase + nb);
