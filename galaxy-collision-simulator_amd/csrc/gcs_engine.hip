@@ -147,6 +147,7 @@ struct gcs_engine {
     // Bucket sort: buckets [bkt0[s], bkt0[s+1]) of instance s, their
     // splitters, counts, offsets and fill counters, and each key's bucket.
     int* bkt0 = nullptr;
+    int* bseg = nullptr;  // per bucket: its instance
     uint64_t* bsplit = nullptr;
     uint64_t* bcand = nullptr;
     uint32_t* bcnt = nullptr;
@@ -249,7 +250,7 @@ void free_all(gcs_engine* e)
     dfree(e->gmark); dfree(e->gq); dfree(e->gorder); dfree(e->gchunk); dfree(e->gcnt); dfree(e->ngroups);
     dfree(e->front); dfree(e->nfront);
     dfree(e->tlist); dfree(e->ntop); dfree(e->tslot); dfree(e->mids); dfree(e->nmid); dfree(e->mfail);
-    dfree(e->bkt0); dfree(e->bsplit); dfree(e->bcand); dfree(e->bcnt); dfree(e->boff); dfree(e->bfill); dfree(e->bid);
+    dfree(e->bkt0); dfree(e->bseg); dfree(e->bsplit); dfree(e->bcand); dfree(e->bcnt); dfree(e->boff); dfree(e->bfill); dfree(e->bid);
     dfree(e->seg_bounds); dfree(e->bpart); dfree(e->qctr); dfree(e->counters); dfree(e->wcount); dfree(e->err);
     dfree(e->diag_part); dfree(e->diag_out); dfree(e->amax_part); dfree(e->dt_inst); dfree(e->tsim); dfree(e->slot_mem); dfree(e->staging); dfree(e->mn_dev);
     for (auto& v : e->ev)
@@ -305,11 +306,16 @@ int rebuild_tables(gcs_engine* e)
     e->n_total = start;
     e->nchunks = chunks;
     e->h_bkt0.assign(1, 0);
-    for (const SegInfo& sg : e->h_segs)
-        e->h_bkt0.push_back(e->h_bkt0.back() + std::max(1, (sg.n + kBucketKeys - 1) / kBucketKeys));
+    std::vector<int> bseg;
+    for (size_t s = 0; s < e->h_segs.size(); ++s) {
+        const int nb = std::max(1, (e->h_segs[s].n + kBucketKeys - 1) / kBucketKeys);
+        e->h_bkt0.push_back(e->h_bkt0.back() + nb);
+        bseg.insert(bseg.end(), (size_t)nb, (int)s);
+    }
     if (!e->h_segs.empty()) {
         HIPCHK(hipMemcpy(e->bkt0, e->h_bkt0.data(), sizeof(int) * e->h_bkt0.size(),
                          hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(e->bseg, bseg.data(), sizeof(int) * bseg.size(), hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(e->segs, e->h_segs.data(), sizeof(SegInfo) * e->h_segs.size(),
                          hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(e->tiles, e->h_tiles.data(), sizeof(TileInfo) * e->h_tiles.size(),
@@ -493,8 +499,9 @@ void enqueue_bucket_sort(gcs_engine* e, int nseg)
     const int ntiles = nseg < (int)e->h_segs.size() ? e->h_segs[(size_t)nseg].tile0
                                                     : (int)e->h_tiles.size();
     const int nbkt = e->h_bkt0[(size_t)nseg];
-    k_bucket_cand<<<(nbkt + 255) / 256, 256, 0, e->stream>>>(e->segs, e->bkt0, nseg, e->keys[0],
-                                                             e->bcand, e->bcnt, e->bfill);
+    k_bucket_cand<<<(nbkt + 255) / 256, 256, 0, e->stream>>>(e->segs, e->bkt0, e->bseg, nbkt,
+                                                             e->keys[0], e->bcand, e->bcnt,
+                                                             e->bfill);
     k_bucket_split<<<nseg, 1024, 0, e->stream>>>(e->bkt0, e->bcand, e->bsplit);
     k_bucket_count<<<ntiles, kTileThreads, 0, e->stream>>>(e->tiles, e->segs, e->bkt0, e->keys[0],
                                                            e->bsplit, e->bid, e->bcnt);
@@ -952,7 +959,7 @@ int64_t gcs_device_bytes(int64_t cap_n, int32_t cap_k)
     b += 4 * 4 * (int64_t)max_tiles(cap_n, cap_k) + 12 * (int64_t)cap_k;  // amax, dt, time
     b += 4 * n + 8 * n + 4 * (int64_t)cap_k;  // nodes above the aggregation subtrees
     b += 8 * (n / kATile + (int64_t)cap_k + 1) + 4 + 4 * (int64_t)cap_k;  // mid-level roots
-    b += 4 * ((int64_t)cap_k + 1) + 28 * (int64_t)max_buckets(cap_n, cap_k) + 4 * n;  // buckets
+    b += 4 * ((int64_t)cap_k + 1) + 32 * (int64_t)max_buckets(cap_n, cap_k) + 4 * n;  // buckets
     b += 32 * n;               // staging (also the device generator's scratch)
     b += 8 * ((gcs_gen::kNR + 1) + (gcs_gen::kNZ + 1) + 2 * gcs_gen::kNR * gcs_gen::kNZ);
     return b;
@@ -1022,6 +1029,7 @@ int gcs_create(int64_t cap_n, int32_t cap_k, int32_t device, gcs_engine** out)
     if (!rc) rc = dalloc(&e->nmid, 1);
     if (!rc) rc = dalloc(&e->mfail, (size_t)cap_k);
     if (!rc) rc = dalloc(&e->bkt0, (size_t)cap_k + 1);
+    if (!rc) rc = dalloc(&e->bseg, max_buckets(cap_n, cap_k));
     if (!rc) rc = dalloc(&e->bsplit, max_buckets(cap_n, cap_k));
     if (!rc) rc = dalloc(&e->bcand, max_buckets(cap_n, cap_k));
     if (!rc) rc = dalloc(&e->bcnt, max_buckets(cap_n, cap_k));

@@ -293,25 +293,21 @@ __host__ __device__ __forceinline__ uint64_t split_candidate(const uint64_t* k, 
     return median9(a);
 }
 
-// Candidate of every bucket (one thread each, over all instances), and the
-// bucket counters cleared.
+// Candidate of every bucket b < nbkt (one thread each, over all instances;
+// bseg[b] = its instance), and the bucket counters cleared.
 __global__ __launch_bounds__(256) void k_bucket_cand(const SegInfo* __restrict__ segs,
-                                                     const int* __restrict__ bkt0, int nseg,
+                                                     const int* __restrict__ bkt0,
+                                                     const int* __restrict__ bseg, int nbkt,
                                                      const uint64_t* __restrict__ keys,
                                                      uint64_t* __restrict__ cand,
                                                      uint32_t* __restrict__ bcnt,
                                                      uint32_t* __restrict__ bfill)
 {
     const int b = (int)(blockIdx.x * blockDim.x + threadIdx.x);
-    if (b >= bkt0[nseg]) return;
-    int lo = 0, hi = nseg - 1;  // the instance: the last s with bkt0[s] <= b
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (bkt0[mid] <= b) lo = mid;
-        else hi = mid - 1;
-    }
-    const SegInfo sg = segs[lo];
-    cand[b] = split_candidate(keys + sg.start, sg.n, b - bkt0[lo], bkt0[lo + 1] - bkt0[lo]);
+    if (b >= nbkt) return;
+    const int s = bseg[b];
+    const SegInfo sg = segs[s];
+    cand[b] = split_candidate(keys + sg.start, sg.n, b - bkt0[s], bkt0[s + 1] - bkt0[s]);
     bcnt[b] = 0u;
     bfill[b] = 0u;
 }
@@ -434,9 +430,16 @@ __global__ __launch_bounds__(kTileThreads) void k_bucket_count(
     __shared__ uint64_t ls[kBucketWin + 1];
     __shared__ uint32_t lc[kBucketWin];
     const TileInfo t = tiles[blockIdx.x];
+    const int tid = (int)threadIdx.x;
+    // The tile's keys in flight while the window is staged.
+    uint64_t key[kSortItems];
+#pragma unroll
+    for (int r = 0; r < kSortItems; ++r) {
+        const int idx = r * kTileThreads + tid;
+        key[r] = idx < t.count ? keys[t.start + idx] : 0ull;
+    }
     const SegInfo sg = segs[t.seg];
     const int b0 = bkt0[t.seg], nb = bkt0[t.seg + 1] - b0;
-    const int tid = (int)threadIdx.x;
     int wbase, wn;
     bucket_window(t.start - sg.start, nb, wbase, wn);
     load_window(ls, split + b0, wbase, wn, nb);
@@ -446,7 +449,7 @@ __global__ __launch_bounds__(kTileThreads) void k_bucket_count(
     for (int r = 0; r < kSortItems; ++r) {
         const int idx = r * kTileThreads + tid;
         if (idx < t.count) {
-            const uint64_t k = keys[t.start + idx];
+            const uint64_t k = key[r];
             bool inwin;
             const int b = bucket_of(k, ls, wbase, wn, split + b0, nb, inwin);
             if (inwin) atomicAdd(&lc[b - wbase], 1u);
@@ -500,26 +503,35 @@ __global__ __launch_bounds__(kTileThreads) void k_bucket_scatter(
     __shared__ uint32_t lc[kBucketWin];
     __shared__ uint32_t lb[kBucketWin];
     const TileInfo t = tiles[blockIdx.x];
-    const SegInfo sg = segs[t.seg];
-    const int b0 = bkt0[t.seg], nb = bkt0[t.seg + 1] - b0;
     const int tid = (int)threadIdx.x;
-    int wbase, wn;
-    bucket_window(t.start - sg.start, nb, wbase, wn);
-    for (int w = tid; w < wn; w += kTileThreads) lc[w] = 0u;
-    __syncthreads();
+    // The tile's keys and buckets first, then the buckets' offsets, all in
+    // flight while the window is set up and the ranks are taken.
     uint64_t key[kSortItems];
-    uint32_t b[kSortItems], rank[kSortItems];
+    uint32_t b[kSortItems], rank[kSortItems], base[kSortItems];
     bool win[kSortItems];
 #pragma unroll
     for (int r = 0; r < kSortItems; ++r) {
         const int idx = r * kTileThreads + tid;
         win[r] = false;
-        b[r] = 0u;
         rank[r] = 0u;
-        key[r] = 0ull;
+        key[r] = idx < t.count ? keys[t.start + idx] : 0ull;
+        b[r] = idx < t.count ? bid[t.start + idx] : 0u;
+    }
+#pragma unroll
+    for (int r = 0; r < kSortItems; ++r) {
+        const int idx = r * kTileThreads + tid;
+        base[r] = idx < t.count ? boff[b[r]] : 0u;
+    }
+    const SegInfo sg = segs[t.seg];
+    const int b0 = bkt0[t.seg], nb = bkt0[t.seg + 1] - b0;
+    int wbase, wn;
+    bucket_window(t.start - sg.start, nb, wbase, wn);
+    for (int w = tid; w < wn; w += kTileThreads) lc[w] = 0u;
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < kSortItems; ++r) {
+        const int idx = r * kTileThreads + tid;
         if (idx < t.count) {
-            key[r] = keys[t.start + idx];
-            b[r] = bid[t.start + idx];
             const int w = (int)b[r] - b0 - wbase;
             win[r] = w >= 0 && w < wn;
             rank[r] = win[r] ? atomicAdd(&lc[w], 1u) : atomicAdd(&bfill[b[r]], 1u);
@@ -534,7 +546,7 @@ __global__ __launch_bounds__(kTileThreads) void k_bucket_scatter(
         const int idx = r * kTileThreads + tid;
         if (idx < t.count) {
             const uint32_t pos =
-                boff[b[r]] + rank[r] + (win[r] ? lb[(int)b[r] - b0 - wbase] : 0u);
+                base[r] + rank[r] + (win[r] ? lb[(int)b[r] - b0 - wbase] : 0u);
             kout[pos] = key[r];
             vout[pos] = (uint32_t)(t.start + idx);
         }
