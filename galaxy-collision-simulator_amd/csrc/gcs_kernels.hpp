@@ -818,23 +818,30 @@ __global__ __launch_bounds__(kATile) __attribute__((amdgpu_waves_per_eu(8))) voi
         const int v = ibase + tid;
         bool member = false;
         int first = 0, last = 0, split = 0, pv = -1;
+        float4 pl = make_float4(0.f, 0.f, 0.f, 0.f), pr = pl;
         if (tid < c && t0 - sg.start + tid <= sg.n - 2) {
+            // Two round trips: the node and its parent link, then the
+            // particles at split and split + 1 (inside the node's range, so
+            // inside the instance) whether or not the children are leaves.
             const float4 bb = node[2 * v + 1];
+            const int pvr = parent[v];
             first = __float_as_int(bb.y);
             last = __float_as_int(bb.z);
             split = __float_as_int(bb.w);
+            pl = posm[split];
+            pr = posm[split + 1];
             member = (first >= t0 && last <= e0) || (two && first > e0 && last <= t0 + c - 1);
             const bool root = (t.count > 1 && v == t.root) || (two && a1.count > 1 && v == a1.root);
-            if (member && !root) pv = parent[v];
+            if (member && !root) pv = pvr;
         }
         if (member) {
             const bool lleaf = first == split, rleaf = last == split + 1;
             ltopo[tid] = topo_word(split - t0, lleaf, rleaf, pv < 0 ? -1 : pv - ibase);
             if (lleaf && rleaf) {
-                lds_put(ldat, lqt, tid, merge(leaf_data(posm[split]), leaf_data(posm[split + 1])));
+                lds_put(ldat, lqt, tid, merge(leaf_data(pl), leaf_data(pr)));
                 climb = true;
             } else if (lleaf || rleaf) {
-                lds_put(ldat, lqt, tid, leaf_data(posm[lleaf ? split : split + 1]));
+                lds_put(ldat, lqt, tid, leaf_data(lleaf ? pl : pr));
             }
         }
         __syncthreads();  // topology, parked leaves and the first merges are in LDS
@@ -1048,26 +1055,20 @@ __device__ __forceinline__ NodeData child_data(int idx, bool leaf, const SegInfo
                                                const float4* node, const float4* nodeQ,
                                                const float4* boxLo, const float4* boxHi)
 {
-    // The five vectors of either record, then one conversion (a choice
-    // between two whole records would go through private memory).
-    float4 A, lo, hi, qa, qb;
-    if (leaf) {
-        A = posm[idx - sg.leaf_base];
-        lo = hi = make_float4(A.x, A.y, A.z, 0.0f);
-        qa = qb = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    } else {
-        A = node[2 * idx];
-        lo = boxLo[idx];
-        hi = boxHi[idx];
-        qa = nodeQ[2 * idx];
-        qb = nodeQ[2 * idx + 1];
-    }
+    // Both forms are read and one is chosen, so no branch holds back the
+    // next loads: the node arrays have an entry for every node index, leaves
+    // included, and an internal child reads the instance's first particle.
+    const float4 P = posm[leaf ? idx - sg.leaf_base : sg.start];
+    const float4 A = node[2 * idx], lo = boxLo[idx], hi = boxHi[idx];
+    const float4 qa = nodeQ[2 * idx], qb = nodeQ[2 * idx + 1];
     NodeData d;
-    d.cx = A.x; d.cy = A.y; d.cz = A.z; d.m = A.w;
-    d.lx = lo.x; d.ly = lo.y; d.lz = lo.z;
-    d.hx = hi.x; d.hy = hi.y; d.hz = hi.z;
-    d.q0 = qa.x; d.q1 = qa.y; d.q2 = qa.z; d.q3 = qa.w;
-    d.q4 = qb.x; d.q5 = qb.y; d.qt = qb.w;
+    d.cx = leaf ? P.x : A.x; d.cy = leaf ? P.y : A.y; d.cz = leaf ? P.z : A.z;
+    d.m = leaf ? P.w : A.w;
+    d.lx = leaf ? P.x : lo.x; d.ly = leaf ? P.y : lo.y; d.lz = leaf ? P.z : lo.z;
+    d.hx = leaf ? P.x : hi.x; d.hy = leaf ? P.y : hi.y; d.hz = leaf ? P.z : hi.z;
+    d.q0 = leaf ? 0.0f : qa.x; d.q1 = leaf ? 0.0f : qa.y; d.q2 = leaf ? 0.0f : qa.z;
+    d.q3 = leaf ? 0.0f : qa.w; d.q4 = leaf ? 0.0f : qb.x; d.q5 = leaf ? 0.0f : qb.y;
+    d.qt = leaf ? 0.0f : qb.w;
     return d;
 }
 
@@ -1108,36 +1109,42 @@ __device__ __forceinline__ void top_lds(TopLds& T, const SegInfo& sg, int nt, co
         const int v = list[k];
         vk[r] = v;
         const float4 bb = node[2 * v + 1];
+        const int pv = parent[v];
         const int first = __float_as_int(bb.y), last = __float_as_int(bb.z),
                   split = __float_as_int(bb.w);
-        const int pv = parent[v];
         const bool lleaf = first == split, rleaf = last == split + 1;
         const int lidx = lleaf ? split + sg.leaf_base : split + sg.int_base;
         const int ridx = rleaf ? split + 1 + sg.leaf_base : split + 1 + sg.int_base;
         const int nl = split - first + 1, nr = last - split;
+        // Every load that needs the node's record or its parent link, in one
+        // round trip before the first use: the parent's range (phase test),
+        // the slots of the parent and the children (tslot has an entry for
+        // every node; the unused ones are not read further), and both
+        // children's records (a listed child's is taken from LDS instead).
+        const int pq = pv >= 0 ? pv : v;
+        const float4 pb = node[2 * pq + 1];
+        const int sp = tslot[pq], sl = tslot[lidx], sr = tslot[ridx];
+        const NodeData dl = child_data(lidx, lleaf, sg, posm, node, nodeQ, boxLo, boxHi);
+        const NodeData dr = child_data(ridx, rleaf, sg, posm, node, nodeQ, boxLo, boxHi);
         bool ltop = nl > kATile, rtop = nr > kATile, pin = pv >= 0;
         if (phase_of) {
             ltop = ltop && phase_of[top_bucket(nl)] == phase;
             rtop = rtop && phase_of[top_bucket(nr)] == phase;
-            if (pin) {
-                const float4 pb = node[2 * pv + 1];
-                pin = phase_of[top_bucket(__float_as_int(pb.z) - __float_as_int(pb.y) + 1)] ==
-                      phase;
-            }
+            pin = pin &&
+                  phase_of[top_bucket(__float_as_int(pb.z) - __float_as_int(pb.y) + 1)] == phase;
         }
-        T.par[k] = (uint16_t)(pin ? tslot[pv] + 1 : 0);
-        T.cl[k] = (uint16_t)(ltop ? tslot[lidx] + 1 : 0);
-        T.cr[k] = (uint16_t)(rtop ? tslot[ridx] + 1 : 0);
+        T.par[k] = (uint16_t)(pin ? sp + 1 : 0);
+        T.cl[k] = (uint16_t)(ltop ? sl + 1 : 0);
+        T.cr[k] = (uint16_t)(rtop ? sr + 1 : 0);
         if (!ltop || !rtop) {
             // The child (or the left of two) that is not listed: merged with
             // the other when both are not listed, else parked.
-            NodeData a = child_data(ltop ? ridx : lidx, ltop ? rleaf : lleaf, sg, posm, node, nodeQ,
-                                    boxLo, boxHi);
             if (!ltop && !rtop) {
-                a = merge(a, child_data(ridx, rleaf, sg, posm, node, nodeQ, boxLo, boxHi));
+                lds_put(T.dat, T.qt, k, merge(dl, dr));
                 start[r] = true;
+            } else {
+                lds_put(T.dat, T.qt, k, ltop ? dr : dl);
             }
-            lds_put(T.dat, T.qt, k, a);
         }
     }
     __syncthreads();  // topology, parked children and the first merges are in LDS
@@ -1234,14 +1241,25 @@ __global__ __launch_bounds__(1024) void k_aggregate_top(
         __syncthreads();
         int* const plist = front + 2 * (size_t)sg.start + sg.n;
         const int nt = ntop[s];
-        for (int k = tid; k < nt; k += blockDim.x) {
+        // Two listed nodes per thread and round, their loads in flight together.
+        for (int k = tid; k < nt; k += 2 * (int)blockDim.x) {
+            const int k2 = k + (int)blockDim.x;
             const int v = tlist[sg.start + k];
+            const int v2 = tlist[sg.start + (k2 < nt ? k2 : k)];
             const float4 bb = node[2 * v + 1];
+            const float4 bb2 = node[2 * v2 + 1];
             if (__float_as_int(bb.z) - __float_as_int(bb.y) + 1 >= kMidSize) {
                 const int slot = atomicAdd(&lcount, 1);
                 if (slot < kTopCap) {
                     plist[slot] = v;
                     tslot[v] = slot;
+                }
+            }
+            if (k2 < nt && __float_as_int(bb2.z) - __float_as_int(bb2.y) + 1 >= kMidSize) {
+                const int slot = atomicAdd(&lcount, 1);
+                if (slot < kTopCap) {
+                    plist[slot] = v2;
+                    tslot[v2] = slot;
                 }
             }
         }
@@ -1386,13 +1404,23 @@ __global__ __launch_bounds__(1024) void k_aggregate_mid(
         if (tid == 0) lcount = 0;
         __syncthreads();  // the previous root's use of the LDS is over
         const int nt = ntop[s];
-        for (int k = tid; k < nt; k += blockDim.x) {
+        // Two listed nodes per thread and round: both lookups, then both
+        // ranges, in flight together.
+        for (int k = tid; k < nt; k += 2 * (int)blockDim.x) {
+            const int k2 = k + (int)blockDim.x;
             const int v = tlist[sg.start + k];
+            const int v2 = tlist[sg.start + (k2 < nt ? k2 : k)];
             const float4 bb = node[2 * v + 1];
+            const float4 bb2 = node[2 * v2 + 1];
             if (__float_as_int(bb.y) >= rf && __float_as_int(bb.z) <= rl) {
                 const int slot = atomicAdd(&lcount, 1);
                 mlist[slot] = v;
                 tslot[v] = slot;
+            }
+            if (k2 < nt && __float_as_int(bb2.y) >= rf && __float_as_int(bb2.z) <= rl) {
+                const int slot = atomicAdd(&lcount, 1);
+                mlist[slot] = v2;
+                tslot[v2] = slot;
             }
         }
         __syncthreads();  // the list and its slots are visible to the block
