@@ -184,12 +184,11 @@ __global__ __launch_bounds__(256) void k_morton(const TileInfo* __restrict__ til
         }
     }
     if (!q.valid) return;
-    float mn = gmin, mx = gmax;
-    if (!use_given) {
-        mn = fdec(seg_bounds[2 * q.seg]);
-        mx = fdec(seg_bounds[2 * q.seg + 1]);
-    }
+    // The instance's bounds and the position in one round trip (with given
+    // bounds the loaded pair is not used).
+    const uint2 sb = reinterpret_cast<const uint2*>(seg_bounds)[q.seg];
     const float4 p = posm[q.i];
+    const float mn = use_given ? gmin : fdec(sb.x), mx = use_given ? gmax : fdec(sb.y);
     keys[q.i] = morton_key(p.x, p.y, p.z, mn, mx);
     // The radix sort's payload; the bucket sort writes its own.
     if (q.seg >= vals_seg0) vals[q.i] = (uint32_t)q.i;
@@ -208,11 +207,11 @@ struct PermuteArgs {
     int compensated, write_leaves;
 };
 
-__device__ __forceinline__ void permute_one(const QuarterTile& q, const SegInfo* __restrict__ segs,
+__device__ __forceinline__ void permute_one(const QuarterTile& q, int src,
+                                            const SegInfo* __restrict__ segs,
                                             const PermuteArgs& pa, float4* __restrict__ node,
                                             float4* __restrict__ nodeQ)
 {
-    const uint32_t* __restrict__ perm = pa.perm;
     const float4* __restrict__ posm_in = pa.posm_in;
     const float4* __restrict__ vel_in = pa.vel_in;
     const float4* __restrict__ plo_in = pa.plo_in;
@@ -220,11 +219,15 @@ __device__ __forceinline__ void permute_one(const QuarterTile& q, const SegInfo*
     float4* __restrict__ vel_out = pa.vel_out;
     float4* __restrict__ plo_out = pa.plo_out;
     const int compensated = pa.compensated, write_leaves = pa.write_leaves;
-    const int src = (int)perm[q.i];
+    // Every gather before the first store: the streams come through a
+    // struct, so the compiler cannot tell the outputs from the inputs, and a
+    // store placed first would hold back the loads after it.
     const float4 p = posm_in[src];
+    const float4 vv = vel_in[src];
+    const float4 lo = compensated ? plo_in[src] : make_float4(0.f, 0.f, 0.f, 0.f);
     posm_out[q.i] = p;
-    vel_out[q.i] = vel_in[src];
-    if (compensated) plo_out[q.i] = plo_in[src];
+    vel_out[q.i] = vv;
+    if (compensated) plo_out[q.i] = lo;
     // The group traversal reads no leaf records; an instance of one particle
     // gets its leaf (= root) record anyway, the root the COM readback copies.
     if (write_leaves || segs[q.seg].n == 1) {
@@ -415,10 +418,13 @@ __global__ __launch_bounds__(256) void k_permute_build(
     int* __restrict__ parent, int* __restrict__ flag, Marks gm, Marks am, int gmax, TopList tl)
 {
     __shared__ uint64_t wk[kKeyWin];
-    const KeyWin kw = stage_keys(tiles, segs, skeys, wk);
     const QuarterTile q = quarter_tile(tiles, segs);
+    // The particle's place in the old order goes out with the key window's
+    // loads (one wait for both; a lane past the tile's end reads entry 0).
+    const int src = (int)pa.perm[q.valid ? q.i : 0];
+    const KeyWin kw = stage_keys(tiles, segs, skeys, wk);
     if (!q.valid) return;
-    permute_one(q, segs, pa, node, nodeQ);
+    permute_one(q, src, segs, pa, node, nodeQ);
     build_one(q, segs, kw, node, parent, flag, gm, am, gmax, tl);
 }
 
@@ -432,8 +438,12 @@ __global__ __launch_bounds__(kGroupChunk) void k_group_chunksum(const uint32_t* 
     static_assert(kGroupChunk == 2 * kWave, "two waves per chunk");
     __shared__ uint32_t w1[2];
     const int q = blockIdx.x * kGroupChunk + (int)threadIdx.x;
-    const uint32_t v = (uint32_t)wave_sum(q < nq ? (int)gq[q] : 0);
-    const uint32_t u = (uint32_t)wave_sum(q < nq ? (int)aq[q] : 0);
+    // Both counts in flight together (a lane past the end reads entry 0).
+    const int qc = q < nq ? q : 0;
+    const int gv = (int)gq[qc], av = (int)aq[qc];
+    __builtin_amdgcn_sched_barrier(0);  // keep the two loads ahead of their uses
+    const uint32_t v = (uint32_t)wave_sum(q < nq ? gv : 0);
+    const uint32_t u = (uint32_t)wave_sum(q < nq ? av : 0);
     if (threadIdx.x == kWave) {
         w1[0] = v;
         w1[1] = u;
@@ -459,6 +469,7 @@ __global__ __launch_bounds__(256) void k_group_emit(const TileInfo* __restrict__
                                                     int* __restrict__ natiles, int n_total)
 {
     static_assert(kTileThreads == 4 * kWave, "four waves per quarter tile");
+    static_assert(kGroupChunk <= kTileThreads, "a chunk's quarter tiles fit one pass");
     __shared__ uint32_t ws[4][4];
     const QuarterTile q = quarter_tile(tiles, segs);
     const int b = blockIdx.x;
@@ -466,21 +477,27 @@ __global__ __launch_bounds__(256) void k_group_emit(const TileInfo* __restrict__
     // Every load first: the counts before this quarter tile (the chunks
     // before its chunk, the quarter tiles before it in its chunk), its marks
     // and the marked subtrees' sizes and roots.
-    const int c = b / kGroupChunk;
-    uint32_t pg = 0u, pa = 0u;
-    for (int k = tid; k < c; k += kTileThreads) {
+    // In one round trip (places clamped to 0 where a thread has nothing to
+    // read; those values are not used): the thread's first chunk sum, its
+    // quarter tile count (the quarter tiles before this one in its chunk are
+    // fewer than the block's threads), this quarter tile's marks and the
+    // marked subtrees' sizes and roots.
+    const int c = b / kGroupChunk, kq = c * kGroupChunk + tid;
+    const bool hc = tid < c, hq = kq < b;
+    const int qi = q.valid ? q.i : 0;
+    const uint32_t gc = gchunk[hc ? tid : 0], ac = achunk[hc ? tid : 0];
+    const uint32_t gp = gm.perq[hq ? kq : 0], ap = am.perq[hq ? kq : 0];
+    const uint32_t gmk = gm.mark[qi], amk = am.mark[qi];
+    const int gcount = gm.cnt[qi], acount = am.cnt[qi], aroot = am.root[qi];
+    __builtin_amdgcn_sched_barrier(0);  // keep the loads ahead of their uses
+    uint32_t pg = (hc ? gc : 0u) + (hq ? gp : 0u);
+    uint32_t pa = (hc ? ac : 0u) + (hq ? ap : 0u);
+    for (int k = tid + kTileThreads; k < c; k += kTileThreads) {
         pg += gchunk[k];
         pa += achunk[k];
     }
-    for (int k = c * kGroupChunk + tid; k < b; k += kTileThreads) {
-        pg += gm.perq[k];
-        pa += am.perq[k];
-    }
-    const uint32_t m = q.valid ? gm.mark[q.i] : 0u;
-    const uint32_t ma = q.valid ? am.mark[q.i] : 0u;
-    const int gcount = q.valid ? gm.cnt[q.i] : 0;
-    const int acount = q.valid ? am.cnt[q.i] : 0;
-    const int aroot = q.valid ? am.root[q.i] : 0;
+    const uint32_t m = q.valid ? gmk : 0u;
+    const uint32_t ma = q.valid ? amk : 0u;
     // One exchange: per wave the partial counts' sums and the marks' totals.
     const uint32_t sg = (uint32_t)wave_sum((int)pg), sa = (uint32_t)wave_sum((int)pa);
     const uint32_t im = (uint32_t)wave_scan_incl_dpp((int)m);
@@ -1530,18 +1547,22 @@ __global__ __launch_bounds__(256) void k_integrate(
     const float dt = dt_inst ? dt_inst[q.seg] : dt_fixed;
     float4 pn = make_float4(0.f, 0.f, 0.f, 0.f);  // the new position
     double d[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    // Every stream of the particle in one round trip, before the block's
+    // maximum is taken (a lane past the tile's end reads particle 0 and uses
+    // none of it).
+    const int qi = q.valid ? q.i : 0;
+    const float4 a = acc[qi];
+    float4 v = vel[qi];
+    float4 p = posm[qi];
+    float4 lo = compensated ? plo[qi] : make_float4(0.f, 0.f, 0.f, 0.f);
     {
-        const float4 a0 = q.valid ? acc[q.i] : make_float4(0.f, 0.f, 0.f, 0.f);
-        const float m = block_max_accel(a0, q.valid);
+        const float m = block_max_accel(a, q.valid);
         if (threadIdx.x == 0) {
             amax_part[blockIdx.x] = m;
             if ((int)blockIdx.x == 4 * segs[q.seg].tile0) tsim[q.seg] += (double)dt;
         }
     }
     if (q.valid) {
-        const float4 a = acc[q.i];
-        float4 v = vel[q.i];
-        float4 p = posm[q.i];
         if (!(isfinite(a.x) && isfinite(a.y) && isfinite(a.z))) atomicOr(err, kErrNonFinite);
         if (diagnostics) {
             const double hdt = 0.5 * (double)dt;
@@ -1559,7 +1580,6 @@ __global__ __launch_bounds__(256) void k_integrate(
         v.y = __builtin_fmaf(a.y, dt, v.y);
         v.z = __builtin_fmaf(a.z, dt, v.z);
         if (compensated) {
-            float4 lo = plo[q.i];
             const double X = (double)p.x + ((double)lo.x + (double)v.x * (double)dt);
             const double Y = (double)p.y + ((double)lo.y + (double)v.y * (double)dt);
             const double Z = (double)p.z + ((double)lo.z + (double)v.z * (double)dt);

@@ -279,18 +279,19 @@ __host__ __device__ __forceinline__ uint64_t median9(uint64_t* a)
 // = 0 and +inf for b = nb.  Keys that crossed a coarse octree boundary
 // together move as runs of neighbours in the current order; nine points
 // 256 places either side keep the median off such a run.
+// The nine keys are read whatever b is (places clamped into the instance,
+// n >= 1) and the result chosen after, so no branch holds the loads back.
 __host__ __device__ __forceinline__ uint64_t split_candidate(const uint64_t* k, int n, int b,
                                                              int nb)
 {
-    if (b <= 0) return 0ull;
-    if (b >= nb) return ~0ull;
     uint64_t a[9];
 #pragma unroll
     for (int m = 0; m < 9; ++m) {
         const int q = b * kBucketKeys + 64 * (m - 4);
         a[m] = k[q < 0 ? 0 : (q >= n ? n - 1 : q)];
     }
-    return median9(a);
+    const uint64_t med = median9(a);
+    return b <= 0 ? 0ull : (b >= nb ? ~0ull : med);
 }
 
 // Candidate of every bucket b < nbkt (one thread each, over all instances;
@@ -307,7 +308,8 @@ __global__ __launch_bounds__(256) void k_bucket_cand(const SegInfo* __restrict__
     if (b >= nbkt) return;
     const int s = bseg[b];
     const SegInfo sg = segs[s];
-    cand[b] = split_candidate(keys + sg.start, sg.n, b - bkt0[s], bkt0[s + 1] - bkt0[s]);
+    const int lo = bkt0[s], hi = bkt0[s + 1];
+    cand[b] = split_candidate(keys + sg.start, sg.n, b - lo, hi - lo);
     bcnt[b] = 0u;
     bfill[b] = 0u;
 }
@@ -353,6 +355,11 @@ __global__ __launch_bounds__(1024) void k_bucket_split(const int* __restrict__ b
     uint64_t carry = 0ull;
     for (int c0 = 0; c0 < nb; c0 += 4 * 1024) {
         const int bb = c0 + 4 * (int)threadIdx.x;
+        // The six candidates around the thread's four buckets in one round
+        // trip (places clamped into the instance; the clamped ones are not used).
+        uint64_t w[6];
+#pragma unroll
+        for (int e = 0; e < 6; ++e) w[e] = c[min(max(bb - 1 + e, 0), nb - 1)];
         uint64_t v[4];
         uint64_t run = 0ull;
 #pragma unroll
@@ -360,9 +367,9 @@ __global__ __launch_bounds__(1024) void k_bucket_split(const int* __restrict__ b
             const int b = bb + e;
             uint64_t x = 0ull;
             if (b > 0 && b < nb) {
-                x = c[b - 1];
-                uint64_t y = c[b];
-                const uint64_t z = b + 1 < nb ? c[b + 1] : ~0ull;
+                x = w[e];
+                uint64_t y = w[e + 1];
+                const uint64_t z = b + 1 < nb ? w[e + 2] : ~0ull;
                 if (x > y) swap_u64(x, y);
                 x = max_u64(x, min_u64(y, z));  // median of three
             }
@@ -517,11 +524,9 @@ __global__ __launch_bounds__(kTileThreads) void k_bucket_scatter(
         key[r] = idx < t.count ? keys[t.start + idx] : 0ull;
         b[r] = idx < t.count ? bid[t.start + idx] : 0u;
     }
+    // Unconditional, so the four go out together (b is 0 past the tile's end).
 #pragma unroll
-    for (int r = 0; r < kSortItems; ++r) {
-        const int idx = r * kTileThreads + tid;
-        base[r] = idx < t.count ? boff[b[r]] : 0u;
-    }
+    for (int r = 0; r < kSortItems; ++r) base[r] = boff[b[r]];
     const SegInfo sg = segs[t.seg];
     const int b0 = bkt0[t.seg], nb = bkt0[t.seg + 1] - b0;
     int wbase, wn;
