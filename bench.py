@@ -75,8 +75,8 @@ TUNE_TABLE = os.path.join(REPO, "galaxy-collision-simulator_amd", "gcs_amd",
 # HBM bytes per traversal launch from rocprofv3 FETCH_SIZE (x2.000) and
 # WRITE_SIZE (x1.000) passes, scaled by a device copy of known size; valid for
 # the (workload, leaf, block, waves per SIMD, group, team, local_kappa)
-# measured: 702,942 KiB fetched, 625,051 KiB written per launch.
-TRAVERSE_TRAFFIC = {("batched-256", 4, 256, 4, 96, 1, 4.0): 2.080e9}
+# measured: 704,515 KiB fetched, 625,298 KiB written per launch.
+TRAVERSE_TRAFFIC = {("batched-256", 4, 256, 4, 96, 1, 4.0): 2.083e9}
 
 
 def say(*a):
